@@ -1,0 +1,223 @@
+/*
+ * qublar.h -- C ABI of libqublar_hip.so, the MI355X simulated-annealing QUBO engine.
+ *
+ * Plain pointers and sizes; every function returns a status code (QB_OK = 0) and
+ * leaves a message for qb_last_error() on failure. Nothing calls exit(). Input
+ * arrays are copied; output buffers belong to the caller. A handle owns one HIP
+ * stream and is thread-compatible (one caller at a time per handle).
+ *
+ * Trajectory: sweeps visit bits by colour class (bits in one class share no ray
+ * and are not neighbours, so their updates commute), and every Metropolis draw is
+ * Philox4x32-10 keyed by (branch seed, sweep, bit). Branch b's seed is
+ * b + 1 + seed_offset (float path) or seed_base + r (integer path), as in the
+ * reference. Results are bit-identical to the CPU oracle run with the same
+ * colouring and keys.
+ *
+ * Reference interfaces each entry replaces are cited per declaration
+ * (paths relative to the QuBLAR tree).
+ */
+#ifndef QUBLAR_H
+#define QUBLAR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QB_OK 0
+#define QB_E_ARG 1      /* invalid argument */
+#define QB_E_HIP 2      /* HIP runtime error (replaces GISING_CUDA's exit(3), ising_gpu.cuh:44-52) */
+#define QB_E_REFUSED 3  /* the plan refuses the problem (Plan::refused, ising_gpu.cuh:93-94) */
+#define QB_E_NOMEM 4    /* device or host allocation failed */
+#define QB_E_LIMIT 5    /* a stated size limit is exceeded */
+#define QB_E_TIMEOUT 6  /* a device-side barrier wait exceeded its spin limit */
+
+#define QB_FP64 0       /* residuals carried in FP64 (the reference's arithmetic) */
+#define QB_FP32 1       /* residuals carried in FP32, FP64 recompute every k sweeps */
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* qb_last_error(void);
+/* Library ABI version (major * 10000 + minor * 100 + patch). */
+int qb_version(void);
+
+typedef struct {
+    char name[64];
+    char arch[32];
+    int compute_units;
+    int xcds;
+    int64_t hbm_bytes;
+    int lds_bytes_per_cu;
+} qb_device_info;
+int qb_get_device_info(qb_device_info* out);
+
+/* ==== float path: the ray x voxel QUBO ================================================ */
+
+/* BinaryProblem (src/engine.hpp:30-49): bit -> (ray, a) CSR, per-ray d and w, the
+   neighbour CSR, lambda, kappa and an optional per-bit linear field. */
+typedef struct {
+    int n_vars, n_rays;
+    const int32_t* row_ptr;   /* n_vars + 1 */
+    const int32_t* ray;       /* row_ptr[n_vars] */
+    const float* a;
+    const double* d;          /* n_rays */
+    const double* w;          /* n_rays */
+    const int32_t* nbr_ptr;   /* n_vars + 1 */
+    const int32_t* nbr;
+    double lambda, kappa;
+    const double* field;      /* n_vars, or NULL: lin(i) = kappa */
+} qb_problem_desc;
+
+typedef struct qb_problem qb_problem;
+
+/* Copies the problem to the device in colour-class layout. Replaces constructing a
+   BinaryProblem for branch_fractions (src/ising_recon.hpp:148-151). */
+int qb_problem_create(const qb_problem_desc* desc, qb_problem** out);
+int qb_problem_destroy(qb_problem* p);
+
+typedef struct {
+    int n_vars, n_rays;
+    int64_t nnz, n_nbr;
+    int n_classes;            /* colour classes = dependent steps per sweep */
+    int max_class, min_class;
+    int max_entries_per_bit;
+    int max_bits_per_ray;     /* lower bound on n_classes */
+    int64_t device_bytes;     /* structure bytes resident in HBM */
+} qb_problem_info;
+int qb_problem_get_info(const qb_problem* p, qb_problem_info* out);
+/* The colouring: order[n_vars] (bits grouped by class, ascending inside a class)
+   and class_ptr[n_classes + 1]. */
+int qb_problem_colouring(const qb_problem* p, int32_t* order, int32_t* class_ptr);
+/* The same colouring computed on the host without a device (class_ptr needs room for
+   n_vars + 1 entries). The visit order of the sweep is new; the reference visits bits
+   in index order (src/ising_recon.hpp:132). */
+int qb_colour_problem(const qb_problem_desc* desc, int32_t* order, int32_t* class_ptr, int* n_classes);
+
+/* Schedule (src/ising_recon.hpp:81-86) and Budget (src/ising_weighted.hpp:227-232). */
+typedef struct {
+    double t_hot, t_cold;
+    int anneal_sweeps, hold_sweeps;
+} qb_schedule;
+
+typedef struct {
+    int precision;            /* QB_FP64 or QB_FP32 */
+    int recompute_every;      /* > 0: residuals recomputed from scratch in FP64 after every k-th sweep */
+    int workgroups;           /* persistent workgroups; 0 = one per compute unit */
+    int sweeps_per_launch;    /* 0 = all sweeps in one launch; results do not depend on it */
+} qb_anneal_opts;
+
+typedef struct {
+    double wall_s;            /* state init to outputs on the host (excludes the problem upload) */
+    double kernel_ms;         /* sweep kernels only, HIP events on the handle's stream */
+    int launches;
+    double spin_updates;      /* branches x sweeps x n_vars */
+    double fp32_residual_err; /* max |r_carried - r_scratch_fp64| over rays and branches (FP32 carry) */
+} qb_report;
+
+/* n_branches branches with seeds b + 1 + seed_offset annealed from all-zero x:
+   branch_fractions (src/ising_recon.hpp:148-166) and anneal_branch (:121-144).
+   Outputs (each nullable): branches n_branches x n_vars bytes (0/1), p_void n_vars
+   floats (sum of branches / n_branches), tri n_vars bytes (classify, :302-309:
+   0 Exists, 1 NotThere, 2 Undecided), certainty n_vars floats (max(p, 1 - p)),
+   energy n_branches doubles (binary_energy of each final branch, FP64, :65-79). */
+int qb_anneal_branches(qb_problem* p, const qb_schedule* s, int n_branches, uint64_t seed_offset,
+                       const qb_anneal_opts* opts, uint8_t* branches, float* p_void, uint8_t* tri,
+                       float* certainty, double* energy, qb_report* rep);
+
+/* binary_energy (src/ising_recon.hpp:65-79) of K states (K x n_vars bytes, 0/1),
+   computed on the device in FP64; data and prior parts separately (each nullable). */
+int qb_energy(qb_problem* p, const uint8_t* X, int K, double* total, double* data, double* prior);
+
+/* ==== integer path: weighted MaxCut / Ising ========================================== */
+
+/* WeightedGraph edge list (src/ising_weighted.hpp:35-42), 0-indexed. */
+typedef struct {
+    int n;
+    int64_t m;
+    const int32_t* eu;
+    const int32_t* ev;
+    const int32_t* ew;
+} qb_graph_desc;
+
+typedef struct qb_graph qb_graph;
+
+/* Plan (src/ising_gpu.cuh:59-101): the same bound and statistics fields. */
+typedef struct {
+    int n;
+    int64_t m;
+    int n_pad;
+    int64_t B;                /* max_i sum_j |w_ij| */
+    int64_t max_w;
+    int field_bytes, weight_bytes;
+    int dense, shared_state;
+    double sigma;             /* sqrt(2 sum w^2 / n) */
+    int64_t w_min;
+    char refused[64];         /* "" when accepted */
+    int n_classes;
+} qb_plan;
+
+/* make_plan + DeviceGraph (ising_gpu.cuh:74-101, 278-343). */
+int qb_graph_create(const qb_graph_desc* desc, int force_global, qb_graph** out);
+int qb_graph_destroy(qb_graph* g);
+int qb_graph_get_plan(const qb_graph* g, qb_plan* out);
+int qb_graph_colouring(const qb_graph* g, int32_t* order, int32_t* class_ptr);
+/* The graph colouring on the host without a device (class_ptr: room for n + 1);
+   the reference visits vertices in index order (src/ising_weighted.hpp:258). */
+int qb_colour_graph(const qb_graph_desc* desc, int32_t* order, int32_t* class_ptr, int* n_classes);
+
+typedef struct {
+    double wall_s;
+    double kernel_ms;
+    int launches;
+    int sweeps_per_launch;
+} qb_batch_report;
+
+/* run_batch (ising_gpu.cuh:396-458): replicas with seeds seed_base .. seed_base + R - 1.
+   s_out: R x n_pad spins (+1 / -1, original bit order); energy_out: R tracked
+   energies (-cut). updates_per_launch bounds one launch's work and never changes results. */
+int qb_run_batch(qb_graph* g, const qb_schedule* b, uint64_t seed_base, int R, double updates_per_launch,
+                 int8_t* s_out, int64_t* energy_out, qb_batch_report* rep);
+
+/* device_energies (ising_gpu.cuh:461-475): -cut of K states (K x n bytes, 0/1). */
+int qb_device_energies(qb_graph* g, const uint8_t* X, int K, int64_t* energy_out);
+
+/* ==== problem builders (host) ======================================================== */
+
+typedef struct qb_host_problem qb_host_problem;
+
+/* The muon-tomography scene: a square pyramid (height 139 m, half-base 115 m) in a
+   128^3 grid of 2 m voxels, three point chambers, 360 x 70 sky bins to 70 degrees,
+   counts det ~ Poisson(N0 cos^2(theta) exp(-mu L)) per bin, a cubic void, and the
+   QUBO of src/ising_recon.hpp:52-62 over the rock voxels outside 3 m rooms. */
+typedef struct {
+    int log2_muons;           /* muons per chamber = 2^log2_muons (27) */
+    double rho;               /* rock density, g/cm^3 (2.0) */
+    double mu_per_rho;        /* attenuation per density, 1/m per g/cm^3 (0.023: mu = 0.046 /m) */
+    double void_side;         /* cube edge, m (30) */
+    double void_z;            /* cube centre height, m (77) */
+    double lambda, kappa;     /* prior (2, ln 999) */
+    double min_open;          /* bins with fewer open-sky muons are dropped (30) */
+    uint64_t seed;            /* Poisson draws */
+    int grid;                 /* voxels per axis (128); smaller grids scale the scene */
+    int sky_az, sky_th;       /* sky bins per chamber: azimuth x zenith (360 x 70) */
+} qb_muon_scene;
+
+int qb_muon_scene_defaults(qb_muon_scene* s);
+int qb_scene_muon(const qb_muon_scene* s, qb_host_problem** out);
+/* Pointers into the host problem, valid until it is destroyed. */
+int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* view);
+/* truth[n_vars]: 1 where the bit's voxel is inside the void. */
+int qb_host_problem_truth(const qb_host_problem* h, uint8_t* truth);
+int qb_host_problem_destroy(qb_host_problem* h);
+
+/* The Erdos-Renyi max-cut workload: splitmix64 from `seed` over pairs i < j in order;
+   a pair is an edge when u < p, and its weight is +1 when the next draw's top bit is
+   0, else -1. Fills up to cap edges; *m_out = edge count. */
+int qb_scene_er_graph(int n, double p, uint64_t seed, int32_t* eu, int32_t* ev, int32_t* ew, int64_t cap,
+                      int64_t* m_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -46,7 +46,8 @@ class Schedule(C.Structure):
 
 class Mode(C.Structure):
     _fields_ = [("rng", C.c_int), ("order", C.c_void_p), ("class_ptr", C.c_void_p),
-                ("n_classes", C.c_int), ("precision", C.c_int), ("recompute_every", C.c_int)]
+                ("n_classes", C.c_int), ("precision", C.c_int), ("recompute_every", C.c_int),
+                ("segment", C.c_int)]
 
 
 class Graph(C.Structure):
@@ -181,14 +182,20 @@ class IntGraph:
                        _ptr(self.col), _ptr(self.w))
 
 
-def make_mode(rng=RNG_REFERENCE, order=None, class_ptr=None, precision=FP64, recompute_every=0):
+ENGINE_SEGMENT = 16   # the MI355X engine sums dE in chains of 16 entries
+
+
+def make_mode(rng=RNG_REFERENCE, order=None, class_ptr=None, precision=FP64, recompute_every=0, segment=None):
+    """segment=None: 16 (the engine's order) for the Philox trajectory, 0 for the reference's."""
+    if segment is None:
+        segment = ENGINE_SEGMENT if rng == RNG_PHILOX else 0
     keep = []
     if order is not None:
         order = np.ascontiguousarray(order, np.int32)
         class_ptr = np.ascontiguousarray(class_ptr, np.int32)
         keep = [order, class_ptr]
     m = Mode(rng, _ptr(order), _ptr(class_ptr), 0 if class_ptr is None else len(class_ptr) - 1,
-             precision, recompute_every)
+             precision, recompute_every, segment)
     m._keep = keep
     return m
 

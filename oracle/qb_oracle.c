@@ -152,13 +152,32 @@ double orc_local_field_dE(const orc_problem* p, const uint8_t* x, const double* 
     return dE;
 }
 
-/* The same dE, on FP32 residuals promoted to double (the FP32 carry). */
-static double dE_fp32(const orc_problem* p, const uint8_t* x, const float* r, int i) {
+/* The data term of entry k: w (r a + a^2 / 2) with a = a_k delta (ising_recon.hpp:110-111);
+   r64 or r32 holds the carried residuals. */
+static double term(const orc_problem* p, const double* r64, const float* r32, int k, double delta) {
+    const double a = (double)p->a[k] * delta;
+    const double rv = r32 ? (double)r32[p->ray[k]] : r64[p->ray[k]];
+    return p->w[p->ray[k]] * (rv * a + 0.5 * a * a);
+}
+
+/* dE under a mode: the entries summed in one chain (segment == 0, the reference's
+   order) or in chains of `segment` entries whose partial sums are added in order,
+   the first chain seeded with lin * delta; then the neighbour terms in CSR order. */
+static double mode_dE(const orc_problem* p, const uint8_t* x, const double* r64, const float* r32, int i,
+                      int segment) {
     const double delta = x[i] ? -1.0 : 1.0;
+    const int k0 = p->row_ptr[i], k1 = p->row_ptr[i + 1];
     double dE = lin_of(p, i) * delta;
-    for (int k = p->row_ptr[i]; k < p->row_ptr[i + 1]; ++k) {
-        const double a = (double)p->a[k] * delta;
-        dE += p->w[p->ray[k]] * ((double)r[p->ray[k]] * a + 0.5 * a * a);
+    if (segment <= 0 || k1 - k0 <= segment) {
+        for (int k = k0; k < k1; ++k) dE += term(p, r64, r32, k, delta);
+    } else {
+        for (int k = k0; k < k0 + segment; ++k) dE += term(p, r64, r32, k, delta);
+        for (int s0 = k0 + segment; s0 < k1; s0 += segment) {
+            const int s1 = s0 + segment < k1 ? s0 + segment : k1;
+            double part = term(p, r64, r32, s0, delta);
+            for (int k = s0 + 1; k < s1; ++k) part += term(p, r64, r32, k, delta);
+            dE += part;
+        }
     }
     for (int k = p->nbr_ptr[i]; k < p->nbr_ptr[i + 1]; ++k) {
         const int before = x[p->nbr[k]] != x[i];
@@ -186,7 +205,7 @@ void orc_anneal_branch(const orc_problem* p, const orc_schedule* s, uint64_t see
         for (int j = 0; j < n_steps; ++j) {
             const int i = m->order ? m->order[j] : j;
             const double delta = x[i] ? -1.0 : 1.0;
-            const double dE = r32 ? dE_fp32(p, x, r32, i) : orc_local_field_dE(p, x, r64, i);
+            const double dE = mode_dE(p, x, r64, r32, i, m->segment);
             int accept;
             if (m->rng == ORC_RNG_REFERENCE)
                 accept = dE <= 0.0 || orc_uniform(&state) < exp(-dE / temp);

@@ -66,6 +66,9 @@ typedef struct {
     int n_classes;
     int precision;         /* ORC_FP64 / ORC_FP32 residual carry */
     int recompute_every;   /* >0: FP64 from-scratch residuals after every k-th sweep */
+    int segment;           /* 0: dE sums a bit's entries in one sequential chain (the
+                              reference); S > 0: in chains of S entries, the first seeded
+                              with lin*delta, the chains then added in order */
 } orc_mode;
 
 /* ---- primitives ---- */

@@ -1,0 +1,353 @@
+"""Python binding of libqublar_hip.so, the MI355X simulated-annealing QUBO engine.
+
+The engine is the C ABI of include/qublar.h; this module is a thin ctypes layer for
+tests, benchmarks and scripting. Every call runs the HIP library: there is no CPU
+fallback, and importing fails loudly when the library has not been built.
+
+Mirrors the reference's entry points:
+  Problem.anneal_branches   branch_fractions / anneal_branch (src/ising_recon.hpp:121-166)
+  Problem.energy            binary_energy (src/ising_recon.hpp:65-79)
+  Graph.run_batch           gising::run_batch (src/ising_gpu.cuh:396-458)
+  Graph.device_energies     gising::device_energies (src/ising_gpu.cuh:461-475)
+  Graph.plan                gising::make_plan (src/ising_gpu.cuh:74-101)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqublar_hip.so")
+
+QB_OK, QB_E_ARG, QB_E_HIP, QB_E_REFUSED, QB_E_NOMEM, QB_E_LIMIT, QB_E_TIMEOUT = range(7)
+FP64, FP32 = 0, 1
+
+
+class QublarError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qublar error {code}: {msg}")
+        self.code = code
+
+
+def build(jobs: int = 8) -> str:
+    """Compile libqublar_hip.so for gfx950 in-tree (always recompiles what changed)."""
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc"), f"-j{jobs}"], check=True)
+    return LIB_PATH
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [("n_vars", C.c_int), ("n_rays", C.c_int),
+                ("row_ptr", C.c_void_p), ("ray", C.c_void_p), ("a", C.c_void_p),
+                ("d", C.c_void_p), ("w", C.c_void_p),
+                ("nbr_ptr", C.c_void_p), ("nbr", C.c_void_p),
+                ("lam", C.c_double), ("kappa", C.c_double), ("field", C.c_void_p)]
+
+
+class ProblemInfo(C.Structure):
+    _fields_ = [("n_vars", C.c_int), ("n_rays", C.c_int), ("nnz", C.c_int64), ("n_nbr", C.c_int64),
+                ("n_classes", C.c_int), ("max_class", C.c_int), ("min_class", C.c_int),
+                ("max_entries_per_bit", C.c_int), ("max_bits_per_ray", C.c_int), ("device_bytes", C.c_int64)]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("t_hot", C.c_double), ("t_cold", C.c_double),
+                ("anneal_sweeps", C.c_int), ("hold_sweeps", C.c_int)]
+
+
+class AnnealOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("recompute_every", C.c_int), ("workgroups", C.c_int),
+                ("sweeps_per_launch", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
+                ("spin_updates", C.c_double), ("fp32_residual_err", C.c_double)]
+
+
+class GraphDesc(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", C.c_int64), ("eu", C.c_void_p), ("ev", C.c_void_p), ("ew", C.c_void_p)]
+
+
+class Plan(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", C.c_int64), ("n_pad", C.c_int), ("B", C.c_int64), ("max_w", C.c_int64),
+                ("field_bytes", C.c_int), ("weight_bytes", C.c_int), ("dense", C.c_int),
+                ("shared_state", C.c_int), ("sigma", C.c_double), ("w_min", C.c_int64),
+                ("refused", C.c_char * 64), ("n_classes", C.c_int)]
+
+
+class BatchReport(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
+                ("sweeps_per_launch", C.c_int)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int),
+                ("xcds", C.c_int), ("hbm_bytes", C.c_int64), ("lds_bytes_per_cu", C.c_int)]
+
+
+class MuonScene(C.Structure):
+    _fields_ = [("log2_muons", C.c_int), ("rho", C.c_double), ("mu_per_rho", C.c_double),
+                ("void_side", C.c_double), ("void_z", C.c_double), ("lam", C.c_double), ("kappa", C.c_double),
+                ("min_open", C.c_double), ("seed", C.c_uint64), ("grid", C.c_int), ("sky_az", C.c_int),
+                ("sky_th", C.c_int)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise QublarError(QB_E_ARG, f"{LIB_PATH} is not built; run qublar_amd.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, u64, dbl = C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_double
+        sig = {
+            "qb_last_error": (C.c_char_p, []),
+            "qb_version": (i32, []),
+            "qb_get_device_info": (i32, [C.POINTER(DeviceInfo)]),
+            "qb_problem_create": (i32, [C.POINTER(ProblemDesc), C.POINTER(vp)]),
+            "qb_problem_destroy": (i32, [vp]),
+            "qb_problem_get_info": (i32, [vp, C.POINTER(ProblemInfo)]),
+            "qb_problem_colouring": (i32, [vp, vp, vp]),
+            "qb_colour_problem": (i32, [C.POINTER(ProblemDesc), vp, vp, C.POINTER(i32)]),
+            "qb_anneal_branches": (i32, [vp, C.POINTER(Schedule), i32, u64, C.POINTER(AnnealOpts), vp, vp, vp, vp,
+                                         vp, C.POINTER(Report)]),
+            "qb_energy": (i32, [vp, vp, i32, vp, vp, vp]),
+            "qb_graph_create": (i32, [C.POINTER(GraphDesc), i32, C.POINTER(vp)]),
+            "qb_graph_destroy": (i32, [vp]),
+            "qb_graph_get_plan": (i32, [vp, C.POINTER(Plan)]),
+            "qb_graph_colouring": (i32, [vp, vp, vp]),
+            "qb_colour_graph": (i32, [C.POINTER(GraphDesc), vp, vp, C.POINTER(i32)]),
+            "qb_run_batch": (i32, [vp, C.POINTER(Schedule), u64, i32, dbl, vp, vp, C.POINTER(BatchReport)]),
+            "qb_device_energies": (i32, [vp, vp, i32, vp]),
+            "qb_muon_scene_defaults": (i32, [C.POINTER(MuonScene)]),
+            "qb_scene_muon": (i32, [C.POINTER(MuonScene), C.POINTER(vp)]),
+            "qb_host_problem_view": (i32, [vp, C.POINTER(ProblemDesc)]),
+            "qb_host_problem_truth": (i32, [vp, vp]),
+            "qb_host_problem_destroy": (i32, [vp]),
+            "qb_scene_er_graph": (i32, [i32, dbl, u64, vp, vp, vp, i64, C.POINTER(i64)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != QB_OK:
+        raise QublarError(rc, lib().qb_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_info():
+    d = DeviceInfo()
+    _check(lib().qb_get_device_info(C.byref(d)))
+    return dict(name=d.name.decode(), arch=d.arch.decode(), compute_units=d.compute_units, xcds=d.xcds,
+                hbm_bytes=d.hbm_bytes, lds_bytes_per_cu=d.lds_bytes_per_cu)
+
+
+def schedule(t_hot, t_cold, anneal, hold):
+    return Schedule(float(t_hot), float(t_cold), int(anneal), int(hold))
+
+
+class ProblemArrays:
+    """A BinaryProblem held as contiguous numpy arrays (src/engine.hpp:30-49)."""
+
+    def __init__(self, row_ptr, ray, a, d, w, nbr_ptr, nbr, lam, kappa, field=None, truth=None):
+        self.row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+        self.ray = np.ascontiguousarray(ray, np.int32)
+        self.a = np.ascontiguousarray(a, np.float32)
+        self.d = np.ascontiguousarray(d, np.float64)
+        self.w = np.ascontiguousarray(w, np.float64)
+        self.nbr_ptr = np.ascontiguousarray(nbr_ptr, np.int32)
+        self.nbr = np.ascontiguousarray(nbr, np.int32)
+        self.lam = float(lam)
+        self.kappa = float(kappa)
+        self.field = None if field is None else np.ascontiguousarray(field, np.float64)
+        self.truth = truth
+        self.n_vars = len(self.row_ptr) - 1
+        self.n_rays = len(self.d)
+
+    def desc(self):
+        return ProblemDesc(self.n_vars, self.n_rays, _p(self.row_ptr), _p(self.ray), _p(self.a), _p(self.d),
+                           _p(self.w), _p(self.nbr_ptr), _p(self.nbr), self.lam, self.kappa, _p(self.field))
+
+    def kwargs(self):
+        return dict(row_ptr=self.row_ptr, ray=self.ray, a=self.a, d=self.d, w=self.w, nbr_ptr=self.nbr_ptr,
+                    nbr=self.nbr, lam=self.lam, kappa=self.kappa, field=self.field)
+
+    def colouring(self):
+        """(order, class_ptr) of the engine's colour classes, computed on the host."""
+        order = np.zeros(self.n_vars, np.int32)
+        cp = np.zeros(self.n_vars + 1, np.int32)
+        k = C.c_int()
+        d = self.desc()
+        _check(lib().qb_colour_problem(C.byref(d), _p(order), _p(cp), C.byref(k)))
+        return order, cp[:k.value + 1].copy()
+
+
+class Problem:
+    """A float ray x voxel QUBO resident on the device."""
+
+    def __init__(self, arrays: ProblemArrays):
+        self.arrays = arrays
+        self.h = C.c_void_p()
+        d = arrays.desc()
+        _check(lib().qb_problem_create(C.byref(d), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qb_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = ProblemInfo()
+        _check(lib().qb_problem_get_info(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in ProblemInfo._fields_}
+
+    def colouring(self):
+        n = self.arrays.n_vars
+        info = self.info()
+        order = np.zeros(n, np.int32)
+        cp = np.zeros(info["n_classes"] + 1, np.int32)
+        _check(lib().qb_problem_colouring(self.h, _p(order), _p(cp)))
+        return order, cp
+
+    def anneal_branches(self, sched, n_branches, seed_offset=0, precision=FP64, recompute_every=0, workgroups=0,
+                        sweeps_per_launch=0, branches=True, outputs=True, energy=True):
+        n = self.arrays.n_vars
+        br = np.zeros((n_branches, n), np.uint8) if branches else None
+        pv = np.zeros(n, np.float32) if outputs else None
+        tri = np.zeros(n, np.uint8) if outputs else None
+        cert = np.zeros(n, np.float32) if outputs else None
+        en = np.zeros(n_branches, np.float64) if energy else None
+        opts = AnnealOpts(precision, recompute_every, workgroups, sweeps_per_launch)
+        rep = Report()
+        _check(lib().qb_anneal_branches(self.h, C.byref(sched), n_branches, C.c_uint64(seed_offset), C.byref(opts),
+                                        _p(br), _p(pv), _p(tri), _p(cert), _p(en), C.byref(rep)))
+        return dict(branches=br, p_void=pv, tri=tri, certainty=cert, energy=en,
+                    report={f: getattr(rep, f) for f, _ in Report._fields_})
+
+    def energy(self, X):
+        X = np.ascontiguousarray(X, np.uint8)
+        K = X.shape[0]
+        tot, dat, pri = np.zeros(K), np.zeros(K), np.zeros(K)
+        _check(lib().qb_energy(self.h, _p(X), K, _p(tot), _p(dat), _p(pri)))
+        return tot, dat, pri
+
+
+class GraphArrays:
+    def __init__(self, n, eu, ev, ew):
+        self.n = int(n)
+        self.eu = np.ascontiguousarray(eu, np.int32)
+        self.ev = np.ascontiguousarray(ev, np.int32)
+        self.ew = np.ascontiguousarray(ew, np.int32)
+        self.m = len(self.eu)
+
+    def desc(self):
+        return GraphDesc(self.n, self.m, _p(self.eu), _p(self.ev), _p(self.ew))
+
+    def colouring(self):
+        order = np.zeros(self.n, np.int32)
+        cp = np.zeros(self.n + 1, np.int32)
+        k = C.c_int()
+        d = self.desc()
+        _check(lib().qb_colour_graph(C.byref(d), _p(order), _p(cp), C.byref(k)))
+        return order, cp[:k.value + 1].copy()
+
+
+class Graph:
+    """A weighted integer graph resident on the device (DeviceGraph)."""
+
+    def __init__(self, arrays: GraphArrays, force_global=False):
+        self.arrays = arrays
+        self.h = C.c_void_p()
+        d = arrays.desc()
+        _check(lib().qb_graph_create(C.byref(d), int(force_global), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qb_graph_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self):
+        p = Plan()
+        _check(lib().qb_graph_get_plan(self.h, C.byref(p)))
+        out = {f: getattr(p, f) for f, _ in Plan._fields_}
+        out["refused"] = p.refused.decode()
+        return out
+
+    def run_batch(self, sched, seed_base, R, updates_per_launch=1e9):
+        n_pad = self.plan()["n_pad"]
+        s = np.zeros((R, n_pad), np.int8)
+        e = np.zeros(R, np.int64)
+        rep = BatchReport()
+        _check(lib().qb_run_batch(self.h, C.byref(sched), C.c_uint64(seed_base), R, float(updates_per_launch),
+                                  _p(s), _p(e), C.byref(rep)))
+        return s, e, {f: getattr(rep, f) for f, _ in BatchReport._fields_}
+
+    def device_energies(self, X):
+        X = np.ascontiguousarray(X, np.uint8)
+        out = np.zeros(X.shape[0], np.int64)
+        _check(lib().qb_device_energies(self.h, _p(X), X.shape[0], _p(out)))
+        return out
+
+
+def muon_scene(**overrides):
+    """The muon-tomography QUBO (host builder); returns ProblemArrays with .truth."""
+    s = MuonScene()
+    _check(lib().qb_muon_scene_defaults(C.byref(s)))
+    for k, v in overrides.items():
+        setattr(s, "lam" if k == "lambda" else k, v)
+    h = C.c_void_p()
+    _check(lib().qb_scene_muon(C.byref(s), C.byref(h)))
+    try:
+        v = ProblemDesc()
+        _check(lib().qb_host_problem_view(h, C.byref(v)))
+        n, nr = v.n_vars, v.n_rays
+
+        def arr(ptr, count, dt):
+            if count == 0:
+                return np.zeros(0, dt)
+            ct = np.ctypeslib.as_ctypes_type(dt)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(count,)).copy()
+
+        row_ptr = arr(v.row_ptr, n + 1, np.int32)
+        nbr_ptr = arr(v.nbr_ptr, n + 1, np.int32)
+        truth = np.zeros(n, np.uint8)
+        _check(lib().qb_host_problem_truth(h, _p(truth)))
+        return ProblemArrays(row_ptr, arr(v.ray, int(row_ptr[-1]), np.int32), arr(v.a, int(row_ptr[-1]), np.float32),
+                             arr(v.d, nr, np.float64), arr(v.w, nr, np.float64), nbr_ptr,
+                             arr(v.nbr, int(nbr_ptr[-1]), np.int32), v.lam, v.kappa, None, truth)
+    finally:
+        lib().qb_host_problem_destroy(h)
+
+
+def er_graph(n=800, p=0.06, seed=1):
+    """The Erdos-Renyi +-1 max-cut workload (host builder)."""
+    cap = int(n * (n - 1) // 2 * min(1.0, p * 1.5 + 0.01)) + 1024
+    eu = np.zeros(cap, np.int32); ev = np.zeros(cap, np.int32); ew = np.zeros(cap, np.int32)
+    m = C.c_int64()
+    _check(lib().qb_scene_er_graph(n, float(p), C.c_uint64(seed), _p(eu), _p(ev), _p(ew), cap, C.byref(m)))
+    return GraphArrays(n, eu[:m.value], ev[:m.value], ew[:m.value])

@@ -1,0 +1,785 @@
+// qb_host.cpp -- host side of libqublar_hip.so: argument checks, the colouring and
+// colour-class layout, device residency, and the C ABI of include/qublar.h.
+#include "qublar.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "qb_internal.h"
+
+namespace qb {
+int ray_sweep_max_workgroups();
+}
+
+using namespace qb;
+
+// ---- errors -------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define QB_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(QB_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                \
+    } while (0)
+
+extern "C" const char* qb_last_error(void) { return g_err.c_str(); }
+extern "C" int qb_version(void) { return 100; }
+
+// Device buffer that frees itself.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int reserve(size_t n) {
+        if (n <= bytes) return QB_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipMalloc(&p, n ? n : 1) != hipSuccess) { p = nullptr; return fail(QB_E_NOMEM, "hipMalloc(%zu) failed", n); }
+        bytes = n;
+        return QB_OK;
+    }
+    template <typename T>
+    int upload(const std::vector<T>& v, hipStream_t st) {
+        int rc = reserve(sizeof(T) * v.size());
+        if (rc) return rc;
+        if (!v.empty() && hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(QB_E_HIP, "hipMemcpyAsync H2D failed");
+        return QB_OK;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+static int device_props(hipDeviceProp_t& prop) {
+    int dev = 0;
+    QB_HIP(hipGetDevice(&dev));
+    QB_HIP(hipGetDeviceProperties(&prop, dev));
+    return QB_OK;
+}
+
+extern "C" int qb_get_device_info(qb_device_info* out) {
+    if (!out) return fail(QB_E_ARG, "qb_get_device_info: null output");
+    hipDeviceProp_t prop;
+    int rc = device_props(prop);
+    if (rc) return rc;
+    std::memset(out, 0, sizeof *out);
+    std::snprintf(out->name, sizeof out->name, "%s", prop.name);
+    std::snprintf(out->arch, sizeof out->arch, "%s", prop.gcnArchName);
+    out->compute_units = prop.multiProcessorCount;
+    out->xcds = prop.multiProcessorCount >= 256 ? 8 : 1;
+    out->hbm_bytes = int64_t(prop.totalGlobalMem);
+    out->lds_bytes_per_cu = int(prop.maxSharedMemoryPerMultiProcessor);
+    return QB_OK;
+}
+
+// ---- schedule -----------------------------------------------------------------------
+// T(s) = t_hot (t_cold / t_hot)^min(1, s / max(1, A - 1)) for s < A, then t_cold
+// (src/ising_recon.hpp:129-131); the engine uses beta = 1 / T.
+static std::vector<double> betas_of(const qb_schedule& s) {
+    const int total = s.anneal_sweeps + s.hold_sweeps;
+    std::vector<double> beta(size_t(std::max(total, 0)));
+    for (int sweep = 0; sweep < total; ++sweep) {
+        const double frac = std::min(1.0, double(sweep) / double(std::max(1, s.anneal_sweeps - 1)));
+        const double temp = sweep < s.anneal_sweeps ? s.t_hot * std::pow(s.t_cold / s.t_hot, frac) : s.t_cold;
+        beta[size_t(sweep)] = 1.0 / temp;
+    }
+    return beta;
+}
+
+static int check_schedule(const qb_schedule* s) {
+    if (!s) return fail(QB_E_ARG, "null schedule");
+    if (s->anneal_sweeps < 0 || s->hold_sweeps < 0 || s->anneal_sweeps + s->hold_sweeps <= 0)
+        return fail(QB_E_ARG, "schedule needs at least one sweep (anneal %d, hold %d)", s->anneal_sweeps,
+                    s->hold_sweeps);
+    if (!(s->t_hot > 0.0) || !(s->t_cold > 0.0) || !std::isfinite(s->t_hot) || !std::isfinite(s->t_cold))
+        return fail(QB_E_ARG, "temperatures must be positive and finite (t_hot %g, t_cold %g)", s->t_hot, s->t_cold);
+    return QB_OK;
+}
+
+// ---- greedy colouring ---------------------------------------------------------------
+// Bits in `visit` order take the smallest colour not used by any bit they conflict
+// with; conflicts(i, mark) calls mark(j) for every conflicting bit j.
+template <typename Conflicts>
+static int greedy_colour(int n, const std::vector<int>& visit, Conflicts conflicts, std::vector<int>& colour) {
+    colour.assign(size_t(n), -1);
+    std::vector<int> stamp;
+    int n_colours = 0;
+    for (int step = 0; step < n; ++step) {
+        const int i = visit[size_t(step)];
+        conflicts(i, [&](int j) {
+            const int c = colour[size_t(j)];
+            if (c >= 0 && j != i) {
+                if (size_t(c) >= stamp.size()) stamp.resize(size_t(c) + 64, -1);
+                stamp[size_t(c)] = i;
+            }
+        });
+        int c = 0;
+        while (size_t(c) < stamp.size() && stamp[size_t(c)] == i) ++c;
+        colour[size_t(i)] = c;
+        n_colours = std::max(n_colours, c + 1);
+    }
+    return n_colours;
+}
+
+// Positions: bits sorted by (colour, index).
+static void order_by_colour(const std::vector<int>& colour, int n_colours, std::vector<int32_t>& pos_bit,
+                            std::vector<int32_t>& cls_ptr) {
+    cls_ptr.assign(size_t(n_colours) + 1, 0);
+    for (int c : colour) cls_ptr[size_t(c) + 1]++;
+    for (int c = 0; c < n_colours; ++c) cls_ptr[size_t(c) + 1] += cls_ptr[size_t(c)];
+    std::vector<int32_t> fill(cls_ptr.begin(), cls_ptr.end() - 1);
+    pos_bit.assign(colour.size(), 0);
+    for (size_t i = 0; i < colour.size(); ++i) pos_bit[size_t(fill[size_t(colour[i])]++)] = int32_t(i);
+}
+
+// ==== float path =======================================================================
+
+struct qb_problem {
+    hipStream_t stream = nullptr;
+    DevProblem dev{};
+    int n = 0, n_rays = 0, n_classes = 0;
+    int64_t nnz = 0, n_nbr = 0;
+    int max_class = 0, min_class = 0, max_entries = 0, max_bits_per_ray = 0;
+    int64_t device_bytes = 0;
+    std::vector<int32_t> pos_bit, cls_ptr;
+    DevBuf b_cls, b_heavy, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq;
+    // workspace
+    DevBuf w_x, w_r, w_beta, w_bar, w_err, w_part, w_pv, w_tri, w_cert, w_br, w_data, w_prior, w_rerr, w_X;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~qb_problem() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static int validate_problem(const qb_problem_desc* d) {
+    if (!d) return fail(QB_E_ARG, "null problem description");
+    if (d->n_vars < 0 || d->n_rays < 0) return fail(QB_E_ARG, "negative sizes (n_vars %d, n_rays %d)", d->n_vars, d->n_rays);
+    if (!d->row_ptr || !d->nbr_ptr) return fail(QB_E_ARG, "row_ptr and nbr_ptr are required");
+    if (d->n_rays > 0 && (!d->d || !d->w)) return fail(QB_E_ARG, "d and w are required when n_rays > 0");
+    if (d->row_ptr[0] != 0 || d->nbr_ptr[0] != 0) return fail(QB_E_ARG, "row_ptr[0] and nbr_ptr[0] must be 0");
+    for (int i = 0; i < d->n_vars; ++i) {
+        if (d->row_ptr[i + 1] < d->row_ptr[i]) return fail(QB_E_ARG, "row_ptr decreases at bit %d", i);
+        if (d->nbr_ptr[i + 1] < d->nbr_ptr[i]) return fail(QB_E_ARG, "nbr_ptr decreases at bit %d", i);
+    }
+    const int64_t nnz = d->row_ptr[d->n_vars], nn = d->nbr_ptr[d->n_vars];
+    if (nnz > 0 && (!d->ray || !d->a)) return fail(QB_E_ARG, "ray and a are required when entries exist");
+    if (nn > 0 && !d->nbr) return fail(QB_E_ARG, "nbr is required when neighbours exist");
+    for (int64_t k = 0; k < nnz; ++k)
+        if (d->ray[k] < 0 || d->ray[k] >= d->n_rays) return fail(QB_E_ARG, "entry %lld names ray %d of %d", (long long)k, d->ray[k], d->n_rays);
+    for (int64_t k = 0; k < nn; ++k)
+        if (d->nbr[k] < 0 || d->nbr[k] >= d->n_vars) return fail(QB_E_ARG, "neighbour entry %lld names bit %d of %d", (long long)k, d->nbr[k], d->n_vars);
+    return QB_OK;
+}
+
+// Distance-2 colouring of the bit-ray incidence plus the neighbour graph: two bits
+// share a colour only if they share no ray and are not neighbours. Bits are visited
+// by decreasing conflict weight (their rays' lengths plus neighbours), ties by index.
+static int colour_problem(const qb_problem_desc* desc, const std::vector<int32_t>& rt_ptr,
+                          const std::vector<int32_t>& rt_bit, std::vector<int32_t>& pos_bit,
+                          std::vector<int32_t>& cls_ptr, int& max_bits_per_ray) {
+    const int n = desc->n_vars, nr = desc->n_rays;
+    std::vector<int64_t> weight(static_cast<size_t>(n), 0);
+    max_bits_per_ray = 0;
+    for (int b = 0; b < nr; ++b) max_bits_per_ray = std::max(max_bits_per_ray, rt_ptr[size_t(b) + 1] - rt_ptr[size_t(b)]);
+    for (int i = 0; i < n; ++i) {
+        int64_t wsum = desc->nbr_ptr[i + 1] - desc->nbr_ptr[i];
+        for (int k = desc->row_ptr[i]; k < desc->row_ptr[i + 1]; ++k)
+            wsum += rt_ptr[size_t(desc->ray[k]) + 1] - rt_ptr[size_t(desc->ray[k])];
+        weight[size_t(i)] = wsum;
+    }
+    std::vector<int> visit(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) visit[size_t(i)] = i;
+    std::stable_sort(visit.begin(), visit.end(), [&](int a, int b) { return weight[size_t(a)] > weight[size_t(b)]; });
+    std::vector<int> colour;
+    const int K = greedy_colour(n, visit, [&](int i, auto mark) {
+        for (int k = desc->row_ptr[i]; k < desc->row_ptr[i + 1]; ++k) {
+            const int b = desc->ray[k];
+            for (int q = rt_ptr[size_t(b)]; q < rt_ptr[size_t(b) + 1]; ++q) mark(rt_bit[size_t(q)]);
+        }
+        for (int k = desc->nbr_ptr[i]; k < desc->nbr_ptr[i + 1]; ++k) mark(desc->nbr[k]);
+    }, colour);
+    order_by_colour(colour, K, pos_bit, cls_ptr);
+    // inside a class (order there is free): heavy bits first, each part by index
+    for (int c = 0; c < K; ++c)
+        std::stable_partition(pos_bit.begin() + cls_ptr[size_t(c)], pos_bit.begin() + cls_ptr[size_t(c) + 1],
+                              [&](int32_t i) { return desc->row_ptr[i + 1] - desc->row_ptr[i] > kSeg; });
+    return K;
+}
+
+// ray -> bits in increasing bit index (then CSR order), with each entry's a.
+static void ray_lists(const qb_problem_desc* desc, std::vector<int32_t>& rt_ptr, std::vector<int32_t>& rt_bit,
+                      std::vector<float>& rt_a) {
+    const int n = desc->n_vars, nr = desc->n_rays;
+    const int64_t nnz = desc->row_ptr[n];
+    rt_ptr.assign(size_t(nr) + 1, 0);
+    for (int64_t k = 0; k < nnz; ++k) rt_ptr[size_t(desc->ray[k]) + 1]++;
+    for (int b = 0; b < nr; ++b) rt_ptr[size_t(b) + 1] += rt_ptr[size_t(b)];
+    rt_bit.assign(size_t(nnz), 0);
+    rt_a.assign(size_t(nnz), 0.0f);
+    std::vector<int32_t> fill(rt_ptr.begin(), rt_ptr.end() - 1);
+    for (int i = 0; i < n; ++i)
+        for (int k = desc->row_ptr[i]; k < desc->row_ptr[i + 1]; ++k) {
+            const int b = desc->ray[k];
+            rt_bit[size_t(fill[size_t(b)])] = i;
+            rt_a[size_t(fill[size_t(b)]++)] = desc->a[k];
+        }
+}
+
+extern "C" int qb_colour_problem(const qb_problem_desc* desc, int32_t* order, int32_t* class_ptr, int* n_classes) {
+    if (!order || !class_ptr || !n_classes) return fail(QB_E_ARG, "qb_colour_problem: null output");
+    int rc = validate_problem(desc);
+    if (rc) return rc;
+    std::vector<int32_t> rt_ptr, rt_bit, pos_bit, cls_ptr;
+    std::vector<float> rt_a;
+    ray_lists(desc, rt_ptr, rt_bit, rt_a);
+    int mbr = 0;
+    const int K = colour_problem(desc, rt_ptr, rt_bit, pos_bit, cls_ptr, mbr);
+    std::memcpy(order, pos_bit.data(), sizeof(int32_t) * pos_bit.size());
+    std::memcpy(class_ptr, cls_ptr.data(), sizeof(int32_t) * cls_ptr.size());
+    *n_classes = K;
+    return QB_OK;
+}
+
+extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) {
+    if (!out) return fail(QB_E_ARG, "qb_problem_create: null output");
+    *out = nullptr;
+    int rc = validate_problem(desc);
+    if (rc) return rc;
+    const int n = desc->n_vars, nr = desc->n_rays;
+    const int64_t nnz = desc->row_ptr[n], nn = desc->nbr_ptr[n];
+    std::vector<int32_t> rt_ptr, rt_bit;
+    std::vector<float> rt_a;
+    ray_lists(desc, rt_ptr, rt_bit, rt_a);
+    std::vector<int32_t> pos_bit, cls_ptr;
+    int max_bits_per_ray = 0;
+    const int K = colour_problem(desc, rt_ptr, rt_bit, pos_bit, cls_ptr, max_bits_per_ray);
+
+    auto P = new (std::nothrow) qb_problem();
+    if (!P) return fail(QB_E_NOMEM, "out of host memory");
+    std::unique_ptr<qb_problem> guard(P);
+    P->n = n;
+    P->n_rays = nr;
+    P->n_classes = K;
+    P->nnz = nnz;
+    P->n_nbr = nn;
+    P->max_bits_per_ray = max_bits_per_ray;
+    P->pos_bit = std::move(pos_bit);
+    P->cls_ptr = std::move(cls_ptr);
+    P->max_class = 0;
+    P->min_class = n;
+    for (int c = 0; c < K; ++c) {
+        const int sz = P->cls_ptr[size_t(c) + 1] - P->cls_ptr[size_t(c)];
+        P->max_class = std::max(P->max_class, sz);
+        P->min_class = std::min(P->min_class, sz);
+    }
+    std::vector<int32_t> pos_of(static_cast<size_t>(n));
+    for (int p = 0; p < n; ++p) pos_of[size_t(P->pos_bit[size_t(p)])] = p;
+    // colour-major CSC columns and neighbour lists
+    std::vector<int32_t> col_ptr(size_t(n) + 1, 0), nb_ptr(size_t(n) + 1, 0);
+    std::vector<RayEnt> ent(static_cast<size_t>(nnz));
+    std::vector<int32_t> nb(static_cast<size_t>(nn)), nb_bit(static_cast<size_t>(nn));
+    std::vector<double> lin(static_cast<size_t>(n));
+    for (int p = 0; p < n; ++p) {
+        const int i = P->pos_bit[size_t(p)];
+        int k0 = col_ptr[size_t(p)];
+        for (int k = desc->row_ptr[i]; k < desc->row_ptr[i + 1]; ++k, ++k0) {
+            const int b = desc->ray[k];
+            ent[size_t(k0)] = RayEnt{b, desc->a[k], desc->w[b]};
+        }
+        col_ptr[size_t(p) + 1] = k0;
+        P->max_entries = std::max(P->max_entries, desc->row_ptr[i + 1] - desc->row_ptr[i]);
+        int q0 = nb_ptr[size_t(p)];
+        for (int k = desc->nbr_ptr[i]; k < desc->nbr_ptr[i + 1]; ++k, ++q0) {
+            nb[size_t(q0)] = pos_of[size_t(desc->nbr[k])];
+            nb_bit[size_t(q0)] = desc->nbr[k];
+        }
+        nb_ptr[size_t(p) + 1] = q0;
+        lin[size_t(p)] = desc->field ? desc->field[i] : desc->kappa;
+    }
+    std::vector<int32_t> cls_heavy(static_cast<size_t>(K), 0);
+    for (int c = 0; c < K; ++c)
+        for (int q = P->cls_ptr[size_t(c)]; q < P->cls_ptr[size_t(c) + 1]; ++q)
+            if (col_ptr[size_t(q) + 1] - col_ptr[size_t(q)] > kSeg) cls_heavy[size_t(c)]++;
+    std::vector<uint8_t> seq_update(static_cast<size_t>(n), 0);
+    {
+        std::vector<int32_t> seen(static_cast<size_t>(nr), -1);
+        for (int p = 0; p < n; ++p)
+            for (int k = col_ptr[size_t(p)]; k < col_ptr[size_t(p) + 1]; ++k) {
+                const int b = ent[size_t(k)].ray;
+                if (seen[size_t(b)] == p) seq_update[size_t(p)] = 1;
+                seen[size_t(b)] = p;
+            }
+    }
+    std::vector<BitEnt> rt(static_cast<size_t>(nnz));
+    for (size_t q = 0; q < rt.size(); ++q) rt[q] = BitEnt{pos_of[size_t(rt_bit[q])], rt_a[q]};
+    std::vector<double> dv(desc->d, desc->d + nr), wv(desc->w, desc->w + nr);
+
+    QB_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    QB_HIP(hipEventCreate(&P->ev0));
+    QB_HIP(hipEventCreate(&P->ev1));
+    hipStream_t st = P->stream;
+    if ((rc = P->b_cls.upload(P->cls_ptr, st)) || (rc = P->b_heavy.upload(cls_heavy, st)) ||
+        (rc = P->b_seq.upload(seq_update, st)) || (rc = P->b_pos.upload(P->pos_bit, st)) ||
+        (rc = P->b_col.upload(col_ptr, st)) || (rc = P->b_ent.upload(ent, st)) || (rc = P->b_nbp.upload(nb_ptr, st)) ||
+        (rc = P->b_nb.upload(nb, st)) || (rc = P->b_nbbit.upload(nb_bit, st)) || (rc = P->b_lin.upload(lin, st)) ||
+        (rc = P->b_d.upload(dv, st)) || (rc = P->b_w.upload(wv, st)) || (rc = P->b_rtp.upload(rt_ptr, st)) ||
+        (rc = P->b_rt.upload(rt, st)))
+        return rc;
+    QB_HIP(hipStreamSynchronize(st));
+    P->device_bytes = int64_t(P->b_cls.bytes + P->b_heavy.bytes + P->b_seq.bytes + P->b_pos.bytes + P->b_col.bytes + P->b_ent.bytes + P->b_nbp.bytes +
+                              P->b_nb.bytes + P->b_nbbit.bytes + P->b_lin.bytes + P->b_d.bytes + P->b_w.bytes +
+                              P->b_rtp.bytes + P->b_rt.bytes);
+    DevProblem& D = P->dev;
+    D.n = n;
+    D.n_rays = nr;
+    D.n_classes = K;
+    D.nnz = nnz;
+    D.n_nbr = nn;
+    D.cls_ptr = P->b_cls.as<int32_t>();
+    D.cls_heavy = P->b_heavy.as<int32_t>();
+    D.seq_update = P->b_seq.as<uint8_t>();
+    D.pos_bit = P->b_pos.as<int32_t>();
+    D.col_ptr = P->b_col.as<int32_t>();
+    D.ent = P->b_ent.as<RayEnt>();
+    D.nb_ptr = P->b_nbp.as<int32_t>();
+    D.nb = P->b_nb.as<int32_t>();
+    D.nb_bit = P->b_nbbit.as<int32_t>();
+    D.lin = P->b_lin.as<double>();
+    D.d = P->b_d.as<double>();
+    D.w = P->b_w.as<double>();
+    D.rt_ptr = P->b_rtp.as<int32_t>();
+    D.rt = P->b_rt.as<BitEnt>();
+    D.lambda = desc->lambda;
+    *out = guard.release();
+    return QB_OK;
+}
+
+extern "C" int qb_problem_destroy(qb_problem* p) {
+    delete p;
+    return QB_OK;
+}
+
+extern "C" int qb_problem_get_info(const qb_problem* p, qb_problem_info* out) {
+    if (!p || !out) return fail(QB_E_ARG, "qb_problem_get_info: null argument");
+    out->n_vars = p->n;
+    out->n_rays = p->n_rays;
+    out->nnz = p->nnz;
+    out->n_nbr = p->n_nbr;
+    out->n_classes = p->n_classes;
+    out->max_class = p->max_class;
+    out->min_class = p->min_class;
+    out->max_entries_per_bit = p->max_entries;
+    out->max_bits_per_ray = p->max_bits_per_ray;
+    out->device_bytes = p->device_bytes;
+    return QB_OK;
+}
+
+extern "C" int qb_problem_colouring(const qb_problem* p, int32_t* order, int32_t* class_ptr) {
+    if (!p || !order || !class_ptr) return fail(QB_E_ARG, "qb_problem_colouring: null argument");
+    std::memcpy(order, p->pos_bit.data(), sizeof(int32_t) * p->pos_bit.size());
+    std::memcpy(class_ptr, p->cls_ptr.data(), sizeof(int32_t) * p->cls_ptr.size());
+    return QB_OK;
+}
+
+static int sweep_workgroups(int requested) {
+    hipDeviceProp_t prop;
+    int rc = device_props(prop);
+    if (rc) return -rc;
+    const int per_cu = ray_sweep_max_workgroups();
+    if (per_cu <= 0) { fail(QB_E_HIP, "occupancy query failed"); return -QB_E_HIP; }
+    const int cap = per_cu * prop.multiProcessorCount;
+    int g = requested > 0 ? requested : prop.multiProcessorCount;
+    return std::min(g, cap);
+}
+
+extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, uint64_t seed_offset,
+                                  const qb_anneal_opts* opts, uint8_t* branches, float* p_void, uint8_t* tri,
+                                  float* certainty, double* energy, qb_report* rep) {
+    if (!P) return fail(QB_E_ARG, "qb_anneal_branches: null problem");
+    int rc = check_schedule(s);
+    if (rc) return rc;
+    if (R <= 0) return fail(QB_E_ARG, "n_branches must be positive (got %d)", R);
+    qb_anneal_opts o{};
+    if (opts) o = *opts;
+    if (o.precision != QB_FP64 && o.precision != QB_FP32) return fail(QB_E_ARG, "precision must be QB_FP64 or QB_FP32");
+    if (o.recompute_every < 0 || o.sweeps_per_launch < 0 || o.workgroups < 0)
+        return fail(QB_E_ARG, "negative option");
+    const int total = s->anneal_sweeps + s->hold_sweeps;
+    const int n = P->n, nr = P->n_rays;
+    const int n_groups = (R + kGroup - 1) / kGroup;
+    const size_t tsz = o.precision == QB_FP32 ? sizeof(float) : sizeof(double);
+    const int G = sweep_workgroups(o.workgroups);
+    if (G <= 0) return -G;
+    const int n_part = ray_energy_blocks(P->dev);
+    if ((rc = P->w_x.reserve(sizeof(uint16_t) * size_t(n_groups) * size_t(std::max(n, 1)))) ||
+        (rc = P->w_r.reserve(tsz * size_t(n_groups) * size_t(std::max(nr, 1)) * kGroup)) ||
+        (rc = P->w_beta.reserve(sizeof(double) * size_t(total))) || (rc = P->w_bar.reserve(sizeof(unsigned))) ||
+        (rc = P->w_err.reserve(sizeof(int))) ||
+        (rc = P->w_part.reserve(sizeof(double) * size_t(n_part) * size_t(n_groups) * kGroup)) ||
+        (rc = P->w_pv.reserve(sizeof(float) * size_t(std::max(n, 1)))) || (rc = P->w_tri.reserve(size_t(std::max(n, 1)))) ||
+        (rc = P->w_cert.reserve(sizeof(float) * size_t(std::max(n, 1)))) ||
+        (rc = P->w_br.reserve(branches ? size_t(R) * size_t(std::max(n, 1)) : 1)) ||
+        (rc = P->w_data.reserve(sizeof(double) * size_t(R))) || (rc = P->w_prior.reserve(sizeof(double) * size_t(R))) ||
+        (rc = P->w_rerr.reserve(sizeof(float))))
+        return rc;
+    hipStream_t st = P->stream;
+    const std::vector<double> beta = betas_of(*s);
+    QB_HIP(hipStreamSynchronize(st));
+    const auto t0 = std::chrono::steady_clock::now();
+    QB_HIP(hipMemcpyAsync(P->w_beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice, st));
+    QB_HIP(launch_ray_init(P->dev, o.precision, P->w_x.p, P->w_r.p, n_groups, st));
+    QB_HIP(hipMemsetAsync(P->w_err.p, 0, sizeof(int), st));
+    const int chunk = o.sweeps_per_launch > 0 ? std::min(o.sweeps_per_launch, total) : total;
+    int launches = 0;
+    QB_HIP(hipEventRecord(P->ev0, st));
+    for (int sb = 0; sb < total; sb += chunk) {
+        QB_HIP(hipMemsetAsync(P->w_bar.p, 0, sizeof(unsigned), st));
+        QB_HIP(launch_ray_sweeps(P->dev, o.precision, P->w_x.p, P->w_r.p, n_groups, R, seed_offset,
+                                 P->w_beta.as<double>(), sb, std::min(total, sb + chunk), o.recompute_every,
+                                 P->w_bar.as<unsigned>(), P->w_err.as<int>(), G, st));
+        ++launches;
+    }
+    QB_HIP(hipEventRecord(P->ev1, st));
+    const bool want_outputs = p_void || tri || certainty || branches;
+    if (want_outputs)
+        QB_HIP(launch_ray_outputs(P->dev, P->w_x.as<uint16_t>(), n_groups, R, P->w_pv.as<float>(),
+                                  P->w_tri.as<uint8_t>(), P->w_cert.as<float>(),
+                                  branches ? P->w_br.as<uint8_t>() : nullptr, st));
+    const bool want_energy = energy || rep;
+    if (want_energy) {
+        QB_HIP(hipMemsetAsync(P->w_rerr.p, 0, sizeof(float), st));
+        QB_HIP(launch_ray_energy(P->dev, P->w_x.as<uint16_t>(), n_groups, R, P->w_r.p, o.precision,
+                                 P->w_part.as<double>(), n_part, P->w_data.as<double>(), P->w_prior.as<double>(),
+                                 P->w_rerr.as<float>(), st));
+    }
+    if (p_void) QB_HIP(hipMemcpyAsync(p_void, P->w_pv.p, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, st));
+    if (tri) QB_HIP(hipMemcpyAsync(tri, P->w_tri.p, size_t(n), hipMemcpyDeviceToHost, st));
+    if (certainty) QB_HIP(hipMemcpyAsync(certainty, P->w_cert.p, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, st));
+    if (branches) QB_HIP(hipMemcpyAsync(branches, P->w_br.p, size_t(R) * size_t(n), hipMemcpyDeviceToHost, st));
+    std::vector<double> dat(static_cast<size_t>(R)), pri(static_cast<size_t>(R));
+    float rerr = 0.0f;
+    int err_flag = 0;
+    if (want_energy) {
+        QB_HIP(hipMemcpyAsync(dat.data(), P->w_data.p, sizeof(double) * size_t(R), hipMemcpyDeviceToHost, st));
+        QB_HIP(hipMemcpyAsync(pri.data(), P->w_prior.p, sizeof(double) * size_t(R), hipMemcpyDeviceToHost, st));
+        QB_HIP(hipMemcpyAsync(&rerr, P->w_rerr.p, sizeof(float), hipMemcpyDeviceToHost, st));
+    }
+    QB_HIP(hipMemcpyAsync(&err_flag, P->w_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (err_flag) return fail(QB_E_TIMEOUT, "a sweep barrier wait exceeded its limit (%d workgroups)", G);
+    if (energy)
+        for (int b = 0; b < R; ++b) energy[b] = dat[size_t(b)] + pri[size_t(b)];
+    if (rep) {
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+        rep->wall_s = wall;
+        rep->kernel_ms = ms;
+        rep->launches = launches;
+        rep->spin_updates = double(R) * double(total) * double(n);
+        rep->fp32_residual_err = rerr;
+    }
+    return QB_OK;
+}
+
+extern "C" int qb_energy(qb_problem* P, const uint8_t* X, int K, double* total, double* data, double* prior) {
+    if (!P || (!X && K > 0)) return fail(QB_E_ARG, "qb_energy: null argument");
+    if (K <= 0) return fail(QB_E_ARG, "qb_energy: K must be positive");
+    const int n = P->n;
+    const int n_groups = (K + kGroup - 1) / kGroup;
+    const int n_part = ray_energy_blocks(P->dev);
+    int rc;
+    if ((rc = P->w_X.reserve(size_t(K) * size_t(std::max(n, 1)))) ||
+        (rc = P->w_x.reserve(sizeof(uint16_t) * size_t(n_groups) * size_t(std::max(n, 1)))) ||
+        (rc = P->w_part.reserve(sizeof(double) * size_t(n_part) * size_t(n_groups) * kGroup)) ||
+        (rc = P->w_data.reserve(sizeof(double) * size_t(K))) || (rc = P->w_prior.reserve(sizeof(double) * size_t(K))))
+        return rc;
+    hipStream_t st = P->stream;
+    QB_HIP(hipMemcpyAsync(P->w_X.p, X, size_t(K) * size_t(n), hipMemcpyHostToDevice, st));
+    if (n > 0) QB_HIP(launch_pack_states(P->w_X.as<uint8_t>(), K, n, P->dev.pos_bit, P->w_x.as<uint16_t>(), st));
+    QB_HIP(launch_ray_energy(P->dev, P->w_x.as<uint16_t>(), n_groups, K, nullptr, 0, P->w_part.as<double>(), n_part,
+                             P->w_data.as<double>(), P->w_prior.as<double>(), nullptr, st));
+    std::vector<double> dat(static_cast<size_t>(K)), pri(static_cast<size_t>(K));
+    QB_HIP(hipMemcpyAsync(dat.data(), P->w_data.p, sizeof(double) * size_t(K), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipMemcpyAsync(pri.data(), P->w_prior.p, sizeof(double) * size_t(K), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < K; ++k) {
+        if (total) total[k] = dat[size_t(k)] + pri[size_t(k)];
+        if (data) data[k] = dat[size_t(k)];
+        if (prior) prior[k] = pri[size_t(k)];
+    }
+    return QB_OK;
+}
+
+// ==== integer path =====================================================================
+
+struct qb_graph {
+    hipStream_t stream = nullptr;
+    DevGraph dev{};
+    qb_plan plan{};
+    bool lds_state = false;
+    std::vector<int32_t> pos_bit, cls_ptr;
+    DevBuf b_cls, b_pos, b_nbp, b_nb, b_nw, b_f0, b_eu, b_ev, b_ew;
+    DevBuf w_s, w_f, w_e, w_sout, w_beta, w_X, w_out;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~qb_graph() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// make_plan (src/ising_gpu.cuh:74-101): bounds and statistics of the weights.
+static void make_plan(int n, int64_t m, const int32_t* ew, const std::vector<int64_t>& row, const std::vector<int32_t>& w,
+                      qb_plan& p) {
+    std::memset(&p, 0, sizeof p);
+    p.n = n;
+    p.m = m;
+    p.n_pad = (n + 15) / 16 * 16;
+    double sum_w2 = 0.0;
+    for (int64_t e = 0; e < m; ++e) {
+        const int64_t a = ew[e] < 0 ? -int64_t(ew[e]) : int64_t(ew[e]);
+        sum_w2 += double(a) * double(a);
+        if (a > p.max_w) p.max_w = a;
+        if (a != 0 && (p.w_min == 0 || a < p.w_min)) p.w_min = a;
+    }
+    for (int i = 0; i < n; ++i) {
+        int64_t acc = 0;
+        for (int64_t k = row[size_t(i)]; k < row[size_t(i) + 1]; ++k) acc += w[size_t(k)] < 0 ? -int64_t(w[size_t(k)]) : int64_t(w[size_t(k)]);
+        p.B = std::max(p.B, acc);
+    }
+    if (p.B > 2147483647LL) std::snprintf(p.refused, sizeof p.refused, "field bound exceeds int32");
+    if (p.w_min == 0) std::snprintf(p.refused, sizeof p.refused, "all weights are zero");
+    p.field_bytes = p.B <= 127 ? 1 : (p.B <= 32767 ? 2 : 4);
+    p.weight_bytes = p.max_w <= 127 ? 1 : 4;
+    p.dense = n <= 16384 && double(2 * m) / double(n) >= double(n) / 4.0;
+    p.sigma = std::sqrt(2.0 * sum_w2 / double(n));
+}
+
+// Symmetric CSR in parse_rudy's fill order (src/ising_weighted.hpp:134-147).
+static void graph_csr(const qb_graph_desc* d, std::vector<int64_t>& row, std::vector<int32_t>& col,
+                      std::vector<int32_t>& w) {
+    const int n = d->n;
+    const int64_t m = d->m;
+    row.assign(size_t(n) + 1, 0);
+    for (int64_t e = 0; e < m; ++e) { row[size_t(d->eu[e]) + 1]++; row[size_t(d->ev[e]) + 1]++; }
+    for (int i = 0; i < n; ++i) row[size_t(i) + 1] += row[size_t(i)];
+    col.assign(size_t(2 * m), 0);
+    w.assign(size_t(2 * m), 0);
+    std::vector<int64_t> fill(row.begin(), row.end() - 1);
+    for (int64_t e = 0; e < m; ++e) {
+        const int32_t a = d->eu[e], b = d->ev[e], wt = d->ew[e];
+        col[size_t(fill[size_t(a)])] = b; w[size_t(fill[size_t(a)]++)] = wt;
+        col[size_t(fill[size_t(b)])] = a; w[size_t(fill[size_t(b)]++)] = wt;
+    }
+}
+
+// Distance-1 colouring, vertices by decreasing degree (ties by index).
+static int colour_graph(int n, const std::vector<int64_t>& row, const std::vector<int32_t>& col,
+                        std::vector<int32_t>& pos_bit, std::vector<int32_t>& cls_ptr) {
+    std::vector<int> visit(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) visit[size_t(i)] = i;
+    std::stable_sort(visit.begin(), visit.end(), [&](int a, int b) {
+        return row[size_t(a) + 1] - row[size_t(a)] > row[size_t(b) + 1] - row[size_t(b)];
+    });
+    std::vector<int> colour;
+    const int K = greedy_colour(n, visit, [&](int i, auto mark) {
+        for (int64_t k = row[size_t(i)]; k < row[size_t(i) + 1]; ++k) mark(col[size_t(k)]);
+    }, colour);
+    order_by_colour(colour, K, pos_bit, cls_ptr);
+    return K;
+}
+
+static int validate_graph(const qb_graph_desc* d) {
+    if (!d) return fail(QB_E_ARG, "null graph description");
+    if (d->n <= 0 || d->m < 0) return fail(QB_E_ARG, "bad graph sizes (n %d, m %lld)", d->n, (long long)d->m);
+    if (d->m > 0 && (!d->eu || !d->ev || !d->ew)) return fail(QB_E_ARG, "edge arrays are required");
+    if (d->m > (int64_t(1) << 30)) return fail(QB_E_LIMIT, "more than 2^30 edges");
+    for (int64_t e = 0; e < d->m; ++e) {
+        if (d->eu[e] < 0 || d->eu[e] >= d->n || d->ev[e] < 0 || d->ev[e] >= d->n)
+            return fail(QB_E_ARG, "edge %lld names a vertex out of range", (long long)e);
+        if (d->eu[e] == d->ev[e]) return fail(QB_E_ARG, "edge %lld is a self-loop", (long long)e);
+    }
+    return QB_OK;
+}
+
+extern "C" int qb_colour_graph(const qb_graph_desc* d, int32_t* order, int32_t* class_ptr, int* n_classes) {
+    if (!order || !class_ptr || !n_classes) return fail(QB_E_ARG, "qb_colour_graph: null output");
+    int rc = validate_graph(d);
+    if (rc) return rc;
+    std::vector<int64_t> row;
+    std::vector<int32_t> col, w, pos_bit, cls_ptr;
+    graph_csr(d, row, col, w);
+    const int K = colour_graph(d->n, row, col, pos_bit, cls_ptr);
+    std::memcpy(order, pos_bit.data(), sizeof(int32_t) * pos_bit.size());
+    std::memcpy(class_ptr, cls_ptr.data(), sizeof(int32_t) * cls_ptr.size());
+    *n_classes = K;
+    return QB_OK;
+}
+
+extern "C" int qb_graph_create(const qb_graph_desc* d, int force_global, qb_graph** out) {
+    if (!out) return fail(QB_E_ARG, "qb_graph_create: null output");
+    *out = nullptr;
+    int rc0 = validate_graph(d);
+    if (rc0) return rc0;
+    const int n = d->n;
+    const int64_t m = d->m;
+    std::vector<int64_t> row;
+    std::vector<int32_t> col, w;
+    graph_csr(d, row, col, w);
+    auto G = new (std::nothrow) qb_graph();
+    if (!G) return fail(QB_E_NOMEM, "out of host memory");
+    std::unique_ptr<qb_graph> guard(G);
+    make_plan(n, m, d->ew, row, w, G->plan);
+    const int K = colour_graph(n, row, col, G->pos_bit, G->cls_ptr);
+    G->plan.n_classes = K;
+    std::vector<int32_t> pos_of(static_cast<size_t>(n));
+    for (int p = 0; p < n; ++p) pos_of[size_t(G->pos_bit[size_t(p)])] = p;
+    std::vector<int32_t> nb_ptr(size_t(n) + 1, 0), nb(size_t(2 * m)), nw(size_t(2 * m)), f0(static_cast<size_t>(n));
+    for (int p = 0; p < n; ++p) {
+        const int i = G->pos_bit[size_t(p)];
+        int64_t q = nb_ptr[size_t(p)];
+        int64_t acc = 0;
+        for (int64_t k = row[size_t(i)]; k < row[size_t(i) + 1]; ++k, ++q) {
+            nb[size_t(q)] = pos_of[size_t(col[size_t(k)])];
+            nw[size_t(q)] = w[size_t(k)];
+            acc -= w[size_t(k)];
+        }
+        nb_ptr[size_t(p) + 1] = int32_t(q);
+        f0[size_t(p)] = int32_t(acc);
+    }
+    const int64_t lds_need = int64_t(G->plan.n_pad) * 5;
+    G->lds_state = !force_global && G->plan.n_pad <= int_lds_capacity_spins() && lds_need <= 64 * 1024;
+    G->plan.shared_state = G->lds_state ? 1 : 0;
+    QB_HIP(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
+    QB_HIP(hipEventCreate(&G->ev0));
+    QB_HIP(hipEventCreate(&G->ev1));
+    hipStream_t st = G->stream;
+    std::vector<int32_t> eu(d->eu, d->eu + m), ev(d->ev, d->ev + m), ewv(d->ew, d->ew + m);
+    int rc;
+    if ((rc = G->b_cls.upload(G->cls_ptr, st)) || (rc = G->b_pos.upload(G->pos_bit, st)) ||
+        (rc = G->b_nbp.upload(nb_ptr, st)) || (rc = G->b_nb.upload(nb, st)) || (rc = G->b_nw.upload(nw, st)) ||
+        (rc = G->b_f0.upload(f0, st)) || (rc = G->b_eu.upload(eu, st)) || (rc = G->b_ev.upload(ev, st)) ||
+        (rc = G->b_ew.upload(ewv, st)))
+        return rc;
+    QB_HIP(hipStreamSynchronize(st));
+    DevGraph& D = G->dev;
+    D.n = n;
+    D.n_pad = G->plan.n_pad;
+    D.n_classes = K;
+    D.m = m;
+    D.cls_ptr = G->b_cls.as<int32_t>();
+    D.pos_bit = G->b_pos.as<int32_t>();
+    D.nb_ptr = G->b_nbp.as<int32_t>();
+    D.nb = G->b_nb.as<int32_t>();
+    D.nw = G->b_nw.as<int32_t>();
+    D.f0 = G->b_f0.as<int32_t>();
+    D.eu = G->b_eu.as<int32_t>();
+    D.ev = G->b_ev.as<int32_t>();
+    D.ew = G->b_ew.as<int32_t>();
+    *out = guard.release();
+    return QB_OK;
+}
+
+extern "C" int qb_graph_destroy(qb_graph* g) {
+    delete g;
+    return QB_OK;
+}
+
+extern "C" int qb_graph_get_plan(const qb_graph* g, qb_plan* out) {
+    if (!g || !out) return fail(QB_E_ARG, "qb_graph_get_plan: null argument");
+    *out = g->plan;
+    return QB_OK;
+}
+
+extern "C" int qb_graph_colouring(const qb_graph* g, int32_t* order, int32_t* class_ptr) {
+    if (!g || !order || !class_ptr) return fail(QB_E_ARG, "qb_graph_colouring: null argument");
+    std::memcpy(order, g->pos_bit.data(), sizeof(int32_t) * g->pos_bit.size());
+    std::memcpy(class_ptr, g->cls_ptr.data(), sizeof(int32_t) * g->cls_ptr.size());
+    return QB_OK;
+}
+
+extern "C" int qb_run_batch(qb_graph* G, const qb_schedule* b, uint64_t seed_base, int R, double updates_per_launch,
+                            int8_t* s_out, int64_t* energy_out, qb_batch_report* rep) {
+    if (!G) return fail(QB_E_ARG, "qb_run_batch: null graph");
+    int rc = check_schedule(b);
+    if (rc) return rc;
+    if (R <= 0) return fail(QB_E_ARG, "R must be positive (got %d)", R);
+    if (G->plan.refused[0]) return fail(QB_E_REFUSED, "plan refused: %s", G->plan.refused);
+    const int total = b->anneal_sweeps + b->hold_sweeps;
+    const int np = G->plan.n_pad, n = G->plan.n;
+    const size_t cells = size_t(R) * size_t(np);
+    if ((rc = G->w_s.reserve(cells)) || (rc = G->w_f.reserve(cells * 4)) || (rc = G->w_e.reserve(sizeof(long long) * size_t(R))) ||
+        (rc = G->w_sout.reserve(cells)) || (rc = G->w_beta.reserve(sizeof(double) * size_t(total))))
+        return rc;
+    const std::vector<double> beta = betas_of(*b);
+    hipStream_t st = G->stream;
+    int chunk = int(std::floor(updates_per_launch / (double(R) * double(n))));
+    chunk = std::max(1, std::min(total, chunk));
+    QB_HIP(hipStreamSynchronize(st));
+    const auto t0 = std::chrono::steady_clock::now();
+    QB_HIP(hipMemcpyAsync(G->w_beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice, st));
+    QB_HIP(hipEventRecord(G->ev0, st));
+    int launches = 0;
+    for (int sb = 0; sb < total; sb += chunk) {
+        QB_HIP(launch_int_sweeps(G->dev, R, seed_base, G->w_beta.as<double>(), sb, std::min(total, sb + chunk),
+                                 sb == 0 ? 1 : 0, G->w_s.as<int8_t>(), G->w_f.as<int32_t>(), G->w_e.as<long long>(),
+                                 G->lds_state, st));
+        ++launches;
+    }
+    QB_HIP(hipEventRecord(G->ev1, st));
+    QB_HIP(launch_int_unpermute(G->dev, R, G->w_s.as<int8_t>(), G->w_sout.as<int8_t>(), st));
+    if (s_out) QB_HIP(hipMemcpyAsync(s_out, G->w_sout.p, cells, hipMemcpyDeviceToHost, st));
+    if (energy_out) QB_HIP(hipMemcpyAsync(energy_out, G->w_e.p, sizeof(int64_t) * size_t(R), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    if (rep) {
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, G->ev0, G->ev1));
+        rep->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rep->kernel_ms = ms;
+        rep->launches = launches;
+        rep->sweeps_per_launch = chunk;
+    }
+    return QB_OK;
+}
+
+extern "C" int qb_device_energies(qb_graph* G, const uint8_t* X, int K, int64_t* energy_out) {
+    if (!G || !X || !energy_out) return fail(QB_E_ARG, "qb_device_energies: null argument");
+    if (K <= 0) return fail(QB_E_ARG, "K must be positive");
+    int rc;
+    if ((rc = G->w_X.reserve(size_t(K) * size_t(G->plan.n))) || (rc = G->w_out.reserve(sizeof(long long) * size_t(K))))
+        return rc;
+    hipStream_t st = G->stream;
+    QB_HIP(hipMemcpyAsync(G->w_X.p, X, size_t(K) * size_t(G->plan.n), hipMemcpyHostToDevice, st));
+    QB_HIP(launch_int_energies(G->dev, G->w_X.as<uint8_t>(), K, G->w_out.as<long long>(), st));
+    QB_HIP(hipMemcpyAsync(energy_out, G->w_out.p, sizeof(int64_t) * size_t(K), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    return QB_OK;
+}

@@ -1,0 +1,112 @@
+// qb_internal.h -- device layouts shared by the host orchestration (qb_host.cpp)
+// and the gfx950 kernels (qb_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace qb {
+
+// Branches are annealed in groups of 16: a group's spins for one bit form a 16-bit
+// mask, and its residuals for one ray are 16 consecutive values, so the 16 lanes
+// that evaluate one bit for the 16 branches of a group read one contiguous line.
+constexpr int kGroup = 16;
+
+// A bit's dE sums its CSR entries in chains of kSeg entries, the first seeded with
+// lin * delta, the chain sums then added in order. Bits with more than kSeg entries
+// ("heavy") are evaluated by a whole workgroup; they come first inside their class.
+constexpr int kSeg = 16;
+
+// One CSC entry of a bit in colour-class position order: the ray, the bit's share
+// a of it, and the ray's weight w (copied per entry so the sweep streams it).
+struct RayEnt {
+    int32_t ray;
+    float a;
+    double w;
+};
+
+// One entry of a ray's bit list, in increasing original bit index (the order the
+// from-scratch residual sum of binary_energy visits them).
+struct BitEnt {
+    int32_t pos;
+    float a;
+};
+
+// The float problem on the device, in colour-class position order: position p
+// holds original bit pos_bit[p]; class c is positions [cls_ptr[c], cls_ptr[c+1]).
+struct DevProblem {
+    int n, n_rays, n_classes;
+    int64_t nnz, n_nbr;
+    const int32_t* cls_ptr;
+    const int32_t* cls_heavy;   // heavy bits at the start of each class
+    const int32_t* pos_bit;
+    const int32_t* col_ptr;
+    const RayEnt* ent;
+    const int32_t* nb_ptr;
+    const int32_t* nb;      // neighbour positions, reference CSR order
+    const int32_t* nb_bit;  // neighbour original indices (for the energy's j > i rule)
+    const double* lin;
+    const double* d;
+    const double* w;
+    const int32_t* rt_ptr;
+    const BitEnt* rt;
+    const uint8_t* seq_update;  // 1 where a ray repeats in the column: scatter in CSR order
+    double lambda;
+};
+
+// Mutable state of one anneal: n_groups groups of 16 branches.
+template <typename T>
+struct DevState {
+    int n_groups, R;
+    uint64_t seed_offset;
+    uint16_t* x;            // [g][n] spin masks, bit b = branch g*16 + b
+    T* r;                   // [g][n_rays][16] residuals
+    const double* beta;     // 1 / T per sweep
+    unsigned* bar;          // barrier counter (monotone)
+    int* err;               // set to 1 when a barrier wait times out
+};
+
+// The integer graph on the device, in colour-class position order.
+struct DevGraph {
+    int n, n_pad, n_classes;
+    int64_t m;
+    const int32_t* cls_ptr;
+    const int32_t* pos_bit;
+    const int32_t* nb_ptr;
+    const int32_t* nb;
+    const int32_t* nw;
+    const int32_t* f0;      // fields of the all-zero assignment
+    const int32_t* eu;      // edge list, original indices
+    const int32_t* ev;
+    const int32_t* ew;
+};
+
+// ---- launchers (qb_kernels.hip) ----
+hipError_t launch_ray_init(const DevProblem& p, int precision, void* x, void* r, int n_groups,
+                           hipStream_t st);
+hipError_t launch_ray_sweeps(const DevProblem& p, int precision, void* x, void* r, int n_groups, int R,
+                             uint64_t seed_offset, const double* beta, int sweep_begin, int sweep_end,
+                             int recompute_every, unsigned* bar, int* err, int workgroups,
+                             hipStream_t st);
+hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int R, float* p_void,
+                              uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st);
+// FP64 energy of every branch: data and prior partial sums per block, reduced in a
+// fixed order; resid_err receives max |r_carried - r_scratch| when r_carried != null.
+hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_groups, int R,
+                             const void* r_carried, int precision, double* part, int n_part_blocks,
+                             double* data, double* prior, float* resid_err, hipStream_t st);
+int ray_energy_blocks(const DevProblem& p);
+hipError_t launch_pack_states(const uint8_t* X, int K, int n, const int32_t* pos_bit, uint16_t* x,
+                              hipStream_t st);
+
+hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const double* beta,
+                             int sweep_begin, int sweep_end, int init, int8_t* s_state, int32_t* f_state,
+                             long long* energy, bool lds_state, hipStream_t st);
+hipError_t launch_int_unpermute(const DevGraph& g, int R, const int8_t* s_state, int8_t* s_out,
+                                hipStream_t st);
+hipError_t launch_int_energies(const DevGraph& g, const uint8_t* X, int K, long long* out, hipStream_t st);
+int int_lds_capacity_spins();
+
+}  // namespace qb

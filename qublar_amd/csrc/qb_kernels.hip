@@ -1,0 +1,605 @@
+// qb_kernels.hip -- gfx950 kernels of the simulated-annealing QUBO engine.
+//
+// Float path (ray x voxel QUBO, src/ising_recon.hpp:105-166):
+//   ray_sweep_kernel    persistent; every colour class is one step. Sixteen lanes
+//                       evaluate one bit for the 16 branches of a group: dE from the
+//                       bit's CSC column (ray, a, w) and the group's residual line
+//                       r[ray][0..15], the neighbour term from 16-bit spin masks, one
+//                       Philox draw per lane, and on acceptance the residual scatter.
+//                       Bits of one class share no ray and are not neighbours, so
+//                       a step needs no atomics and its result does not depend on
+//                       the order inside it. Steps are separated by a grid barrier.
+//   ray_outputs_kernel  branch agreement: p_void, tri-state, certainty, branches.
+//   ray_energy_*        FP64 binary_energy of every branch, from scratch.
+// Integer path (src/ising_weighted.hpp:193-267, src/ising_gpu.cuh:149-272):
+//   int_sweep_kernel    one workgroup per replica, state in LDS (or global memory
+//                       when it does not fit), fields updated with integer atomics.
+//   int_energy_kernel   -cut over the edge list, one block per state.
+//
+// Every file of the engine is compiled with -ffp-contract=off: each floating-point
+// expression is evaluated as written, in the order of the CPU oracle.
+#include "qb_internal.h"
+#include "qb_rng.h"
+
+namespace qb {
+
+// ---------------------------------------------------------------------------------
+// Grid barrier for the persistent sweep. Monotone counter: the k-th barrier of a
+// launch waits until the counter reaches k * gridDim.x. Release/acquire at agent
+// scope make the residual and spin stores of one step visible to every workgroup
+// before the next step reads them. A wait longer than ~2 s (s_memrealtime runs at
+// 100 MHz) sets *err and every workgroup leaves its loops, so the grid drains.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned target, int* err) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        if (ok) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            unsigned polls = 0;
+            while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((++polls & 255u) == 0) {
+                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                        __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        break;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T residual_add(T r, float a, double delta);
+template <>
+__device__ __forceinline__ double residual_add<double>(double r, float a, double delta) {
+    return r + double(a) * delta;                       // r[ray] += a * delta (ising_recon.hpp:139)
+}
+template <>
+__device__ __forceinline__ float residual_add<float>(float r, float a, double delta) {
+    return r + (delta > 0.0 ? a : -a);                  // the FP32 carry: one float add
+}
+
+// The dE of a bit sums its CSR entries in chains of kSeg: the first chain starts
+// from lin * delta, later chains start from their first term, and the chain sums
+// are added in order; the neighbour terms follow in CSR order. For a bit with at
+// most kSeg entries this is local_field_dE's order exactly (ising_recon.hpp:105-118).
+
+// One chain: entries [ka, kb) (kb - ka <= kSeg) for branch lane b. All loads are
+// issued before the first add, so a chain costs two dependent memory round trips.
+template <typename T>
+__device__ __forceinline__ double chain_sum(const RayEnt* __restrict__ ent, int ka, int kb, const T* rg, int b,
+                                            double delta, double seed, bool seeded) {
+    RayEnt e[kSeg];
+    double rv[kSeg];
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u)
+        if (ka + u < kb) e[u] = ent[ka + u];
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u)
+        if (ka + u < kb) rv[u] = double(rg[size_t(e[u].ray) * kGroup + b]);
+    double acc = seed;
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u)
+        if (ka + u < kb) {
+            const double ad = double(e[u].a) * delta;
+            const double t = e[u].w * (rv[u] * ad + 0.5 * ad * ad);
+            acc = (u == 0 && !seeded) ? t : acc + t;
+        }
+    return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ double neighbour_terms(const DevProblem& p, const uint16_t* xg, int pos, int b, int xb,
+                                                  double acc) {
+    for (int k = p.nb_ptr[pos]; k < p.nb_ptr[pos + 1]; ++k) {
+        const int before = ((xg[p.nb[k]] >> b) & 1) != xb;
+        acc += p.lambda * (before ? -1.0 : 1.0);
+    }
+    return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ bool decide(const DevProblem& p, const DevState<T>& s, int pos, int branch, int sweep,
+                                       double beta, double dE) {
+    if (branch >= s.R) return false;
+    return dE <= 0.0 || metropolis_uphill(dE, beta, s.seed_offset + uint64_t(branch) + 1u, uint32_t(sweep),
+                                          uint32_t(p.pos_bit[pos]));
+}
+
+// A bit with at most kSeg entries, for the 16 branches of group g (lane b).
+template <typename T>
+__device__ __forceinline__ void light_update(const DevProblem& p, const DevState<T>& s, int pos, int g, int b,
+                                             int sweep, double beta) {
+    uint16_t* xg = s.x + size_t(g) * size_t(p.n);
+    T* rg = s.r + size_t(g) * size_t(p.n_rays) * kGroup;
+    const uint16_t xm = xg[pos];
+    const int xb = (xm >> b) & 1;
+    const double delta = xb ? -1.0 : 1.0;
+    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
+    double dE = chain_sum<T>(p.ent, k0, k1, rg, b, delta, p.lin[pos] * delta, true);
+    dE = neighbour_terms<T>(p, xg, pos, b, xb, dE);
+    const bool acc = decide<T>(p, s, pos, g * kGroup + b, sweep, beta, dE);
+    if (acc)
+        for (int k = k0; k < k1; ++k) {
+            const RayEnt e = p.ent[k];
+            T* q = rg + size_t(e.ray) * kGroup + b;
+            *q = residual_add<T>(*q, e.a, delta);
+        }
+    const uint64_t bal = __ballot(acc);
+    const uint16_t flips = uint16_t(bal >> (threadIdx.x & 48u));
+    if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
+}
+
+// A bit with more than kSeg entries, by a whole workgroup: the chains are spread over
+// the block's 16-lane slots, their sums meet in LDS, slot 0 adds them in order and
+// decides, and the residual scatter is spread over the block again.
+template <typename T, int BLOCK>
+__device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState<T>& s, int pos, int g, int sweep,
+                                             double beta) {
+    constexpr int kSlots = BLOCK / kGroup;
+    __shared__ double part[kSlots][kGroup];
+    __shared__ int accepted[kGroup];
+    const int b = threadIdx.x & (kGroup - 1);
+    const int slot = threadIdx.x / kGroup;
+    uint16_t* xg = s.x + size_t(g) * size_t(p.n);
+    T* rg = s.r + size_t(g) * size_t(p.n_rays) * kGroup;
+    const uint16_t xm = xg[pos];
+    const int xb = (xm >> b) & 1;
+    const double delta = xb ? -1.0 : 1.0;
+    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
+    const int n_chain = (k1 - k0 + kSeg - 1) / kSeg;
+    double dE = 0.0;
+    for (int j0 = 0; j0 < n_chain; j0 += kSlots) {
+        const int j = j0 + slot;
+        if (j < n_chain) {
+            const int ka = k0 + j * kSeg;
+            const int kb = ka + kSeg < k1 ? ka + kSeg : k1;
+            part[slot][b] = chain_sum<T>(p.ent, ka, kb, rg, b, delta, p.lin[pos] * delta, j == 0);
+        }
+        __syncthreads();
+        if (slot == 0) {
+            const int m = n_chain - j0 < kSlots ? n_chain - j0 : kSlots;
+            for (int q = 0; q < m; ++q) dE = (j0 + q == 0) ? part[0][b] : dE + part[q][b];
+        }
+        __syncthreads();
+    }
+    if (slot == 0) {
+        dE = neighbour_terms<T>(p, xg, pos, b, xb, dE);
+        const bool acc = decide<T>(p, s, pos, g * kGroup + b, sweep, beta, dE);
+        accepted[b] = acc ? 1 : 0;
+        const uint64_t bal = __ballot(acc);
+        const uint16_t flips = uint16_t(bal >> (threadIdx.x & 48u));
+        if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
+    }
+    __syncthreads();
+    if (accepted[b]) {
+        if (p.seq_update[pos]) {
+            // a ray repeats in this column: its updates must land in CSR order
+            if (slot == 0)
+                for (int k = k0; k < k1; ++k) {
+                    const RayEnt e = p.ent[k];
+                    T* q = rg + size_t(e.ray) * kGroup + b;
+                    *q = residual_add<T>(*q, e.a, delta);
+                }
+        } else {
+            for (int k = k0 + slot; k < k1; k += kSlots) {
+                const RayEnt e = p.ent[k];
+                T* q = rg + size_t(e.ray) * kGroup + b;
+                *q = residual_add<T>(*q, e.a, delta);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Residuals of ray `ray` for branch lane b, from scratch: -d + sum of a over the
+// ray's set bits in increasing original bit index (binary_energy's order).
+template <typename T>
+__device__ __forceinline__ void ray_recompute(const DevProblem& p, const DevState<T>& s, int ray, int g, int b) {
+    const uint16_t* xg = s.x + size_t(g) * size_t(p.n);
+    double r = -p.d[ray];
+    for (int k = p.rt_ptr[ray]; k < p.rt_ptr[ray + 1]; ++k) {
+        const BitEnt e = p.rt[k];
+        if ((xg[e.pos] >> b) & 1) r += double(e.a);
+    }
+    s.r[(size_t(g) * size_t(p.n_rays) + size_t(ray)) * kGroup + b] = T(r);
+}
+
+template <typename T, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+ray_sweep_kernel(DevProblem p, DevState<T> s, int sweep_begin, int sweep_end, int recompute_every) {
+    constexpr int kSlots = BLOCK / kGroup;
+    const int b = threadIdx.x & (kGroup - 1);
+    const int n_slots = int(gridDim.x) * kSlots;
+    unsigned target = 0;
+    for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
+        const double beta = s.beta[sweep];
+        for (int c = 0; c < p.n_classes; ++c) {
+            const int c0 = p.cls_ptr[c];
+            const int nh = p.cls_heavy[c];
+            const int nl = p.cls_ptr[c + 1] - c0 - nh;
+            const int heavy_tasks = nh * s.n_groups;
+            for (int h = int(blockIdx.x); h < heavy_tasks; h += int(gridDim.x)) {
+                const int g = h / nh;
+                heavy_update<T, BLOCK>(p, s, c0 + (h - g * nh), g, sweep, beta);
+            }
+            // light tasks start on the blocks after those that took heavy bits
+            const int first = (heavy_tasks % int(gridDim.x)) * kSlots;
+            const int slot = (int(blockIdx.x) * kSlots + int(threadIdx.x) / kGroup - first + n_slots) % n_slots;
+            const int light_tasks = nl * s.n_groups;
+            for (int t = slot; t < light_tasks; t += n_slots) {
+                const int g = t / nl;
+                light_update<T>(p, s, c0 + nh + (t - g * nl), g, b, sweep, beta);
+            }
+            target += gridDim.x;
+            if (!grid_barrier(s.bar, target, s.err)) return;
+        }
+        if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
+            const int slot = int(blockIdx.x) * kSlots + int(threadIdx.x) / kGroup;
+            const int tasks = p.n_rays * s.n_groups;
+            for (int t = slot; t < tasks; t += n_slots) {
+                const int g = t / p.n_rays;
+                ray_recompute<T>(p, s, t - g * p.n_rays, g, b);
+            }
+            target += gridDim.x;
+            if (!grid_barrier(s.bar, target, s.err)) return;
+        }
+    }
+}
+
+template <typename T>
+__global__ void ray_init_kernel(DevProblem p, uint16_t* x, T* r, int n_groups) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t nx = size_t(n_groups) * size_t(p.n);
+    const size_t nr = size_t(n_groups) * size_t(p.n_rays) * kGroup;
+    if (i < nx) x[i] = 0;
+    if (i < nr) r[i] = T(-p.d[(i / kGroup) % size_t(p.n_rays)]);
+}
+
+// Branch agreement (branch_fractions' p_v, ising_recon.hpp:160-163, and classify,
+// :305-309) plus certainty max(p, 1 - p); outputs in original bit order.
+__global__ void ray_outputs_kernel(DevProblem p, const uint16_t* x, int n_groups, int R, float* p_void,
+                                   uint8_t* tri, float* certainty, uint8_t* branches) {
+    const int pos = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (pos >= p.n) return;
+    const int bit = p.pos_bit[pos];
+    int cnt = 0;
+    for (int g = 0; g < n_groups; ++g) {
+        const int live = R - g * kGroup < kGroup ? R - g * kGroup : kGroup;
+        const unsigned mask = live >= 16 ? 0xFFFFu : ((1u << live) - 1u);
+        const unsigned m = unsigned(x[size_t(g) * size_t(p.n) + pos]) & mask;
+        cnt += __popc(m);
+        if (branches)
+            for (int b = 0; b < live; ++b)
+                branches[size_t(g * kGroup + b) * size_t(p.n) + size_t(bit)] = uint8_t((m >> b) & 1u);
+    }
+    const float pv = float(cnt) / float(R);
+    if (p_void) p_void[bit] = pv;
+    if (tri) tri[bit] = pv >= 0.9f ? 1 : (pv <= 0.1f ? 0 : 2);
+    if (certainty) certainty[bit] = pv > 1.0f - pv ? pv : 1.0f - pv;
+}
+
+// ---- FP64 energies -------------------------------------------------------------------
+constexpr int kEnergyBlock = 256;
+constexpr int kEnergyItems = kEnergyBlock / kGroup;   // rays or positions per block
+
+// Data term per (ray, branch lane): 1/2 w r^2 with r from scratch, summed over the
+// block's rays per lane; also max |r_carried - r_scratch|.
+template <typename T>
+__global__ void __launch_bounds__(kEnergyBlock)
+ray_energy_data_kernel(DevProblem p, const uint16_t* x, int n_groups, const T* r_carried, double* part,
+                       int part_stride, float* resid_err) {
+    __shared__ double red[kEnergyBlock];
+    const int b = threadIdx.x & (kGroup - 1);
+    const int g = blockIdx.y;
+    const int ray = blockIdx.x * kEnergyItems + int(threadIdx.x / kGroup);
+    double v = 0.0;
+    float err = 0.0f;
+    if (ray < p.n_rays) {
+        const uint16_t* xg = x + size_t(g) * size_t(p.n);
+        double r = -p.d[ray];
+        for (int k = p.rt_ptr[ray]; k < p.rt_ptr[ray + 1]; ++k) {
+            const BitEnt e = p.rt[k];
+            if ((xg[e.pos] >> b) & 1) r += double(e.a);
+        }
+        v = 0.5 * p.w[ray] * r * r;
+        if (r_carried) {
+            const double rc = double(r_carried[(size_t(g) * size_t(p.n_rays) + size_t(ray)) * kGroup + b]);
+            err = float(fabs(rc - r));
+        }
+    }
+    red[threadIdx.x] = v;
+    if (r_carried && err > 0.0f) atomicMax(reinterpret_cast<unsigned*>(resid_err), __float_as_uint(err));
+    __syncthreads();
+    for (int stride = kEnergyBlock / 2; stride >= kGroup; stride >>= 1) {
+        if (int(threadIdx.x) < stride) red[threadIdx.x] += red[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x < kGroup) part[size_t(blockIdx.x) * part_stride + size_t(g) * kGroup + threadIdx.x] = red[threadIdx.x];
+}
+
+// Prior term per (position, branch lane): lin x + lambda per unlike neighbour j > i.
+__global__ void __launch_bounds__(kEnergyBlock)
+ray_energy_prior_kernel(DevProblem p, const uint16_t* x, int n_groups, double* part, int part_stride) {
+    __shared__ double red[kEnergyBlock];
+    const int b = threadIdx.x & (kGroup - 1);
+    const int g = blockIdx.y;
+    const int pos = blockIdx.x * kEnergyItems + int(threadIdx.x / kGroup);
+    double v = 0.0;
+    if (pos < p.n) {
+        const uint16_t* xg = x + size_t(g) * size_t(p.n);
+        const int xi = (xg[pos] >> b) & 1;
+        const int bit = p.pos_bit[pos];
+        v = p.lin[pos] * double(xi);
+        for (int k = p.nb_ptr[pos]; k < p.nb_ptr[pos + 1]; ++k)
+            if (p.nb_bit[k] > bit && ((xg[p.nb[k]] >> b) & 1) != xi) v += p.lambda;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int stride = kEnergyBlock / 2; stride >= kGroup; stride >>= 1) {
+        if (int(threadIdx.x) < stride) red[threadIdx.x] += red[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x < kGroup) part[size_t(blockIdx.x) * part_stride + size_t(g) * kGroup + threadIdx.x] = red[threadIdx.x];
+}
+
+// Sums the partials of each branch in block order (deterministic).
+__global__ void ray_energy_final_kernel(const double* part_data, int blocks_data, const double* part_prior,
+                                        int blocks_prior, int stride, int R, double* data, double* prior) {
+    const int br = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (br >= R) return;
+    double d = 0.0, q = 0.0;
+    for (int k = 0; k < blocks_data; ++k) d += part_data[size_t(k) * stride + br];
+    for (int k = 0; k < blocks_prior; ++k) q += part_prior[size_t(k) * stride + br];
+    data[br] = d;
+    prior[br] = q;
+}
+
+__global__ void pack_states_kernel(const uint8_t* X, int K, int n, const int32_t* pos_bit, uint16_t* x) {
+    const int pos = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int g = blockIdx.y;
+    if (pos >= n) return;
+    const int bit = pos_bit[pos];
+    unsigned m = 0;
+    for (int b = 0; b < kGroup; ++b) {
+        const int k = g * kGroup + b;
+        if (k < K && X[size_t(k) * size_t(n) + size_t(bit)]) m |= 1u << b;
+    }
+    x[size_t(g) * size_t(n) + pos] = uint16_t(m);
+}
+
+// ---------------------------------------------------------------------------------
+// Integer path
+// ---------------------------------------------------------------------------------
+template <int BLOCK, bool LDS>
+__global__ void __launch_bounds__(BLOCK)
+int_sweep_kernel(DevGraph g, uint64_t seed_base, const double* beta, int sweep_begin, int sweep_end, int init,
+                 int8_t* s_state, int32_t* f_state, long long* energy) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ long long red[BLOCK / 64];
+    const int r = blockIdx.x;
+    const int n = g.n, np = g.n_pad;
+    int8_t* gs = s_state + size_t(r) * size_t(np);
+    int32_t* gf = f_state + size_t(r) * size_t(np);
+    int32_t* f = LDS ? reinterpret_cast<int32_t*>(smem) : gf;
+    int8_t* s = LDS ? reinterpret_cast<int8_t*>(smem + size_t(np) * 4) : gs;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    if (init) {
+        for (int i = tid; i < n; i += BLOCK) { f[i] = g.f0[i]; s[i] = -1; }
+    } else if (LDS) {
+        for (int i = tid; i < n; i += BLOCK) { f[i] = gf[i]; s[i] = gs[i]; }
+    }
+    __syncthreads();
+    const uint64_t seed = seed_base + uint64_t(r);
+    long long dsum = 0;
+    for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
+        const double bt = beta[sweep];
+        for (int c = 0; c < g.n_classes; ++c) {
+            const int c0 = g.cls_ptr[c], c1 = g.cls_ptr[c + 1];
+            for (int base = c0; base < c1; base += BLOCK) {
+                const int pos = base + tid;
+                int si = 0;
+                bool acc = false;
+                long long dE = 0;
+                if (pos < c1) {
+                    int fi;
+                    if (LDS) {
+                        si = s[pos];
+                        fi = f[pos];
+                    } else {
+                        si = __hip_atomic_load(&s[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        fi = __hip_atomic_load(&f[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    dE = -(long long)si * (long long)fi;          // State::dE, ising_weighted.hpp:209
+                    acc = dE <= 0 || metropolis_uphill(double(dE), bt, seed, uint32_t(sweep),
+                                                       uint32_t(g.pos_bit[pos]));
+                    if (acc) dsum += dE;
+                }
+                uint64_t m = __ballot(acc);
+                while (m) {                                      // State::flip, :210-217
+                    const int L = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int pj = __shfl(pos, L);
+                    const int sj = __shfl(si, L);
+                    for (int k = g.nb_ptr[pj] + lane; k < g.nb_ptr[pj + 1]; k += 64)
+                        atomicAdd(&f[g.nb[k]], -2 * g.nw[k] * sj);
+                    if (lane == 0) s[pj] = int8_t(-sj);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (LDS)
+        for (int i = tid; i < n; i += BLOCK) { gf[i] = f[i]; gs[i] = s[i]; }
+    for (int o = 32; o > 0; o >>= 1) dsum += __shfl_down(dsum, o);
+    if (lane == 0) red[tid >> 6] = dsum;
+    __syncthreads();
+    if (tid == 0) {
+        long long t = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
+        energy[r] = (init ? 0 : energy[r]) + t;
+    }
+}
+
+__global__ void int_unpermute_kernel(DevGraph g, int R, const int8_t* s_state, int8_t* s_out) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t total = size_t(R) * size_t(g.n_pad);
+    if (i >= total) return;
+    const int r = int(i / size_t(g.n_pad));
+    const int pos = int(i % size_t(g.n_pad));
+    if (pos < g.n)
+        s_out[size_t(r) * g.n_pad + size_t(g.pos_bit[pos])] = s_state[size_t(r) * g.n_pad + pos];
+    else
+        s_out[size_t(r) * g.n_pad + size_t(pos)] = 0;
+}
+
+// -cut of state blockIdx.x over the edge list (oracle_kernel, ising_gpu.cuh:257-272).
+__global__ void __launch_bounds__(256) int_energy_kernel(DevGraph g, const uint8_t* X, long long* out) {
+    __shared__ long long red[4];
+    const uint8_t* x = X + size_t(blockIdx.x) * size_t(g.n);
+    long long acc = 0;
+    for (int64_t e = threadIdx.x; e < g.m; e += 256)
+        if (x[g.eu[e]] != x[g.ev[e]]) acc += g.ew[e];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = -(red[0] + red[1] + red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------
+constexpr int kSweepBlock = 256;
+
+hipError_t launch_ray_init(const DevProblem& p, int precision, void* x, void* r, int n_groups, hipStream_t st) {
+    const size_t nx = size_t(n_groups) * size_t(p.n);
+    const size_t nr = size_t(n_groups) * size_t(p.n_rays) * kGroup;
+    const size_t total = nx > nr ? nx : nr;
+    const unsigned blocks = unsigned((total + 255) / 256);
+    if (precision == 1)
+        ray_init_kernel<float><<<blocks, 256, 0, st>>>(p, static_cast<uint16_t*>(x), static_cast<float*>(r), n_groups);
+    else
+        ray_init_kernel<double><<<blocks, 256, 0, st>>>(p, static_cast<uint16_t*>(x), static_cast<double*>(r), n_groups);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t sweeps_t(const DevProblem& p, void* x, void* r, int n_groups, int R, uint64_t seed_offset,
+                           const double* beta, int sb, int se, int recompute_every, unsigned* bar, int* err,
+                           int workgroups, hipStream_t st) {
+    DevState<T> s;
+    s.n_groups = n_groups;
+    s.R = R;
+    s.seed_offset = seed_offset;
+    s.x = static_cast<uint16_t*>(x);
+    s.r = static_cast<T*>(r);
+    s.beta = beta;
+    s.bar = bar;
+    s.err = err;
+    void* args[] = {const_cast<DevProblem*>(&p), &s, &sb, &se, &recompute_every};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&ray_sweep_kernel<T, kSweepBlock>),
+                                      dim3(unsigned(workgroups)), dim3(kSweepBlock), args, 0, st);
+}
+
+hipError_t launch_ray_sweeps(const DevProblem& p, int precision, void* x, void* r, int n_groups, int R,
+                             uint64_t seed_offset, const double* beta, int sb, int se, int recompute_every,
+                             unsigned* bar, int* err, int workgroups, hipStream_t st) {
+    if (precision == 1)
+        return sweeps_t<float>(p, x, r, n_groups, R, seed_offset, beta, sb, se, recompute_every, bar, err,
+                               workgroups, st);
+    return sweeps_t<double>(p, x, r, n_groups, R, seed_offset, beta, sb, se, recompute_every, bar, err,
+                            workgroups, st);
+}
+
+int ray_sweep_max_workgroups() {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&ray_sweep_kernel<double, kSweepBlock>),
+                                                     kSweepBlock, 0) != hipSuccess)
+        return 0;
+    return per_cu;
+}
+
+hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int R, float* p_void,
+                              uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st) {
+    ray_outputs_kernel<<<unsigned((p.n + 255) / 256), 256, 0, st>>>(p, x, n_groups, R, p_void, tri, certainty,
+                                                                     branches);
+    return hipGetLastError();
+}
+
+int ray_energy_blocks(const DevProblem& p) {
+    return (p.n_rays + kEnergyItems - 1) / kEnergyItems + (p.n + kEnergyItems - 1) / kEnergyItems;
+}
+
+hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_groups, int R, const void* r_carried,
+                             int precision, double* part, int n_part_blocks, double* data, double* prior,
+                             float* resid_err, hipStream_t st) {
+    const int bd = (p.n_rays + kEnergyItems - 1) / kEnergyItems;
+    const int bp = (p.n + kEnergyItems - 1) / kEnergyItems;
+    if (bd + bp > n_part_blocks) return hipErrorInvalidValue;
+    const int stride = n_groups * kGroup;
+    double* part_prior = part + size_t(bd) * stride;
+    if (bd > 0) {
+        if (precision == 1)
+            ray_energy_data_kernel<float><<<dim3(unsigned(bd), unsigned(n_groups)), kEnergyBlock, 0, st>>>(
+                p, x, n_groups, static_cast<const float*>(r_carried), part, stride, resid_err);
+        else
+            ray_energy_data_kernel<double><<<dim3(unsigned(bd), unsigned(n_groups)), kEnergyBlock, 0, st>>>(
+                p, x, n_groups, static_cast<const double*>(r_carried), part, stride, resid_err);
+    }
+    ray_energy_prior_kernel<<<dim3(unsigned(bp), unsigned(n_groups)), kEnergyBlock, 0, st>>>(p, x, n_groups,
+                                                                                           part_prior, stride);
+    ray_energy_final_kernel<<<unsigned((R + 63) / 64), 64, 0, st>>>(part, bd, part_prior, bp, stride, R, data, prior);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_states(const uint8_t* X, int K, int n, const int32_t* pos_bit, uint16_t* x, hipStream_t st) {
+    const int n_groups = (K + kGroup - 1) / kGroup;
+    pack_states_kernel<<<dim3(unsigned((n + 255) / 256), unsigned(n_groups)), 256, 0, st>>>(X, K, n, pos_bit, x);
+    return hipGetLastError();
+}
+
+constexpr int kIntLdsBytes = 64 * 1024;
+
+int int_lds_capacity_spins() { return kIntLdsBytes / 5 - 16; }
+
+hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const double* beta, int sb, int se,
+                             int init, int8_t* s_state, int32_t* f_state, long long* energy, bool lds_state,
+                             hipStream_t st) {
+    const int max_class_block = 256;
+    (void)max_class_block;
+    if (lds_state) {
+        const size_t smem = size_t(g.n_pad) * 5;
+        if (smem > size_t(kIntLdsBytes)) return hipErrorInvalidValue;
+        int_sweep_kernel<64, true><<<unsigned(R), 64, smem, st>>>(g, seed_base, beta, sb, se, init, s_state, f_state,
+                                                                  energy);
+    } else {
+        int_sweep_kernel<256, false><<<unsigned(R), 256, 0, st>>>(g, seed_base, beta, sb, se, init, s_state,
+                                                                  f_state, energy);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_int_unpermute(const DevGraph& g, int R, const int8_t* s_state, int8_t* s_out, hipStream_t st) {
+    const size_t total = size_t(R) * size_t(g.n_pad);
+    int_unpermute_kernel<<<unsigned((total + 255) / 256), 256, 0, st>>>(g, R, s_state, s_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_int_energies(const DevGraph& g, const uint8_t* X, int K, long long* out, hipStream_t st) {
+    int_energy_kernel<<<unsigned(K), 256, 0, st>>>(g, X, out);
+    return hipGetLastError();
+}
+
+}  // namespace qb

@@ -1,0 +1,106 @@
+"""CPU checks of libqublar_hip.so that need no GPU: the library loads and exports every
+function include/qublar.h declares; the host problem builders and the colourings."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import qublar_amd as qa
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "qublar.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qb_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = qa.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_library_refuses_bad_arguments_without_a_device():
+    import ctypes as C
+    L = qa.lib()
+    assert L.qb_problem_create(None, C.byref(C.c_void_p())) == qa.QB_E_ARG
+    assert b"null" in L.qb_last_error()
+    bad = qa.ProblemArrays([0, 1], [5], [0.1], [0.0], [1.0], [0, 0], [], 0.0, 0.0)   # ray 5 of 1
+    d = bad.desc()
+    assert L.qb_problem_create(C.byref(d), C.byref(C.c_void_p())) == qa.QB_E_ARG
+    assert b"ray" in L.qb_last_error()
+
+
+def test_muon_scene_matches_the_reference_problem_size():
+    """The builder's 128^3 scene has the reference's published size: 306,328 variables,
+    75,600 rays, 3,829,376 ray-voxel entries (experiments/p1_bench/check_log.txt:162)."""
+    P = qa.muon_scene()
+    assert (P.n_vars, P.n_rays, int(P.row_ptr[-1])) == (306328, 75600, 3829376)
+    assert int(P.truth.sum()) > 0
+    assert np.all(P.w >= 1.0) and np.all(np.isfinite(P.d))
+    # 6-neighbour lists are symmetric
+    i = np.repeat(np.arange(P.n_vars), np.diff(P.nbr_ptr))
+    fwd = set(zip(i.tolist(), P.nbr.tolist()))
+    assert all((b, a) in fwd for a, b in list(fwd)[:20000])
+
+
+def colours_of(order, cp, n):
+    col = np.empty(n, np.int64)
+    for c in range(len(cp) - 1):
+        col[order[cp[c]:cp[c + 1]]] = c
+    return col
+
+
+def assert_distance2_colouring(P, order, cp):
+    n = P.n_vars
+    assert np.array_equal(np.sort(order), np.arange(n))
+    col = colours_of(order, cp, n)
+    bit = np.repeat(np.arange(n), np.diff(P.row_ptr))
+    key = P.ray.astype(np.int64) * (len(cp) + 1) + col[bit]
+    assert len(np.unique(key)) == len(key), "two bits of one class share a ray"
+    i = np.repeat(np.arange(n), np.diff(P.nbr_ptr))
+    assert np.all(col[i] != col[P.nbr]), "neighbours share a class"
+
+
+def test_muon_colouring_is_distance_two():
+    P = qa.muon_scene()
+    order, cp = P.colouring()
+    assert_distance2_colouring(P, order, cp)
+    lower = np.bincount(P.ray).max()
+    assert lower <= len(cp) - 1 <= 2 * lower
+    # heavy bits (more than 16 entries) come first inside every class
+    deg = np.diff(P.row_ptr)[order]
+    for c in range(len(cp) - 1):
+        h = deg[cp[c]:cp[c + 1]] > 16
+        assert not np.any(h[1:] & ~h[:-1])
+
+
+def test_small_scene_colouring_is_distance_two():
+    P = qa.muon_scene(grid=32, sky_az=90, sky_th=20)
+    assert_distance2_colouring(P, *P.colouring())
+
+
+def test_er_graph_matches_the_test_generator(orc):
+    from test_oracle_pins import er_graph
+    g = qa.er_graph(300, 0.05, 7)
+    ref = er_graph(orc, 300, 0.05, 7)
+    assert np.array_equal(g.eu, ref.eu) and np.array_equal(g.ev, ref.ev) and np.array_equal(g.ew, ref.ew)
+
+
+def test_er_graph_workload_size():
+    g = qa.er_graph(800, 0.06, 1)
+    assert abs(g.m - 19176) < 4 * np.sqrt(19176)
+    assert set(np.unique(g.ew).tolist()) == {-1, 1}
+
+
+def test_graph_colouring_is_proper():
+    g = qa.er_graph(800, 0.06, 1)
+    order, cp = g.colouring()
+    col = colours_of(order, cp, g.n)
+    assert np.all(col[g.eu] != col[g.ev])
+    assert len(cp) - 1 < 40

@@ -1,0 +1,216 @@
+"""GPU parity: every result of the HIP engine, called through the C ABI, against the
+CPU oracle run with the engine's own colouring and the same Philox keys.
+
+Bar: integer problems bit-exact (spins, tracked energies, device energies); float
+problems identical spins, p_void and tri-state maps, final FP64 energies within
+1e-9 relative (the reduction order differs from the oracle's sequential sum).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as orc_mod
+import qublar_amd as qa
+
+pytestmark = pytest.mark.gpu
+
+ENERGY_RTOL = 1e-9
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    info = qa.device_info()
+    say(f"device {info}")
+    assert info["arch"].startswith("gfx950")
+    return info
+
+
+def oracle_graph(g):
+    return orc_mod.IntGraph(g.n, g.eu, g.ev, g.ew)
+
+
+def oracle_problem(P):
+    return orc_mod.FloatProblem(P.row_ptr, P.ray, P.a, P.d, P.w, P.nbr_ptr, P.nbr, P.lam, P.kappa, P.field)
+
+
+def osched(s):
+    return orc_mod.schedule(s.t_hot, s.t_cold, s.anneal_sweeps, s.hold_sweeps)
+
+
+# ---- integer path -------------------------------------------------------------------
+
+def check_int_batch(g, sched, R, seed_base=1, force_global=False, updates_per_launch=1e9):
+    G = qa.Graph(g, force_global=force_global)
+    order = np.zeros(g.n, np.int32)
+    plan = G.plan()
+    cp = np.zeros(plan["n_classes"] + 1, np.int32)
+    qa._check(qa.lib().qb_graph_colouring(G.h, order.ctypes.data, cp.ctypes.data))
+    say(f"run_batch n={g.n} m={g.m} R={R} classes={plan['n_classes']} global={force_global}")
+    s, e, rep = G.run_batch(sched, seed_base, R, updates_per_launch)
+    og = oracle_graph(g)
+    mode = orc_mod.make_mode(orc_mod.RNG_PHILOX, order, cp)
+    for r in range(R):
+        x, en = orc_mod.anneal_int(og, osched(sched), seed_base + r, mode)
+        xg = (s[r, :g.n] > 0).astype(np.uint8)
+        assert np.array_equal(xg, x), f"replica {r}: {int((xg != x).sum())} spins differ"
+        assert e[r] == en, f"replica {r}: energy {e[r]} vs {en}"
+        assert np.all(np.abs(s[r, :g.n]) == 1)
+    X = (s[:, :g.n] > 0).astype(np.uint8)
+    assert np.array_equal(G.device_energies(X), e)
+    return s, e, rep
+
+
+def test_int_maxcut_800_sixteen_replicas(gpu):
+    """The max-cut workload: ER n=800 p=0.06 +-1, 16 replicas, T 3.0 -> 0.05 over 1000 sweeps."""
+    g = qa.er_graph(800, 0.06, 1)
+    s, e, rep = check_int_batch(g, qa.schedule(3.0, 0.05, 1000, 0), 16)
+    say(f"kernel_ms {rep['kernel_ms']:.2f} best cut {-e.min()}")
+
+
+def test_int_chunked_launches_and_global_state(gpu):
+    """The launch split and the state placement never change results (ising_gpu.cuh:19-21)."""
+    g = qa.er_graph(300, 0.1, 4)
+    sched = qa.schedule(3.0, 0.05, 120, 10)
+    s0, e0, _ = check_int_batch(g, sched, 8, 5)
+    s1, e1, rep = check_int_batch(g, sched, 8, 5, updates_per_launch=8 * 300 * 7)
+    assert rep["launches"] > 10
+    s2, e2, _ = check_int_batch(g, sched, 8, 5, force_global=True)
+    assert np.array_equal(s0, s1) and np.array_equal(e0, e1)
+    assert np.array_equal(s0, s2) and np.array_equal(e0, e2)
+
+
+def test_int_k16_and_signed_weights(gpu):
+    """K16 with weights in [-1000, 1000] (gpu_anneal.cu:166-169) and a signed sparse graph."""
+    rng = np.random.default_rng(3)
+    iu = np.triu_indices(16, 1)
+    g = qa.GraphArrays(16, iu[0], iu[1], rng.integers(-1000, 1001, len(iu[0])))
+    check_int_batch(g, qa.schedule(1000.0, 0.1, 400, 20), 8)
+    eu, ev = [], []
+    seen = set()
+    while len(eu) < 3000:
+        u, v = rng.integers(0, 300, 2)
+        if u != v and (min(u, v), max(u, v)) not in seen:
+            seen.add((min(u, v), max(u, v))); eu.append(min(u, v)); ev.append(max(u, v))
+    w = rng.integers(1, 1001, 3000) * rng.choice([-1, 1], 3000)
+    check_int_batch(qa.GraphArrays(300, eu, ev, w), qa.schedule(500.0, 0.1, 200, 20), 8)
+
+
+# ---- float path ---------------------------------------------------------------------
+
+def check_branches(P, sched, R, seed_offset=0, precision=qa.FP64, recompute_every=0, **kw):
+    prob = qa.Problem(P)
+    order, cp = prob.colouring()
+    info = prob.info()
+    say(f"anneal n={P.n_vars} rays={P.n_rays} classes={info['n_classes']} max_entries={info['max_entries_per_bit']}"
+        f" R={R} prec={precision} k={recompute_every} {kw}")
+    out = prob.anneal_branches(sched, R, seed_offset, precision, recompute_every, **kw)
+    op = oracle_problem(P)
+    mode = orc_mod.make_mode(orc_mod.RNG_PHILOX, order, cp, precision, recompute_every)
+    pv, br = orc_mod.branch_fractions(op, osched(sched), R, 16, seed_offset, mode)
+    diff = (out["branches"] != br).sum(axis=1)
+    assert not diff.any(), f"branches differ at {np.flatnonzero(diff)[:8]}: counts {diff[diff > 0][:8]}"
+    assert np.array_equal(out["p_void"], pv)
+    tri = np.where(pv >= np.float32(0.9), 1, np.where(pv <= np.float32(0.1), 0, 2))
+    assert np.array_equal(out["tri"], tri.astype(np.uint8))
+    assert np.array_equal(out["certainty"], np.maximum(pv, np.float32(1.0) - pv))
+    e_ref = np.array([orc_mod.binary_energy(op, br[b]) for b in range(R)])
+    assert np.allclose(out["energy"], e_ref, rtol=ENERGY_RTOL, atol=1e-12 * max(1.0, np.abs(e_ref).max()))
+    return prob, out
+
+
+def small_muon(**kw):
+    base = dict(grid=32, sky_az=90, sky_th=20, log2_muons=22)
+    base.update(kw)
+    return qa.muon_scene(**base)
+
+
+def test_ray_muon_small_fp64(gpu):
+    P = small_muon()
+    prob, out = check_branches(P, qa.schedule(50.0, 1.0, 30, 10), 16)
+    assert 0 < out["branches"].sum() < out["branches"].size
+    say(f"kernel_ms {out['report']['kernel_ms']:.2f}")
+
+
+def test_ray_muon_small_fp32_recompute(gpu):
+    P = small_muon()
+    _, out = check_branches(P, qa.schedule(50.0, 1.0, 30, 10), 16, precision=qa.FP32, recompute_every=7)
+    assert out["report"]["fp32_residual_err"] < 1e-3
+
+
+def test_ray_uneven_branch_groups_and_offset(gpu):
+    """20 branches (a full group and a partial one) with seed_offset 5."""
+    P = small_muon(grid=24, sky_az=60, sky_th=15)
+    check_branches(P, qa.schedule(20.0, 1.0, 20, 5), 20, seed_offset=5, precision=qa.FP32)
+
+
+def test_ray_launch_split_and_grid_size_do_not_change_results(gpu):
+    P = small_muon(grid=24, sky_az=60, sky_th=15)
+    sched = qa.schedule(20.0, 1.0, 20, 5)
+    _, a = check_branches(P, sched, 16, precision=qa.FP32, recompute_every=4)
+    _, b = check_branches(P, sched, 16, precision=qa.FP32, recompute_every=4, sweeps_per_launch=3, workgroups=37)
+    assert b["report"]["launches"] == 9
+    assert np.array_equal(a["branches"], b["branches"])
+
+
+def synthetic_problem(n=300, n_rays=400, seed=0, dup=False, field=True):
+    """Mixed light and heavy columns, a linear field, 6-neighbour-like adjacency, and
+    (dup=True) repeated rays inside some heavy columns."""
+    rng = np.random.default_rng(seed)
+    row_ptr, ray, a = [0], [], []
+    for i in range(n):
+        k = int(rng.integers(1, 12)) if rng.random() < 0.8 else int(rng.integers(17, 90))
+        rs = list(rng.choice(n_rays, size=k, replace=False))
+        if dup and k > 16 and rng.random() < 0.5:
+            rs[-1] = rs[0]
+        for b in rs:
+            ray.append(b)
+            a.append(rng.uniform(0.005, 0.08))
+        row_ptr.append(len(ray))
+    side = 17
+    nbr_ptr, nbr = [0], []
+    for i in range(n):
+        for j in (i - 1, i + 1, i - side, i + side):
+            if 0 <= j < n and j != i and abs(j % side - i % side) <= 1:
+                nbr.append(j)
+        nbr_ptr.append(len(nbr))
+    d = rng.uniform(0.0, 0.4, n_rays)
+    w = rng.uniform(5.0, 60.0, n_rays)
+    fld = rng.normal(0.5, 0.8, n) if field else None
+    return qa.ProblemArrays(row_ptr, ray, a, d, w, nbr_ptr, nbr, 0.7, 1.3, fld)
+
+
+@pytest.mark.parametrize("precision", [qa.FP64, qa.FP32])
+def test_ray_synthetic_heavy_and_duplicate_columns(gpu, precision):
+    P = synthetic_problem(dup=True)
+    check_branches(P, qa.schedule(8.0, 0.3, 40, 10), 16, precision=precision, recompute_every=6)
+
+
+def test_energy_of_arbitrary_states(gpu):
+    P = synthetic_problem(seed=4)
+    prob = qa.Problem(P)
+    rng = np.random.default_rng(9)
+    X = rng.integers(0, 2, (37, P.n_vars)).astype(np.uint8)
+    say("qb_energy K=37")
+    tot, dat, pri = prob.energy(X)
+    op = oracle_problem(P)
+    for k in range(37):
+        d, q = orc_mod.energy_parts(op, X[k])
+        assert abs(dat[k] - d) <= ENERGY_RTOL * max(1.0, abs(d))
+        assert abs(pri[k] - q) <= ENERGY_RTOL * max(1.0, abs(q))
+        assert abs(tot[k] - (d + q)) <= ENERGY_RTOL * max(1.0, abs(d + q))
+
+
+def test_errors_are_reported_not_fatal(gpu):
+    P = synthetic_problem(seed=2)
+    prob = qa.Problem(P)
+    with pytest.raises(qa.QublarError) as e:
+        prob.anneal_branches(qa.schedule(-1.0, 1.0, 10, 0), 16)
+    assert e.value.code == qa.QB_E_ARG
+    with pytest.raises(qa.QublarError):
+        prob.anneal_branches(qa.schedule(5.0, 1.0, 10, 0), 0)
