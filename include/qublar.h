@@ -103,8 +103,9 @@ typedef struct {
 typedef struct {
     int precision;            /* QB_FP64 or QB_FP32 */
     int recompute_every;      /* > 0: residuals recomputed from scratch in FP64 after every k-th sweep */
-    int workgroups;           /* persistent workgroups; 0 = one per compute unit */
+    int workgroups;           /* persistent workgroups, a multiple of 8 (one team per XCD); 0 = one per CU */
     int sweeps_per_launch;    /* 0 = all sweeps in one launch; results do not depend on it */
+    int group_width;          /* branches per group: 2, 4, 8 or 16; 0 = chosen from n_branches */
 } qb_anneal_opts;
 
 typedef struct {

@@ -59,7 +59,7 @@ class Schedule(C.Structure):
 
 class AnnealOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("recompute_every", C.c_int), ("workgroups", C.c_int),
-                ("sweeps_per_launch", C.c_int)]
+                ("sweeps_per_launch", C.c_int), ("group_width", C.c_int)]
 
 
 class Report(C.Structure):
@@ -229,14 +229,14 @@ class Problem:
         return order, cp
 
     def anneal_branches(self, sched, n_branches, seed_offset=0, precision=FP64, recompute_every=0, workgroups=0,
-                        sweeps_per_launch=0, branches=True, outputs=True, energy=True):
+                        sweeps_per_launch=0, group_width=0, branches=True, outputs=True, energy=True):
         n = self.arrays.n_vars
         br = np.zeros((n_branches, n), np.uint8) if branches else None
         pv = np.zeros(n, np.float32) if outputs else None
         tri = np.zeros(n, np.uint8) if outputs else None
         cert = np.zeros(n, np.float32) if outputs else None
         en = np.zeros(n_branches, np.float64) if energy else None
-        opts = AnnealOpts(precision, recompute_every, workgroups, sweeps_per_launch)
+        opts = AnnealOpts(precision, recompute_every, workgroups, sweeps_per_launch, group_width)
         rep = Report()
         _check(lib().qb_anneal_branches(self.h, C.byref(sched), n_branches, C.c_uint64(seed_offset), C.byref(opts),
                                         _p(br), _p(pv), _p(tri), _p(cert), _p(en), C.byref(rep)))

@@ -16,9 +16,6 @@
 
 #include "qb_internal.h"
 
-namespace qb {
-int ray_sweep_max_workgroups();
-}
 
 using namespace qb;
 
@@ -408,15 +405,27 @@ extern "C" int qb_problem_colouring(const qb_problem* p, int32_t* order, int32_t
     return QB_OK;
 }
 
-static int sweep_workgroups(int requested) {
+// Workgroups per XCD team: `requested` total (a multiple of 8), or one per CU.
+static int team_size_for(int requested) {
     hipDeviceProp_t prop;
     int rc = device_props(prop);
     if (rc) return -rc;
-    const int per_cu = ray_sweep_max_workgroups();
-    if (per_cu <= 0) { fail(QB_E_HIP, "occupancy query failed"); return -QB_E_HIP; }
+    const int per_cu = ray_sweep_blocks_per_cu();
+    if (per_cu <= 0) return -fail(QB_E_HIP, "occupancy query for the sweep kernel failed");
     const int cap = per_cu * prop.multiProcessorCount;
-    int g = requested > 0 ? requested : prop.multiProcessorCount;
-    return std::min(g, cap);
+    const int total = requested > 0 ? requested : prop.multiProcessorCount;
+    if (total % 8 != 0 || total > cap)
+        return -fail(QB_E_ARG, "workgroups must be a positive multiple of 8 and at most %d (got %d)", cap, total);
+    return total / 8;
+}
+
+// Branches per group: the smallest of 2, 4, 8, 16 that spreads the branches over
+// the 8 XCD teams in at most 8 groups, unless the caller fixes it.
+static int group_width_for(int R, int requested) {
+    if (requested) return requested;
+    int bw = 2;
+    while (bw < 16 && (R + bw - 1) / bw > 8) bw *= 2;
+    return bw;
 }
 
 extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, uint64_t seed_offset,
@@ -431,18 +440,22 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     if (o.precision != QB_FP64 && o.precision != QB_FP32) return fail(QB_E_ARG, "precision must be QB_FP64 or QB_FP32");
     if (o.recompute_every < 0 || o.sweeps_per_launch < 0 || o.workgroups < 0)
         return fail(QB_E_ARG, "negative option");
+    if (o.group_width != 0 && o.group_width != 2 && o.group_width != 4 && o.group_width != 8 && o.group_width != 16)
+        return fail(QB_E_ARG, "group_width must be 0, 2, 4, 8 or 16 (got %d)", o.group_width);
     const int total = s->anneal_sweeps + s->hold_sweeps;
     const int n = P->n, nr = P->n_rays;
-    const int n_groups = (R + kGroup - 1) / kGroup;
+    const int bw = group_width_for(R, o.group_width);
+    const int n_groups = (R + bw - 1) / bw;
     const size_t tsz = o.precision == QB_FP32 ? sizeof(float) : sizeof(double);
-    const int G = sweep_workgroups(o.workgroups);
-    if (G <= 0) return -G;
-    const int n_part = ray_energy_blocks(P->dev);
+    const int team_size = team_size_for(o.workgroups);
+    if (team_size <= 0) return -team_size;
+    const int n_part = ray_energy_blocks(P->dev, bw);
     if ((rc = P->w_x.reserve(sizeof(uint16_t) * size_t(n_groups) * size_t(std::max(n, 1)))) ||
-        (rc = P->w_r.reserve(tsz * size_t(n_groups) * size_t(std::max(nr, 1)) * kGroup)) ||
-        (rc = P->w_beta.reserve(sizeof(double) * size_t(total))) || (rc = P->w_bar.reserve(sizeof(unsigned))) ||
+        (rc = P->w_r.reserve(tsz * size_t(n_groups) * size_t(std::max(nr, 1)) * size_t(bw))) ||
+        (rc = P->w_beta.reserve(sizeof(double) * size_t(total))) ||
+        (rc = P->w_bar.reserve(sizeof(unsigned) * (kBarWords + kTicketWords))) ||
         (rc = P->w_err.reserve(sizeof(int))) ||
-        (rc = P->w_part.reserve(sizeof(double) * size_t(n_part) * size_t(n_groups) * kGroup)) ||
+        (rc = P->w_part.reserve(sizeof(double) * size_t(n_part) * size_t(n_groups) * size_t(bw))) ||
         (rc = P->w_pv.reserve(sizeof(float) * size_t(std::max(n, 1)))) || (rc = P->w_tri.reserve(size_t(std::max(n, 1)))) ||
         (rc = P->w_cert.reserve(sizeof(float) * size_t(std::max(n, 1)))) ||
         (rc = P->w_br.reserve(branches ? size_t(R) * size_t(std::max(n, 1)) : 1)) ||
@@ -454,28 +467,29 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     QB_HIP(hipStreamSynchronize(st));
     const auto t0 = std::chrono::steady_clock::now();
     QB_HIP(hipMemcpyAsync(P->w_beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice, st));
-    QB_HIP(launch_ray_init(P->dev, o.precision, P->w_x.p, P->w_r.p, n_groups, st));
+    QB_HIP(launch_ray_init(P->dev, o.precision, bw, P->w_x.p, P->w_r.p, n_groups, st));
     QB_HIP(hipMemsetAsync(P->w_err.p, 0, sizeof(int), st));
     const int chunk = o.sweeps_per_launch > 0 ? std::min(o.sweeps_per_launch, total) : total;
     int launches = 0;
     QB_HIP(hipEventRecord(P->ev0, st));
     for (int sb = 0; sb < total; sb += chunk) {
-        QB_HIP(hipMemsetAsync(P->w_bar.p, 0, sizeof(unsigned), st));
-        QB_HIP(launch_ray_sweeps(P->dev, o.precision, P->w_x.p, P->w_r.p, n_groups, R, seed_offset,
+        QB_HIP(hipMemsetAsync(P->w_bar.p, 0, sizeof(unsigned) * (kBarWords + kTicketWords), st));
+        QB_HIP(launch_ray_sweeps(P->dev, o.precision, bw, P->w_x.p, P->w_r.p, n_groups, R, seed_offset,
                                  P->w_beta.as<double>(), sb, std::min(total, sb + chunk), o.recompute_every,
-                                 P->w_bar.as<unsigned>(), P->w_err.as<int>(), G, st));
+                                 P->w_bar.as<unsigned>(), P->w_bar.as<unsigned>() + kBarWords, P->w_err.as<int>(),
+                                 team_size, st));
         ++launches;
     }
     QB_HIP(hipEventRecord(P->ev1, st));
     const bool want_outputs = p_void || tri || certainty || branches;
     if (want_outputs)
-        QB_HIP(launch_ray_outputs(P->dev, P->w_x.as<uint16_t>(), n_groups, R, P->w_pv.as<float>(),
+        QB_HIP(launch_ray_outputs(P->dev, P->w_x.as<uint16_t>(), n_groups, bw, R, P->w_pv.as<float>(),
                                   P->w_tri.as<uint8_t>(), P->w_cert.as<float>(),
                                   branches ? P->w_br.as<uint8_t>() : nullptr, st));
     const bool want_energy = energy || rep;
     if (want_energy) {
         QB_HIP(hipMemsetAsync(P->w_rerr.p, 0, sizeof(float), st));
-        QB_HIP(launch_ray_energy(P->dev, P->w_x.as<uint16_t>(), n_groups, R, P->w_r.p, o.precision,
+        QB_HIP(launch_ray_energy(P->dev, P->w_x.as<uint16_t>(), n_groups, bw, R, P->w_r.p, o.precision,
                                  P->w_part.as<double>(), n_part, P->w_data.as<double>(), P->w_prior.as<double>(),
                                  P->w_rerr.as<float>(), st));
     }
@@ -494,7 +508,9 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     QB_HIP(hipMemcpyAsync(&err_flag, P->w_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     QB_HIP(hipStreamSynchronize(st));
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (err_flag) return fail(QB_E_TIMEOUT, "a sweep barrier wait exceeded its limit (%d workgroups)", G);
+    if (err_flag == 2)
+        return fail(QB_E_TIMEOUT, "an XCD team came up short of %d workgroups (workgroup placement)", team_size);
+    if (err_flag) return fail(QB_E_TIMEOUT, "a sweep barrier wait exceeded its limit (%d workgroups)", 8 * team_size);
     if (energy)
         for (int b = 0; b < R; ++b) energy[b] = dat[size_t(b)] + pri[size_t(b)];
     if (rep) {
@@ -514,7 +530,7 @@ extern "C" int qb_energy(qb_problem* P, const uint8_t* X, int K, double* total, 
     if (K <= 0) return fail(QB_E_ARG, "qb_energy: K must be positive");
     const int n = P->n;
     const int n_groups = (K + kGroup - 1) / kGroup;
-    const int n_part = ray_energy_blocks(P->dev);
+    const int n_part = ray_energy_blocks(P->dev, kGroup);
     int rc;
     if ((rc = P->w_X.reserve(size_t(K) * size_t(std::max(n, 1)))) ||
         (rc = P->w_x.reserve(sizeof(uint16_t) * size_t(n_groups) * size_t(std::max(n, 1)))) ||
@@ -524,7 +540,7 @@ extern "C" int qb_energy(qb_problem* P, const uint8_t* X, int K, double* total, 
     hipStream_t st = P->stream;
     QB_HIP(hipMemcpyAsync(P->w_X.p, X, size_t(K) * size_t(n), hipMemcpyHostToDevice, st));
     if (n > 0) QB_HIP(launch_pack_states(P->w_X.as<uint8_t>(), K, n, P->dev.pos_bit, P->w_x.as<uint16_t>(), st));
-    QB_HIP(launch_ray_energy(P->dev, P->w_x.as<uint16_t>(), n_groups, K, nullptr, 0, P->w_part.as<double>(), n_part,
+    QB_HIP(launch_ray_energy(P->dev, P->w_x.as<uint16_t>(), n_groups, kGroup, K, nullptr, 0, P->w_part.as<double>(), n_part,
                              P->w_data.as<double>(), P->w_prior.as<double>(), nullptr, st));
     std::vector<double> dat(static_cast<size_t>(K)), pri(static_cast<size_t>(K));
     QB_HIP(hipMemcpyAsync(dat.data(), P->w_data.p, sizeof(double) * size_t(K), hipMemcpyDeviceToHost, st));

@@ -56,17 +56,22 @@ struct DevProblem {
     double lambda;
 };
 
-// Mutable state of one anneal: n_groups groups of 16 branches.
+// Mutable state of one anneal: n_groups groups of BW branches (BW in {2, 4, 8, 16}).
 template <typename T>
 struct DevState {
     int n_groups, R;
     uint64_t seed_offset;
-    uint16_t* x;            // [g][n] spin masks, bit b = branch g*16 + b
-    T* r;                   // [g][n_rays][16] residuals
+    uint16_t* x;            // [g][n] spin masks, bit b = branch g*BW + b
+    T* r;                   // [g][n_rays][BW] residuals
     const double* beta;     // 1 / T per sweep
-    unsigned* bar;          // barrier counter (monotone)
-    int* err;               // set to 1 when a barrier wait times out
+    unsigned* bar;          // [0]: grid barrier; [64 + 64 t]: team t's barrier (monotone)
+    unsigned* tickets;      // [64 t]: team t's member count
+    int* err;               // 1: a barrier wait timed out; 2: a team came up short
 };
+
+// Unsigned words of the barrier array and the ticket array (zeroed before a launch).
+constexpr int kBarWords = 64 * 9;
+constexpr int kTicketWords = 64 * 8;
 
 // The integer graph on the device, in colour-class position order.
 struct DevGraph {
@@ -84,20 +89,22 @@ struct DevGraph {
 };
 
 // ---- launchers (qb_kernels.hip) ----
-hipError_t launch_ray_init(const DevProblem& p, int precision, void* x, void* r, int n_groups,
+hipError_t launch_ray_init(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups,
                            hipStream_t st);
-hipError_t launch_ray_sweeps(const DevProblem& p, int precision, void* x, void* r, int n_groups, int R,
+// The persistent sweep on team_size * 8 workgroups (team_size per XCD).
+hipError_t launch_ray_sweeps(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups, int R,
                              uint64_t seed_offset, const double* beta, int sweep_begin, int sweep_end,
-                             int recompute_every, unsigned* bar, int* err, int workgroups,
+                             int recompute_every, unsigned* bar, unsigned* tickets, int* err, int team_size,
                              hipStream_t st);
-hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int R, float* p_void,
+int ray_sweep_blocks_per_cu();
+hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, float* p_void,
                               uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st);
 // FP64 energy of every branch: data and prior partial sums per block, reduced in a
 // fixed order; resid_err receives max |r_carried - r_scratch| when r_carried != null.
-hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_groups, int R,
+hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R,
                              const void* r_carried, int precision, double* part, int n_part_blocks,
                              double* data, double* prior, float* resid_err, hipStream_t st);
-int ray_energy_blocks(const DevProblem& p);
+int ray_energy_blocks(const DevProblem& p, int bw);
 hipError_t launch_pack_states(const uint8_t* X, int K, int n, const int32_t* pos_bit, uint16_t* x,
                               hipStream_t st);
 

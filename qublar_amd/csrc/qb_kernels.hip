@@ -1,14 +1,16 @@
 // qb_kernels.hip -- gfx950 kernels of the simulated-annealing QUBO engine.
 //
 // Float path (ray x voxel QUBO, src/ising_recon.hpp:105-166):
-//   ray_sweep_kernel    persistent; every colour class is one step. Sixteen lanes
-//                       evaluate one bit for the 16 branches of a group: dE from the
-//                       bit's CSC column (ray, a, w) and the group's residual line
-//                       r[ray][0..15], the neighbour term from 16-bit spin masks, one
-//                       Philox draw per lane, and on acceptance the residual scatter.
-//                       Bits of one class share no ray and are not neighbours, so
-//                       a step needs no atomics and its result does not depend on
-//                       the order inside it. Steps are separated by a grid barrier.
+//   ray_team_kernel     persistent; one team of workgroups per XCD, each team
+//                       annealing its own groups of BW branches. Every colour class
+//                       is one step: BW lanes evaluate one bit for the BW branches
+//                       of a group -- dE from the bit's CSC column (ray, a, w) and the
+//                       group's residual line r[ray][0..BW), the neighbour term from
+//                       spin masks, one Philox draw per lane, and on acceptance the
+//                       residual scatter. Bits of one class share no ray and are not
+//                       neighbours, so a step needs no atomics and its result does not
+//                       depend on the order inside it. Steps are separated by a team
+//                       barrier in the XCD's L2.
 //   ray_outputs_kernel  branch agreement: p_void, tri-state, certainty, branches.
 //   ray_energy_*        FP64 binary_energy of every branch, from scratch.
 // Integer path (src/ising_weighted.hpp:193-267, src/ising_gpu.cuh:149-272):
@@ -25,13 +27,17 @@ namespace qb {
 
 // ---------------------------------------------------------------------------------
 // Grid barrier for the persistent sweep. Monotone counter: the k-th barrier of a
-// launch waits until the counter reaches k * gridDim.x. Release/acquire at agent
-// scope make the residual and spin stores of one step visible to every workgroup
-// before the next step reads them. A wait longer than ~2 s (s_memrealtime runs at
-// 100 MHz) sets *err and every workgroup leaves its loops, so the grid drains.
+// launch waits until the counter reaches k * gridDim.x. Every wave first waits for
+// its own global stores to complete (s_barrier does not wait for other waves'
+// stores on this target), so when thread 0's agent-scope release writes the L2
+// back, the whole block's residual and spin stores of the step are in it; the
+// acquire after the wait makes every block's stores visible before the next step
+// reads them. A wait longer than ~2 s (s_memrealtime runs at 100 MHz) sets *err and
+// every workgroup leaves its loops, so the grid drains.
 // ---------------------------------------------------------------------------------
 __device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned target, int* err) {
     __shared__ int s_ok;
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
@@ -70,116 +76,217 @@ __device__ __forceinline__ float residual_add<float>(float r, float a, double de
     return r + (delta > 0.0 ? a : -a);                  // the FP32 carry: one float add
 }
 
+// ---------------------------------------------------------------------------------
+// XCD teams. The sweep's workgroups form one team per XCD (the hardware XCC_ID), and
+// a team anneals its own branch groups alone. Inside an XCD the L2 is the point of
+// coherence: stores write through the CU's L1 to it, so a team shares its mutable
+// state (residuals, spin masks) by draining stores before each barrier and reading
+// that state with loads that skip L1 and hit L2 (scope bit sc1). No cache is written
+// back or invalidated between steps, and teams never touch each other's state.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
+
+template <typename V>
+__device__ __forceinline__ V ld_l2(const V* p) {
+    return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Team barrier: monotone counter in L2; the k-th barrier waits for k * team size
+// arrivals. Same drain rule and ~2 s limit as grid_barrier.
+__device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int* err) {
+    __shared__ int s_ok;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = ld_l2(err) == 0;
+        if (ok) {
+            __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            unsigned polls = 0;
+            while (ld_l2(bar) < target) {
+                if ((++polls & 1023u) == 0) {
+                    if (ld_l2(err) != 0 || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        break;
+                    }
+                }
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
 // The dE of a bit sums its CSR entries in chains of kSeg: the first chain starts
 // from lin * delta, later chains start from their first term, and the chain sums
 // are added in order; the neighbour terms follow in CSR order. For a bit with at
 // most kSeg entries this is local_field_dE's order exactly (ising_recon.hpp:105-118).
 
-// One chain: entries [ka, kb) (kb - ka <= kSeg) for branch lane b. All loads are
-// issued before the first add, so a chain costs two dependent memory round trips.
+// Loads of one chain: the entries [ka, kb) (1 <= kb - ka <= kSeg) and the group's
+// residuals on their rays. Every load is unconditional -- past kb the index is
+// clamped to kb - 1 -- so all of them issue back to back and the chain costs two
+// dependent memory round trips.
 template <typename T>
-__device__ __forceinline__ double chain_sum(const RayEnt* __restrict__ ent, int ka, int kb, const T* rg, int b,
-                                            double delta, double seed, bool seeded) {
+struct Chain {
     RayEnt e[kSeg];
-    double rv[kSeg];
+    T rv[kSeg];
+};
+
+template <typename T, int BW>
+__device__ __forceinline__ void chain_load(const RayEnt* __restrict__ ent, int ka, int kb, const T* rg, int b,
+                                           Chain<T>& ch) {
 #pragma unroll
-    for (int u = 0; u < kSeg; ++u)
-        if (ka + u < kb) e[u] = ent[ka + u];
+    for (int u = 0; u < kSeg; ++u) ch.e[u] = ent[min(ka + u, kb - 1)];
 #pragma unroll
-    for (int u = 0; u < kSeg; ++u)
-        if (ka + u < kb) rv[u] = double(rg[size_t(e[u].ray) * kGroup + b]);
+    for (int u = 0; u < kSeg; ++u) ch.rv[u] = ld_l2(rg + size_t(ch.e[u].ray) * BW + b);
+}
+
+// The chain's sum: seeded with `seed` (the first chain of a bit, seed = lin * delta)
+// or started from its first term; terms w (r a + a^2 / 2), a = a_k delta, in order.
+template <typename T>
+__device__ __forceinline__ double chain_sum(const Chain<T>& ch, int len, double delta, double seed, bool seeded) {
     double acc = seed;
 #pragma unroll
-    for (int u = 0; u < kSeg; ++u)
-        if (ka + u < kb) {
-            const double ad = double(e[u].a) * delta;
-            const double t = e[u].w * (rv[u] * ad + 0.5 * ad * ad);
-            acc = (u == 0 && !seeded) ? t : acc + t;
-        }
+    for (int u = 0; u < kSeg; ++u) {
+        const double ad = double(ch.e[u].a) * delta;
+        const double t = ch.e[u].w * (double(ch.rv[u]) * ad + 0.5 * ad * ad);
+        const double nxt = (u == 0 && !seeded) ? t : acc + t;
+        acc = u < len ? nxt : acc;
+    }
     return acc;
 }
 
-template <typename T>
-__device__ __forceinline__ double neighbour_terms(const DevProblem& p, const uint16_t* xg, int pos, int b, int xb,
-                                                  double acc) {
-    for (int k = p.nb_ptr[pos]; k < p.nb_ptr[pos + 1]; ++k) {
-        const int before = ((xg[p.nb[k]] >> b) & 1) != xb;
+// On acceptance: r[ray] += a * delta for the chain's entries, from the loaded values
+// (no other bit of the class touches these rays).
+template <typename T, int BW>
+__device__ __forceinline__ void chain_scatter(const Chain<T>& ch, int len, T* rg, int b, double delta) {
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u)
+        if (u < len) rg[size_t(ch.e[u].ray) * BW + b] = residual_add<T>(ch.rv[u], ch.e[u].a, delta);
+}
+
+// lambda * (+-1) per neighbour, in CSR order; the first 8 neighbours' masks are loaded
+// together (the 6-neighbour grid never needs the tail loop).
+__device__ __forceinline__ double neighbour_terms(const DevProblem& p, const uint16_t* xg, int n0, int n1, int b,
+                                                  int xb, double acc) {
+    if (n1 <= n0) return acc;
+    constexpr int kNb = 8;
+    int q[kNb];
+    uint16_t m[kNb];
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) q[u] = p.nb[min(n0 + u, n1 - 1)];
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + q[u]);
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) {
+        const int before = ((m[u] >> b) & 1) != xb;
+        const double nxt = acc + p.lambda * (before ? -1.0 : 1.0);
+        acc = n0 + u < n1 ? nxt : acc;
+    }
+    for (int k = n0 + kNb; k < n1; ++k) {
+        const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
         acc += p.lambda * (before ? -1.0 : 1.0);
     }
     return acc;
 }
 
 template <typename T>
-__device__ __forceinline__ bool decide(const DevProblem& p, const DevState<T>& s, int pos, int branch, int sweep,
-                                       double beta, double dE) {
+__device__ __forceinline__ bool decide(const DevState<T>& s, int bit, int branch, int sweep, double beta, double dE) {
     if (branch >= s.R) return false;
-    return dE <= 0.0 || metropolis_uphill(dE, beta, s.seed_offset + uint64_t(branch) + 1u, uint32_t(sweep),
-                                          uint32_t(p.pos_bit[pos]));
+    return dE <= 0.0 ||
+           metropolis_uphill(dE, beta, s.seed_offset + uint64_t(branch) + 1u, uint32_t(sweep), uint32_t(bit));
 }
 
-// A bit with at most kSeg entries, for the 16 branches of group g (lane b).
-template <typename T>
+// The accepted lanes of this lane's BW-wide segment, as a mask of the group's branches.
+template <int BW>
+__device__ __forceinline__ uint16_t segment_flips(bool acc) {
+    const uint64_t bal = __ballot(acc);
+    const unsigned base = threadIdx.x & 63u & ~unsigned(BW - 1);
+    return uint16_t((bal >> base) & ((uint64_t(1) << BW) - 1));
+}
+
+// A bit with at most kSeg entries, for the BW branches of group g (lane b).
+template <typename T, int BW>
 __device__ __forceinline__ void light_update(const DevProblem& p, const DevState<T>& s, int pos, int g, int b,
                                              int sweep, double beta) {
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
-    T* rg = s.r + size_t(g) * size_t(p.n_rays) * kGroup;
-    const uint16_t xm = xg[pos];
+    T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
+    const uint16_t xm = ld_l2(xg + pos);
+    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
+    const int n0 = p.nb_ptr[pos], n1 = p.nb_ptr[pos + 1];
+    const double lin = p.lin[pos];
+    const int bit = p.pos_bit[pos];
     const int xb = (xm >> b) & 1;
     const double delta = xb ? -1.0 : 1.0;
-    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
-    double dE = chain_sum<T>(p.ent, k0, k1, rg, b, delta, p.lin[pos] * delta, true);
-    dE = neighbour_terms<T>(p, xg, pos, b, xb, dE);
-    const bool acc = decide<T>(p, s, pos, g * kGroup + b, sweep, beta, dE);
-    if (acc)
-        for (int k = k0; k < k1; ++k) {
-            const RayEnt e = p.ent[k];
-            T* q = rg + size_t(e.ray) * kGroup + b;
-            *q = residual_add<T>(*q, e.a, delta);
+    const int len = k1 - k0;
+    Chain<T> ch;
+    double dE = lin * delta;
+    if (len > 0) {
+        chain_load<T, BW>(p.ent, k0, k1, rg, b, ch);
+        dE = chain_sum<T>(ch, len, delta, dE, true);
+    }
+    dE = neighbour_terms(p, xg, n0, n1, b, xb, dE);
+    const bool acc = decide<T>(s, bit, g * BW + b, sweep, beta, dE);
+    if (acc && len > 0) {
+        if (p.seq_update[pos]) {
+            // a ray repeats in this column: its updates must land in CSR order
+            for (int k = k0; k < k1; ++k) {
+                const RayEnt e = p.ent[k];
+                T* q = rg + size_t(e.ray) * BW + b;
+                *q = residual_add<T>(ld_l2(q), e.a, delta);
+                __builtin_amdgcn_s_waitcnt(0);
+            }
+        } else {
+            chain_scatter<T, BW>(ch, len, rg, b, delta);
         }
-    const uint64_t bal = __ballot(acc);
-    const uint16_t flips = uint16_t(bal >> (threadIdx.x & 48u));
+    }
+    const uint16_t flips = segment_flips<BW>(acc);
     if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
 }
 
 // A bit with more than kSeg entries, by a whole workgroup: the chains are spread over
-// the block's 16-lane slots, their sums meet in LDS, slot 0 adds them in order and
+// the block's BW-lane slots, their sums meet in LDS, slot 0 adds them in order and
 // decides, and the residual scatter is spread over the block again.
-template <typename T, int BLOCK>
+template <typename T, int BW, int BLOCK>
 __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState<T>& s, int pos, int g, int sweep,
                                              double beta) {
-    constexpr int kSlots = BLOCK / kGroup;
-    __shared__ double part[kSlots][kGroup];
-    __shared__ int accepted[kGroup];
-    const int b = threadIdx.x & (kGroup - 1);
-    const int slot = threadIdx.x / kGroup;
+    constexpr int kSlots = BLOCK / BW;
+    __shared__ double part[kSlots][BW];
+    __shared__ int accepted[BW];
+    const int b = threadIdx.x % BW;
+    const int slot = threadIdx.x / BW;
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
-    T* rg = s.r + size_t(g) * size_t(p.n_rays) * kGroup;
-    const uint16_t xm = xg[pos];
+    T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
+    const uint16_t xm = ld_l2(xg + pos);
+    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
+    const double lin = p.lin[pos];
     const int xb = (xm >> b) & 1;
     const double delta = xb ? -1.0 : 1.0;
-    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
     const int n_chain = (k1 - k0 + kSeg - 1) / kSeg;
     double dE = 0.0;
+    Chain<T> ch;
     for (int j0 = 0; j0 < n_chain; j0 += kSlots) {
         const int j = j0 + slot;
         if (j < n_chain) {
             const int ka = k0 + j * kSeg;
-            const int kb = ka + kSeg < k1 ? ka + kSeg : k1;
-            part[slot][b] = chain_sum<T>(p.ent, ka, kb, rg, b, delta, p.lin[pos] * delta, j == 0);
+            const int kb = min(ka + kSeg, k1);
+            chain_load<T, BW>(p.ent, ka, kb, rg, b, ch);
+            part[slot][b] = chain_sum<T>(ch, kb - ka, delta, lin * delta, j == 0);
         }
         __syncthreads();
         if (slot == 0) {
-            const int m = n_chain - j0 < kSlots ? n_chain - j0 : kSlots;
+            const int m = min(n_chain - j0, kSlots);
             for (int q = 0; q < m; ++q) dE = (j0 + q == 0) ? part[0][b] : dE + part[q][b];
         }
         __syncthreads();
     }
     if (slot == 0) {
-        dE = neighbour_terms<T>(p, xg, pos, b, xb, dE);
-        const bool acc = decide<T>(p, s, pos, g * kGroup + b, sweep, beta, dE);
+        dE = neighbour_terms(p, xg, p.nb_ptr[pos], p.nb_ptr[pos + 1], b, xb, dE);
+        const bool acc = decide<T>(s, p.pos_bit[pos], g * BW + b, sweep, beta, dE);
         accepted[b] = acc ? 1 : 0;
-        const uint64_t bal = __ballot(acc);
-        const uint16_t flips = uint16_t(bal >> (threadIdx.x & 48u));
+        const uint16_t flips = segment_flips<BW>(acc);
         if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
     }
     __syncthreads();
@@ -189,14 +296,23 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
             if (slot == 0)
                 for (int k = k0; k < k1; ++k) {
                     const RayEnt e = p.ent[k];
-                    T* q = rg + size_t(e.ray) * kGroup + b;
-                    *q = residual_add<T>(*q, e.a, delta);
+                    T* q = rg + size_t(e.ray) * BW + b;
+                    *q = residual_add<T>(ld_l2(q), e.a, delta);
+                    __builtin_amdgcn_s_waitcnt(0);
                 }
         } else {
-            for (int k = k0 + slot; k < k1; k += kSlots) {
-                const RayEnt e = p.ent[k];
-                T* q = rg + size_t(e.ray) * kGroup + b;
-                *q = residual_add<T>(*q, e.a, delta);
+            // the chain this slot evaluated in the last pass is still in registers:
+            // scatter it first, then load and scatter the slot's earlier chains
+            const int last_pass = ((n_chain - 1) / kSlots) * kSlots;
+            if (last_pass + slot < n_chain) {
+                const int ka = k0 + (last_pass + slot) * kSeg;
+                chain_scatter<T, BW>(ch, min(ka + kSeg, k1) - ka, rg, b, delta);
+            }
+            for (int j = slot; j < last_pass; j += kSlots) {
+                const int ka = k0 + j * kSeg;
+                const int kb = min(ka + kSeg, k1);
+                chain_load<T, BW>(p.ent, ka, kb, rg, b, ch);
+                chain_scatter<T, BW>(ch, kb - ka, rg, b, delta);
             }
         }
     }
@@ -205,23 +321,57 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
 
 // Residuals of ray `ray` for branch lane b, from scratch: -d + sum of a over the
 // ray's set bits in increasing original bit index (binary_energy's order).
-template <typename T>
+template <typename T, int BW>
 __device__ __forceinline__ void ray_recompute(const DevProblem& p, const DevState<T>& s, int ray, int g, int b) {
     const uint16_t* xg = s.x + size_t(g) * size_t(p.n);
     double r = -p.d[ray];
-    for (int k = p.rt_ptr[ray]; k < p.rt_ptr[ray + 1]; ++k) {
-        const BitEnt e = p.rt[k];
-        if ((xg[e.pos] >> b) & 1) r += double(e.a);
+    const int q0 = p.rt_ptr[ray], q1 = p.rt_ptr[ray + 1];
+    for (int qa = q0; qa < q1; qa += kSeg) {
+        BitEnt e[kSeg];
+        uint16_t m[kSeg];
+#pragma unroll
+        for (int u = 0; u < kSeg; ++u) e[u] = p.rt[min(qa + u, q1 - 1)];
+#pragma unroll
+        for (int u = 0; u < kSeg; ++u) m[u] = ld_l2(xg + e[u].pos);
+#pragma unroll
+        for (int u = 0; u < kSeg; ++u) {
+            const double nxt = r + double(e[u].a);
+            r = (qa + u < q1 && ((m[u] >> b) & 1)) ? nxt : r;
+        }
     }
-    s.r[(size_t(g) * size_t(p.n_rays) + size_t(ray)) * kGroup + b] = T(r);
+    s.r[(size_t(g) * size_t(p.n_rays) + size_t(ray)) * BW + b] = T(r);
 }
 
-template <typename T, int BLOCK>
+// The persistent sweep. Block b runs on XCD b mod 8 (hardware round-robin); the
+// blocks read their XCC_ID to form the teams, take a rank by ticket, and check
+// once, behind an agent-scope grid barrier, that every team is complete. Team t
+// anneals groups t, t + 8, ... through every class of every sweep.
+constexpr int kTeams = 8;
+
+template <typename T, int BW, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-ray_sweep_kernel(DevProblem p, DevState<T> s, int sweep_begin, int sweep_end, int recompute_every) {
-    constexpr int kSlots = BLOCK / kGroup;
-    const int b = threadIdx.x & (kGroup - 1);
-    const int n_slots = int(gridDim.x) * kSlots;
+ray_team_kernel(DevProblem p, DevState<T> s, int sweep_begin, int sweep_end, int recompute_every, int team_size) {
+    constexpr int kSlots = BLOCK / BW;
+    __shared__ int s_team, s_rank, s_ok;
+    if (threadIdx.x == 0) {
+        s_team = int(xcc_id() & (kTeams - 1));
+        s_rank = int(__hip_atomic_fetch_add(s.tickets + s_team * 64, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    if (!grid_barrier(s.bar, gridDim.x, s.err)) return;
+    if (threadIdx.x == 0) {
+        const unsigned members = __hip_atomic_load(s.tickets + s_team * 64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = members >= unsigned(team_size) ? 1 : 0;
+        if (!s_ok) __hip_atomic_store(s.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const int team = s_team, rank = s_rank;
+    if (!s_ok || rank >= team_size) return;   // extra blocks on a crowded XCD leave
+    unsigned* bar = s.bar + 64 + team * 64;
+    const int my_groups = s.n_groups > team ? (s.n_groups - team + kTeams - 1) / kTeams : 0;
+    if (my_groups == 0) return;
+    const int b = threadIdx.x % BW;
+    const int n_slots = team_size * kSlots;
+    const int my_slot = rank * kSlots + int(threadIdx.x) / BW;
     unsigned target = 0;
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         const double beta = s.beta[sweep];
@@ -229,60 +379,59 @@ ray_sweep_kernel(DevProblem p, DevState<T> s, int sweep_begin, int sweep_end, in
             const int c0 = p.cls_ptr[c];
             const int nh = p.cls_heavy[c];
             const int nl = p.cls_ptr[c + 1] - c0 - nh;
-            const int heavy_tasks = nh * s.n_groups;
-            for (int h = int(blockIdx.x); h < heavy_tasks; h += int(gridDim.x)) {
-                const int g = h / nh;
-                heavy_update<T, BLOCK>(p, s, c0 + (h - g * nh), g, sweep, beta);
+            const int heavy_tasks = nh * my_groups;
+            for (int h = rank; h < heavy_tasks; h += team_size) {
+                const int gi = h / nh;
+                heavy_update<T, BW, BLOCK>(p, s, c0 + (h - gi * nh), team + gi * kTeams, sweep, beta);
             }
             // light tasks start on the blocks after those that took heavy bits
-            const int first = (heavy_tasks % int(gridDim.x)) * kSlots;
-            const int slot = (int(blockIdx.x) * kSlots + int(threadIdx.x) / kGroup - first + n_slots) % n_slots;
-            const int light_tasks = nl * s.n_groups;
+            const int first = (heavy_tasks % team_size) * kSlots;
+            const int slot = (my_slot - first + n_slots) % n_slots;
+            const int light_tasks = nl * my_groups;
             for (int t = slot; t < light_tasks; t += n_slots) {
-                const int g = t / nl;
-                light_update<T>(p, s, c0 + nh + (t - g * nl), g, b, sweep, beta);
+                const int gi = t / nl;
+                light_update<T, BW>(p, s, c0 + nh + (t - gi * nl), team + gi * kTeams, b, sweep, beta);
             }
-            target += gridDim.x;
-            if (!grid_barrier(s.bar, target, s.err)) return;
+            target += unsigned(team_size);
+            if (!team_barrier(bar, target, s.err)) return;
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
-            const int slot = int(blockIdx.x) * kSlots + int(threadIdx.x) / kGroup;
-            const int tasks = p.n_rays * s.n_groups;
-            for (int t = slot; t < tasks; t += n_slots) {
-                const int g = t / p.n_rays;
-                ray_recompute<T>(p, s, t - g * p.n_rays, g, b);
+            const int tasks = p.n_rays * my_groups;
+            for (int t = my_slot; t < tasks; t += n_slots) {
+                const int gi = t / p.n_rays;
+                ray_recompute<T, BW>(p, s, t - gi * p.n_rays, team + gi * kTeams, b);
             }
-            target += gridDim.x;
-            if (!grid_barrier(s.bar, target, s.err)) return;
+            target += unsigned(team_size);
+            if (!team_barrier(bar, target, s.err)) return;
         }
     }
 }
 
-template <typename T>
+template <typename T, int BW>
 __global__ void ray_init_kernel(DevProblem p, uint16_t* x, T* r, int n_groups) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t nx = size_t(n_groups) * size_t(p.n);
-    const size_t nr = size_t(n_groups) * size_t(p.n_rays) * kGroup;
+    const size_t nr = size_t(n_groups) * size_t(p.n_rays) * BW;
     if (i < nx) x[i] = 0;
-    if (i < nr) r[i] = T(-p.d[(i / kGroup) % size_t(p.n_rays)]);
+    if (i < nr) r[i] = T(-p.d[(i / BW) % size_t(p.n_rays)]);
 }
 
 // Branch agreement (branch_fractions' p_v, ising_recon.hpp:160-163, and classify,
 // :305-309) plus certainty max(p, 1 - p); outputs in original bit order.
-__global__ void ray_outputs_kernel(DevProblem p, const uint16_t* x, int n_groups, int R, float* p_void,
+__global__ void ray_outputs_kernel(DevProblem p, const uint16_t* x, int n_groups, int bw, int R, float* p_void,
                                    uint8_t* tri, float* certainty, uint8_t* branches) {
     const int pos = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (pos >= p.n) return;
     const int bit = p.pos_bit[pos];
     int cnt = 0;
     for (int g = 0; g < n_groups; ++g) {
-        const int live = R - g * kGroup < kGroup ? R - g * kGroup : kGroup;
-        const unsigned mask = live >= 16 ? 0xFFFFu : ((1u << live) - 1u);
+        const int live = R - g * bw < bw ? R - g * bw : bw;
+        const unsigned mask = (1u << live) - 1u;
         const unsigned m = unsigned(x[size_t(g) * size_t(p.n) + pos]) & mask;
         cnt += __popc(m);
         if (branches)
             for (int b = 0; b < live; ++b)
-                branches[size_t(g * kGroup + b) * size_t(p.n) + size_t(bit)] = uint8_t((m >> b) & 1u);
+                branches[size_t(g * bw + b) * size_t(p.n) + size_t(bit)] = uint8_t((m >> b) & 1u);
     }
     const float pv = float(cnt) / float(R);
     if (p_void) p_void[bit] = pv;
@@ -292,18 +441,18 @@ __global__ void ray_outputs_kernel(DevProblem p, const uint16_t* x, int n_groups
 
 // ---- FP64 energies -------------------------------------------------------------------
 constexpr int kEnergyBlock = 256;
-constexpr int kEnergyItems = kEnergyBlock / kGroup;   // rays or positions per block
 
 // Data term per (ray, branch lane): 1/2 w r^2 with r from scratch, summed over the
 // block's rays per lane; also max |r_carried - r_scratch|.
-template <typename T>
+template <typename T, int BW>
 __global__ void __launch_bounds__(kEnergyBlock)
-ray_energy_data_kernel(DevProblem p, const uint16_t* x, int n_groups, const T* r_carried, double* part,
-                       int part_stride, float* resid_err) {
+ray_energy_data_kernel(DevProblem p, const uint16_t* x, const T* r_carried, double* part, int part_stride,
+                       float* resid_err) {
+    constexpr int kItems = kEnergyBlock / BW;
     __shared__ double red[kEnergyBlock];
-    const int b = threadIdx.x & (kGroup - 1);
+    const int b = threadIdx.x % BW;
     const int g = blockIdx.y;
-    const int ray = blockIdx.x * kEnergyItems + int(threadIdx.x / kGroup);
+    const int ray = blockIdx.x * kItems + int(threadIdx.x / BW);
     double v = 0.0;
     float err = 0.0f;
     if (ray < p.n_rays) {
@@ -315,27 +464,29 @@ ray_energy_data_kernel(DevProblem p, const uint16_t* x, int n_groups, const T* r
         }
         v = 0.5 * p.w[ray] * r * r;
         if (r_carried) {
-            const double rc = double(r_carried[(size_t(g) * size_t(p.n_rays) + size_t(ray)) * kGroup + b]);
+            const double rc = double(r_carried[(size_t(g) * size_t(p.n_rays) + size_t(ray)) * BW + b]);
             err = float(fabs(rc - r));
         }
     }
     red[threadIdx.x] = v;
     if (r_carried && err > 0.0f) atomicMax(reinterpret_cast<unsigned*>(resid_err), __float_as_uint(err));
     __syncthreads();
-    for (int stride = kEnergyBlock / 2; stride >= kGroup; stride >>= 1) {
+    for (int stride = kEnergyBlock / 2; stride >= BW; stride >>= 1) {
         if (int(threadIdx.x) < stride) red[threadIdx.x] += red[threadIdx.x + stride];
         __syncthreads();
     }
-    if (threadIdx.x < kGroup) part[size_t(blockIdx.x) * part_stride + size_t(g) * kGroup + threadIdx.x] = red[threadIdx.x];
+    if (int(threadIdx.x) < BW) part[size_t(blockIdx.x) * part_stride + size_t(g) * BW + threadIdx.x] = red[threadIdx.x];
 }
 
 // Prior term per (position, branch lane): lin x + lambda per unlike neighbour j > i.
+template <int BW>
 __global__ void __launch_bounds__(kEnergyBlock)
-ray_energy_prior_kernel(DevProblem p, const uint16_t* x, int n_groups, double* part, int part_stride) {
+ray_energy_prior_kernel(DevProblem p, const uint16_t* x, double* part, int part_stride) {
+    constexpr int kItems = kEnergyBlock / BW;
     __shared__ double red[kEnergyBlock];
-    const int b = threadIdx.x & (kGroup - 1);
+    const int b = threadIdx.x % BW;
     const int g = blockIdx.y;
-    const int pos = blockIdx.x * kEnergyItems + int(threadIdx.x / kGroup);
+    const int pos = blockIdx.x * kItems + int(threadIdx.x / BW);
     double v = 0.0;
     if (pos < p.n) {
         const uint16_t* xg = x + size_t(g) * size_t(p.n);
@@ -347,11 +498,11 @@ ray_energy_prior_kernel(DevProblem p, const uint16_t* x, int n_groups, double* p
     }
     red[threadIdx.x] = v;
     __syncthreads();
-    for (int stride = kEnergyBlock / 2; stride >= kGroup; stride >>= 1) {
+    for (int stride = kEnergyBlock / 2; stride >= BW; stride >>= 1) {
         if (int(threadIdx.x) < stride) red[threadIdx.x] += red[threadIdx.x + stride];
         __syncthreads();
     }
-    if (threadIdx.x < kGroup) part[size_t(blockIdx.x) * part_stride + size_t(g) * kGroup + threadIdx.x] = red[threadIdx.x];
+    if (int(threadIdx.x) < BW) part[size_t(blockIdx.x) * part_stride + size_t(g) * BW + threadIdx.x] = red[threadIdx.x];
 }
 
 // Sums the partials of each branch in block order (deterministic).
@@ -366,6 +517,7 @@ __global__ void ray_energy_final_kernel(const double* part_data, int blocks_data
     prior[br] = q;
 }
 
+// K states (original bit order, one byte per bit) into 16-bit group masks.
 __global__ void pack_states_kernel(const uint8_t* X, int K, int n, const int32_t* pos_bit, uint16_t* x) {
     const int pos = int(blockIdx.x * blockDim.x + threadIdx.x);
     const int g = blockIdx.y;
@@ -438,6 +590,9 @@ int_sweep_kernel(DevGraph g, uint64_t seed_base, const double* beta, int sweep_b
                     if (lane == 0) s[pj] = int8_t(-sj);
                 }
             }
+            // global state: this wave's field atomics and spin stores complete before
+            // the barrier (s_barrier alone does not wait for them)
+            if (!LDS) __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         }
     }
@@ -483,85 +638,103 @@ __global__ void __launch_bounds__(256) int_energy_kernel(DevGraph g, const uint8
 // ---------------------------------------------------------------------------------
 constexpr int kSweepBlock = 256;
 
-hipError_t launch_ray_init(const DevProblem& p, int precision, void* x, void* r, int n_groups, hipStream_t st) {
+// Dispatch on the group width: fn.template operator()<BW>().
+template <typename F>
+static hipError_t with_bw(int bw, F&& fn) {
+    switch (bw) {
+        case 2: return fn.template operator()<2>();
+        case 4: return fn.template operator()<4>();
+        case 8: return fn.template operator()<8>();
+        case 16: return fn.template operator()<16>();
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ray_init(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups, hipStream_t st) {
     const size_t nx = size_t(n_groups) * size_t(p.n);
-    const size_t nr = size_t(n_groups) * size_t(p.n_rays) * kGroup;
+    const size_t nr = size_t(n_groups) * size_t(p.n_rays) * size_t(bw);
     const size_t total = nx > nr ? nx : nr;
     const unsigned blocks = unsigned((total + 255) / 256);
-    if (precision == 1)
-        ray_init_kernel<float><<<blocks, 256, 0, st>>>(p, static_cast<uint16_t*>(x), static_cast<float*>(r), n_groups);
-    else
-        ray_init_kernel<double><<<blocks, 256, 0, st>>>(p, static_cast<uint16_t*>(x), static_cast<double*>(r), n_groups);
-    return hipGetLastError();
+    return with_bw(bw, [&]<int BW>() {
+        if (precision == 1)
+            ray_init_kernel<float, BW><<<blocks, 256, 0, st>>>(p, static_cast<uint16_t*>(x), static_cast<float*>(r), n_groups);
+        else
+            ray_init_kernel<double, BW><<<blocks, 256, 0, st>>>(p, static_cast<uint16_t*>(x), static_cast<double*>(r), n_groups);
+        return hipGetLastError();
+    });
 }
 
-template <typename T>
-static hipError_t sweeps_t(const DevProblem& p, void* x, void* r, int n_groups, int R, uint64_t seed_offset,
-                           const double* beta, int sb, int se, int recompute_every, unsigned* bar, int* err,
-                           int workgroups, hipStream_t st) {
-    DevState<T> s;
-    s.n_groups = n_groups;
-    s.R = R;
-    s.seed_offset = seed_offset;
-    s.x = static_cast<uint16_t*>(x);
-    s.r = static_cast<T*>(r);
-    s.beta = beta;
-    s.bar = bar;
-    s.err = err;
-    void* args[] = {const_cast<DevProblem*>(&p), &s, &sb, &se, &recompute_every};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&ray_sweep_kernel<T, kSweepBlock>),
-                                      dim3(unsigned(workgroups)), dim3(kSweepBlock), args, 0, st);
-}
-
-hipError_t launch_ray_sweeps(const DevProblem& p, int precision, void* x, void* r, int n_groups, int R,
+hipError_t launch_ray_sweeps(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups, int R,
                              uint64_t seed_offset, const double* beta, int sb, int se, int recompute_every,
-                             unsigned* bar, int* err, int workgroups, hipStream_t st) {
-    if (precision == 1)
-        return sweeps_t<float>(p, x, r, n_groups, R, seed_offset, beta, sb, se, recompute_every, bar, err,
-                               workgroups, st);
-    return sweeps_t<double>(p, x, r, n_groups, R, seed_offset, beta, sb, se, recompute_every, bar, err,
-                            workgroups, st);
+                             unsigned* bar, unsigned* tickets, int* err, int team_size, hipStream_t st) {
+    return with_bw(bw, [&]<int BW>() {
+        auto go = [&](auto zero) -> hipError_t {
+            using T = decltype(zero);
+            DevState<T> s;
+            s.n_groups = n_groups;
+            s.R = R;
+            s.seed_offset = seed_offset;
+            s.x = static_cast<uint16_t*>(x);
+            s.r = static_cast<T*>(r);
+            s.beta = beta;
+            s.bar = bar;
+            s.tickets = tickets;
+            s.err = err;
+            void* args[] = {const_cast<DevProblem*>(&p), &s, &sb, &se, &recompute_every, &team_size};
+            return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock>),
+                                              dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), args, 0, st);
+        };
+        return precision == 1 ? go(0.0f) : go(0.0);
+    });
 }
 
-int ray_sweep_max_workgroups() {
+int ray_sweep_blocks_per_cu() {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&ray_sweep_kernel<double, kSweepBlock>),
-                                                     kSweepBlock, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&ray_team_kernel<double, 16, kSweepBlock>), kSweepBlock, 0) !=
+        hipSuccess)
         return 0;
     return per_cu;
 }
 
-hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int R, float* p_void,
+hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, float* p_void,
                               uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st) {
-    ray_outputs_kernel<<<unsigned((p.n + 255) / 256), 256, 0, st>>>(p, x, n_groups, R, p_void, tri, certainty,
+    ray_outputs_kernel<<<unsigned((p.n + 255) / 256), 256, 0, st>>>(p, x, n_groups, bw, R, p_void, tri, certainty,
                                                                      branches);
     return hipGetLastError();
 }
 
-int ray_energy_blocks(const DevProblem& p) {
-    return (p.n_rays + kEnergyItems - 1) / kEnergyItems + (p.n + kEnergyItems - 1) / kEnergyItems;
+// Partial-sum blocks of the energy kernels at group width bw.
+int ray_energy_blocks(const DevProblem& p, int bw) {
+    const int items = kEnergyBlock / bw;
+    return (p.n_rays + items - 1) / items + (p.n + items - 1) / items;
 }
 
-hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_groups, int R, const void* r_carried,
+hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, const void* r_carried,
                              int precision, double* part, int n_part_blocks, double* data, double* prior,
                              float* resid_err, hipStream_t st) {
-    const int bd = (p.n_rays + kEnergyItems - 1) / kEnergyItems;
-    const int bp = (p.n + kEnergyItems - 1) / kEnergyItems;
+    const int items = kEnergyBlock / bw;
+    const int bd = (p.n_rays + items - 1) / items;
+    const int bp = (p.n + items - 1) / items;
     if (bd + bp > n_part_blocks) return hipErrorInvalidValue;
-    const int stride = n_groups * kGroup;
+    const int stride = n_groups * bw;
     double* part_prior = part + size_t(bd) * stride;
-    if (bd > 0) {
-        if (precision == 1)
-            ray_energy_data_kernel<float><<<dim3(unsigned(bd), unsigned(n_groups)), kEnergyBlock, 0, st>>>(
-                p, x, n_groups, static_cast<const float*>(r_carried), part, stride, resid_err);
-        else
-            ray_energy_data_kernel<double><<<dim3(unsigned(bd), unsigned(n_groups)), kEnergyBlock, 0, st>>>(
-                p, x, n_groups, static_cast<const double*>(r_carried), part, stride, resid_err);
-    }
-    ray_energy_prior_kernel<<<dim3(unsigned(bp), unsigned(n_groups)), kEnergyBlock, 0, st>>>(p, x, n_groups,
-                                                                                           part_prior, stride);
-    ray_energy_final_kernel<<<unsigned((R + 63) / 64), 64, 0, st>>>(part, bd, part_prior, bp, stride, R, data, prior);
-    return hipGetLastError();
+    return with_bw(bw, [&]<int BW>() {
+        if (bd > 0) {
+            const dim3 grid{unsigned(bd), unsigned(n_groups), 1u};
+            if (precision == 1)
+                ray_energy_data_kernel<float, BW><<<grid, kEnergyBlock, 0, st>>>(
+                    p, x, static_cast<const float*>(r_carried), part, stride, resid_err);
+            else
+                ray_energy_data_kernel<double, BW><<<grid, kEnergyBlock, 0, st>>>(
+                    p, x, static_cast<const double*>(r_carried), part, stride, resid_err);
+        }
+        if (bp > 0)
+            ray_energy_prior_kernel<BW><<<dim3(unsigned(bp), unsigned(n_groups)), kEnergyBlock, 0, st>>>(p, x, part_prior,
+                                                                                                      stride);
+        ray_energy_final_kernel<<<unsigned((R + 63) / 64), 64, 0, st>>>(part, bd, part_prior, bp, stride, R, data, prior);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_pack_states(const uint8_t* X, int K, int n, const int32_t* pos_bit, uint16_t* x, hipStream_t st) {

@@ -149,13 +149,22 @@ def test_ray_uneven_branch_groups_and_offset(gpu):
     check_branches(P, qa.schedule(20.0, 1.0, 20, 5), 20, seed_offset=5, precision=qa.FP32)
 
 
-def test_ray_launch_split_and_grid_size_do_not_change_results(gpu):
+def test_ray_launch_split_team_size_and_group_width_do_not_change_results(gpu):
     P = small_muon(grid=24, sky_az=60, sky_th=15)
     sched = qa.schedule(20.0, 1.0, 20, 5)
     _, a = check_branches(P, sched, 16, precision=qa.FP32, recompute_every=4)
-    _, b = check_branches(P, sched, 16, precision=qa.FP32, recompute_every=4, sweeps_per_launch=3, workgroups=37)
+    _, b = check_branches(P, sched, 16, precision=qa.FP32, recompute_every=4, sweeps_per_launch=3, workgroups=40)
     assert b["report"]["launches"] == 9
     assert np.array_equal(a["branches"], b["branches"])
+    for bw in (4, 8, 16):
+        _, c = check_branches(P, sched, 16, precision=qa.FP32, recompute_every=4, group_width=bw, workgroups=64)
+        assert np.array_equal(a["branches"], c["branches"])
+
+
+def test_ray_many_branches(gpu):
+    """40 branches: 20 groups of 2 over the 8 XCD teams (uneven load per team)."""
+    P = small_muon(grid=20, sky_az=45, sky_th=12)
+    check_branches(P, qa.schedule(20.0, 1.0, 12, 4), 40, group_width=2, precision=qa.FP64, recompute_every=5)
 
 
 def synthetic_problem(n=300, n_rays=400, seed=0, dup=False, field=True):
@@ -166,8 +175,8 @@ def synthetic_problem(n=300, n_rays=400, seed=0, dup=False, field=True):
     for i in range(n):
         k = int(rng.integers(1, 12)) if rng.random() < 0.8 else int(rng.integers(17, 90))
         rs = list(rng.choice(n_rays, size=k, replace=False))
-        if dup and k > 16 and rng.random() < 0.5:
-            rs[-1] = rs[0]
+        if dup and k > 1 and rng.random() < 0.3:
+            rs[-1] = rs[0]     # a repeated ray, in light and heavy columns
         for b in rs:
             ray.append(b)
             a.append(rng.uniform(0.005, 0.08))
