@@ -78,7 +78,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--recompute", type=int, default=50, help="FP64 residual recompute every k sweeps")
-    ap.add_argument("--workgroups", type=int, default=0)
+    ap.add_argument("--workgroups", type=int, default=0, help="persistent workgroups (multiple of 8); 0 = one per CU")
+    ap.add_argument("--group-width", type=int, default=0, help="branches per group (2, 4, 8, 16); 0 = auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -106,7 +107,8 @@ def main():
     k = args.recompute if prec == qa.FP32 else 0
 
     def step():
-        return prob.anneal_branches(sched, R, 0, prec, k, args.workgroups, branches=False)
+        return prob.anneal_branches(sched, R, 0, prec, k, args.workgroups, group_width=args.group_width,
+                                    branches=False)
 
     for w in range(args.warmup):
         out = step()

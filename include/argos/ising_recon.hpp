@@ -1,0 +1,319 @@
+// argos/ising_recon.hpp -- binary branches on the MI355X engine, drop-in for QuBLAR's
+// src/ising_recon.hpp (same names and signatures).
+//
+// anneal_branch and branch_fractions run on the GPU through libqublar_hip.so (FP64
+// residuals, the reference's arithmetic); their trajectory visits bits by colour class
+// with Philox draws keyed by (branch seed, sweep, bit) instead of index order with one
+// splitmix64 stream (src/ising_recon.hpp:121-144). Branch b's seed is still
+// b + 1 + seed_offset. The host helpers binary_energy, local_field_dE,
+// freeze_provable_rock, roi_qubo and classify keep the reference's definitions.
+// Errors of the engine throw argos::EngineError.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../qublar.h"
+#include "engine.hpp"
+
+namespace argos {
+
+struct EngineError : std::runtime_error {
+    int code;
+    EngineError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+namespace engine_detail {
+inline void check(int rc) {
+    if (rc != QB_OK) throw EngineError(rc, std::string("qublar: ") + qb_last_error());
+}
+}  // namespace engine_detail
+
+// Energy of a configuration from scratch (src/ising_recon.hpp:65-79).
+inline double binary_energy(const BinaryProblem& p, const std::vector<uint8_t>& x) {
+    std::vector<double> r(static_cast<size_t>(p.n_rays()));
+    for (int b = 0; b < p.n_rays(); ++b) r[static_cast<size_t>(b)] = -p.d[static_cast<size_t>(b)];
+    double prior = 0.0;
+    for (int i = 0; i < p.n_vars(); ++i) {
+        const size_t si = static_cast<size_t>(i);
+        if (x[si])
+            for (int k = p.row_ptr[si]; k < p.row_ptr[si + 1]; ++k)
+                r[static_cast<size_t>(p.ray[static_cast<size_t>(k)])] += p.a[static_cast<size_t>(k)];
+        prior += p.lin(i) * x[si];
+        for (int k = p.nbr_ptr[si]; k < p.nbr_ptr[si + 1]; ++k) {
+            const int j = p.nbr[static_cast<size_t>(k)];
+            if (j > i && x[static_cast<size_t>(j)] != x[si]) prior += p.lambda;
+        }
+    }
+    double data = 0.0;
+    for (int b = 0; b < p.n_rays(); ++b) {
+        const double rb = r[static_cast<size_t>(b)];
+        data += 0.5 * p.w[static_cast<size_t>(b)] * rb * rb;
+    }
+    return data + prior;
+}
+
+// src/ising_recon.hpp:81-86
+struct Schedule {
+    double t_hot = 50.0;
+    double t_cold = 1.0;
+    int anneal_sweeps = 200;
+    int hold_sweeps = 50;
+};
+
+namespace detail {
+// SplitMix64 (src/ising_recon.hpp:89-95): drivers use it to build their own inputs.
+inline uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double uniform(uint64_t& s) { return static_cast<double>(splitmix(s) >> 11) * 0x1.0p-53; }
+}  // namespace detail
+
+// dE of flipping bit i given residuals r that match x (src/ising_recon.hpp:105-118).
+inline double local_field_dE(const BinaryProblem& p, const std::vector<uint8_t>& x, const std::vector<double>& r, int i) {
+    const size_t si = static_cast<size_t>(i);
+    const double delta = x[si] ? -1.0 : 1.0;
+    double dE = p.lin(i) * delta;
+    for (int k = p.row_ptr[si]; k < p.row_ptr[si + 1]; ++k) {
+        const size_t b = static_cast<size_t>(p.ray[static_cast<size_t>(k)]);
+        const double a = static_cast<double>(p.a[static_cast<size_t>(k)]) * delta;
+        dE += p.w[b] * (r[b] * a + 0.5 * a * a);
+    }
+    for (int k = p.nbr_ptr[si]; k < p.nbr_ptr[si + 1]; ++k)
+        dE += p.lambda * (x[static_cast<size_t>(p.nbr[static_cast<size_t>(k)])] != x[si] ? -1.0 : 1.0);
+    return dE;
+}
+
+// The problem resident on the device; reuse it to anneal one problem many times.
+class DeviceProblem {
+public:
+    explicit DeviceProblem(const BinaryProblem& p) : n_(p.n_vars()) {
+        qb_problem_desc d{};
+        d.n_vars = p.n_vars();
+        d.n_rays = p.n_rays();
+        d.row_ptr = p.row_ptr.data();
+        d.ray = p.ray.data();
+        d.a = p.a.data();
+        d.d = p.d.data();
+        d.w = p.w.data();
+        d.nbr_ptr = p.nbr_ptr.data();
+        d.nbr = p.nbr.data();
+        d.lambda = p.lambda;
+        d.kappa = p.kappa;
+        d.field = p.field.empty() ? nullptr : p.field.data();
+        if (n_ > 0) engine_detail::check(qb_problem_create(&d, &h_));
+    }
+    ~DeviceProblem() { if (h_) qb_problem_destroy(h_); }
+    DeviceProblem(const DeviceProblem&) = delete;
+    DeviceProblem& operator=(const DeviceProblem&) = delete;
+
+    int n_vars() const { return n_; }
+    qb_problem* handle() const { return h_; }
+
+    // R branches (seeds b + 1 + seed_offset). Fills whichever outputs are non-null.
+    void anneal(const Schedule& s, int R, uint64_t seed_offset, std::vector<std::vector<uint8_t>>* branches,
+                std::vector<float>* p_void, std::vector<uint8_t>* tri = nullptr, std::vector<float>* certainty = nullptr,
+                std::vector<double>* energy = nullptr, qb_report* rep = nullptr,
+                const qb_anneal_opts* opts = nullptr) const {
+        if (R <= 0) throw EngineError(QB_E_ARG, "qublar: n_branches must be positive");
+        const size_t n = static_cast<size_t>(n_);
+        if (n_ == 0) {
+            if (branches) branches->assign(static_cast<size_t>(R), {});
+            if (p_void) p_void->clear();
+            if (tri) tri->clear();
+            if (certainty) certainty->clear();
+            if (energy) energy->assign(static_cast<size_t>(R), 0.0);
+            return;
+        }
+        const qb_schedule qs{s.t_hot, s.t_cold, s.anneal_sweeps, s.hold_sweeps};
+        const qb_anneal_opts o = opts ? *opts : qb_anneal_opts{QB_FP64, 0, 0, 0, 0};
+        std::vector<uint8_t> flat(branches ? static_cast<size_t>(R) * n : 0);
+        if (p_void) p_void->assign(n, 0.0f);
+        if (tri) tri->assign(n, 0);
+        if (certainty) certainty->assign(n, 0.0f);
+        if (energy) energy->assign(static_cast<size_t>(R), 0.0);
+        engine_detail::check(qb_anneal_branches(h_, &qs, R, seed_offset, &o, branches ? flat.data() : nullptr,
+                                                p_void ? p_void->data() : nullptr, tri ? tri->data() : nullptr,
+                                                certainty ? certainty->data() : nullptr,
+                                                energy ? energy->data() : nullptr, rep));
+        if (branches) {
+            branches->resize(static_cast<size_t>(R));
+            for (size_t b = 0; b < static_cast<size_t>(R); ++b)
+                (*branches)[b].assign(flat.begin() + static_cast<std::ptrdiff_t>(b * n),
+                                      flat.begin() + static_cast<std::ptrdiff_t>((b + 1) * n));
+        }
+    }
+
+private:
+    int n_ = 0;
+    qb_problem* h_ = nullptr;
+};
+
+// One branch from all-zero x (src/ising_recon.hpp:121-144), on the GPU.
+inline std::vector<uint8_t> anneal_branch(const BinaryProblem& p, const Schedule& s, uint64_t seed) {
+    std::vector<std::vector<uint8_t>> br;
+    DeviceProblem(p).anneal(s, 1, seed - 1, &br, nullptr);   // branch 0 has seed 0 + 1 + (seed - 1)
+    return br.empty() ? std::vector<uint8_t>() : std::move(br[0]);
+}
+
+// R branches and the per-bit void fraction (src/ising_recon.hpp:148-166), on the GPU;
+// n_threads is accepted for the reference's signature and has no effect.
+inline std::vector<float> branch_fractions(const BinaryProblem& p, const Schedule& s, int n_branches,
+                                           unsigned n_threads, std::vector<std::vector<uint8_t>>* keep = nullptr,
+                                           uint64_t seed_offset = 0) {
+    (void)n_threads;
+    std::vector<float> frac;
+    DeviceProblem(p).anneal(s, n_branches, seed_offset, keep, &frac);
+    if (p.n_vars() == 0) frac.clear();
+    return frac;
+}
+
+// src/ising_recon.hpp:181-234: freeze every bit whose flip 0 -> 1 costs at least
+// `margin` in every context (data bound sum w a (a/2 - d), prior bound kappa +
+// lambda (frozen - free neighbours)), to a fixpoint; the reduced problem keeps the
+// free bits with bonds to frozen rock folded into their field.
+struct FreezeResult {
+    BinaryProblem reduced;
+    std::vector<int> kept;
+    int rounds = 0;
+};
+
+inline FreezeResult freeze_provable_rock(const BinaryProblem& p, double margin) {
+    const int n = p.n_vars();
+    std::vector<double> data_bound(static_cast<size_t>(n), 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = p.row_ptr[static_cast<size_t>(i)]; k < p.row_ptr[static_cast<size_t>(i) + 1]; ++k) {
+            const size_t b = static_cast<size_t>(p.ray[static_cast<size_t>(k)]);
+            const double a = p.a[static_cast<size_t>(k)];
+            data_bound[static_cast<size_t>(i)] += p.w[b] * a * (0.5 * a - p.d[b]);
+        }
+    std::vector<uint8_t> frozen(static_cast<size_t>(n), 0);
+    FreezeResult out;
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        for (int i = 0; i < n; ++i) {
+            if (frozen[static_cast<size_t>(i)]) continue;
+            int rock = 0, open = 0;
+            for (int k = p.nbr_ptr[static_cast<size_t>(i)]; k < p.nbr_ptr[static_cast<size_t>(i) + 1]; ++k)
+                (frozen[static_cast<size_t>(p.nbr[static_cast<size_t>(k)])] ? rock : open)++;
+            if (data_bound[static_cast<size_t>(i)] + p.lin(i) + p.lambda * (rock - open) >= margin) {
+                frozen[static_cast<size_t>(i)] = 1;
+                changed = true;
+            }
+        }
+        ++out.rounds;
+    }
+    std::vector<int> index(static_cast<size_t>(n), -1);
+    for (int i = 0; i < n; ++i)
+        if (!frozen[static_cast<size_t>(i)]) {
+            index[static_cast<size_t>(i)] = static_cast<int>(out.kept.size());
+            out.kept.push_back(i);
+        }
+    BinaryProblem& r = out.reduced;
+    r.lambda = p.lambda;
+    r.kappa = p.kappa;
+    r.d = p.d;
+    r.w = p.w;
+    r.row_ptr.push_back(0);
+    r.nbr_ptr.push_back(0);
+    for (int i : out.kept) {
+        const size_t si = static_cast<size_t>(i);
+        r.var_voxel.push_back(p.var_voxel[si]);
+        for (int k = p.row_ptr[si]; k < p.row_ptr[si + 1]; ++k) {
+            r.ray.push_back(p.ray[static_cast<size_t>(k)]);
+            r.a.push_back(p.a[static_cast<size_t>(k)]);
+        }
+        r.row_ptr.push_back(static_cast<int>(r.ray.size()));
+        int rock = 0;
+        for (int k = p.nbr_ptr[si]; k < p.nbr_ptr[si + 1]; ++k) {
+            const int j = p.nbr[static_cast<size_t>(k)];
+            if (frozen[static_cast<size_t>(j)]) ++rock;
+            else r.nbr.push_back(index[static_cast<size_t>(j)]);
+        }
+        r.nbr_ptr.push_back(static_cast<int>(r.nbr.size()));
+        r.field.push_back(p.lin(i) + p.lambda * rock);
+    }
+    r.voxel_var.assign(p.voxel_var.size(), -1);
+    for (int j = 0; j < r.n_vars(); ++j) r.voxel_var[static_cast<size_t>(r.var_voxel[static_cast<size_t>(j)])] = j;
+    return out;
+}
+
+// src/ising_recon.hpp:245-300: the exact conditional QUBO of a region R with every
+// other bit held at x_bar: E(x_R) = const + sum h_i x_i + sum_{i<j} J_ij x_i x_j.
+struct RoiQubo {
+    std::vector<int> vars;
+    std::vector<double> h;
+    std::vector<double> J;   // n x n, upper triangle
+};
+
+inline RoiQubo roi_qubo(const BinaryProblem& p, const std::vector<int>& roi, const std::vector<uint8_t>& x_bar) {
+    const int n = static_cast<int>(roi.size());
+    RoiQubo q;
+    q.vars = roi;
+    q.h.assign(static_cast<size_t>(n), 0.0);
+    q.J.assign(static_cast<size_t>(n) * static_cast<size_t>(n), 0.0);
+    std::vector<int> pos(static_cast<size_t>(p.n_vars()), -1);
+    for (int i = 0; i < n; ++i) pos[static_cast<size_t>(roi[static_cast<size_t>(i)])] = i;
+    std::vector<double> r(static_cast<size_t>(p.n_rays()));
+    for (int b = 0; b < p.n_rays(); ++b) r[static_cast<size_t>(b)] = -p.d[static_cast<size_t>(b)];
+    for (int v = 0; v < p.n_vars(); ++v)
+        if (x_bar[static_cast<size_t>(v)] && pos[static_cast<size_t>(v)] < 0)
+            for (int k = p.row_ptr[static_cast<size_t>(v)]; k < p.row_ptr[static_cast<size_t>(v) + 1]; ++k)
+                r[static_cast<size_t>(p.ray[static_cast<size_t>(k)])] += p.a[static_cast<size_t>(k)];
+    std::vector<std::vector<std::pair<int, double>>> members(static_cast<size_t>(p.n_rays()));
+    for (int i = 0; i < n; ++i) {
+        const size_t vi = static_cast<size_t>(roi[static_cast<size_t>(i)]);
+        for (int k = p.row_ptr[vi]; k < p.row_ptr[vi + 1]; ++k) {
+            const size_t b = static_cast<size_t>(p.ray[static_cast<size_t>(k)]);
+            const double a = p.a[static_cast<size_t>(k)];
+            q.h[static_cast<size_t>(i)] += p.w[b] * (r[b] * a + 0.5 * a * a);
+            members[b].push_back({i, a});
+        }
+    }
+    for (size_t b = 0; b < members.size(); ++b)
+        for (size_t u = 0; u < members[b].size(); ++u)
+            for (size_t v = u + 1; v < members[b].size(); ++v) {
+                int i = members[b][u].first, j = members[b][v].first;
+                if (i > j) std::swap(i, j);
+                q.J[static_cast<size_t>(i) * static_cast<size_t>(n) + static_cast<size_t>(j)] +=
+                    p.w[b] * members[b][u].second * members[b][v].second;
+            }
+    for (int i = 0; i < n; ++i) {
+        const int vi = roi[static_cast<size_t>(i)];
+        q.h[static_cast<size_t>(i)] += p.lin(vi);
+        for (int k = p.nbr_ptr[static_cast<size_t>(vi)]; k < p.nbr_ptr[static_cast<size_t>(vi) + 1]; ++k) {
+            const int vj = p.nbr[static_cast<size_t>(k)];
+            const int pj = pos[static_cast<size_t>(vj)];
+            if (pj >= 0) {
+                if (pj > i) {
+                    q.h[static_cast<size_t>(i)] += p.lambda;
+                    q.h[static_cast<size_t>(pj)] += p.lambda;
+                    q.J[static_cast<size_t>(i) * static_cast<size_t>(n) + static_cast<size_t>(pj)] -= 2.0 * p.lambda;
+                }
+            } else {
+                q.h[static_cast<size_t>(i)] += p.lambda * (x_bar[static_cast<size_t>(vj)] ? -1.0 : 1.0);
+            }
+        }
+    }
+    return q;
+}
+
+// src/ising_recon.hpp:302-309
+enum class Bit : uint8_t { Exists = 0, NotThere = 1, Undecided = 2 };
+
+inline Bit classify(float p_void, float lo = 0.1f, float hi = 0.9f) {
+    if (p_void >= hi) return Bit::NotThere;
+    if (p_void <= lo) return Bit::Exists;
+    return Bit::Undecided;
+}
+
+}  // namespace argos

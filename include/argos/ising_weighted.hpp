@@ -1,0 +1,421 @@
+// argos/ising_weighted.hpp -- the integer path (weighted MaxCut / Ising with exact
+// int32 weights and int64 energies), drop-in for QuBLAR's src/ising_weighted.hpp.
+//
+//   x in {0,1}^n, s = 2x - 1, cut(x) = sum_edges w_uv [x_u != x_v], E = -cut
+//   f_i = sum_j w_ij s_j, dE_i = -s_i f_i
+//
+// anneal() runs on the GPU through libqublar_hip.so (one replica; run_batch in
+// argos/ising_gpu.cuh runs many); its trajectory visits vertices by colour class with
+// Philox draws keyed by (seed, sweep, vertex) instead of index order with one
+// splitmix64 stream (src/ising_weighted.hpp:250-267). The loader, exact energies,
+// State, TTS99 and SHA-256 are host code with the reference's behaviour.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../qublar.h"
+
+namespace argos {
+namespace wising {
+
+// src/ising_weighted.hpp:35-42
+struct WeightedGraph {
+    int n = 0;
+    int64_t m = 0;
+    std::vector<int32_t> eu, ev, ew;   // edges as read, 0-indexed
+    std::vector<int64_t> row;          // CSR offsets (n + 1), both directions
+    std::vector<int32_t> col, w;
+    int64_t sum_w = 0, n_pos = 0, n_neg = 0, n_other = 0;
+};
+
+namespace loader {
+inline bool blank(char c) { return c == ' ' || c == '\t' || c == '\r'; }
+
+// A signed decimal integer with magnitude at most 2^40; anything else is refused.
+inline bool to_int(const char* s, size_t len, int64_t& out) {
+    size_t i = 0;
+    bool neg = false;
+    if (len > 0 && (s[0] == '+' || s[0] == '-')) {
+        neg = s[0] == '-';
+        i = 1;
+    }
+    if (i == len) return false;
+    int64_t v = 0;
+    for (; i < len; ++i) {
+        if (s[i] < '0' || s[i] > '9') return false;
+        v = v * 10 + (s[i] - '0');
+        if (v > (int64_t(1) << 40)) return false;
+    }
+    out = neg ? -v : v;
+    return true;
+}
+
+// Tokens of one line at a time; a token ends at a blank or a newline.
+struct Reader {
+    const char* at;
+    const char* end;
+    void skip_blank() { while (at < end && blank(*at)) ++at; }
+    void skip_space() { while (at < end && (blank(*at) || *at == '\n')) ++at; }
+    bool next(const char*& s, size_t& len) {
+        skip_blank();
+        if (at >= end || *at == '\n') return false;
+        s = at;
+        while (at < end && !blank(*at) && *at != '\n') ++at;
+        len = static_cast<size_t>(at - s);
+        return true;
+    }
+    bool line_done() { skip_blank(); return at >= end || *at == '\n'; }
+};
+}  // namespace loader
+
+// rudy / Gset text: "n m", then m lines "u v [w]" (1-indexed, missing w = +1).
+// Refuses a non-integer weight, a self-loop, a duplicate edge (either orientation),
+// an index out of range, or an edge count that differs from the header
+// (src/ising_weighted.hpp:83-149).
+inline bool parse_rudy(const char* data, size_t size, WeightedGraph& g, std::string& err) {
+    g = WeightedGraph();
+    loader::Reader rd{data, data + size};
+    const char* t = nullptr;
+    size_t tl = 0;
+    int64_t n = 0, m = 0;
+    rd.skip_space();
+    if (!rd.next(t, tl) || !loader::to_int(t, tl, n) || n <= 0 || n > (int64_t(1) << 30)) {
+        err = "bad header: n";
+        return false;
+    }
+    if (!rd.next(t, tl) || !loader::to_int(t, tl, m) || m < 0) {
+        err = "bad header: m";
+        return false;
+    }
+    g.n = static_cast<int>(n);
+    g.m = m;
+    int64_t count = 0;
+    for (;;) {
+        rd.skip_space();
+        if (rd.at >= rd.end) break;
+        int64_t u = 0, v = 0, w = 1;
+        if (!rd.next(t, tl) || !loader::to_int(t, tl, u) || !rd.next(t, tl) || !loader::to_int(t, tl, v)) {
+            err = "bad vertex token '" + std::string(t ? t : "", t ? tl : 0) + "'";
+            return false;
+        }
+        if (rd.next(t, tl)) {
+            if (!loader::to_int(t, tl, w)) {
+                err = "weight '" + std::string(t, tl) + "' is not an integer; the float path is BinaryProblem";
+                return false;
+            }
+            if (w < std::numeric_limits<int32_t>::min() || w > std::numeric_limits<int32_t>::max()) {
+                err = "weight out of int32 range";
+                return false;
+            }
+            if (!rd.line_done()) {
+                err = "trailing token on an edge line";
+                return false;
+            }
+        }
+        if (u < 1 || v < 1 || u > n || v > n) {
+            err = "vertex index out of range";
+            return false;
+        }
+        if (u == v) {
+            err = "self-loop";
+            return false;
+        }
+        g.eu.push_back(static_cast<int32_t>(u - 1));
+        g.ev.push_back(static_cast<int32_t>(v - 1));
+        g.ew.push_back(static_cast<int32_t>(w));
+        ++count;
+    }
+    if (count != m) {
+        err = "header says " + std::to_string(m) + " edges, file has " + std::to_string(count);
+        return false;
+    }
+    std::vector<uint64_t> keys(static_cast<size_t>(m));
+    for (size_t e = 0; e < keys.size(); ++e) {
+        const uint64_t lo = static_cast<uint64_t>(std::min(g.eu[e], g.ev[e]));
+        const uint64_t hi = static_cast<uint64_t>(std::max(g.eu[e], g.ev[e]));
+        keys[e] = (lo << 32) | hi;
+    }
+    std::sort(keys.begin(), keys.end());
+    if (std::adjacent_find(keys.begin(), keys.end()) != keys.end()) {
+        err = "duplicate edge";
+        return false;
+    }
+    g.row.assign(static_cast<size_t>(n) + 1, 0);
+    for (size_t e = 0; e < keys.size(); ++e) {
+        ++g.row[static_cast<size_t>(g.eu[e]) + 1];
+        ++g.row[static_cast<size_t>(g.ev[e]) + 1];
+    }
+    for (size_t i = 0; i < static_cast<size_t>(n); ++i) g.row[i + 1] += g.row[i];
+    g.col.resize(static_cast<size_t>(2 * m));
+    g.w.resize(static_cast<size_t>(2 * m));
+    std::vector<int64_t> next(g.row.begin(), g.row.end() - 1);
+    for (size_t e = 0; e < keys.size(); ++e) {
+        const int32_t a = g.eu[e], b = g.ev[e], wt = g.ew[e];
+        g.col[static_cast<size_t>(next[static_cast<size_t>(a)])] = b;
+        g.w[static_cast<size_t>(next[static_cast<size_t>(a)]++)] = wt;
+        g.col[static_cast<size_t>(next[static_cast<size_t>(b)])] = a;
+        g.w[static_cast<size_t>(next[static_cast<size_t>(b)]++)] = wt;
+        g.sum_w += wt;
+        if (wt == 1) ++g.n_pos;
+        else if (wt == -1) ++g.n_neg;
+        else ++g.n_other;
+    }
+    return true;
+}
+
+// The whole file into buf (src/ising_weighted.hpp:151-168).
+inline bool read_file(const std::string& path, std::vector<char>& buf) {
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    buf.clear();
+    char chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+    const bool ok = !std::ferror(f);
+    std::fclose(f);
+    return ok;
+}
+
+// cut over the edge list (src/ising_weighted.hpp:173-178)
+inline int64_t cut_of(const WeightedGraph& g, const std::vector<uint8_t>& x) {
+    int64_t cut = 0;
+    for (size_t e = 0; e < g.eu.size(); ++e)
+        if (x[static_cast<size_t>(g.eu[e])] != x[static_cast<size_t>(g.ev[e])]) cut += g.ew[e];
+    return cut;
+}
+
+// -cut over the CSR, each edge once (j > i) (src/ising_weighted.hpp:181-189)
+inline int64_t energy_scratch(const WeightedGraph& g, const std::vector<uint8_t>& x) {
+    int64_t cut = 0;
+    for (int i = 0; i < g.n; ++i)
+        for (int64_t k = g.row[static_cast<size_t>(i)]; k < g.row[static_cast<size_t>(i) + 1]; ++k) {
+            const int j = g.col[static_cast<size_t>(k)];
+            if (j > i && x[static_cast<size_t>(i)] != x[static_cast<size_t>(j)]) cut += g.w[static_cast<size_t>(k)];
+        }
+    return -cut;
+}
+
+// Spins with a maintained local field (src/ising_weighted.hpp:193-223).
+struct State {
+    const WeightedGraph* g = nullptr;
+    std::vector<int8_t> s;     // +-1
+    std::vector<int64_t> f;    // f_i = sum_j w_ij s_j
+    int64_t energy = 0;        // -cut, tracked
+
+    explicit State(const WeightedGraph& graph)
+        : g(&graph), s(static_cast<size_t>(graph.n), int8_t(-1)), f(static_cast<size_t>(graph.n), 0) {
+        for (int i = 0; i < graph.n; ++i)
+            for (int64_t k = graph.row[static_cast<size_t>(i)]; k < graph.row[static_cast<size_t>(i) + 1]; ++k)
+                f[static_cast<size_t>(i)] -= graph.w[static_cast<size_t>(k)];
+    }
+    int64_t dE(int i) const { return -int64_t(s[static_cast<size_t>(i)]) * f[static_cast<size_t>(i)]; }
+    void flip(int i) {
+        const int64_t change = dE(i);
+        const int64_t si = s[static_cast<size_t>(i)];
+        for (int64_t k = g->row[static_cast<size_t>(i)]; k < g->row[static_cast<size_t>(i) + 1]; ++k)
+            f[static_cast<size_t>(g->col[static_cast<size_t>(k)])] -= 2 * int64_t(g->w[static_cast<size_t>(k)]) * si;
+        s[static_cast<size_t>(i)] = int8_t(-si);
+        energy += change;
+    }
+    std::vector<uint8_t> x() const {
+        std::vector<uint8_t> out(s.size());
+        for (size_t i = 0; i < s.size(); ++i) out[i] = s[i] > 0 ? 1u : 0u;
+        return out;
+    }
+};
+
+// src/ising_weighted.hpp:227-232
+struct Budget {
+    double t_hot = 5.0;
+    double t_cold = 1e-3;
+    int anneal_sweeps = 400;
+    int hold_sweeps = 20;
+};
+
+namespace detail {
+inline uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double uniform(uint64_t& s) { return static_cast<double>(splitmix(s) >> 11) * 0x1.0p-53; }
+}  // namespace detail
+
+struct RunOut {
+    std::vector<uint8_t> x;
+    int64_t energy_tracked = 0;
+};
+
+struct EngineError : std::runtime_error {
+    int code;
+    EngineError(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+inline qb_graph* upload_graph(const WeightedGraph& g, bool force_global) {
+    qb_graph_desc d{g.n, g.m, g.eu.data(), g.ev.data(), g.ew.data()};
+    qb_graph* h = nullptr;
+    const int rc = qb_graph_create(&d, force_global ? 1 : 0, &h);
+    if (rc != QB_OK) throw EngineError(rc, std::string("qublar: ") + qb_last_error());
+    return h;
+}
+
+// One replica with seed `seed` on the GPU (src/ising_weighted.hpp:250-267).
+inline RunOut anneal(const WeightedGraph& g, const Budget& b, uint64_t seed) {
+    qb_graph* h = upload_graph(g, false);
+    qb_plan plan{};
+    qb_graph_get_plan(h, &plan);
+    std::vector<int8_t> s(static_cast<size_t>(plan.n_pad));
+    int64_t e = 0;
+    const qb_schedule qs{b.t_hot, b.t_cold, b.anneal_sweeps, b.hold_sweeps};
+    const int rc = qb_run_batch(h, &qs, seed, 1, 1e18, s.data(), &e, nullptr);
+    const std::string msg = rc == QB_OK ? "" : qb_last_error();
+    qb_graph_destroy(h);
+    if (rc != QB_OK) throw EngineError(rc, "qublar: " + msg);
+    RunOut out;
+    out.x.resize(static_cast<size_t>(g.n));
+    for (int i = 0; i < g.n; ++i) out.x[static_cast<size_t>(i)] = s[static_cast<size_t>(i)] > 0 ? 1u : 0u;
+    out.energy_tracked = e;
+    return out;
+}
+
+// Time to reach the target with 99% confidence (src/ising_weighted.hpp:271-275).
+inline double tts99(double t_mean, double p) {
+    if (p <= 0.0) return std::numeric_limits<double>::infinity();
+    if (p >= 0.99) return t_mean;
+    return t_mean * std::log(0.01) / std::log(1.0 - p);
+}
+
+struct TtsCi { double lo, hi; };
+
+// Bootstrap 95% interval of TTS99: `resamples` resamplings of the runs, indices drawn
+// by splitmix64 from `seed`, percentiles by linear interpolation, infinite when a
+// bracketing resample never succeeds (src/ising_weighted.hpp:279-303).
+inline TtsCi tts99_bootstrap(const std::vector<int>& success, const std::vector<double>& time, int resamples = 2000,
+                             uint64_t seed = 20260928ull) {
+    const size_t r = success.size();
+    std::vector<double> est;
+    est.reserve(static_cast<size_t>(resamples));
+    uint64_t st = seed;
+    for (int k = 0; k < resamples; ++k) {
+        int hits = 0;
+        double total = 0.0;
+        for (size_t q = 0; q < r; ++q) {
+            const size_t j = static_cast<size_t>(detail::splitmix(st) % static_cast<uint64_t>(r));
+            hits += success[j];
+            total += time[j];
+        }
+        est.push_back(tts99(total / static_cast<double>(r), static_cast<double>(hits) / static_cast<double>(r)));
+    }
+    std::sort(est.begin(), est.end());
+    auto quantile = [&](double q) {
+        const double at = q * static_cast<double>(est.size() - 1);
+        const size_t lo = static_cast<size_t>(std::floor(at)), hi = static_cast<size_t>(std::ceil(at));
+        if (std::isinf(est[lo]) || std::isinf(est[hi])) return std::numeric_limits<double>::infinity();
+        return est[lo] + (est[hi] - est[lo]) * (at - static_cast<double>(lo));
+    };
+    return {quantile(0.025), quantile(0.975)};
+}
+
+// SHA-256 (FIPS 180-4) for assignment and file hashes (src/ising_weighted.hpp:307-379).
+class Sha256 {
+public:
+    Sha256() { reset(); }
+    void reset() {
+        static const uint32_t iv[8] = {0x6a09e667u, 0xbb67ae85u, 0x3c6ef372u, 0xa54ff53au,
+                                       0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
+        std::memcpy(state_, iv, sizeof state_);
+        total_ = 0;
+        used_ = 0;
+    }
+    void update(const void* data, size_t n) {
+        const uint8_t* p = static_cast<const uint8_t*>(data);
+        total_ += n;
+        while (n > 0) {
+            const size_t take = std::min(n, sizeof block_ - used_);
+            std::memcpy(block_ + used_, p, take);
+            used_ += take;
+            p += take;
+            n -= take;
+            if (used_ == sizeof block_) {
+                compress(block_);
+                used_ = 0;
+            }
+        }
+    }
+    std::string hex() {
+        const uint64_t bits = total_ * 8u;
+        const uint8_t one = 0x80, zero = 0;
+        update(&one, 1);
+        while (used_ != 56) update(&zero, 1);
+        uint8_t len[8];
+        for (int i = 0; i < 8; ++i) len[i] = static_cast<uint8_t>(bits >> (56 - 8 * i));
+        update(len, 8);
+        static const char digits[] = "0123456789abcdef";
+        std::string out;
+        for (uint32_t word : state_)
+            for (int sh = 28; sh >= 0; sh -= 4) out.push_back(digits[(word >> sh) & 15u]);
+        return out;
+    }
+    static std::string of(const void* data, size_t n) {
+        Sha256 h;
+        h.update(data, n);
+        return h.hex();
+    }
+
+private:
+    static uint32_t rotr(uint32_t v, int k) { return (v >> k) | (v << (32 - k)); }
+    void compress(const uint8_t* blk) {
+        static const uint32_t K[64] = {
+            0x428a2f98u, 0x71374491u, 0xb5c0fbcfu, 0xe9b5dba5u, 0x3956c25bu, 0x59f111f1u, 0x923f82a4u, 0xab1c5ed5u,
+            0xd807aa98u, 0x12835b01u, 0x243185beu, 0x550c7dc3u, 0x72be5d74u, 0x80deb1feu, 0x9bdc06a7u, 0xc19bf174u,
+            0xe49b69c1u, 0xefbe4786u, 0x0fc19dc6u, 0x240ca1ccu, 0x2de92c6fu, 0x4a7484aau, 0x5cb0a9dcu, 0x76f988dau,
+            0x983e5152u, 0xa831c66du, 0xb00327c8u, 0xbf597fc7u, 0xc6e00bf3u, 0xd5a79147u, 0x06ca6351u, 0x14292967u,
+            0x27b70a85u, 0x2e1b2138u, 0x4d2c6dfcu, 0x53380d13u, 0x650a7354u, 0x766a0abbu, 0x81c2c92eu, 0x92722c85u,
+            0xa2bfe8a1u, 0xa81a664bu, 0xc24b8b70u, 0xc76c51a3u, 0xd192e819u, 0xd6990624u, 0xf40e3585u, 0x106aa070u,
+            0x19a4c116u, 0x1e376c08u, 0x2748774cu, 0x34b0bcb5u, 0x391c0cb3u, 0x4ed8aa4au, 0x5b9cca4fu, 0x682e6ff3u,
+            0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u, 0xc67178f2u};
+        uint32_t w[64];
+        for (int i = 0; i < 16; ++i)
+            w[i] = uint32_t(blk[4 * i]) << 24 | uint32_t(blk[4 * i + 1]) << 16 | uint32_t(blk[4 * i + 2]) << 8 |
+                   uint32_t(blk[4 * i + 3]);
+        for (int i = 16; i < 64; ++i)
+            w[i] = w[i - 16] + (rotr(w[i - 15], 7) ^ rotr(w[i - 15], 18) ^ (w[i - 15] >> 3)) + w[i - 7] +
+                   (rotr(w[i - 2], 17) ^ rotr(w[i - 2], 19) ^ (w[i - 2] >> 10));
+        uint32_t v[8];
+        std::memcpy(v, state_, sizeof v);
+        for (int i = 0; i < 64; ++i) {
+            const uint32_t t1 = v[7] + (rotr(v[4], 6) ^ rotr(v[4], 11) ^ rotr(v[4], 25)) +
+                                ((v[4] & v[5]) ^ (~v[4] & v[6])) + K[i] + w[i];
+            const uint32_t t2 = (rotr(v[0], 2) ^ rotr(v[0], 13) ^ rotr(v[0], 22)) +
+                                ((v[0] & v[1]) ^ (v[0] & v[2]) ^ (v[1] & v[2]));
+            std::memmove(v + 1, v, 7 * sizeof(uint32_t));
+            v[4] += t1;
+            v[0] = t1 + t2;
+        }
+        for (int i = 0; i < 8; ++i) state_[i] += v[i];
+    }
+    uint32_t state_[8];
+    uint8_t block_[64];
+    uint64_t total_ = 0;
+    size_t used_ = 0;
+};
+
+// SHA-256 of the '0'/'1' string of x (src/ising_weighted.hpp:381-385).
+inline std::string assignment_sha256(const std::vector<uint8_t>& x) {
+    std::string bits(x.size(), '0');
+    for (size_t i = 0; i < x.size(); ++i)
+        if (x[i]) bits[i] = '1';
+    return Sha256::of(bits.data(), bits.size());
+}
+
+}  // namespace wising
+}  // namespace argos

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -164,7 +165,13 @@ struct qb_problem {
     int max_class = 0, min_class = 0, max_entries = 0, max_bits_per_ray = 0;
     int64_t device_bytes = 0;
     std::vector<int32_t> pos_bit, cls_ptr;
-    DevBuf b_cls, b_heavy, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq;
+    DevBuf b_cls, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq;
+    std::vector<int32_t> col_ptr;   // host copy, position space
+    struct Batches {
+        DevBuf cls, first, count, pos, slot0;
+        DevBatches dev{};
+    };
+    std::map<int, std::unique_ptr<Batches>> batches;   // by group width
     // workspace
     DevBuf w_x, w_r, w_beta, w_bar, w_err, w_part, w_pv, w_tri, w_cert, w_br, w_data, w_prior, w_rerr, w_X;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -320,10 +327,6 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
         nb_ptr[size_t(p) + 1] = q0;
         lin[size_t(p)] = desc->field ? desc->field[i] : desc->kappa;
     }
-    std::vector<int32_t> cls_heavy(static_cast<size_t>(K), 0);
-    for (int c = 0; c < K; ++c)
-        for (int q = P->cls_ptr[size_t(c)]; q < P->cls_ptr[size_t(c) + 1]; ++q)
-            if (col_ptr[size_t(q) + 1] - col_ptr[size_t(q)] > kSeg) cls_heavy[size_t(c)]++;
     std::vector<uint8_t> seq_update(static_cast<size_t>(n), 0);
     {
         std::vector<int32_t> seen(static_cast<size_t>(nr), -1);
@@ -342,7 +345,7 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     QB_HIP(hipEventCreate(&P->ev0));
     QB_HIP(hipEventCreate(&P->ev1));
     hipStream_t st = P->stream;
-    if ((rc = P->b_cls.upload(P->cls_ptr, st)) || (rc = P->b_heavy.upload(cls_heavy, st)) ||
+    if ((rc = P->b_cls.upload(P->cls_ptr, st)) ||
         (rc = P->b_seq.upload(seq_update, st)) || (rc = P->b_pos.upload(P->pos_bit, st)) ||
         (rc = P->b_col.upload(col_ptr, st)) || (rc = P->b_ent.upload(ent, st)) || (rc = P->b_nbp.upload(nb_ptr, st)) ||
         (rc = P->b_nb.upload(nb, st)) || (rc = P->b_nbbit.upload(nb_bit, st)) || (rc = P->b_lin.upload(lin, st)) ||
@@ -350,7 +353,8 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
         (rc = P->b_rt.upload(rt, st)))
         return rc;
     QB_HIP(hipStreamSynchronize(st));
-    P->device_bytes = int64_t(P->b_cls.bytes + P->b_heavy.bytes + P->b_seq.bytes + P->b_pos.bytes + P->b_col.bytes + P->b_ent.bytes + P->b_nbp.bytes +
+    P->col_ptr = col_ptr;
+    P->device_bytes = int64_t(P->b_cls.bytes + P->b_seq.bytes + P->b_pos.bytes + P->b_col.bytes + P->b_ent.bytes + P->b_nbp.bytes +
                               P->b_nb.bytes + P->b_nbbit.bytes + P->b_lin.bytes + P->b_d.bytes + P->b_w.bytes +
                               P->b_rtp.bytes + P->b_rt.bytes);
     DevProblem& D = P->dev;
@@ -360,7 +364,6 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     D.nnz = nnz;
     D.n_nbr = nn;
     D.cls_ptr = P->b_cls.as<int32_t>();
-    D.cls_heavy = P->b_heavy.as<int32_t>();
     D.seq_update = P->b_seq.as<uint8_t>();
     D.pos_bit = P->b_pos.as<int32_t>();
     D.col_ptr = P->b_col.as<int32_t>();
@@ -428,6 +431,65 @@ static int group_width_for(int R, int requested) {
     return bw;
 }
 
+// The work batches of every class at group width bw (cached per width): bits sorted
+// by chain count, the largest first; a bit needing more chains than a workgroup has
+// slots is a batch of its own, the rest are packed into workgroups next-fit.
+static int batches_for(qb_problem* P, int bw, const DevBatches** out) {
+    auto found = P->batches.find(bw);
+    if (found != P->batches.end()) {
+        *out = &found->second->dev;
+        return QB_OK;
+    }
+    const int slots = kSweepBlock / bw;
+    const int K = P->n_classes;
+    std::vector<int32_t> cls_batch(static_cast<size_t>(K) + 1, 0), first, count, pos, slot0;
+    std::vector<int32_t> items;
+    for (int c = 0; c < K; ++c) {
+        cls_batch[size_t(c)] = int32_t(first.size());
+        items.clear();
+        for (int q = P->cls_ptr[size_t(c)]; q < P->cls_ptr[size_t(c) + 1]; ++q) items.push_back(q);
+        auto chains = [&](int q) {
+            return std::max(1, (P->col_ptr[size_t(q) + 1] - P->col_ptr[size_t(q)] + kSeg - 1) / kSeg);
+        };
+        std::stable_sort(items.begin(), items.end(), [&](int a, int b) { return chains(a) > chains(b); });
+        int open = -1, used = 0;
+        for (int q : items) {
+            const int k = chains(q);
+            if (k > slots) {
+                first.push_back(int32_t(pos.size()));
+                count.push_back(-1);
+                pos.push_back(q);
+                slot0.push_back(0);
+                open = -1;
+                continue;
+            }
+            if (open < 0 || used + k > slots) {
+                first.push_back(int32_t(pos.size()));
+                count.push_back(0);
+                open = int(count.size()) - 1;
+                used = 0;
+            }
+            pos.push_back(q);
+            slot0.push_back(used);
+            used += k;
+            count[size_t(open)]++;
+        }
+    }
+    cls_batch[size_t(K)] = int32_t(first.size());
+    auto B = std::make_unique<qb_problem::Batches>();
+    int rc;
+    if ((rc = B->cls.upload(cls_batch, P->stream)) || (rc = B->first.upload(first, P->stream)) ||
+        (rc = B->count.upload(count, P->stream)) || (rc = B->pos.upload(pos, P->stream)) ||
+        (rc = B->slot0.upload(slot0, P->stream)))
+        return rc;
+    QB_HIP(hipStreamSynchronize(P->stream));
+    B->dev = DevBatches{B->cls.as<int32_t>(), B->first.as<int32_t>(), B->count.as<int32_t>(), B->pos.as<int32_t>(),
+                        B->slot0.as<int32_t>()};
+    *out = &B->dev;
+    P->batches[bw] = std::move(B);
+    return QB_OK;
+}
+
 extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, uint64_t seed_offset,
                                   const qb_anneal_opts* opts, uint8_t* branches, float* p_void, uint8_t* tri,
                                   float* certainty, double* energy, qb_report* rep) {
@@ -462,6 +524,8 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
         (rc = P->w_data.reserve(sizeof(double) * size_t(R))) || (rc = P->w_prior.reserve(sizeof(double) * size_t(R))) ||
         (rc = P->w_rerr.reserve(sizeof(float))))
         return rc;
+    const DevBatches* bt = nullptr;
+    if ((rc = batches_for(P, bw, &bt))) return rc;
     hipStream_t st = P->stream;
     const std::vector<double> beta = betas_of(*s);
     QB_HIP(hipStreamSynchronize(st));
@@ -474,7 +538,7 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     QB_HIP(hipEventRecord(P->ev0, st));
     for (int sb = 0; sb < total; sb += chunk) {
         QB_HIP(hipMemsetAsync(P->w_bar.p, 0, sizeof(unsigned) * (kBarWords + kTicketWords), st));
-        QB_HIP(launch_ray_sweeps(P->dev, o.precision, bw, P->w_x.p, P->w_r.p, n_groups, R, seed_offset,
+        QB_HIP(launch_ray_sweeps(P->dev, *bt, o.precision, bw, P->w_x.p, P->w_r.p, n_groups, R, seed_offset,
                                  P->w_beta.as<double>(), sb, std::min(total, sb + chunk), o.recompute_every,
                                  P->w_bar.as<unsigned>(), P->w_bar.as<unsigned>() + kBarWords, P->w_err.as<int>(),
                                  team_size, st));

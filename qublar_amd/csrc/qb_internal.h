@@ -40,7 +40,6 @@ struct DevProblem {
     int n, n_rays, n_classes;
     int64_t nnz, n_nbr;
     const int32_t* cls_ptr;
-    const int32_t* cls_heavy;   // heavy bits at the start of each class
     const int32_t* pos_bit;
     const int32_t* col_ptr;
     const RayEnt* ent;
@@ -54,6 +53,19 @@ struct DevProblem {
     const BitEnt* rt;
     const uint8_t* seq_update;  // 1 where a ray repeats in the column: scatter in CSR order
     double lambda;
+};
+
+// Work batches of one sweep step at one group width: a workgroup of BLOCK threads
+// has BLOCK / BW slots of BW lanes, and a bit with k entries needs
+// max(1, ceil(k / kSeg)) slots (one per chain). A batch packs whole bits into one
+// workgroup's slots; a bit needing more slots than a workgroup has is a batch of its
+// own (count -1) and is evaluated in several passes.
+struct DevBatches {
+    const int32_t* cls_batch;   // n_classes + 1: class c's batches
+    const int32_t* first;       // per batch: first item
+    const int32_t* count;       // per batch: items, or -1 for one bit in several passes
+    const int32_t* pos;         // per item: the bit's position
+    const int32_t* slot0;       // per item: its first slot in the batch
 };
 
 // Mutable state of one anneal: n_groups groups of BW branches (BW in {2, 4, 8, 16}).
@@ -92,11 +104,13 @@ struct DevGraph {
 hipError_t launch_ray_init(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups,
                            hipStream_t st);
 // The persistent sweep on team_size * 8 workgroups (team_size per XCD).
-hipError_t launch_ray_sweeps(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups, int R,
+hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int precision, int bw, void* x, void* r,
+                             int n_groups, int R,
                              uint64_t seed_offset, const double* beta, int sweep_begin, int sweep_end,
                              int recompute_every, unsigned* bar, unsigned* tickets, int* err, int team_size,
                              hipStream_t st);
 int ray_sweep_blocks_per_cu();
+constexpr int kSweepBlock = 256;   // threads per sweep workgroup
 hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, float* p_void,
                               uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st);
 // FP64 energy of every branch: data and prior partial sums per block, reduced in a

@@ -207,48 +207,93 @@ __device__ __forceinline__ uint16_t segment_flips(bool acc) {
     return uint16_t((bal >> base) & ((uint64_t(1) << BW) - 1));
 }
 
-// A bit with at most kSeg entries, for the BW branches of group g (lane b).
-template <typename T, int BW>
-__device__ __forceinline__ void light_update(const DevProblem& p, const DevState<T>& s, int pos, int g, int b,
-                                             int sweep, double beta) {
+// One batch of whole bits for the BW branches of group g: every slot evaluates one
+// chain of one bit; the chain sums meet in LDS, the bit's first slot adds them in
+// order, adds the neighbour terms and decides; each slot then scatters its own
+// chain from registers.
+template <typename T, int BW, int BLOCK>
+__device__ __forceinline__ void packed_update(const DevProblem& p, const DevBatches& bt, const DevState<T>& s,
+                                              int batch, int g, int sweep, double beta) {
+    constexpr int kSlots = BLOCK / BW;
+    __shared__ int item_pos[kSlots], item_k0[kSlots], item_k1[kSlots], item_s0[kSlots];
+    __shared__ int slot_item[kSlots];
+    __shared__ double part[kSlots][BW];
+    __shared__ int accepted[kSlots][BW];
+    const int tid = int(threadIdx.x);
+    const int b = tid % BW, slot = tid / BW;
+    const int first = bt.first[batch], count = bt.count[batch];
+    __syncthreads();   // the previous batch is done with the tables
+    if (tid < kSlots) slot_item[tid] = -1;
+    __syncthreads();
+    if (tid < count) {
+        const int pos = bt.pos[first + tid];
+        const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
+        const int s0 = bt.slot0[first + tid];
+        const int nch = max(1, (k1 - k0 + kSeg - 1) / kSeg);
+        item_pos[tid] = pos;
+        item_k0[tid] = k0;
+        item_k1[tid] = k1;
+        item_s0[tid] = s0;
+        for (int q = 0; q < nch; ++q) slot_item[s0 + q] = tid;
+    }
+    __syncthreads();
+    const int it = slot_item[slot];
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
     T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
-    const uint16_t xm = ld_l2(xg + pos);
-    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
-    const int n0 = p.nb_ptr[pos], n1 = p.nb_ptr[pos + 1];
-    const double lin = p.lin[pos];
-    const int bit = p.pos_bit[pos];
-    const int xb = (xm >> b) & 1;
-    const double delta = xb ? -1.0 : 1.0;
-    const int len = k1 - k0;
     Chain<T> ch;
-    double dE = lin * delta;
-    if (len > 0) {
-        chain_load<T, BW>(p.ent, k0, k1, rg, b, ch);
-        dE = chain_sum<T>(ch, len, delta, dE, true);
+    int pos = 0, k0 = 0, k1 = 0, j = 0, len = 0, xb = 0;
+    uint16_t xm = 0;
+    double delta = 1.0;
+    if (it >= 0) {
+        pos = item_pos[it];
+        k0 = item_k0[it];
+        k1 = item_k1[it];
+        j = slot - item_s0[it];
+        xm = ld_l2(xg + pos);
+        xb = (xm >> b) & 1;
+        delta = xb ? -1.0 : 1.0;
+        const int ka = k0 + j * kSeg;
+        const int kb = min(ka + kSeg, k1);
+        len = kb - ka;
+        double partial = p.lin[pos] * delta;
+        if (len > 0) {
+            chain_load<T, BW>(p.ent, ka, kb, rg, b, ch);
+            partial = chain_sum<T>(ch, len, delta, partial, j == 0);
+        }
+        part[slot][b] = partial;
     }
-    dE = neighbour_terms(p, xg, n0, n1, b, xb, dE);
-    const bool acc = decide<T>(s, bit, g * BW + b, sweep, beta, dE);
-    if (acc && len > 0) {
+    __syncthreads();
+    if (it >= 0 && j == 0) {
+        const int nch = max(1, (k1 - k0 + kSeg - 1) / kSeg);
+        double dE = part[slot][b];
+        for (int q = 1; q < nch; ++q) dE += part[slot + q][b];
+        dE = neighbour_terms(p, xg, p.nb_ptr[pos], p.nb_ptr[pos + 1], b, xb, dE);
+        const bool acc = decide<T>(s, p.pos_bit[pos], g * BW + b, sweep, beta, dE);
+        accepted[it][b] = acc ? 1 : 0;
+        const uint16_t flips = segment_flips<BW>(acc);
+        if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
+    }
+    __syncthreads();
+    if (it >= 0 && accepted[it][b]) {
         if (p.seq_update[pos]) {
-            // a ray repeats in this column: its updates must land in CSR order
-            for (int k = k0; k < k1; ++k) {
-                const RayEnt e = p.ent[k];
-                T* q = rg + size_t(e.ray) * BW + b;
-                *q = residual_add<T>(ld_l2(q), e.a, delta);
-                __builtin_amdgcn_s_waitcnt(0);
-            }
-        } else {
+            // a ray repeats in this column: the bit's first slot scatters it in CSR order
+            if (j == 0)
+                for (int k = k0; k < k1; ++k) {
+                    const RayEnt e = p.ent[k];
+                    T* q = rg + size_t(e.ray) * BW + b;
+                    *q = residual_add<T>(ld_l2(q), e.a, delta);
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
+        } else if (len > 0) {
             chain_scatter<T, BW>(ch, len, rg, b, delta);
         }
     }
-    const uint16_t flips = segment_flips<BW>(acc);
-    if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
 }
 
-// A bit with more than kSeg entries, by a whole workgroup: the chains are spread over
-// the block's BW-lane slots, their sums meet in LDS, slot 0 adds them in order and
-// decides, and the residual scatter is spread over the block again.
+// A bit with more chains than a workgroup has slots, by a whole workgroup in several
+// passes: the chains are spread over the block's BW-lane slots, their sums meet in
+// LDS, slot 0 adds them in order and decides, and the residual scatter is spread
+// over the block again.
 template <typename T, int BW, int BLOCK>
 __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState<T>& s, int pos, int g, int sweep,
                                              double beta) {
@@ -350,7 +395,8 @@ constexpr int kTeams = 8;
 
 template <typename T, int BW, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-ray_team_kernel(DevProblem p, DevState<T> s, int sweep_begin, int sweep_end, int recompute_every, int team_size) {
+ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int sweep_end, int recompute_every,
+                int team_size) {
     constexpr int kSlots = BLOCK / BW;
     __shared__ int s_team, s_rank, s_ok;
     if (threadIdx.x == 0) {
@@ -376,21 +422,17 @@ ray_team_kernel(DevProblem p, DevState<T> s, int sweep_begin, int sweep_end, int
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         const double beta = s.beta[sweep];
         for (int c = 0; c < p.n_classes; ++c) {
-            const int c0 = p.cls_ptr[c];
-            const int nh = p.cls_heavy[c];
-            const int nl = p.cls_ptr[c + 1] - c0 - nh;
-            const int heavy_tasks = nh * my_groups;
-            for (int h = rank; h < heavy_tasks; h += team_size) {
-                const int gi = h / nh;
-                heavy_update<T, BW, BLOCK>(p, s, c0 + (h - gi * nh), team + gi * kTeams, sweep, beta);
-            }
-            // light tasks start on the blocks after those that took heavy bits
-            const int first = (heavy_tasks % team_size) * kSlots;
-            const int slot = (my_slot - first + n_slots) % n_slots;
-            const int light_tasks = nl * my_groups;
-            for (int t = slot; t < light_tasks; t += n_slots) {
-                const int gi = t / nl;
-                light_update<T, BW>(p, s, c0 + nh + (t - gi * nl), team + gi * kTeams, b, sweep, beta);
+            const int b0 = bt.cls_batch[c];
+            const int nbc = bt.cls_batch[c + 1] - b0;
+            const int tasks = nbc * my_groups;
+            for (int t = rank; t < tasks; t += team_size) {
+                const int gi = t / nbc;
+                const int batch = b0 + (t - gi * nbc);
+                const int g = team + gi * kTeams;
+                if (bt.count[batch] < 0)
+                    heavy_update<T, BW, BLOCK>(p, s, bt.pos[bt.first[batch]], g, sweep, beta);
+                else
+                    packed_update<T, BW, BLOCK>(p, bt, s, batch, g, sweep, beta);
             }
             target += unsigned(team_size);
             if (!team_barrier(bar, target, s.err)) return;
@@ -636,8 +678,6 @@ __global__ void __launch_bounds__(256) int_energy_kernel(DevGraph g, const uint8
 // ---------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------
-constexpr int kSweepBlock = 256;
-
 // Dispatch on the group width: fn.template operator()<BW>().
 template <typename F>
 static hipError_t with_bw(int bw, F&& fn) {
@@ -664,7 +704,8 @@ hipError_t launch_ray_init(const DevProblem& p, int precision, int bw, void* x, 
     });
 }
 
-hipError_t launch_ray_sweeps(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups, int R,
+hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int precision, int bw, void* x, void* r,
+                             int n_groups, int R,
                              uint64_t seed_offset, const double* beta, int sb, int se, int recompute_every,
                              unsigned* bar, unsigned* tickets, int* err, int team_size, hipStream_t st) {
     return with_bw(bw, [&]<int BW>() {
@@ -680,7 +721,8 @@ hipError_t launch_ray_sweeps(const DevProblem& p, int precision, int bw, void* x
             s.bar = bar;
             s.tickets = tickets;
             s.err = err;
-            void* args[] = {const_cast<DevProblem*>(&p), &s, &sb, &se, &recompute_every, &team_size};
+            void* args[] = {const_cast<DevProblem*>(&p), const_cast<DevBatches*>(&bt), &s, &sb, &se, &recompute_every,
+                            &team_size};
             return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock>),
                                               dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), args, 0, st);
         };
