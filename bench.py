@@ -26,6 +26,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "spin updates/s, 306k-bit muon QUBO ×16 branches; p50/p99 batched solve ms"
 HBM_PEAK_GBS = 8000.0
+# The reference's published rate for this workload: branch_fractions, 16 branches x 250
+# sweeps of the 306,328-bit muon QUBO in 7.0 s on 16 host threads
+# (experiments/p1_bench/check_log.txt:162-163): 16 * 250 * 306,328 / 7.0.
+REFERENCE_RATE = 16 * 250 * 306328 / 7.0
 
 
 def log(*a):
@@ -82,6 +86,8 @@ def main():
     ap.add_argument("--group-width", type=int, default=0, help="branches per group (2, 4, 8, 16); 0 = auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the plain 1 GiB device copy (HBM counter calibration) and print its rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this engine has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -92,6 +98,12 @@ def main():
 
     info = qa.device_info()
     log("device", info)
+    if args.copy_probe:
+        nbytes, reps = 1 << 30, 5
+        ms = qa.copy_probe(nbytes, reps)
+        print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ms": ms, "read_plus_write_GBs": 2 * nbytes / ms / 1e6}),
+              flush=True)
+        return
     t0 = time.perf_counter()
     P = qa.muon_scene()
     t_scene = time.perf_counter() - t0
@@ -140,7 +152,7 @@ def main():
         "ms_per_step": total / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": value / REFERENCE_RATE,
         "dtype": "f32" if prec == qa.FP32 else "f64",
         "data": "synthetic",
         "config": {

@@ -52,6 +52,13 @@ typedef struct {
 } qb_device_info;
 int qb_get_device_info(qb_device_info* out);
 
+/* No counterpart in the reference. A plain device-to-device streaming copy of `bytes`
+   (a positive multiple of 16) between two buffers the call allocates, repeated `reps`
+   times; *ms_per_copy is the mean duration from HIP events. It calibrates HBM byte
+   counters against a kernel of known traffic (2 x bytes) and gives the bandwidth a pure
+   read + write stream sustains. */
+int qb_copy_probe(int64_t bytes, int reps, double* ms_per_copy);
+
 /* ==== float path: the ray x voxel QUBO ================================================ */
 
 /* BinaryProblem (src/engine.hpp:30-49): bit -> (ray, a) CSR, per-ray d and w, the

@@ -109,6 +109,7 @@ def lib():
             "qb_last_error": (C.c_char_p, []),
             "qb_version": (i32, []),
             "qb_get_device_info": (i32, [C.POINTER(DeviceInfo)]),
+            "qb_copy_probe": (i32, [C.c_int64, i32, C.POINTER(C.c_double)]),
             "qb_problem_create": (i32, [C.POINTER(ProblemDesc), C.POINTER(vp)]),
             "qb_problem_destroy": (i32, [vp]),
             "qb_problem_get_info": (i32, [vp, C.POINTER(ProblemInfo)]),
@@ -153,6 +154,13 @@ def device_info():
     _check(lib().qb_get_device_info(C.byref(d)))
     return dict(name=d.name.decode(), arch=d.arch.decode(), compute_units=d.compute_units, xcds=d.xcds,
                 hbm_bytes=d.hbm_bytes, lds_bytes_per_cu=d.lds_bytes_per_cu)
+
+
+def copy_probe(nbytes, reps=10):
+    """Mean ms of a plain device copy of `nbytes` (reads and writes nbytes each)."""
+    ms = C.c_double()
+    _check(lib().qb_copy_probe(C.c_int64(int(nbytes)), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def schedule(t_hot, t_cold, anneal, hold):

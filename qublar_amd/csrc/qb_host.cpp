@@ -95,6 +95,34 @@ extern "C" int qb_get_device_info(qb_device_info* out) {
     return QB_OK;
 }
 
+extern "C" int qb_copy_probe(int64_t bytes, int reps, double* ms_per_copy) {
+    if (bytes < 16 || bytes % 16 != 0 || reps < 1 || !ms_per_copy)
+        return fail(QB_E_ARG, "qb_copy_probe: bytes must be a positive multiple of 16, reps >= 1");
+    DevBuf src, dst;
+    int rc = src.reserve(size_t(bytes));
+    if (!rc) rc = dst.reserve(size_t(bytes));
+    if (rc) return rc;
+    hipStream_t st = nullptr;
+    QB_HIP(hipMemsetAsync(src.p, 1, size_t(bytes), st));
+    const size_t n16 = size_t(bytes) / 16;
+    QB_HIP(launch_copy_probe(src.p, dst.p, n16, st));   // first touch, untimed
+    hipEvent_t e0, e1;
+    QB_HIP(hipEventCreate(&e0));
+    QB_HIP(hipEventCreate(&e1));
+    (void)hipEventRecord(e0, st);
+    for (int i = 0; i < reps; ++i) (void)launch_copy_probe(src.p, dst.p, n16, st);
+    (void)hipEventRecord(e1, st);
+    const hipError_t se = hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (se != hipSuccess) return fail(QB_E_HIP, "qb_copy_probe: %s", hipGetErrorString(se));
+    QB_HIP(hipGetLastError());
+    *ms_per_copy = double(ms) / reps;
+    return QB_OK;
+}
+
 // ---- schedule -----------------------------------------------------------------------
 // T(s) = t_hot (t_cold / t_hot)^min(1, s / max(1, A - 1)) for s < A, then t_cold
 // (src/ising_recon.hpp:129-131); the engine uses beta = 1 / T.
@@ -166,12 +194,13 @@ struct qb_problem {
     int64_t device_bytes = 0;
     std::vector<int32_t> pos_bit, cls_ptr;
     DevBuf b_cls, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq;
-    std::vector<int32_t> col_ptr;   // host copy, position space
+    std::vector<int32_t> col_ptr, nb_ptr;   // host copies, position space
+    std::vector<uint8_t> seq_update;
     struct Batches {
-        DevBuf cls, first, count, pos, slot0;
+        DevBuf cls, cls_heavy, first, count, pos, slot0, desc;
         DevBatches dev{};
     };
-    std::map<int, std::unique_ptr<Batches>> batches;   // by group width
+    std::map<int, std::unique_ptr<Batches>> batches;   // by group width and team size
     // workspace
     DevBuf w_x, w_r, w_beta, w_bar, w_err, w_part, w_pv, w_tri, w_cert, w_br, w_data, w_prior, w_rerr, w_X;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -354,6 +383,8 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
         return rc;
     QB_HIP(hipStreamSynchronize(st));
     P->col_ptr = col_ptr;
+    P->nb_ptr = nb_ptr;
+    P->seq_update = seq_update;
     P->device_bytes = int64_t(P->b_cls.bytes + P->b_seq.bytes + P->b_pos.bytes + P->b_col.bytes + P->b_ent.bytes + P->b_nbp.bytes +
                               P->b_nb.bytes + P->b_nbbit.bytes + P->b_lin.bytes + P->b_d.bytes + P->b_w.bytes +
                               P->b_rtp.bytes + P->b_rt.bytes);
@@ -409,11 +440,11 @@ extern "C" int qb_problem_colouring(const qb_problem* p, int32_t* order, int32_t
 }
 
 // Workgroups per XCD team: `requested` total (a multiple of 8), or one per CU.
-static int team_size_for(int requested) {
+static int team_size_for(int requested, size_t table_bytes) {
     hipDeviceProp_t prop;
     int rc = device_props(prop);
     if (rc) return -rc;
-    const int per_cu = ray_sweep_blocks_per_cu();
+    const int per_cu = ray_sweep_blocks_per_cu(table_bytes);
     if (per_cu <= 0) return -fail(QB_E_HIP, "occupancy query for the sweep kernel failed");
     const int cap = per_cu * prop.multiProcessorCount;
     const int total = requested > 0 ? requested : prop.multiProcessorCount;
@@ -434,15 +465,22 @@ static int group_width_for(int R, int requested) {
 // The work batches of every class at group width bw (cached per width): bits sorted
 // by chain count, the largest first; a bit needing more chains than a workgroup has
 // slots is a batch of its own, the rest are packed into workgroups next-fit.
-static int batches_for(qb_problem* P, int bw, const DevBatches** out) {
-    auto found = P->batches.find(bw);
+// Packs each class's bits into batches for a team of `team_size` workgroups. A bit
+// with more chains than a workgroup has slots is a multi-pass batch of its own. A
+// class whose other bits fit the remaining workgroups in one round is spread over
+// them evenly (largest bit first into the least filled batch), so its step costs one
+// batch per workgroup; a larger class is cut into full batches, next fit.
+static int batches_for(qb_problem* P, int bw, int team_size, const DevBatches** out) {
+    const int key = bw * 65536 + team_size;
+    auto found = P->batches.find(key);
     if (found != P->batches.end()) {
         *out = &found->second->dev;
         return QB_OK;
     }
     const int slots = kSweepBlock / bw;
     const int K = P->n_classes;
-    std::vector<int32_t> cls_batch(static_cast<size_t>(K) + 1, 0), first, count, pos, slot0;
+    std::vector<int32_t> cls_batch(static_cast<size_t>(K) + 1, 0), cls_heavy(static_cast<size_t>(K), 0), first,
+        count, pos, slot0;
     std::vector<int32_t> items;
     for (int c = 0; c < K; ++c) {
         cls_batch[size_t(c)] = int32_t(first.size());
@@ -452,7 +490,10 @@ static int batches_for(qb_problem* P, int bw, const DevBatches** out) {
             return std::max(1, (P->col_ptr[size_t(q) + 1] - P->col_ptr[size_t(q)] + kSeg - 1) / kSeg);
         };
         std::stable_sort(items.begin(), items.end(), [&](int a, int b) { return chains(a) > chains(b); });
-        int open = -1, used = 0;
+        // a bit needing more slots than a workgroup has: a multi-pass batch of its own
+        std::vector<int32_t> packable;
+        int n_heavy = 0;
+        int64_t total = 0;
         for (int q : items) {
             const int k = chains(q);
             if (k > slots) {
@@ -460,33 +501,101 @@ static int batches_for(qb_problem* P, int bw, const DevBatches** out) {
                 count.push_back(-1);
                 pos.push_back(q);
                 slot0.push_back(0);
-                open = -1;
-                continue;
+                ++n_heavy;
+            } else {
+                packable.push_back(q);
+                total += k;
             }
-            if (open < 0 || used + k > slots) {
-                first.push_back(int32_t(pos.size()));
-                count.push_back(0);
-                open = int(count.size()) - 1;
-                used = 0;
+        }
+        auto emit = [&](const std::vector<int32_t>& bin) {
+            first.push_back(int32_t(pos.size()));
+            count.push_back(int32_t(bin.size()));
+            int used = 0;
+            for (int q : bin) {
+                pos.push_back(q);
+                slot0.push_back(used);
+                used += chains(q);
             }
-            pos.push_back(q);
-            slot0.push_back(used);
-            used += k;
-            count[size_t(open)]++;
+        };
+        cls_heavy[size_t(c)] = n_heavy;
+        const int room = team_size - heavy_ranks(n_heavy, team_size);
+        if (!packable.empty() && room >= 1 && total <= int64_t(room) * slots) {
+            // the class fits the team in one round: largest bit first into the least
+            // filled of at most `room` batches, so every workgroup gets one batch
+            const int B = std::min<int>(room, int(packable.size()));
+            std::vector<std::vector<int32_t>> bins(static_cast<size_t>(B));
+            std::vector<int> load(static_cast<size_t>(B), 0);
+            for (int q : packable) {
+                const int k = chains(q);
+                size_t best = 0;
+                for (size_t i = 1; i < load.size(); ++i)
+                    if (load[i] < load[best]) best = i;
+                if (load[best] + k > slots) {
+                    bins.emplace_back();
+                    load.push_back(0);
+                    best = load.size() - 1;
+                }
+                bins[best].push_back(q);
+                load[best] += k;
+            }
+            for (const auto& bin : bins)
+                if (!bin.empty()) emit(bin);
+        } else {
+            // more than one round: full batches, next fit
+            std::vector<int32_t> bin;
+            int used = 0;
+            for (int q : packable) {
+                const int k = chains(q);
+                if (!bin.empty() && used + k > slots) {
+                    emit(bin);
+                    bin.clear();
+                    used = 0;
+                }
+                bin.push_back(q);
+                used += k;
+            }
+            if (!bin.empty()) emit(bin);
         }
     }
     cls_batch[size_t(K)] = int32_t(first.size());
+    // slot descriptors: one per chain of every packed bit, empty slots pos = -1
+    std::vector<SlotDesc> desc(first.size() * size_t(slots), SlotDesc{-1, 0, 0, 0u});
+    for (size_t bi = 0; bi < first.size(); ++bi) {
+        if (count[bi] < 0) continue;
+        for (int it = 0; it < count[bi]; ++it) {
+            const int q = pos[size_t(first[bi] + it)];
+            const int k0 = P->col_ptr[size_t(q)], k1 = P->col_ptr[size_t(q) + 1];
+            const int nch = std::max(1, (k1 - k0 + kSeg - 1) / kSeg);
+            const int nn = P->nb_ptr[size_t(q) + 1] - P->nb_ptr[size_t(q)];
+            if (nn > kMaxPackedNeighbours)
+                return fail(QB_E_LIMIT, "bit %d has %d neighbours; the sweep's limit is %d", P->pos_bit[size_t(q)], nn,
+                            kMaxPackedNeighbours);
+            for (int j = 0; j < nch; ++j) {
+                const int ka = k0 + j * kSeg;
+                const int len = std::max(0, std::min(kSeg, k1 - ka));
+                const uint32_t info = uint32_t(len) | (uint32_t(j) << 5) | (uint32_t(nch) << 13) |
+                                      (uint32_t(nn) << 21) | (uint32_t(P->seq_update[size_t(q)] ? 1u : 0u) << 31);
+                desc[bi * size_t(slots) + size_t(slot0[size_t(first[bi] + it)] + j)] =
+                    SlotDesc{q, ka, P->nb_ptr[size_t(q)], info};
+            }
+        }
+    }
+    const int n_batches = int(first.size());
+    if (sweep_table_bytes(K) > size_t(kSweepTableLimit))
+        return fail(QB_E_LIMIT, "%d colour classes exceed the sweep's %d-byte LDS table", K, kSweepTableLimit);
     auto B = std::make_unique<qb_problem::Batches>();
     int rc;
     if ((rc = B->cls.upload(cls_batch, P->stream)) || (rc = B->first.upload(first, P->stream)) ||
         (rc = B->count.upload(count, P->stream)) || (rc = B->pos.upload(pos, P->stream)) ||
-        (rc = B->slot0.upload(slot0, P->stream)))
+        (rc = B->slot0.upload(slot0, P->stream)) || (rc = B->desc.upload(desc, P->stream)) ||
+        (rc = B->cls_heavy.upload(cls_heavy, P->stream)))
         return rc;
     QB_HIP(hipStreamSynchronize(P->stream));
-    B->dev = DevBatches{B->cls.as<int32_t>(), B->first.as<int32_t>(), B->count.as<int32_t>(), B->pos.as<int32_t>(),
-                        B->slot0.as<int32_t>()};
+    B->dev = DevBatches{B->cls.as<int32_t>(), B->cls_heavy.as<int32_t>(), B->first.as<int32_t>(),
+                        B->count.as<int32_t>(), B->pos.as<int32_t>(), B->slot0.as<int32_t>(),
+                        B->desc.as<SlotDesc>(), n_batches};
     *out = &B->dev;
-    P->batches[bw] = std::move(B);
+    P->batches[key] = std::move(B);
     return QB_OK;
 }
 
@@ -509,8 +618,15 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     const int bw = group_width_for(R, o.group_width);
     const int n_groups = (R + bw - 1) / bw;
     const size_t tsz = o.precision == QB_FP32 ? sizeof(float) : sizeof(double);
-    const int team_size = team_size_for(o.workgroups);
+    const int team_size = team_size_for(o.workgroups, 0);
     if (team_size <= 0) return -team_size;
+    const DevBatches* bt = nullptr;
+    if ((rc = batches_for(P, bw, team_size, &bt))) return rc;
+    {
+        // the LDS tables must not lower the residency the team size was chosen for
+        const int checked = team_size_for(o.workgroups, sweep_table_bytes(P->n_classes));
+        if (checked <= 0) return -checked;
+    }
     const int n_part = ray_energy_blocks(P->dev, bw);
     if ((rc = P->w_x.reserve(sizeof(uint16_t) * size_t(n_groups) * size_t(std::max(n, 1)))) ||
         (rc = P->w_r.reserve(tsz * size_t(n_groups) * size_t(std::max(nr, 1)) * size_t(bw))) ||
@@ -524,8 +640,6 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
         (rc = P->w_data.reserve(sizeof(double) * size_t(R))) || (rc = P->w_prior.reserve(sizeof(double) * size_t(R))) ||
         (rc = P->w_rerr.reserve(sizeof(float))))
         return rc;
-    const DevBatches* bt = nullptr;
-    if ((rc = batches_for(P, bw, &bt))) return rc;
     hipStream_t st = P->stream;
     const std::vector<double> beta = betas_of(*s);
     QB_HIP(hipStreamSynchronize(st));

@@ -60,13 +60,44 @@ struct DevProblem {
 // max(1, ceil(k / kSeg)) slots (one per chain). A batch packs whole bits into one
 // workgroup's slots; a bit needing more slots than a workgroup has is a batch of its
 // own (count -1) and is evaluated in several passes.
+//
+// Each slot of a packed batch has a descriptor holding everything static the slot
+// needs, so the sweep can fetch a step's structure before the step's state is ready:
+//   pos   the bit's position (-1: the slot is empty)
+//   ka    the first CSR entry of the slot's chain
+//   nb0   the bit's first neighbour entry
+//   info  len (bits 0-4, entries in the chain, 0..kSeg) | j (5-12, the chain's index
+//         in its bit; the bit's first slot is the slot's index minus j) | nch (13-20,
+//         the bit's chains) | n_nb (21-30, the bit's neighbours) | seq (31, a ray
+//         repeats in the column)
+struct SlotDesc {
+    int32_t pos, ka, nb0;
+    uint32_t info;
+};
+constexpr int kMaxPackedNeighbours = 1023;
+
 struct DevBatches {
-    const int32_t* cls_batch;   // n_classes + 1: class c's batches
+    const int32_t* cls_batch;   // n_classes + 1: class c's batches, its multi-pass batches first
+    const int32_t* cls_heavy;   // n_classes: class c's multi-pass batches
     const int32_t* first;       // per batch: first item
     const int32_t* count;       // per batch: items, or -1 for one bit in several passes
     const int32_t* pos;         // per item: the bit's position
     const int32_t* slot0;       // per item: its first slot in the batch
+    const SlotDesc* desc;       // per batch: BLOCK / BW slot descriptors (all empty for count -1)
+    int n_batches;
 };
+
+// The sweep keeps its class tables (cls_batch, cls_heavy) in LDS.
+constexpr int kSweepTableLimit = 96 * 1024;
+inline size_t sweep_table_bytes(int n_classes) { return sizeof(int32_t) * size_t(2 * n_classes + 1); }
+
+// Workgroups of a team reserved for a class's multi-pass batches: up to half the team;
+// the packed batches go round-robin over the others. With none reserved every task
+// of the class goes round-robin over the whole team.
+__host__ __device__ inline int heavy_ranks(int heavy_tasks, int team_size) {
+    const int half = team_size / 2;
+    return heavy_tasks < half ? heavy_tasks : half;
+}
 
 // Mutable state of one anneal: n_groups groups of BW branches (BW in {2, 4, 8, 16}).
 template <typename T>
@@ -109,7 +140,7 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
                              uint64_t seed_offset, const double* beta, int sweep_begin, int sweep_end,
                              int recompute_every, unsigned* bar, unsigned* tickets, int* err, int team_size,
                              hipStream_t st);
-int ray_sweep_blocks_per_cu();
+int ray_sweep_blocks_per_cu(size_t table_bytes);
 constexpr int kSweepBlock = 256;   // threads per sweep workgroup
 hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, float* p_void,
                               uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st);
@@ -129,5 +160,7 @@ hipError_t launch_int_unpermute(const DevGraph& g, int R, const int8_t* s_state,
                                 hipStream_t st);
 hipError_t launch_int_energies(const DevGraph& g, const uint8_t* X, int K, long long* out, hipStream_t st);
 int int_lds_capacity_spins();
+
+hipError_t launch_copy_probe(const void* src, void* dst, size_t n16, hipStream_t st);
 
 }  // namespace qb

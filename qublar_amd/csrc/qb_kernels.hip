@@ -92,24 +92,34 @@ __device__ __forceinline__ V ld_l2(const V* p) {
 }
 
 // Team barrier: monotone counter in L2; the k-th barrier waits for k * team size
-// arrivals. Same drain rule and ~2 s limit as grid_barrier.
-__device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int* err) {
+// arrivals. Every wave drains its stores, thread 0 arrives, and then -- while thread 0
+// polls -- every thread runs `after_arrival`, which issues loads of static data for
+// the next step (they may not be waited for before the arrival: the drain would
+// wait for them too). The error flag is read only while polling, so an arrival
+// costs one atomic; a wait longer than ~2 s sets *err, as in grid_barrier.
+template <typename F>
+__device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int* err, F&& after_arrival) {
     __shared__ int s_ok;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    // the first poll is issued before the hook's loads: loads return in order, so the
+    // last workgroup to arrive sees its own arrival without waiting for the prefetch
+    unsigned seen = 0;
     if (threadIdx.x == 0) {
-        int ok = ld_l2(err) == 0;
-        if (ok) {
-            __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-            unsigned polls = 0;
-            while (ld_l2(bar) < target) {
-                if ((++polls & 1023u) == 0) {
-                    if (ld_l2(err) != 0 || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
-                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ok = 0;
-                        break;
-                    }
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        seen = ld_l2(bar);
+    }
+    after_arrival();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        unsigned polls = 0;
+        while (seen < target && (seen = ld_l2(bar)) < target) {
+            if ((++polls & 1023u) == 0) {
+                if (ld_l2(err) != 0 || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
                 }
             }
         }
@@ -117,6 +127,10 @@ __device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int
     }
     __syncthreads();
     return s_ok != 0;
+}
+
+__device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int* err) {
+    return team_barrier(bar, target, err, [] {});
 }
 
 // The dE of a bit sums its CSR entries in chains of kSeg: the first chain starts
@@ -146,25 +160,34 @@ __device__ __forceinline__ void chain_load(const RayEnt* __restrict__ ent, int k
 // The chain's sum: seeded with `seed` (the first chain of a bit, seed = lin * delta)
 // or started from its first term; terms w (r a + a^2 / 2), a = a_k delta, in order.
 template <typename T>
-__device__ __forceinline__ double chain_sum(const Chain<T>& ch, int len, double delta, double seed, bool seeded) {
+__device__ __forceinline__ double chain_sum(const RayEnt* e, const T* rv, int len, double delta, double seed,
+                                            bool seeded) {
     double acc = seed;
 #pragma unroll
     for (int u = 0; u < kSeg; ++u) {
-        const double ad = double(ch.e[u].a) * delta;
-        const double t = ch.e[u].w * (double(ch.rv[u]) * ad + 0.5 * ad * ad);
+        const double ad = double(e[u].a) * delta;
+        const double t = e[u].w * (double(rv[u]) * ad + 0.5 * ad * ad);
         const double nxt = (u == 0 && !seeded) ? t : acc + t;
         acc = u < len ? nxt : acc;
     }
     return acc;
 }
+template <typename T>
+__device__ __forceinline__ double chain_sum(const Chain<T>& ch, int len, double delta, double seed, bool seeded) {
+    return chain_sum<T>(ch.e, ch.rv, len, delta, seed, seeded);
+}
 
 // On acceptance: r[ray] += a * delta for the chain's entries, from the loaded values
 // (no other bit of the class touches these rays).
 template <typename T, int BW>
-__device__ __forceinline__ void chain_scatter(const Chain<T>& ch, int len, T* rg, int b, double delta) {
+__device__ __forceinline__ void chain_scatter(const RayEnt* e, const T* rv, int len, T* rg, int b, double delta) {
 #pragma unroll
     for (int u = 0; u < kSeg; ++u)
-        if (u < len) rg[size_t(ch.e[u].ray) * BW + b] = residual_add<T>(ch.rv[u], ch.e[u].a, delta);
+        if (u < len) rg[size_t(e[u].ray) * BW + b] = residual_add<T>(rv[u], e[u].a, delta);
+}
+template <typename T, int BW>
+__device__ __forceinline__ void chain_scatter(const Chain<T>& ch, int len, T* rg, int b, double delta) {
+    chain_scatter<T, BW>(ch.e, ch.rv, len, rg, b, delta);
 }
 
 // lambda * (+-1) per neighbour, in CSR order; the first 8 neighbours' masks are loaded
@@ -199,6 +222,13 @@ __device__ __forceinline__ bool decide(const DevState<T>& s, int bit, int branch
            metropolis_uphill(dE, beta, s.seed_offset + uint64_t(branch) + 1u, uint32_t(sweep), uint32_t(bit));
 }
 
+// The same test with the branch's draw u = draw_u01(seed, sweep, bit) already made.
+template <typename T>
+__device__ __forceinline__ bool decide_drawn(const DevState<T>& s, int branch, double u, double beta, double dE) {
+    if (branch >= s.R) return false;
+    return dE <= 0.0 || u < qexp(-dE * beta);
+}
+
 // The accepted lanes of this lane's BW-wide segment, as a mask of the group's branches.
 template <int BW>
 __device__ __forceinline__ uint16_t segment_flips(bool acc) {
@@ -207,85 +237,168 @@ __device__ __forceinline__ uint16_t segment_flips(bool acc) {
     return uint16_t((bal >> base) & ((uint64_t(1) << BW) - 1));
 }
 
-// One batch of whole bits for the BW branches of group g: every slot evaluates one
-// chain of one bit; the chain sums meet in LDS, the bit's first slot adds them in
-// order, adds the neighbour terms and decides; each slot then scatters its own
-// chain from registers.
-template <typename T, int BW, int BLOCK>
-__device__ __forceinline__ void packed_update(const DevProblem& p, const DevBatches& bt, const DevState<T>& s,
-                                              int batch, int g, int sweep, double beta) {
+// ---------------------------------------------------------------------------------
+// Packed steps. A slot's static operands -- its descriptor, its chain's entries, and
+// for a bit's first chain the neighbour positions, lin and the original bit index --
+// are fetched a step ahead (fetch_desc while the previous step computes, the rest
+// either then or while the team barrier before this step is pending), so once the
+// barrier opens a step issues only its state loads (the spin mask, the chain's
+// residuals, the neighbours' masks), all at once.
+// ---------------------------------------------------------------------------------
+constexpr int kNb = 6;
+
+struct SlotStatic {
+    SlotDesc d;
+    RayEnt e[kSeg];
+    int32_t q[kNb];
+    double lin;
+    int32_t bit;
+};
+
+__device__ __forceinline__ int slot_len(uint32_t i) { return int(i & 31u); }
+__device__ __forceinline__ int slot_j(uint32_t i) { return int((i >> 5) & 255u); }
+__device__ __forceinline__ int slot_nch(uint32_t i) { return int((i >> 13) & 255u); }
+__device__ __forceinline__ int slot_nnb(uint32_t i) { return int((i >> 21) & 1023u); }
+__device__ __forceinline__ bool slot_seq(uint32_t i) { return (i >> 31) != 0u; }
+
+template <int KSLOTS>
+__device__ __forceinline__ void fetch_desc(const DevBatches& bt, int batch, int slot, SlotStatic& S) {
+    const int4 v = *reinterpret_cast<const int4*>(bt.desc + size_t(batch) * KSLOTS + size_t(slot));
+    S.d = SlotDesc{v.x, v.y, v.z, uint32_t(v.w)};
+}
+
+__device__ __forceinline__ void fetch_static(const DevProblem& p, SlotStatic& S) {
+    if (S.d.pos < 0) return;
+    const int len = slot_len(S.d.info);
+    if (len > 0) {
+        // in quarters: a lane whose chain ends earlier issues no further loads
+        const int last = S.d.ka + len - 1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) S.e[u] = p.ent[min(S.d.ka + u, last)];
+        if (len > 4) {
+#pragma unroll
+            for (int u = 4; u < 8; ++u) S.e[u] = p.ent[min(S.d.ka + u, last)];
+        }
+        if (len > 8) {
+#pragma unroll
+            for (int u = 8; u < 12; ++u) S.e[u] = p.ent[min(S.d.ka + u, last)];
+        }
+        if (len > 12) {
+#pragma unroll
+            for (int u = 12; u < 16; ++u) S.e[u] = p.ent[min(S.d.ka + u, last)];
+        }
+    }
+    if (slot_j(S.d.info) == 0) {
+        const int nn = slot_nnb(S.d.info);
+        if (nn > 0) {
+            const int last = S.d.nb0 + nn - 1;
+#pragma unroll
+            for (int u = 0; u < kNb; ++u) S.q[u] = p.nb[min(S.d.nb0 + u, last)];
+        }
+        S.lin = p.lin[S.d.pos];
+        S.bit = p.pos_bit[S.d.pos];
+    }
+}
+
+// One packed batch for the BW branches of group g: every slot evaluates one chain of
+// one bit; the chain sums meet in LDS, the bit's first slot adds them in order, adds
+// the neighbour terms and decides; each slot then scatters its own chain from
+// registers. `after_issue` runs once the state loads are issued, `after_compute` once
+// this slot's chain sum is in LDS (the caller's prefetch of the next step).
+template <typename T, int BW, int BLOCK, typename F1, typename F2>
+__device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<T>& s, const SlotStatic& S, int g,
+                                            int sweep, double beta, F1&& after_issue, F2&& after_compute) {
     constexpr int kSlots = BLOCK / BW;
-    __shared__ int item_pos[kSlots], item_k0[kSlots], item_k1[kSlots], item_s0[kSlots];
-    __shared__ int slot_item[kSlots];
     __shared__ double part[kSlots][BW];
     __shared__ int accepted[kSlots][BW];
     const int tid = int(threadIdx.x);
     const int b = tid % BW, slot = tid / BW;
-    const int first = bt.first[batch], count = bt.count[batch];
-    __syncthreads();   // the previous batch is done with the tables
-    if (tid < kSlots) slot_item[tid] = -1;
-    __syncthreads();
-    if (tid < count) {
-        const int pos = bt.pos[first + tid];
-        const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
-        const int s0 = bt.slot0[first + tid];
-        const int nch = max(1, (k1 - k0 + kSeg - 1) / kSeg);
-        item_pos[tid] = pos;
-        item_k0[tid] = k0;
-        item_k1[tid] = k1;
-        item_s0[tid] = s0;
-        for (int q = 0; q < nch; ++q) slot_item[s0 + q] = tid;
-    }
-    __syncthreads();
-    const int it = slot_item[slot];
+    const uint32_t info = S.d.info;
+    const int pos = S.d.pos, len = slot_len(info), j = slot_j(info), nn = slot_nnb(info);
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
     T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
-    Chain<T> ch;
-    int pos = 0, k0 = 0, k1 = 0, j = 0, len = 0, xb = 0;
+    T rv[kSeg];
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u) rv[u] = T(0);
+    uint16_t m[kNb];
     uint16_t xm = 0;
-    double delta = 1.0;
-    if (it >= 0) {
-        pos = item_pos[it];
-        k0 = item_k0[it];
-        k1 = item_k1[it];
-        j = slot - item_s0[it];
+    if (pos >= 0) {
         xm = ld_l2(xg + pos);
-        xb = (xm >> b) & 1;
-        delta = xb ? -1.0 : 1.0;
-        const int ka = k0 + j * kSeg;
-        const int kb = min(ka + kSeg, k1);
-        len = kb - ka;
-        double partial = p.lin[pos] * delta;
+        // residual loads in quarters of the chain: lanes past their chain's end are
+        // masked off and send no request
         if (len > 0) {
-            chain_load<T, BW>(p.ent, ka, kb, rg, b, ch);
-            partial = chain_sum<T>(ch, len, delta, partial, j == 0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rv[u] = ld_l2(rg + size_t(S.e[u].ray) * BW + b);
         }
+        if (len > 4) {
+#pragma unroll
+            for (int u = 4; u < 8; ++u) rv[u] = ld_l2(rg + size_t(S.e[u].ray) * BW + b);
+        }
+        if (len > 8) {
+#pragma unroll
+            for (int u = 8; u < 12; ++u) rv[u] = ld_l2(rg + size_t(S.e[u].ray) * BW + b);
+        }
+        if (len > 12) {
+#pragma unroll
+            for (int u = 12; u < 16; ++u) rv[u] = ld_l2(rg + size_t(S.e[u].ray) * BW + b);
+        }
+        if (j == 0 && nn > 0) {
+#pragma unroll
+            for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + S.q[u]);
+        }
+    }
+    after_issue();
+    // the draw depends only on (branch, sweep, bit): made while the loads are in flight
+    double u = 0.0;
+    if (pos >= 0 && j == 0) u = draw_u01(s.seed_offset + uint64_t(g * BW + b) + 1u, uint32_t(sweep), uint32_t(S.bit));
+    const int xb = (xm >> b) & 1;
+    const double delta = xb ? -1.0 : 1.0;
+    if (pos >= 0) {
+        double partial = j == 0 ? S.lin * delta : 0.0;
+        if (len > 0) partial = chain_sum<T>(S.e, rv, len, delta, partial, j == 0);
         part[slot][b] = partial;
     }
+    after_compute();
     __syncthreads();
-    if (it >= 0 && j == 0) {
-        const int nch = max(1, (k1 - k0 + kSeg - 1) / kSeg);
+    if (pos >= 0 && j == 0) {
+        const int nch = slot_nch(info);
         double dE = part[slot][b];
-        for (int q = 1; q < nch; ++q) dE += part[slot + q][b];
-        dE = neighbour_terms(p, xg, p.nb_ptr[pos], p.nb_ptr[pos + 1], b, xb, dE);
-        const bool acc = decide<T>(s, p.pos_bit[pos], g * BW + b, sweep, beta, dE);
-        accepted[it][b] = acc ? 1 : 0;
+        for (int q0 = 1; q0 < nch; q0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[min(slot + q0 + u, kSlots - 1)][b];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) dE = q0 + u < nch ? dE + v[u] : dE;
+        }
+        // lambda * (+-1) per neighbour in CSR order
+#pragma unroll
+        for (int u = 0; u < kNb; ++u) {
+            const int before = ((m[u] >> b) & 1) != xb;
+            const double nxt = dE + p.lambda * (before ? -1.0 : 1.0);
+            dE = u < nn ? nxt : dE;
+        }
+        for (int k = S.d.nb0 + kNb; k < S.d.nb0 + nn; ++k) {
+            const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
+            dE += p.lambda * (before ? -1.0 : 1.0);
+        }
+        const bool acc = decide_drawn<T>(s, g * BW + b, u, beta, dE);
+        accepted[slot][b] = acc ? 1 : 0;
         const uint16_t flips = segment_flips<BW>(acc);
         if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
     }
     __syncthreads();
-    if (it >= 0 && accepted[it][b]) {
-        if (p.seq_update[pos]) {
+    if (pos >= 0 && accepted[slot - j][b]) {
+        if (slot_seq(info)) {
             // a ray repeats in this column: the bit's first slot scatters it in CSR order
             if (j == 0)
-                for (int k = k0; k < k1; ++k) {
+                for (int k = p.col_ptr[pos]; k < p.col_ptr[pos + 1]; ++k) {
                     const RayEnt e = p.ent[k];
                     T* q = rg + size_t(e.ray) * BW + b;
                     *q = residual_add<T>(ld_l2(q), e.a, delta);
                     __builtin_amdgcn_s_waitcnt(0);
                 }
         } else if (len > 0) {
-            chain_scatter<T, BW>(ch, len, rg, b, delta);
+            chain_scatter<T, BW>(S.e, rv, len, rg, b, delta);
         }
     }
 }
@@ -416,26 +529,130 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
     const int my_groups = s.n_groups > team ? (s.n_groups - team + kTeams - 1) / kTeams : 0;
     if (my_groups == 0) return;
     const int b = threadIdx.x % BW;
+    const int slot = int(threadIdx.x) / BW;
     const int n_slots = team_size * kSlots;
-    const int my_slot = rank * kSlots + int(threadIdx.x) / BW;
+    const int my_slot = rank * kSlots + slot;
+    if (p.n_classes == 0) return;
+    // class -> batch table and heavy-batch bits, copied to LDS once
+    extern __shared__ int32_t tables[];
+    int32_t* cls = tables;                         // n_classes + 1 batch offsets
+    int32_t* nheavy = tables + p.n_classes + 1;   // n_classes multi-pass batch counts
+    for (int i = int(threadIdx.x); i <= p.n_classes; i += BLOCK) cls[i] = bt.cls_batch[i];
+    for (int i = int(threadIdx.x); i < p.n_classes; i += BLOCK) nheavy[i] = bt.cls_heavy[i];
+    __syncthreads();
+    // Class c has H = nh * G multi-pass tasks and P = (nb - nh) * G packed tasks, each a
+    // (batch, group) pair; Hr = heavy_ranks(H, team) ranks take the multi-pass tasks
+    // round-robin and the other ranks the packed ones (with Hr = 0 all tasks, multi-pass
+    // first, go round-robin over the team). A rank's k-th task in the class:
+    struct Split {
+        int H, P, Hr, nh, np;
+    };
+    auto split_of = [&](int c) {
+        const int nh = nheavy[c], np = cls[c + 1] - cls[c] - nh;
+        const int H = nh * my_groups, P = np * my_groups;
+        return Split{H, P, heavy_ranks(H, team_size), nh, np};
+    };
+    auto n_tasks = [&](int c) {
+        const Split q = split_of(c);
+        if (q.Hr == 0) {
+            const int total = q.H + q.P;
+            return rank < total ? (total - rank + team_size - 1) / team_size : 0;
+        }
+        if (rank < q.Hr) return (q.H - rank + q.Hr - 1) / q.Hr;
+        const int r = rank - q.Hr, n = team_size - q.Hr;
+        return r < q.P ? (q.P - r + n - 1) / n : 0;
+    };
+    // the batch and group of this rank's k-th task in class c; heavy = multi-pass
+    auto task_of = [&](int c, int k, int& g, bool& heavy) {
+        const Split q = split_of(c);
+        int h = -1, pk = -1;
+        if (q.Hr == 0) {
+            const int t = rank + k * team_size;
+            if (t < q.H) h = t;
+            else pk = t - q.H;
+        } else if (rank < q.Hr) {
+            h = rank + k * q.Hr;
+        } else {
+            pk = (rank - q.Hr) + k * (team_size - q.Hr);
+        }
+        heavy = h >= 0;
+        if (heavy) {
+            const int gi = h / q.nh;
+            g = team + gi * kTeams;
+            return cls[c] + (h - gi * q.nh);
+        }
+        const int gi = pk / q.np;
+        g = team + gi * kTeams;
+        return cls[c] + q.nh + (pk - gi * q.np);
+    };
+    // The lookahead cursor (ls, lc, lk): the task after the one being processed.
+    auto advance = [&](int& sw, int& c, int& k) {
+        ++k;
+        while (k >= n_tasks(c)) {
+            k = 0;
+            if (++c == p.n_classes) {
+                c = 0;
+                if (++sw >= sweep_end) return false;
+            }
+        }
+        return true;
+    };
+    int ls = sweep_begin, lc = 0, lt = -1;
+    bool ahead = advance(ls, lc, lt);
+    // the task at the cursor: its batch, group and kind
+    int cb = 0, cg = 0;
+    bool ch = false;
+    SlotStatic cur;
+    cur.d = SlotDesc{-1, 0, 0, 0u};
+    bool pending = false;   // cur holds a descriptor whose other operands are not yet fetched
+    if (ahead) {
+        cb = task_of(lc, lt, cg, ch);
+        if (!ch) {
+            fetch_desc<kSlots>(bt, cb, slot, cur);
+            fetch_static(p, cur);
+        }
+    }
+    auto prefetch_pending = [&] {
+        if (pending) {
+            fetch_static(p, cur);
+            pending = false;
+        }
+    };
     unsigned target = 0;
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         const double beta = s.beta[sweep];
         for (int c = 0; c < p.n_classes; ++c) {
-            const int b0 = bt.cls_batch[c];
-            const int nbc = bt.cls_batch[c + 1] - b0;
-            const int tasks = nbc * my_groups;
-            for (int t = rank; t < tasks; t += team_size) {
-                const int gi = t / nbc;
-                const int batch = b0 + (t - gi * nbc);
-                const int g = team + gi * kTeams;
-                if (bt.count[batch] < 0)
+            while (ahead && ls == sweep && lc == c) {
+                const int batch = cb, g = cg;
+                const bool heavy = ch;
+                ahead = advance(ls, lc, lt);
+                if (ahead) cb = task_of(lc, lt, cg, ch);
+                const int nbatch = cb;
+                const bool npacked = ahead && !ch;
+                const bool same_class = ahead && ls == sweep && lc == c;
+                SlotStatic nxt;
+                nxt.d = SlotDesc{-1, 0, 0, 0u};
+                if (heavy) {
                     heavy_update<T, BW, BLOCK>(p, s, bt.pos[bt.first[batch]], g, sweep, beta);
-                else
-                    packed_update<T, BW, BLOCK>(p, bt, s, batch, g, sweep, beta);
+                    if (npacked) {
+                        fetch_desc<kSlots>(bt, nbatch, slot, nxt);
+                        if (same_class) fetch_static(p, nxt);
+                    }
+                } else {
+                    packed_step<T, BW, BLOCK>(
+                        p, s, cur, g, sweep, beta,
+                        [&] {
+                            if (npacked) fetch_desc<kSlots>(bt, nbatch, slot, nxt);
+                        },
+                        [&] {
+                            if (npacked && same_class) fetch_static(p, nxt);
+                        });
+                }
+                cur = nxt;
+                pending = npacked && !same_class;
             }
             target += unsigned(team_size);
-            if (!team_barrier(bar, target, s.err)) return;
+            if (!team_barrier(bar, target, s.err, prefetch_pending)) return;
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
             const int tasks = p.n_rays * my_groups;
@@ -444,7 +661,7 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 ray_recompute<T, BW>(p, s, t - gi * p.n_rays, team + gi * kTeams, b);
             }
             target += unsigned(team_size);
-            if (!team_barrier(bar, target, s.err)) return;
+            if (!team_barrier(bar, target, s.err, prefetch_pending)) return;
         }
     }
 }
@@ -724,16 +941,18 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
             void* args[] = {const_cast<DevProblem*>(&p), const_cast<DevBatches*>(&bt), &s, &sb, &se, &recompute_every,
                             &team_size};
             return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock>),
-                                              dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), args, 0, st);
+                                              dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), args,
+                                              unsigned(sweep_table_bytes(p.n_classes)), st);
         };
         return precision == 1 ? go(0.0f) : go(0.0);
     });
 }
 
-int ray_sweep_blocks_per_cu() {
+int ray_sweep_blocks_per_cu(size_t table_bytes) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&ray_team_kernel<double, 16, kSweepBlock>), kSweepBlock, 0) !=
+            &per_cu, reinterpret_cast<const void*>(&ray_team_kernel<double, 16, kSweepBlock>), kSweepBlock,
+            table_bytes) !=
         hipSuccess)
         return 0;
     return per_cu;
@@ -814,6 +1033,19 @@ hipError_t launch_int_unpermute(const DevGraph& g, int R, const int8_t* s_state,
 
 hipError_t launch_int_energies(const DevGraph& g, const uint8_t* X, int K, long long* out, hipStream_t st) {
     int_energy_kernel<<<unsigned(K), 256, 0, st>>>(g, X, out);
+    return hipGetLastError();
+}
+
+// A plain streaming copy, 16 bytes per lane per step: the reference point for HBM byte
+// counters and for the bandwidth the chip sustains on a pure read + write stream.
+__global__ void __launch_bounds__(256) copy_probe_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                         size_t n16) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(const void* src, void* dst, size_t n16, hipStream_t st) {
+    copy_probe_kernel<<<2048, 256, 0, st>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
     return hipGetLastError();
 }
 
