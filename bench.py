@@ -46,6 +46,23 @@ def algorithmic_bytes_per_sweep(P, R, res_bytes):
     return structure + state
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ray_team_kernel_muon16_traffic.md")
+
+
+def profiled_traffic():
+    """HBM-side bytes per sweep-kernel launch of this workload at the default settings,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the same
+    counters on a plain 1 GiB copy); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(P, seconds, threads):
     """The oracle's reference trajectory (index order, splitmix64 drawn on uphill moves,
     FP64 residuals) -- anneal_branch restated -- on `threads` host threads, 16 branches,
@@ -142,6 +159,8 @@ def main():
     bytes_launch = (A + H) * algorithmic_bytes_per_sweep(P, R, res_bytes)
     launch_s = float(np.mean(kms)) / 1e3
     achieved = bytes_launch / launch_s / 1e9
+    default_shape = prec == qa.FP32 and k == 50 and args.workgroups == 0 and args.group_width == 0
+    traffic = profiled_traffic() if default_shape else None
     result = {
         "metric": METRIC,
         "value": value,
@@ -166,8 +185,8 @@ def main():
         "latency_ms": {"p50": float(np.percentile(walls, 50) * 1e3), "p99": float(np.percentile(walls, 99) * 1e3),
                        "kind": "one solve per call, sequential"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ray_sweep_kernel", "launch_ms": launch_s * 1e3,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "ray_team_kernel", "launch_ms": launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_launch},
         "result_check": {"not_there": n_found, "not_there_in_void": truth_hit,
                          "fp32_residual_err": out["report"]["fp32_residual_err"],

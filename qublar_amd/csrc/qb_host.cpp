@@ -518,7 +518,7 @@ static int batches_for(qb_problem* P, int bw, int team_size, const DevBatches** 
             }
         };
         cls_heavy[size_t(c)] = n_heavy;
-        const int room = team_size - heavy_ranks(n_heavy, team_size);
+        const int room = team_size - n_heavy;   // one task per workgroup, multi-pass included
         if (!packable.empty() && room >= 1 && total <= int64_t(room) * slots) {
             // the class fits the team in one round: largest bit first into the least
             // filled of at most `room` batches, so every workgroup gets one batch

@@ -91,12 +91,17 @@ struct DevBatches {
 constexpr int kSweepTableLimit = 96 * 1024;
 inline size_t sweep_table_bytes(int n_classes) { return sizeof(int32_t) * size_t(2 * n_classes + 1); }
 
-// Workgroups of a team reserved for a class's multi-pass batches: up to half the team;
-// the packed batches go round-robin over the others. With none reserved every task
-// of the class goes round-robin over the whole team.
+// Workgroups of a team reserved for a class's multi-pass batches, the packed batches
+// going round-robin over the others; with none reserved every task of the class,
+// multi-pass first, goes round-robin over the whole team. Reserving pays only where
+// a class has more tasks than the team has workgroups and a multi-pass batch costs
+// about one packed batch per reserved workgroup's share -- not the case for the sweep
+// as measured -- so none are reserved: a class that fits the team gets one task per
+// workgroup either way (its packing leaves room for the multi-pass batches).
 __host__ __device__ inline int heavy_ranks(int heavy_tasks, int team_size) {
-    const int half = team_size / 2;
-    return heavy_tasks < half ? heavy_tasks : half;
+    (void)heavy_tasks;
+    (void)team_size;
+    return 0;
 }
 
 // Mutable state of one anneal: n_groups groups of BW branches (BW in {2, 4, 8, 16}).

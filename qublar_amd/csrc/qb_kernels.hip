@@ -477,6 +477,143 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
     __syncthreads();
 }
 
+// A bit with more chains than a workgroup has slots, up to kHeavyPasses chains per
+// slot: its column is staged in LDS by the whole workgroup (kStage coalesced loads
+// in flight per thread), then every slot issues the residual loads of all its chains
+// at once, sums each chain, and after the decision scatters them from registers. A
+// step of such a bit costs the staging plus one round of residual loads, where the
+// pass-by-pass form costs two dependent round trips per pass and again per pass to
+// scatter.
+constexpr int kHeavyPasses = 3;
+
+template <typename T, int BW, int BLOCK>
+__device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevState<T>& s, int pos, int k0, int k1,
+                                                 int g, int sweep, double beta) {
+    constexpr int kSlots = BLOCK / BW;
+    constexpr int kMaxEnt = kHeavyPasses * kSlots * kSeg;
+    constexpr int kStage = 8;
+    __shared__ RayEnt hent[kMaxEnt];
+    __shared__ double hpart[kHeavyPasses * kSlots][BW];
+    __shared__ int hacc[BW];
+    const int tid = int(threadIdx.x);
+    const int b = tid % BW, slot = tid / BW;
+    uint16_t* xg = s.x + size_t(g) * size_t(p.n);
+    T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
+    const int deg = k1 - k0;                       // kSlots * kSeg < deg <= kMaxEnt
+    const int n_chain = (deg + kSeg - 1) / kSeg;
+    const uint16_t xm = ld_l2(xg + pos);
+    const int n0 = p.nb_ptr[pos], n1 = p.nb_ptr[pos + 1];
+    // stage the column
+    for (int base = 0; base < deg; base += BLOCK * kStage) {
+        RayEnt tmp[kStage];
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) tmp[u] = p.ent[k0 + min(base + u * BLOCK + tid, deg - 1)];
+#pragma unroll
+        for (int u = 0; u < kStage; ++u)
+            if (base + u * BLOCK + tid < deg) hent[base + u * BLOCK + tid] = tmp[u];
+    }
+    __syncthreads();
+    const int xb = (xm >> b) & 1;
+    const double delta = xb ? -1.0 : 1.0;
+    // the residuals of every chain of this slot, all in flight together
+    T rv[kHeavyPasses][kSeg];
+#pragma unroll
+    for (int q = 0; q < kHeavyPasses; ++q) {
+        const int j = slot + q * kSlots;
+        const int len = j < n_chain ? min(kSeg, deg - j * kSeg) : 0;
+        const RayEnt* e = hent + j * kSeg;
+#pragma unroll
+        for (int u = 0; u < kSeg; ++u) rv[q][u] = T(0);
+        if (len > 0) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rv[q][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+        }
+        if (len > 4) {
+#pragma unroll
+            for (int u = 4; u < 8; ++u) rv[q][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+        }
+        if (len > 8) {
+#pragma unroll
+            for (int u = 8; u < 12; ++u) rv[q][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+        }
+        if (len > 12) {
+#pragma unroll
+            for (int u = 12; u < 16; ++u) rv[q][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+        }
+    }
+    double u01 = 0.0;
+    if (slot == 0) u01 = draw_u01(s.seed_offset + uint64_t(g * BW + b) + 1u, uint32_t(sweep), uint32_t(p.pos_bit[pos]));
+    const double lin = p.lin[pos];
+#pragma unroll
+    for (int q = 0; q < kHeavyPasses; ++q) {
+        const int j = slot + q * kSlots;
+        if (j < n_chain) {
+            const int len = min(kSeg, deg - j * kSeg);
+            RayEnt e[kSeg];
+#pragma unroll
+            for (int u = 0; u < kSeg; ++u) e[u] = hent[j * kSeg + min(u, len - 1)];
+            hpart[j][b] = chain_sum<T>(e, rv[q], len, delta, lin * delta, j == 0);
+        }
+    }
+    __syncthreads();
+    if (slot == 0) {
+        double dE = hpart[0][b];
+        for (int q0 = 1; q0 < n_chain; q0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = hpart[min(q0 + u, kHeavyPasses * kSlots - 1)][b];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) dE = q0 + u < n_chain ? dE + v[u] : dE;
+        }
+        dE = neighbour_terms(p, xg, n0, n1, b, xb, dE);
+        const bool acc = decide_drawn<T>(s, g * BW + b, u01, beta, dE);
+        hacc[b] = acc ? 1 : 0;
+        const uint16_t flips = segment_flips<BW>(acc);
+        if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
+    }
+    __syncthreads();
+    if (hacc[b]) {
+        if (p.seq_update[pos]) {
+            // a ray repeats in this column: its updates must land in CSR order
+            if (slot == 0)
+                for (int k = 0; k < deg; ++k) {
+                    const RayEnt e = hent[k];
+                    T* q = rg + size_t(e.ray) * BW + b;
+                    *q = residual_add<T>(ld_l2(q), e.a, delta);
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kHeavyPasses; ++q) {
+                const int j = slot + q * kSlots;
+                if (j < n_chain) {
+                    const int len = min(kSeg, deg - j * kSeg);
+#pragma unroll
+                    for (int u = 0; u < kSeg; ++u)
+                        if (u < len) {
+                            const RayEnt e = hent[j * kSeg + u];
+                            rg[size_t(e.ray) * BW + b] = residual_add<T>(rv[q][u], e.a, delta);
+                        }
+                }
+            }
+        }
+    }
+    __syncthreads();   // hent, hpart and hacc are reused by the next heavy step
+}
+
+// Dispatch of a multi-pass batch: staged in LDS when its chains fit kHeavyPasses
+// rounds of the workgroup's slots, else pass by pass.
+template <typename T, int BW, int BLOCK>
+__device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T>& s, int pos, int g, int sweep,
+                                           double beta) {
+    constexpr int kSlots = BLOCK / BW;
+    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
+    if (k1 - k0 <= kHeavyPasses * kSlots * kSeg)
+        heavy_lds_update<T, BW, BLOCK>(p, s, pos, k0, k1, g, sweep, beta);
+    else
+        heavy_update<T, BW, BLOCK>(p, s, pos, g, sweep, beta);
+}
+
 // Residuals of ray `ray` for branch lane b, from scratch: -d + sum of a over the
 // ray's set bits in increasing original bit index (binary_energy's order).
 template <typename T, int BW>
@@ -585,7 +722,11 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
         g = team + gi * kTeams;
         return cls[c] + q.nh + (pk - gi * q.np);
     };
-    // The lookahead cursor (ls, lc, lk): the task after the one being processed.
+    // Two lookahead cursors: (s1, c1, k1) is the task after the one being processed,
+    // (s2, c2, k2) the one after that. A packed step issues its state loads, then the
+    // static operands of task 1 (whose descriptor it fetched a step earlier) and the
+    // descriptor of task 2; by the end of the step they have landed, so neither the
+    // drain before the barrier nor the next step waits for them.
     auto advance = [&](int& sw, int& c, int& k) {
         ++k;
         while (k >= n_tasks(c)) {
@@ -597,62 +738,62 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
         }
         return true;
     };
-    int ls = sweep_begin, lc = 0, lt = -1;
-    bool ahead = advance(ls, lc, lt);
-    // the task at the cursor: its batch, group and kind
-    int cb = 0, cg = 0;
-    bool ch = false;
-    SlotStatic cur;
+    int s1 = sweep_begin, c1 = 0, k1 = -1;
+    bool a1 = advance(s1, c1, k1);
+    int s2 = s1, c2 = c1, k2 = k1;
+    bool a2 = a1 && advance(s2, c2, k2);
+    int b1 = 0, g1 = 0, b2 = 0, g2 = 0;
+    bool h1 = false, h2 = false;
+    if (a1) b1 = task_of(c1, k1, g1, h1);
+    if (a2) b2 = task_of(c2, k2, g2, h2);
+    SlotStatic cur, nxt;
     cur.d = SlotDesc{-1, 0, 0, 0u};
-    bool pending = false;   // cur holds a descriptor whose other operands are not yet fetched
-    if (ahead) {
-        cb = task_of(lc, lt, cg, ch);
-        if (!ch) {
-            fetch_desc<kSlots>(bt, cb, slot, cur);
-            fetch_static(p, cur);
-        }
+    nxt.d = SlotDesc{-1, 0, 0, 0u};
+    if (a1 && !h1) {
+        fetch_desc<kSlots>(bt, b1, slot, cur);
+        fetch_static(p, cur);
     }
-    auto prefetch_pending = [&] {
-        if (pending) {
-            fetch_static(p, cur);
-            pending = false;
-        }
-    };
+    if (a2 && !h2) fetch_desc<kSlots>(bt, b2, slot, nxt);
     unsigned target = 0;
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         const double beta = s.beta[sweep];
         for (int c = 0; c < p.n_classes; ++c) {
-            while (ahead && ls == sweep && lc == c) {
-                const int batch = cb, g = cg;
-                const bool heavy = ch;
-                ahead = advance(ls, lc, lt);
-                if (ahead) cb = task_of(lc, lt, cg, ch);
-                const int nbatch = cb;
-                const bool npacked = ahead && !ch;
-                const bool same_class = ahead && ls == sweep && lc == c;
-                SlotStatic nxt;
-                nxt.d = SlotDesc{-1, 0, 0, 0u};
+            while (a1 && s1 == sweep && c1 == c) {
+                const int batch = b1, g = g1;
+                const bool heavy = h1;
+                // shift the cursors: the next task is the old task 2
+                s1 = s2;
+                c1 = c2;
+                k1 = k2;
+                a1 = a2;
+                b1 = b2;
+                g1 = g2;
+                h1 = h2;
+                if (a2) {
+                    a2 = advance(s2, c2, k2);
+                    if (a2) b2 = task_of(c2, k2, g2, h2);
+                }
+                const bool fetch1 = a1 && !h1, fetch2 = a2 && !h2;
+                SlotStatic nn;
+                nn.d = SlotDesc{-1, 0, 0, 0u};
                 if (heavy) {
-                    heavy_update<T, BW, BLOCK>(p, s, bt.pos[bt.first[batch]], g, sweep, beta);
-                    if (npacked) {
-                        fetch_desc<kSlots>(bt, nbatch, slot, nxt);
-                        if (same_class) fetch_static(p, nxt);
-                    }
+                    if (fetch1) fetch_static(p, nxt);
+                    if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
+                    heavy_step<T, BW, BLOCK>(p, s, bt.pos[bt.first[batch]], g, sweep, beta);
                 } else {
                     packed_step<T, BW, BLOCK>(
                         p, s, cur, g, sweep, beta,
                         [&] {
-                            if (npacked) fetch_desc<kSlots>(bt, nbatch, slot, nxt);
+                            if (fetch1) fetch_static(p, nxt);
+                            if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
                         },
-                        [&] {
-                            if (npacked && same_class) fetch_static(p, nxt);
-                        });
+                        [] {});
                 }
                 cur = nxt;
-                pending = npacked && !same_class;
+                nxt.d = nn.d;
             }
             target += unsigned(team_size);
-            if (!team_barrier(bar, target, s.err, prefetch_pending)) return;
+            if (!team_barrier(bar, target, s.err)) return;
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
             const int tasks = p.n_rays * my_groups;
@@ -661,7 +802,7 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 ray_recompute<T, BW>(p, s, t - gi * p.n_rays, team + gi * kTeams, b);
             }
             target += unsigned(team_size);
-            if (!team_barrier(bar, target, s.err, prefetch_pending)) return;
+            if (!team_barrier(bar, target, s.err)) return;
         }
     }
 }

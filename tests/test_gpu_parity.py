@@ -200,6 +200,41 @@ def test_ray_synthetic_heavy_and_duplicate_columns(gpu, precision):
     check_branches(P, qa.schedule(8.0, 0.3, 40, 10), 16, precision=precision, recompute_every=6)
 
 
+def very_heavy_problem(seed=3):
+    """Light bits plus a few columns longer than a workgroup's slots hold at group
+    width 2 (> 2048 entries): three staged in LDS (2100, 2600, 4400 entries, one with
+    a repeated ray) and one beyond it, evaluated pass by pass (6500 entries)."""
+    rng = np.random.default_rng(seed)
+    n, n_rays = 120, 8000
+    heavy = {5: 2100, 40: 2600, 77: 4400, 101: 6500}
+    row_ptr, ray, a = [0], [], []
+    for i in range(n):
+        k = heavy.get(i, int(rng.integers(1, 40)))
+        rs = list(rng.choice(n_rays, size=k, replace=False))
+        if i == 40:
+            rs[-1] = rs[3]
+        for b in rs:
+            ray.append(b)
+            a.append(rng.uniform(0.001, 0.02))
+        row_ptr.append(len(ray))
+    side = 11
+    nbr_ptr, nbr = [0], []
+    for i in range(n):
+        for j in (i - 1, i + 1, i - side, i + side):
+            if 0 <= j < n and j != i and abs(j % side - i % side) <= 1:
+                nbr.append(j)
+        nbr_ptr.append(len(nbr))
+    d = rng.uniform(0.0, 0.3, n_rays)
+    w = rng.uniform(2.0, 30.0, n_rays)
+    return qa.ProblemArrays(row_ptr, ray, a, d, w, nbr_ptr, nbr, 0.9, 0.4)
+
+
+@pytest.mark.parametrize("precision", [qa.FP64, qa.FP32])
+def test_ray_very_heavy_columns(gpu, precision):
+    P = very_heavy_problem()
+    check_branches(P, qa.schedule(6.0, 0.2, 30, 6), 16, precision=precision, recompute_every=7)
+
+
 def test_energy_of_arbitrary_states(gpu):
     P = synthetic_problem(seed=4)
     prob = qa.Problem(P)
