@@ -112,7 +112,7 @@ struct DevState {
     uint16_t* x;            // [g][n] spin masks, bit b = branch g*BW + b
     T* r;                   // [g][n_rays][BW] residuals
     const double* beta;     // 1 / T per sweep
-    unsigned* bar;          // [0]: grid barrier; [64 + 64 t]: team t's barrier (monotone)
+    unsigned* bar;          // [0]: grid barrier; [64 + 64 t]: team t's arrivals, [96 + 64 t] its release word
     unsigned* tickets;      // [64 t]: team t's member count
     int* err;               // 1: a barrier wait timed out; 2: a team came up short
 };

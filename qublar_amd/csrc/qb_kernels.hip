@@ -91,30 +91,36 @@ __device__ __forceinline__ V ld_l2(const V* p) {
     return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Team barrier: monotone counter in L2; the k-th barrier waits for k * team size
-// arrivals. Every wave drains its stores, thread 0 arrives, and then -- while thread 0
-// polls -- every thread runs `after_arrival`, which issues loads of static data for
-// the next step (they may not be waited for before the arrival: the drain would
-// wait for them too). The error flag is read only while polling, so an arrival
-// costs one atomic; a wait longer than ~2 s sets *err, as in grid_barrier.
+// Team barrier: a monotone arrival counter in L2, bar[0], and a release word on its
+// own 128-byte line, bar[32]; the k-th barrier completes at k * team size arrivals.
+// Every wave drains its stores, then thread 0 arrives with one atomic; the workgroup
+// whose arrival completes the count writes the release word, and the others poll that
+// word, not the counter line the atomics keep busy. While thread 0 polls, every thread
+// runs `after_arrival` (loads that must not be waited for before the arrival, or the
+// drain would wait for them too). The error flag is read only while polling; a wait
+// longer than ~2 s sets *err, as in grid_barrier.
 template <typename F>
 __device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int* err, F&& after_arrival) {
     __shared__ int s_ok;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // the first poll is issued before the hook's loads: loads return in order, so the
-    // last workgroup to arrive sees its own arrival without waiting for the prefetch
     unsigned seen = 0;
+    unsigned* release = bar + 32;
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        seen = ld_l2(bar);
+        const unsigned before = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (before + 1u == target) {
+            __hip_atomic_store(release, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            seen = target;
+        } else {
+            seen = ld_l2(release);
+        }
     }
     after_arrival();
     if (threadIdx.x == 0) {
         int ok = 1;
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
         unsigned polls = 0;
-        while (seen < target && (seen = ld_l2(bar)) < target) {
+        while (seen < target && (seen = ld_l2(release)) < target) {
             if ((++polls & 1023u) == 0) {
                 if (ld_l2(err) != 0 || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -723,10 +729,12 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
         return cls[c] + q.nh + (pk - gi * q.np);
     };
     // Two lookahead cursors: (s1, c1, k1) is the task after the one being processed,
-    // (s2, c2, k2) the one after that. A packed step issues its state loads, then the
-    // static operands of task 1 (whose descriptor it fetched a step earlier) and the
-    // descriptor of task 2; by the end of the step they have landed, so neither the
-    // drain before the barrier nor the next step waits for them.
+    // (s2, c2, k2) the one after that. Once a packed step's state loads have returned
+    // and its chain sums are made, it issues the static operands of task 1 (whose
+    // descriptor it fetched a step earlier) and the descriptor of task 2; they land
+    // while the step combines, decides and scatters, so neither the drain before the
+    // barrier nor the next step waits long for them, and they do not compete with the
+    // step's own state loads.
     auto advance = [&](int& sw, int& c, int& k) {
         ++k;
         while (k >= n_tasks(c)) {
@@ -783,11 +791,11 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 } else {
                     packed_step<T, BW, BLOCK>(
                         p, s, cur, g, sweep, beta,
+                        [] {},
                         [&] {
                             if (fetch1) fetch_static(p, nxt);
                             if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
-                        },
-                        [] {});
+                        });
                 }
                 cur = nxt;
                 nxt.d = nn.d;
