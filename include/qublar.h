@@ -74,6 +74,13 @@ typedef struct {
     const int32_t* nbr;
     double lambda, kappa;
     const double* field;      /* n_vars, or NULL: lin(i) = kappa */
+    /* No counterpart in the reference (both may be NULL). The draw of bit i in sweep s
+       for branch b is keyed by seed b + 1 + seed_offset + draw_key[i] and counter
+       (draw_bit[i], s); NULL means draw_key = 0 and draw_bit[i] = i. A problem
+       assembled from independent requests (block-diagonal, each request's bits keeping
+       its own local index and key) thereby anneals every request exactly as alone. */
+    const int32_t* draw_bit;  /* n_vars */
+    const uint64_t* draw_key; /* n_vars */
 } qb_problem_desc;
 
 typedef struct qb_problem qb_problem;

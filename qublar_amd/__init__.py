@@ -43,7 +43,8 @@ class ProblemDesc(C.Structure):
                 ("row_ptr", C.c_void_p), ("ray", C.c_void_p), ("a", C.c_void_p),
                 ("d", C.c_void_p), ("w", C.c_void_p),
                 ("nbr_ptr", C.c_void_p), ("nbr", C.c_void_p),
-                ("lam", C.c_double), ("kappa", C.c_double), ("field", C.c_void_p)]
+                ("lam", C.c_double), ("kappa", C.c_double), ("field", C.c_void_p),
+                ("draw_bit", C.c_void_p), ("draw_key", C.c_void_p)]
 
 
 class ProblemInfo(C.Structure):
@@ -156,6 +157,41 @@ def device_info():
                 hbm_bytes=d.hbm_bytes, lds_bytes_per_cu=d.lds_bytes_per_cu)
 
 
+def union(problems, keys):
+    """The block-diagonal union of independent problems (one lambda for all): bits,
+    rays and neighbours of problem q follow those of problems 0 .. q-1, each bit keeps
+    its local index as its draw counter and problem q's key as its draw key, and every
+    bit's linear term is its problem's field (or kappa). Annealed with seed_offset 0,
+    problem q's branch b in the union is, spin for spin, its branch b annealed alone
+    with seed_offset keys[q]. Returns the union and each problem's (bit, ray) offsets."""
+    lam = problems[0].lam
+    if any(P.lam != lam for P in problems):
+        raise ValueError("union: every problem must have the same lambda")
+    row_ptr, ray, a, d, w, nbr_ptr, nbr, field, dbit, dkey, offs = [np.zeros(1, np.int32)], [], [], [], [], \
+        [np.zeros(1, np.int32)], [], [], [], [], []
+    bit0 = ray0 = nnz0 = nn0 = 0
+    for P, key in zip(problems, keys):
+        offs.append((bit0, ray0))
+        row_ptr.append(P.row_ptr[1:] + nnz0)
+        ray.append(P.ray + ray0)
+        a.append(P.a)
+        d.append(P.d)
+        w.append(P.w)
+        nbr_ptr.append(P.nbr_ptr[1:] + nn0)
+        nbr.append(P.nbr + bit0)
+        field.append(P.field if P.field is not None else np.full(P.n_vars, P.kappa))
+        dbit.append(P.draw_bit if P.draw_bit is not None else np.arange(P.n_vars, dtype=np.int32))
+        dkey.append((P.draw_key if P.draw_key is not None else np.zeros(P.n_vars, np.uint64)) + np.uint64(key))
+        bit0 += P.n_vars
+        ray0 += P.n_rays
+        nnz0 += int(P.row_ptr[-1])
+        nn0 += int(P.nbr_ptr[-1])
+    cat = np.concatenate
+    U = ProblemArrays(cat(row_ptr), cat(ray), cat(a), cat(d), cat(w), cat(nbr_ptr), cat(nbr), lam, 0.0, cat(field),
+                      draw_bit=cat(dbit), draw_key=cat(dkey))
+    return U, offs
+
+
 def copy_probe(nbytes, reps=10):
     """Mean ms of a plain device copy of `nbytes` (reads and writes nbytes each)."""
     ms = C.c_double()
@@ -170,7 +206,8 @@ def schedule(t_hot, t_cold, anneal, hold):
 class ProblemArrays:
     """A BinaryProblem held as contiguous numpy arrays (src/engine.hpp:30-49)."""
 
-    def __init__(self, row_ptr, ray, a, d, w, nbr_ptr, nbr, lam, kappa, field=None, truth=None):
+    def __init__(self, row_ptr, ray, a, d, w, nbr_ptr, nbr, lam, kappa, field=None, truth=None, draw_bit=None,
+                 draw_key=None):
         self.row_ptr = np.ascontiguousarray(row_ptr, np.int32)
         self.ray = np.ascontiguousarray(ray, np.int32)
         self.a = np.ascontiguousarray(a, np.float32)
@@ -182,12 +219,15 @@ class ProblemArrays:
         self.kappa = float(kappa)
         self.field = None if field is None else np.ascontiguousarray(field, np.float64)
         self.truth = truth
+        self.draw_bit = None if draw_bit is None else np.ascontiguousarray(draw_bit, np.int32)
+        self.draw_key = None if draw_key is None else np.ascontiguousarray(draw_key, np.uint64)
         self.n_vars = len(self.row_ptr) - 1
         self.n_rays = len(self.d)
 
     def desc(self):
         return ProblemDesc(self.n_vars, self.n_rays, _p(self.row_ptr), _p(self.ray), _p(self.a), _p(self.d),
-                           _p(self.w), _p(self.nbr_ptr), _p(self.nbr), self.lam, self.kappa, _p(self.field))
+                           _p(self.w), _p(self.nbr_ptr), _p(self.nbr), self.lam, self.kappa, _p(self.field),
+                           _p(self.draw_bit), _p(self.draw_key))
 
     def kwargs(self):
         return dict(row_ptr=self.row_ptr, ray=self.ray, a=self.a, d=self.d, w=self.w, nbr_ptr=self.nbr_ptr,

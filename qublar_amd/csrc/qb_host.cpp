@@ -193,7 +193,7 @@ struct qb_problem {
     int max_class = 0, min_class = 0, max_entries = 0, max_bits_per_ray = 0;
     int64_t device_bytes = 0;
     std::vector<int32_t> pos_bit, cls_ptr;
-    DevBuf b_cls, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq;
+    DevBuf b_cls, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq, b_dbit, b_dkey;
     std::vector<int32_t> col_ptr, nb_ptr;   // host copies, position space
     std::vector<uint8_t> seq_update;
     struct Batches {
@@ -228,6 +228,9 @@ static int validate_problem(const qb_problem_desc* d) {
         if (d->ray[k] < 0 || d->ray[k] >= d->n_rays) return fail(QB_E_ARG, "entry %lld names ray %d of %d", (long long)k, d->ray[k], d->n_rays);
     for (int64_t k = 0; k < nn; ++k)
         if (d->nbr[k] < 0 || d->nbr[k] >= d->n_vars) return fail(QB_E_ARG, "neighbour entry %lld names bit %d of %d", (long long)k, d->nbr[k], d->n_vars);
+    if (d->draw_bit)
+        for (int i = 0; i < d->n_vars; ++i)
+            if (d->draw_bit[i] < 0) return fail(QB_E_ARG, "draw_bit[%d] is negative (%d)", i, d->draw_bit[i]);
     return QB_OK;
 }
 
@@ -366,6 +369,14 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
                 seen[size_t(b)] = p;
             }
     }
+    // draw keys in position order
+    std::vector<int32_t> draw_bit(static_cast<size_t>(n));
+    std::vector<uint64_t> draw_key(static_cast<size_t>(n), 0u);
+    for (int p = 0; p < n; ++p) {
+        const int i = P->pos_bit[size_t(p)];
+        draw_bit[size_t(p)] = desc->draw_bit ? desc->draw_bit[i] : i;
+        if (desc->draw_key) draw_key[size_t(p)] = desc->draw_key[i];
+    }
     std::vector<BitEnt> rt(static_cast<size_t>(nnz));
     for (size_t q = 0; q < rt.size(); ++q) rt[q] = BitEnt{pos_of[size_t(rt_bit[q])], rt_a[q]};
     std::vector<double> dv(desc->d, desc->d + nr), wv(desc->w, desc->w + nr);
@@ -376,6 +387,7 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     hipStream_t st = P->stream;
     if ((rc = P->b_cls.upload(P->cls_ptr, st)) ||
         (rc = P->b_seq.upload(seq_update, st)) || (rc = P->b_pos.upload(P->pos_bit, st)) ||
+        (rc = P->b_dbit.upload(draw_bit, st)) || (rc = P->b_dkey.upload(draw_key, st)) ||
         (rc = P->b_col.upload(col_ptr, st)) || (rc = P->b_ent.upload(ent, st)) || (rc = P->b_nbp.upload(nb_ptr, st)) ||
         (rc = P->b_nb.upload(nb, st)) || (rc = P->b_nbbit.upload(nb_bit, st)) || (rc = P->b_lin.upload(lin, st)) ||
         (rc = P->b_d.upload(dv, st)) || (rc = P->b_w.upload(wv, st)) || (rc = P->b_rtp.upload(rt_ptr, st)) ||
@@ -385,7 +397,8 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     P->col_ptr = col_ptr;
     P->nb_ptr = nb_ptr;
     P->seq_update = seq_update;
-    P->device_bytes = int64_t(P->b_cls.bytes + P->b_seq.bytes + P->b_pos.bytes + P->b_col.bytes + P->b_ent.bytes + P->b_nbp.bytes +
+    P->device_bytes = int64_t(P->b_dbit.bytes + P->b_dkey.bytes +
+                              P->b_cls.bytes + P->b_seq.bytes + P->b_pos.bytes + P->b_col.bytes + P->b_ent.bytes + P->b_nbp.bytes +
                               P->b_nb.bytes + P->b_nbbit.bytes + P->b_lin.bytes + P->b_d.bytes + P->b_w.bytes +
                               P->b_rtp.bytes + P->b_rt.bytes);
     DevProblem& D = P->dev;
@@ -396,6 +409,8 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     D.n_nbr = nn;
     D.cls_ptr = P->b_cls.as<int32_t>();
     D.seq_update = P->b_seq.as<uint8_t>();
+    D.draw_bit = P->b_dbit.as<int32_t>();
+    D.draw_key = P->b_dkey.as<uint64_t>();
     D.pos_bit = P->b_pos.as<int32_t>();
     D.col_ptr = P->b_col.as<int32_t>();
     D.ent = P->b_ent.as<RayEnt>();

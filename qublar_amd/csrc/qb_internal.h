@@ -52,6 +52,8 @@ struct DevProblem {
     const int32_t* rt_ptr;
     const BitEnt* rt;
     const uint8_t* seq_update;  // 1 where a ray repeats in the column: scatter in CSR order
+    const int32_t* draw_bit;    // the counter index of each position's draws
+    const uint64_t* draw_key;   // added to the branch seed of each position's draws
     double lambda;
 };
 

@@ -221,14 +221,8 @@ __device__ __forceinline__ double neighbour_terms(const DevProblem& p, const uin
     return acc;
 }
 
-template <typename T>
-__device__ __forceinline__ bool decide(const DevState<T>& s, int bit, int branch, int sweep, double beta, double dE) {
-    if (branch >= s.R) return false;
-    return dE <= 0.0 ||
-           metropolis_uphill(dE, beta, s.seed_offset + uint64_t(branch) + 1u, uint32_t(sweep), uint32_t(bit));
-}
-
-// The same test with the branch's draw u = draw_u01(seed, sweep, bit) already made.
+// The Metropolis test of one branch with its draw u = draw_u01(seed, sweep, bit)
+// already made: accept iff dE <= 0 or u < qexp(-dE * beta).
 template <typename T>
 __device__ __forceinline__ bool decide_drawn(const DevState<T>& s, int branch, double u, double beta, double dE) {
     if (branch >= s.R) return false;
@@ -258,7 +252,8 @@ struct SlotStatic {
     RayEnt e[kSeg];
     int32_t q[kNb];
     double lin;
-    int32_t bit;
+    int32_t bit;          // draw counter index
+    uint64_t key;         // draw key offset
 };
 
 __device__ __forceinline__ int slot_len(uint32_t i) { return int(i & 31u); }
@@ -302,7 +297,8 @@ __device__ __forceinline__ void fetch_static(const DevProblem& p, SlotStatic& S)
             for (int u = 0; u < kNb; ++u) S.q[u] = p.nb[min(S.d.nb0 + u, last)];
         }
         S.lin = p.lin[S.d.pos];
-        S.bit = p.pos_bit[S.d.pos];
+        S.bit = p.draw_bit[S.d.pos];
+        S.key = p.draw_key[S.d.pos];
     }
 }
 
@@ -356,7 +352,8 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
     after_issue();
     // the draw depends only on (branch, sweep, bit): made while the loads are in flight
     double u = 0.0;
-    if (pos >= 0 && j == 0) u = draw_u01(s.seed_offset + uint64_t(g * BW + b) + 1u, uint32_t(sweep), uint32_t(S.bit));
+    if (pos >= 0 && j == 0)
+        u = draw_u01(s.seed_offset + S.key + uint64_t(g * BW + b) + 1u, uint32_t(sweep), uint32_t(S.bit));
     const int xb = (xm >> b) & 1;
     const double delta = xb ? -1.0 : 1.0;
     if (pos >= 0) {
@@ -448,7 +445,9 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
     }
     if (slot == 0) {
         dE = neighbour_terms(p, xg, p.nb_ptr[pos], p.nb_ptr[pos + 1], b, xb, dE);
-        const bool acc = decide<T>(s, p.pos_bit[pos], g * BW + b, sweep, beta, dE);
+        const double u = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(sweep),
+                                  uint32_t(p.draw_bit[pos]));
+        const bool acc = decide_drawn<T>(s, g * BW + b, u, beta, dE);
         accepted[b] = acc ? 1 : 0;
         const uint16_t flips = segment_flips<BW>(acc);
         if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
@@ -548,7 +547,9 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
         }
     }
     double u01 = 0.0;
-    if (slot == 0) u01 = draw_u01(s.seed_offset + uint64_t(g * BW + b) + 1u, uint32_t(sweep), uint32_t(p.pos_bit[pos]));
+    if (slot == 0)
+        u01 = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(sweep),
+                       uint32_t(p.draw_bit[pos]));
     const double lin = p.lin[pos];
 #pragma unroll
     for (int q = 0; q < kHeavyPasses; ++q) {

@@ -1,0 +1,38 @@
+"""Host side of batched serving: the union of independent requests colours each request
+exactly as it is coloured alone (the precondition for identical trajectories), and the
+union's arrays are the requests' arrays with offsets."""
+import numpy as np
+
+import qublar_amd as qa
+
+
+def colours(P):
+    order, cp = P.colouring()
+    col = np.empty(P.n_vars, np.int64)
+    for c in range(len(cp) - 1):
+        col[order[cp[c]:cp[c + 1]]] = c
+    return col
+
+
+def test_union_colouring_equals_each_request():
+    shapes = [dict(grid=16, sky_az=30, sky_th=10, log2_muons=20, seed=3),
+              dict(grid=22, sky_az=45, sky_th=12, log2_muons=21, seed=8),
+              dict(grid=18, sky_az=36, sky_th=9, log2_muons=20, seed=11)]
+    probs = [qa.muon_scene(**s) for s in shapes]
+    U, offs = qa.union(probs, [5, 17, 99])
+    cu = colours(U)
+    for P, (b0, r0) in zip(probs, offs):
+        assert np.array_equal(cu[b0:b0 + P.n_vars], colours(P))
+
+
+def test_union_arrays():
+    probs = [qa.muon_scene(grid=16, sky_az=30, sky_th=10, log2_muons=20, seed=s) for s in (1, 2)]
+    U, offs = qa.union(probs, [7, 9])
+    assert U.n_vars == sum(P.n_vars for P in probs) and U.n_rays == sum(P.n_rays for P in probs)
+    b1, r1 = offs[1]
+    P = probs[1]
+    k0 = int(U.row_ptr[b1])
+    assert np.array_equal(U.ray[k0:k0 + int(P.row_ptr[-1])], P.ray + r1)
+    assert np.array_equal(U.draw_bit[b1:], np.arange(P.n_vars))
+    assert np.all(U.draw_key[:b1] == 7) and np.all(U.draw_key[b1:] == 9)
+    assert np.allclose(U.field, P.kappa)
