@@ -92,6 +92,59 @@ def cpu_baseline(P, seconds, threads):
                 single_thread_value=single, host_cpus=os.cpu_count())
 
 
+def serving_pool(n, seed):
+    """n distinct ray x voxel requests of the serving stream: voxels log-uniform in
+    [2k, 300k] (pyramid grid 24..128), rays log-uniform in [1k, 75k] (sky bins), the
+    same kappa / lambda prior, each generated from its own seed."""
+    import qublar_amd as qa
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    for i in range(n):
+        voxels = float(np.exp(rng.uniform(np.log(2e3), np.log(3e5))))
+        rays = float(np.exp(rng.uniform(np.log(1e3), np.log(7.5e4))))
+        grid = int(np.clip(round((voxels / 0.146) ** (1.0 / 3.0)), 24, 128))
+        th = int(np.clip(round(np.sqrt(rays / 15.0)), 4, 70))
+        pool.append(qa.muon_scene(grid=grid, sky_az=min(360, 5 * th), sky_th=th, log2_muons=24, seed=1000 + i))
+    return pool
+
+
+def run_serving(args):
+    """Poisson arrivals of ray x voxel requests (8-16 branches, annealed to T = 1) for
+    --serve-seconds, served batched (unions) and then one solve per launch; prints one
+    JSON line per mode with p50 / p99 latency and the served spin-update rate."""
+    import qublar_amd as qa
+    from qublar_amd.serving import Request, Server, latency_summary
+
+    pool = serving_pool(args.pool, 7)
+    log("pool sizes:", sorted(P.n_vars for P in pool))
+    sched = qa.schedule(50.0, 1.0, 200, 50)
+    rng = np.random.default_rng(11)
+    for batched in (True, False):
+        t, reqs = 0.0, []
+        while True:
+            t += rng.exponential(1.0 / args.rate)
+            if t > args.serve_seconds:
+                break
+            P = pool[int(rng.integers(len(pool)))]
+            reqs.append(Request(P, int(rng.integers(8, 17)), 100 + len(reqs), t))
+        srv = Server(sched, qa.FP32, 50)
+        t0 = time.perf_counter()
+        reports = srv.run(reqs, batched=batched)
+        wall = time.perf_counter() - t0
+        updates = sum(q.branches * 250 * q.problem.n_vars for q in reqs)
+        sizes = np.bincount([q.batch for q in reqs])
+        kms = np.array([r["kernel_ms"] for r in reports])
+        out = {"workload": "serving", "mode": "batched unions" if batched else "one solve per launch",
+               "offered_req_per_s": args.rate, "arrival_window_s": args.serve_seconds, "requests": len(reqs),
+               "batches": len(reports), "requests_per_batch": {"mean": float(sizes.mean()), "max": int(sizes.max())},
+               "sweep_kernel_ms": {"mean": float(kms.mean()), "max": float(kms.max())},
+               "gpu_busy_fraction": float(kms.sum() / 1e3 / wall),
+               "wall_s": wall, "spin_updates_per_s": updates / wall, "latency_ms": latency_summary(reqs),
+               "data": "synthetic muon-tomography QUBOs; each request coloured on arrival on a host thread"}
+        print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +156,11 @@ def main():
     ap.add_argument("--group-width", type=int, default=0, help="branches per group (2, 4, 8, 16); 0 = auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["muon", "serving"], default="muon",
+                    help="muon: the headline line (default); serving: the batched serving stream")
+    ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window")
+    ap.add_argument("--pool", type=int, default=24, help="serving: distinct request problems")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain 1 GiB device copy (HBM counter calibration) and print its rate")
     args = ap.parse_args()
@@ -115,6 +173,9 @@ def main():
 
     info = qa.device_info()
     log("device", info)
+    if args.workload == "serving":
+        run_serving(args)
+        return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5
         ms = qa.copy_probe(nbytes, reps)

@@ -81,6 +81,12 @@ typedef struct {
        its own local index and key) thereby anneals every request exactly as alone. */
     const int32_t* draw_bit;  /* n_vars */
     const uint64_t* draw_key; /* n_vars */
+    /* No counterpart in the reference (may be NULL). A colouring to use instead of the
+       engine's own: colour[i] >= 0, no two bits that share a ray or are neighbours
+       with the same colour (checked; QB_E_ARG otherwise). Classes are visited in
+       increasing colour, unused colours skipped. The union of independently coloured
+       requests is coloured by their own colourings side by side. */
+    const int32_t* colour;    /* n_vars */
 } qb_problem_desc;
 
 typedef struct qb_problem qb_problem;

@@ -44,7 +44,7 @@ class ProblemDesc(C.Structure):
                 ("d", C.c_void_p), ("w", C.c_void_p),
                 ("nbr_ptr", C.c_void_p), ("nbr", C.c_void_p),
                 ("lam", C.c_double), ("kappa", C.c_double), ("field", C.c_void_p),
-                ("draw_bit", C.c_void_p), ("draw_key", C.c_void_p)]
+                ("draw_bit", C.c_void_p), ("draw_key", C.c_void_p), ("colour", C.c_void_p)]
 
 
 class ProblemInfo(C.Structure):
@@ -157,20 +157,24 @@ def device_info():
                 hbm_bytes=d.hbm_bytes, lds_bytes_per_cu=d.lds_bytes_per_cu)
 
 
-def union(problems, keys):
+def union(problems, keys, colours=None):
     """The block-diagonal union of independent problems (one lambda for all): bits,
     rays and neighbours of problem q follow those of problems 0 .. q-1, each bit keeps
     its local index as its draw counter and problem q's key as its draw key, and every
     bit's linear term is its problem's field (or kappa). Annealed with seed_offset 0,
     problem q's branch b in the union is, spin for spin, its branch b annealed alone
-    with seed_offset keys[q]. Returns the union and each problem's (bit, ray) offsets."""
+    with seed_offset keys[q]. The union is coloured by the problems' own colourings side
+    by side (`colours`, or each problem's colours()), so its classes are theirs.
+    Returns the union and each problem's (bit, ray) offsets."""
     lam = problems[0].lam
     if any(P.lam != lam for P in problems):
         raise ValueError("union: every problem must have the same lambda")
-    row_ptr, ray, a, d, w, nbr_ptr, nbr, field, dbit, dkey, offs = [np.zeros(1, np.int32)], [], [], [], [], \
-        [np.zeros(1, np.int32)], [], [], [], [], []
+    row_ptr, ray, a, d, w, nbr_ptr, nbr, field, dbit, dkey, colour, offs = [np.zeros(1, np.int32)], [], [], [], \
+        [], [np.zeros(1, np.int32)], [], [], [], [], [], []
     bit0 = ray0 = nnz0 = nn0 = 0
-    for P, key in zip(problems, keys):
+    if colours is None:
+        colours = [P.colours() for P in problems]
+    for P, key, col in zip(problems, keys, colours):
         offs.append((bit0, ray0))
         row_ptr.append(P.row_ptr[1:] + nnz0)
         ray.append(P.ray + ray0)
@@ -182,13 +186,14 @@ def union(problems, keys):
         field.append(P.field if P.field is not None else np.full(P.n_vars, P.kappa))
         dbit.append(P.draw_bit if P.draw_bit is not None else np.arange(P.n_vars, dtype=np.int32))
         dkey.append((P.draw_key if P.draw_key is not None else np.zeros(P.n_vars, np.uint64)) + np.uint64(key))
+        colour.append(col)
         bit0 += P.n_vars
         ray0 += P.n_rays
         nnz0 += int(P.row_ptr[-1])
         nn0 += int(P.nbr_ptr[-1])
     cat = np.concatenate
     U = ProblemArrays(cat(row_ptr), cat(ray), cat(a), cat(d), cat(w), cat(nbr_ptr), cat(nbr), lam, 0.0, cat(field),
-                      draw_bit=cat(dbit), draw_key=cat(dkey))
+                      draw_bit=cat(dbit), draw_key=cat(dkey), colour=cat(colour))
     return U, offs
 
 
@@ -207,7 +212,7 @@ class ProblemArrays:
     """A BinaryProblem held as contiguous numpy arrays (src/engine.hpp:30-49)."""
 
     def __init__(self, row_ptr, ray, a, d, w, nbr_ptr, nbr, lam, kappa, field=None, truth=None, draw_bit=None,
-                 draw_key=None):
+                 draw_key=None, colour=None):
         self.row_ptr = np.ascontiguousarray(row_ptr, np.int32)
         self.ray = np.ascontiguousarray(ray, np.int32)
         self.a = np.ascontiguousarray(a, np.float32)
@@ -221,13 +226,24 @@ class ProblemArrays:
         self.truth = truth
         self.draw_bit = None if draw_bit is None else np.ascontiguousarray(draw_bit, np.int32)
         self.draw_key = None if draw_key is None else np.ascontiguousarray(draw_key, np.uint64)
+        self.colour = None if colour is None else np.ascontiguousarray(colour, np.int32)
         self.n_vars = len(self.row_ptr) - 1
         self.n_rays = len(self.d)
 
     def desc(self):
         return ProblemDesc(self.n_vars, self.n_rays, _p(self.row_ptr), _p(self.ray), _p(self.a), _p(self.d),
                            _p(self.w), _p(self.nbr_ptr), _p(self.nbr), self.lam, self.kappa, _p(self.field),
-                           _p(self.draw_bit), _p(self.draw_key))
+                           _p(self.draw_bit), _p(self.draw_key), _p(self.colour))
+
+    def colours(self):
+        """The engine's colour of every bit (the given colouring when there is one)."""
+        if self.colour is not None:
+            return self.colour
+        order, cp = self.colouring()
+        col = np.empty(self.n_vars, np.int32)
+        for c in range(len(cp) - 1):
+            col[order[cp[c]:cp[c + 1]]] = c
+        return col
 
     def kwargs(self):
         return dict(row_ptr=self.row_ptr, ray=self.ray, a=self.a, d=self.d, w=self.w, nbr_ptr=self.nbr_ptr,

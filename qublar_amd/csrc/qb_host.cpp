@@ -237,13 +237,56 @@ static int validate_problem(const qb_problem_desc* d) {
 // Distance-2 colouring of the bit-ray incidence plus the neighbour graph: two bits
 // share a colour only if they share no ray and are not neighbours. Bits are visited
 // by decreasing conflict weight (their rays' lengths plus neighbours), ties by index.
+// A caller's colouring: verified at distance 2 (bits of one ray, neighbours), unused
+// colours dropped (order kept). Returns the number of classes, or -1 with g_err set.
+static int given_colouring(const qb_problem_desc* desc, const std::vector<int32_t>& rt_ptr,
+                           const std::vector<int32_t>& rt_bit, std::vector<int>& colour) {
+    const int n = desc->n_vars, nr = desc->n_rays;
+    int K = 0;
+    for (int i = 0; i < n; ++i) {
+        if (desc->colour[i] < 0) return -fail(QB_E_ARG, "colour[%d] is negative", i);
+        K = std::max(K, desc->colour[i] + 1);
+    }
+    std::vector<int32_t> seen_ray(static_cast<size_t>(K), -1), seen_bit(static_cast<size_t>(K), -1);
+    for (int b = 0; b < nr; ++b)
+        for (int q = rt_ptr[size_t(b)]; q < rt_ptr[size_t(b) + 1]; ++q) {
+            const int i = rt_bit[size_t(q)], c = desc->colour[i];
+            if (seen_ray[size_t(c)] == b && seen_bit[size_t(c)] != i)
+                return -fail(QB_E_ARG, "colour: bits %d and %d share ray %d and colour %d", seen_bit[size_t(c)], i, b, c);
+            seen_ray[size_t(c)] = b;
+            seen_bit[size_t(c)] = i;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int k = desc->nbr_ptr[i]; k < desc->nbr_ptr[i + 1]; ++k)
+            if (desc->nbr[k] != i && desc->colour[desc->nbr[k]] == desc->colour[i])
+                return -fail(QB_E_ARG, "colour: neighbours %d and %d share colour %d", i, desc->nbr[k], desc->colour[i]);
+    std::vector<int> remap(static_cast<size_t>(K), -1);
+    for (int i = 0; i < n; ++i) remap[size_t(desc->colour[i])] = 0;
+    int used = 0;
+    for (int c = 0; c < K; ++c)
+        if (remap[size_t(c)] == 0) remap[size_t(c)] = used++;
+    colour.resize(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) colour[size_t(i)] = remap[size_t(desc->colour[i])];
+    return used;
+}
+
 static int colour_problem(const qb_problem_desc* desc, const std::vector<int32_t>& rt_ptr,
                           const std::vector<int32_t>& rt_bit, std::vector<int32_t>& pos_bit,
                           std::vector<int32_t>& cls_ptr, int& max_bits_per_ray) {
     const int n = desc->n_vars, nr = desc->n_rays;
-    std::vector<int64_t> weight(static_cast<size_t>(n), 0);
     max_bits_per_ray = 0;
     for (int b = 0; b < nr; ++b) max_bits_per_ray = std::max(max_bits_per_ray, rt_ptr[size_t(b) + 1] - rt_ptr[size_t(b)]);
+    if (desc->colour) {
+        std::vector<int> colour;
+        const int K = given_colouring(desc, rt_ptr, rt_bit, colour);
+        if (K < 0) return K;
+        order_by_colour(colour, K, pos_bit, cls_ptr);
+        for (int c = 0; c < K; ++c)
+            std::stable_partition(pos_bit.begin() + cls_ptr[size_t(c)], pos_bit.begin() + cls_ptr[size_t(c) + 1],
+                                  [&](int32_t i) { return desc->row_ptr[i + 1] - desc->row_ptr[i] > kSeg; });
+        return K;
+    }
+    std::vector<int64_t> weight(static_cast<size_t>(n), 0);
     for (int i = 0; i < n; ++i) {
         int64_t wsum = desc->nbr_ptr[i + 1] - desc->nbr_ptr[i];
         for (int k = desc->row_ptr[i]; k < desc->row_ptr[i + 1]; ++k)
@@ -297,6 +340,7 @@ extern "C" int qb_colour_problem(const qb_problem_desc* desc, int32_t* order, in
     ray_lists(desc, rt_ptr, rt_bit, rt_a);
     int mbr = 0;
     const int K = colour_problem(desc, rt_ptr, rt_bit, pos_bit, cls_ptr, mbr);
+    if (K < 0) return -K;
     std::memcpy(order, pos_bit.data(), sizeof(int32_t) * pos_bit.size());
     std::memcpy(class_ptr, cls_ptr.data(), sizeof(int32_t) * cls_ptr.size());
     *n_classes = K;
@@ -316,6 +360,7 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     std::vector<int32_t> pos_bit, cls_ptr;
     int max_bits_per_ray = 0;
     const int K = colour_problem(desc, rt_ptr, rt_bit, pos_bit, cls_ptr, max_bits_per_ray);
+    if (K < 0) return -K;
 
     auto P = new (std::nothrow) qb_problem();
     if (!P) return fail(QB_E_NOMEM, "out of host memory");

@@ -20,9 +20,31 @@ def test_union_colouring_equals_each_request():
               dict(grid=18, sky_az=36, sky_th=9, log2_muons=20, seed=11)]
     probs = [qa.muon_scene(**s) for s in shapes]
     U, offs = qa.union(probs, [5, 17, 99])
+    U.colour = None            # the engine's own colouring of the whole union
     cu = colours(U)
     for P, (b0, r0) in zip(probs, offs):
         assert np.array_equal(cu[b0:b0 + P.n_vars], colours(P))
+
+
+def test_given_colouring_is_used_and_checked():
+    P = qa.muon_scene(grid=16, sky_az=30, sky_th=10, log2_muons=20, seed=3)
+    own = colours(P)
+    # a valid colouring with the classes renumbered in reverse is taken as given
+    K = int(own.max()) + 1
+    P.colour = (K - 1 - own).astype(np.int32)
+    assert np.array_equal(colours(P), K - 1 - own)
+    # two bits that share a ray may not share a colour
+    bad = own.copy()
+    i = int(np.argmax(np.diff(P.row_ptr) > 0))
+    ray = P.ray[P.row_ptr[i]]
+    j = next(b for b in range(P.n_vars) if b != i and ray in P.ray[P.row_ptr[b]:P.row_ptr[b + 1]])
+    bad[j] = bad[i]
+    P.colour = bad
+    try:
+        P.colouring()
+        raise AssertionError("a conflicting colouring was accepted")
+    except qa.QublarError as e:
+        assert "share ray" in str(e)
 
 
 def test_union_arrays():
