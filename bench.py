@@ -145,6 +145,45 @@ def run_serving(args):
         print(json.dumps(out), flush=True)
 
 
+def run_precision(args):
+    """The FP32 residual carry against full FP64 on the muon QUBO: 64 branches, T 50 -> 1
+    over 400 sweeps + 100 at T = 1, the same keys and order. For FP32 with an FP64
+    recompute every k sweeps (and with none) it reports the final carry error
+    max |r_fp32 - r_fp64(scratch)|, the branches' final energies against the FP64 run's,
+    how many spins and tri-state cells agree, and the time per solve."""
+    import qublar_amd as qa
+
+    P = qa.muon_scene()
+    prob = qa.Problem(P)
+    R = 64
+    sched = qa.schedule(50.0, 1.0, 400, 100)
+    runs = [("fp64", qa.FP64, 0), ("fp32 k=1", qa.FP32, 1), ("fp32 k=10", qa.FP32, 10),
+            ("fp32 k=100", qa.FP32, 100), ("fp32 no recompute", qa.FP32, 0)]
+    base = None
+    rows = []
+    for name, prec, k in runs:
+        prob.anneal_branches(qa.schedule(50.0, 1.0, 2, 0), R, 0, prec, k, branches=False)   # warm
+        t = time.perf_counter()
+        out = prob.anneal_branches(sched, R, 0, prec, k)
+        wall = time.perf_counter() - t
+        log(f"{name}: {wall:.2f} s")
+        row = {"run": name, "solve_s": wall, "kernel_ms": out["report"]["kernel_ms"],
+               "fp32_residual_err": out["report"]["fp32_residual_err"] if prec == qa.FP32 else 0.0,
+               "energy_mean": float(np.mean(out["energy"]))}
+        if base is None:
+            base = out
+        else:
+            de = (out["energy"] - base["energy"]) / np.abs(base["energy"])
+            row.update({"energy_rel_diff_vs_fp64": {"mean": float(de.mean()), "min": float(de.min()),
+                                                    "max": float(de.max())},
+                        "spins_equal_fraction": float(np.mean(out["branches"] == base["branches"])),
+                        "branches_identical": int(np.sum(np.all(out["branches"] == base["branches"], axis=1))),
+                        "tri_state_equal_fraction": float(np.mean(out["tri"] == base["tri"]))})
+        rows.append(row)
+    print(json.dumps({"workload": "precision", "problem": "muon QUBO 306,328 bits, 64 branches, 500 sweeps",
+                      "rows": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,7 +195,7 @@ def main():
     ap.add_argument("--group-width", type=int, default=0, help="branches per group (2, 4, 8, 16); 0 = auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["muon", "serving"], default="muon",
+    ap.add_argument("--workload", choices=["muon", "serving", "precision"], default="muon",
                     help="muon: the headline line (default); serving: the batched serving stream")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window")
@@ -175,6 +214,9 @@ def main():
     log("device", info)
     if args.workload == "serving":
         run_serving(args)
+        return
+    if args.workload == "precision":
+        run_precision(args)
         return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5
