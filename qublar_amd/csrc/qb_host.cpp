@@ -193,7 +193,9 @@ struct qb_problem {
     int max_class = 0, min_class = 0, max_entries = 0, max_bits_per_ray = 0;
     int64_t device_bytes = 0;
     std::vector<int32_t> pos_bit, cls_ptr;
-    DevBuf b_cls, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq, b_dbit, b_dkey;
+    DevBuf b_cls, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq, b_dbit, b_dkey,
+        b_lbeg, b_lnb;
+    std::vector<int32_t> light_begin;   // host copy: each class's first light position
     std::vector<int32_t> col_ptr, nb_ptr;   // host copies, position space
     std::vector<uint8_t> seq_update;
     struct Batches {
@@ -237,6 +239,21 @@ static int validate_problem(const qb_problem_desc* d) {
 // Distance-2 colouring of the bit-ray incidence plus the neighbour graph: two bits
 // share a colour only if they share no ray and are not neighbours. Bits are visited
 // by decreasing conflict weight (their rays' lengths plus neighbours), ties by index.
+// Inside a class the order is free; it is used to group the work: bits with more
+// entries than one chain first, then the others, and last the light bits -- no entry
+// and at most kLightNb neighbours -- which the sweep runs without batches. Each part
+// keeps index order.
+static bool light_bit(const qb_problem_desc* d, int i) {
+    return d->row_ptr[i + 1] == d->row_ptr[i] && d->nbr_ptr[i + 1] - d->nbr_ptr[i] <= kLightNb;
+}
+static void order_within_classes(const qb_problem_desc* desc, int K, std::vector<int32_t>& pos_bit,
+                                 const std::vector<int32_t>& cls_ptr) {
+    auto part = [&](int32_t i) { return desc->row_ptr[i + 1] - desc->row_ptr[i] > kSeg ? 0 : (light_bit(desc, i) ? 2 : 1); };
+    for (int c = 0; c < K; ++c)
+        std::stable_sort(pos_bit.begin() + cls_ptr[size_t(c)], pos_bit.begin() + cls_ptr[size_t(c) + 1],
+                         [&](int32_t a, int32_t b) { return part(a) < part(b); });
+}
+
 // A caller's colouring: verified at distance 2 (bits of one ray, neighbours), unused
 // colours dropped (order kept). Returns the number of classes, or -1 with g_err set.
 static int given_colouring(const qb_problem_desc* desc, const std::vector<int32_t>& rt_ptr,
@@ -281,9 +298,7 @@ static int colour_problem(const qb_problem_desc* desc, const std::vector<int32_t
         const int K = given_colouring(desc, rt_ptr, rt_bit, colour);
         if (K < 0) return K;
         order_by_colour(colour, K, pos_bit, cls_ptr);
-        for (int c = 0; c < K; ++c)
-            std::stable_partition(pos_bit.begin() + cls_ptr[size_t(c)], pos_bit.begin() + cls_ptr[size_t(c) + 1],
-                                  [&](int32_t i) { return desc->row_ptr[i + 1] - desc->row_ptr[i] > kSeg; });
+        order_within_classes(desc, K, pos_bit, cls_ptr);
         return K;
     }
     std::vector<int64_t> weight(static_cast<size_t>(n), 0);
@@ -305,10 +320,7 @@ static int colour_problem(const qb_problem_desc* desc, const std::vector<int32_t
         for (int k = desc->nbr_ptr[i]; k < desc->nbr_ptr[i + 1]; ++k) mark(desc->nbr[k]);
     }, colour);
     order_by_colour(colour, K, pos_bit, cls_ptr);
-    // inside a class (order there is free): heavy bits first, each part by index
-    for (int c = 0; c < K; ++c)
-        std::stable_partition(pos_bit.begin() + cls_ptr[size_t(c)], pos_bit.begin() + cls_ptr[size_t(c) + 1],
-                              [&](int32_t i) { return desc->row_ptr[i + 1] - desc->row_ptr[i] > kSeg; });
+    order_within_classes(desc, K, pos_bit, cls_ptr);
     return K;
 }
 
@@ -414,6 +426,17 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
                 seen[size_t(b)] = p;
             }
     }
+    // light bits: each class's last run of positions, neighbour positions at stride kLightNb
+    std::vector<int32_t> light_begin(static_cast<size_t>(K), 0);
+    std::vector<int32_t> light_nb(static_cast<size_t>(n) * kLightNb, -1);
+    for (int c = 0; c < K; ++c) {
+        int lb = P->cls_ptr[size_t(c) + 1];
+        while (lb > P->cls_ptr[size_t(c)] && light_bit(desc, P->pos_bit[size_t(lb) - 1])) --lb;
+        light_begin[size_t(c)] = lb;
+        for (int p = lb; p < P->cls_ptr[size_t(c) + 1]; ++p)
+            for (int k = nb_ptr[size_t(p)]; k < nb_ptr[size_t(p) + 1]; ++k)
+                light_nb[size_t(p) * kLightNb + size_t(k - nb_ptr[size_t(p)])] = nb[size_t(k)];
+    }
     // draw keys in position order
     std::vector<int32_t> draw_bit(static_cast<size_t>(n));
     std::vector<uint64_t> draw_key(static_cast<size_t>(n), 0u);
@@ -433,6 +456,7 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     if ((rc = P->b_cls.upload(P->cls_ptr, st)) ||
         (rc = P->b_seq.upload(seq_update, st)) || (rc = P->b_pos.upload(P->pos_bit, st)) ||
         (rc = P->b_dbit.upload(draw_bit, st)) || (rc = P->b_dkey.upload(draw_key, st)) ||
+        (rc = P->b_lbeg.upload(light_begin, st)) || (rc = P->b_lnb.upload(light_nb, st)) ||
         (rc = P->b_col.upload(col_ptr, st)) || (rc = P->b_ent.upload(ent, st)) || (rc = P->b_nbp.upload(nb_ptr, st)) ||
         (rc = P->b_nb.upload(nb, st)) || (rc = P->b_nbbit.upload(nb_bit, st)) || (rc = P->b_lin.upload(lin, st)) ||
         (rc = P->b_d.upload(dv, st)) || (rc = P->b_w.upload(wv, st)) || (rc = P->b_rtp.upload(rt_ptr, st)) ||
@@ -442,7 +466,8 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     P->col_ptr = col_ptr;
     P->nb_ptr = nb_ptr;
     P->seq_update = seq_update;
-    P->device_bytes = int64_t(P->b_dbit.bytes + P->b_dkey.bytes +
+    P->light_begin = light_begin;
+    P->device_bytes = int64_t(P->b_dbit.bytes + P->b_dkey.bytes + P->b_lbeg.bytes + P->b_lnb.bytes +
                               P->b_cls.bytes + P->b_seq.bytes + P->b_pos.bytes + P->b_col.bytes + P->b_ent.bytes + P->b_nbp.bytes +
                               P->b_nb.bytes + P->b_nbbit.bytes + P->b_lin.bytes + P->b_d.bytes + P->b_w.bytes +
                               P->b_rtp.bytes + P->b_rt.bytes);
@@ -456,6 +481,8 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     D.seq_update = P->b_seq.as<uint8_t>();
     D.draw_bit = P->b_dbit.as<int32_t>();
     D.draw_key = P->b_dkey.as<uint64_t>();
+    D.light_begin = P->b_lbeg.as<int32_t>();
+    D.light_nb = P->b_lnb.as<int32_t>();
     D.pos_bit = P->b_pos.as<int32_t>();
     D.col_ptr = P->b_col.as<int32_t>();
     D.ent = P->b_ent.as<RayEnt>();
@@ -545,7 +572,7 @@ static int batches_for(qb_problem* P, int bw, int team_size, const DevBatches** 
     for (int c = 0; c < K; ++c) {
         cls_batch[size_t(c)] = int32_t(first.size());
         items.clear();
-        for (int q = P->cls_ptr[size_t(c)]; q < P->cls_ptr[size_t(c) + 1]; ++q) items.push_back(q);
+        for (int q = P->cls_ptr[size_t(c)]; q < P->light_begin[size_t(c)]; ++q) items.push_back(q);
         auto chains = [&](int q) {
             return std::max(1, (P->col_ptr[size_t(q) + 1] - P->col_ptr[size_t(q)] + kSeg - 1) / kSeg);
         };

@@ -54,8 +54,14 @@ struct DevProblem {
     const uint8_t* seq_update;  // 1 where a ray repeats in the column: scatter in CSR order
     const int32_t* draw_bit;    // the counter index of each position's draws
     const uint64_t* draw_key;   // added to the branch seed of each position's draws
+    // Light bits (no entry, at most kLightNb neighbours) close every class, from
+    // light_begin[c]; light_nb[p * kLightNb + k] is light position p's k-th neighbour in
+    // CSR order (-1 past its last).
+    const int32_t* light_begin;
+    const int32_t* light_nb;
     double lambda;
 };
+constexpr int kLightNb = 8;
 
 // Work batches of one sweep step at one group width: a workgroup of BLOCK threads
 // has BLOCK / BW slots of BW lanes, and a bit with k entries needs
@@ -89,9 +95,9 @@ struct DevBatches {
     int n_batches;
 };
 
-// The sweep keeps its class tables (cls_batch, cls_heavy) in LDS.
+// The sweep keeps its class tables (cls_batch, cls_heavy, light_begin, cls_ptr end) in LDS.
 constexpr int kSweepTableLimit = 96 * 1024;
-inline size_t sweep_table_bytes(int n_classes) { return sizeof(int32_t) * size_t(2 * n_classes + 1); }
+inline size_t sweep_table_bytes(int n_classes) { return sizeof(int32_t) * size_t(4 * n_classes + 2); }
 
 // Workgroups of a team reserved for a class's multi-pass batches, the packed batches
 // going round-robin over the others; with none reserved every task of the class,

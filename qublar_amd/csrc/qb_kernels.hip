@@ -621,6 +621,76 @@ __device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T
         heavy_update<T, BW, BLOCK>(p, s, pos, g, sweep, beta);
 }
 
+// The light bits of one class, [lb, le): no ray entry, at most kLightNb neighbours,
+// so dE = lin * delta + lambda * (+-1) per neighbour in CSR order -- the same sum the
+// packed step makes for such a bit. Every slot of the team takes (bit, group) tasks in
+// a grid-stride loop, kLightU of them at a time with all their loads in flight
+// (neighbour positions, lin and draw key with the spin mask, then the neighbours'
+// masks); no descriptor, no LDS and no workgroup sync. Bits of one class are
+// independent, so which slot takes which bit does not matter.
+constexpr int kLightU = 4;
+constexpr int kTeams = 8;   // XCD teams of the sweep (group g belongs to team g mod 8)
+
+template <typename T, int BW>
+__device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<T>& s, int lb, int le, int my_slot,
+                                            int n_slots, int my_groups, int team, int sweep, double beta) {
+    const int n_light = le - lb;
+    if (n_light <= 0) return;
+    const int b = int(threadIdx.x) % BW;
+    const int tasks = n_light * my_groups;
+    for (int t0 = my_slot; t0 < tasks; t0 += n_slots * kLightU) {
+        int pos[kLightU], grp[kLightU];
+        bool valid[kLightU];
+        int4 qa[kLightU], qb[kLightU];
+        double lin[kLightU];
+        int bit[kLightU];
+        uint64_t key[kLightU];
+        uint16_t xm[kLightU];
+#pragma unroll
+        for (int u = 0; u < kLightU; ++u) {
+            const int t = t0 + u * n_slots;
+            valid[u] = t < tasks;
+            const int tt = valid[u] ? t : t0;
+            const int gi = tt / n_light;
+            pos[u] = lb + (tt - gi * n_light);
+            grp[u] = team + gi * kTeams;
+            const int4* q4 = reinterpret_cast<const int4*>(p.light_nb + size_t(pos[u]) * kLightNb);
+            qa[u] = q4[0];
+            qb[u] = q4[1];
+            lin[u] = p.lin[pos[u]];
+            bit[u] = p.draw_bit[pos[u]];
+            key[u] = p.draw_key[pos[u]];
+            xm[u] = ld_l2(s.x + size_t(grp[u]) * size_t(p.n) + pos[u]);
+        }
+        uint16_t m[kLightU][kLightNb];
+#pragma unroll
+        for (int u = 0; u < kLightU; ++u) {
+            const uint16_t* xg = s.x + size_t(grp[u]) * size_t(p.n);
+            const int q[kLightNb] = {qa[u].x, qa[u].y, qa[u].z, qa[u].w, qb[u].x, qb[u].y, qb[u].z, qb[u].w};
+#pragma unroll
+            for (int v = 0; v < kLightNb; ++v) m[u][v] = ld_l2(xg + (q[v] >= 0 ? q[v] : pos[u]));
+        }
+#pragma unroll
+        for (int u = 0; u < kLightU; ++u) {
+            const int q[kLightNb] = {qa[u].x, qa[u].y, qa[u].z, qa[u].w, qb[u].x, qb[u].y, qb[u].z, qb[u].w};
+            const int xb = (xm[u] >> b) & 1;
+            const double delta = xb ? -1.0 : 1.0;
+            double dE = lin[u] * delta;
+#pragma unroll
+            for (int v = 0; v < kLightNb; ++v) {
+                const int before = ((m[u][v] >> b) & 1) != xb;
+                const double nxt = dE + p.lambda * (before ? -1.0 : 1.0);
+                dE = q[v] >= 0 ? nxt : dE;
+            }
+            const int branch = grp[u] * BW + b;
+            const double draw = draw_u01(s.seed_offset + key[u] + uint64_t(branch) + 1u, uint32_t(sweep), uint32_t(bit[u]));
+            const bool acc = valid[u] && decide_drawn<T>(s, branch, draw, beta, dE);
+            const uint16_t flips = segment_flips<BW>(acc);
+            if (valid[u] && b == 0 && flips) s.x[size_t(grp[u]) * size_t(p.n) + pos[u]] = uint16_t(xm[u] ^ flips);
+        }
+    }
+}
+
 // Residuals of ray `ray` for branch lane b, from scratch: -d + sum of a over the
 // ray's set bits in increasing original bit index (binary_energy's order).
 template <typename T, int BW>
@@ -648,7 +718,6 @@ __device__ __forceinline__ void ray_recompute(const DevProblem& p, const DevStat
 // blocks read their XCC_ID to form the teams, take a rank by ticket, and check
 // once, behind an agent-scope grid barrier, that every team is complete. Team t
 // anneals groups t, t + 8, ... through every class of every sweep.
-constexpr int kTeams = 8;
 
 template <typename T, int BW, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
@@ -679,10 +748,14 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
     if (p.n_classes == 0) return;
     // class -> batch table and heavy-batch bits, copied to LDS once
     extern __shared__ int32_t tables[];
-    int32_t* cls = tables;                         // n_classes + 1 batch offsets
-    int32_t* nheavy = tables + p.n_classes + 1;   // n_classes multi-pass batch counts
+    int32_t* cls = tables;                          // n_classes + 1 batch offsets
+    int32_t* nheavy = tables + p.n_classes + 1;    // n_classes multi-pass batch counts
+    int32_t* lbeg = nheavy + p.n_classes;           // n_classes first light positions
+    int32_t* cend = lbeg + p.n_classes;             // n_classes + 1 class position offsets
     for (int i = int(threadIdx.x); i <= p.n_classes; i += BLOCK) cls[i] = bt.cls_batch[i];
     for (int i = int(threadIdx.x); i < p.n_classes; i += BLOCK) nheavy[i] = bt.cls_heavy[i];
+    for (int i = int(threadIdx.x); i < p.n_classes; i += BLOCK) lbeg[i] = p.light_begin[i];
+    for (int i = int(threadIdx.x); i <= p.n_classes; i += BLOCK) cend[i] = p.cls_ptr[i];
     __syncthreads();
     // Class c has H = nh * G multi-pass tasks and P = (nb - nh) * G packed tasks, each a
     // (batch, group) pair; Hr = heavy_ranks(H, team) ranks take the multi-pass tasks
@@ -801,6 +874,7 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 cur = nxt;
                 nxt.d = nn.d;
             }
+            light_sweep<T, BW>(p, s, lbeg[c], cend[c + 1], my_slot, n_slots, my_groups, team, sweep, beta);
             target += unsigned(team_size);
             if (!team_barrier(bar, target, s.err)) return;
         }
