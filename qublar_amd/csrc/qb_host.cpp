@@ -172,6 +172,110 @@ static int greedy_colour(int n, const std::vector<int>& visit, Conflicts conflic
     return n_colours;
 }
 
+// ---- saturation colouring ------------------------------------------------------------
+// DSatur (Brelaz, CACM 22(4), 1979): the next bit coloured is the uncoloured one whose
+// conflicting bits already carry the most distinct colours (its saturation), ties by
+// conflict weight, then by index; it takes the smallest colour none of them carries.
+// On ray x voxel problems it needs markedly fewer classes -- fewer dependent sweep
+// steps -- than one greedy pass in weight order. The queue holds every uncoloured bit
+// in the rank bitmap of its saturation level (rank: position in weight order), so the
+// next bit is the highest rank of the highest non-empty level.
+namespace {
+struct RankSet {
+    std::vector<uint64_t> w, s;   // rank bitmap and its summary (one bit per non-zero word)
+    int count = 0;
+    void init(int n) {
+        w.assign(size_t(n + 63) / 64, 0);
+        s.assign((w.size() + 63) / 64, 0);
+    }
+    void insert(int r) {
+        uint64_t& x = w[size_t(r >> 6)];
+        if (!x) s[size_t(r >> 12)] |= uint64_t(1) << ((r >> 6) & 63);
+        x |= uint64_t(1) << (r & 63);
+        ++count;
+    }
+    void erase(int r) {
+        uint64_t& x = w[size_t(r >> 6)];
+        x &= ~(uint64_t(1) << (r & 63));
+        if (!x) s[size_t(r >> 12)] &= ~(uint64_t(1) << ((r >> 6) & 63));
+        --count;
+    }
+    int top() const {
+        for (size_t k = s.size(); k-- > 0;)
+            if (s[k]) {
+                const size_t wi = k * 64 + size_t(63 - __builtin_clzll(s[k]));
+                return int(wi * 64 + size_t(63 - __builtin_clzll(w[wi])));
+            }
+        return -1;
+    }
+};
+}  // namespace
+
+// `first`: bits coloured before the others, greedily in the given order (they see only
+// each other). conflicts(i, mark) calls mark(j) for every conflicting bit j.
+template <typename Conflicts>
+static int saturation_colour(int n, const std::vector<int>& by_weight, const std::vector<int>& first,
+                             Conflicts conflicts, std::vector<int>& colour) {
+    colour.assign(size_t(n), -1);
+    std::vector<int> rank(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) rank[size_t(by_weight[size_t(k)])] = n - 1 - k;
+    int W = 2;                                    // words per saturation set
+    std::vector<uint64_t> sat(size_t(n) * size_t(W), 0);
+    std::vector<int> level(static_cast<size_t>(n), 0);
+    std::vector<RankSet> queue;
+    bool queued = false;
+    int K = 0, top = 0;
+    auto first_free = [&](int i) {
+        const uint64_t* st = &sat[size_t(i) * size_t(W)];
+        int k = 0;
+        while (k < W && st[k] == ~uint64_t(0)) ++k;
+        return k == W ? 64 * W : 64 * k + __builtin_ctzll(~st[k]);
+    };
+    auto enqueue = [&](int j) {
+        const int l = level[size_t(j)];
+        while (int(queue.size()) <= l) {
+            queue.emplace_back();
+            queue.back().init(n);
+        }
+        queue[size_t(l)].insert(rank[size_t(j)]);
+        top = std::max(top, l);
+    };
+    auto assign = [&](int i, int c) {
+        if (c >= 64 * W) {                        // widen every set
+            std::vector<uint64_t> wider(size_t(n) * size_t(2 * W), 0);
+            for (int j = 0; j < n; ++j)
+                std::memcpy(&wider[size_t(j) * size_t(2 * W)], &sat[size_t(j) * size_t(W)], sizeof(uint64_t) * size_t(W));
+            sat.swap(wider);
+            W *= 2;
+        }
+        colour[size_t(i)] = c;
+        K = std::max(K, c + 1);
+        conflicts(i, [&](int j) {
+            if (colour[size_t(j)] >= 0) return;
+            uint64_t& word = sat[size_t(j) * size_t(W) + size_t(c >> 6)];
+            const uint64_t m = uint64_t(1) << (c & 63);
+            if (word & m) return;
+            word |= m;
+            if (queued) queue[size_t(level[size_t(j)])].erase(rank[size_t(j)]);
+            ++level[size_t(j)];
+            if (queued) enqueue(j);
+        });
+    };
+    for (int i : first) assign(i, first_free(i));
+    for (int i = 0; i < n; ++i)
+        if (colour[size_t(i)] < 0) enqueue(i);
+    queued = true;
+    for (;;) {
+        while (top >= 0 && queue[size_t(top)].count == 0) --top;
+        if (top < 0) break;
+        const int r = queue[size_t(top)].top();
+        queue[size_t(top)].erase(r);
+        const int i = by_weight[size_t(n - 1 - r)];
+        assign(i, first_free(i));
+    }
+    return K;
+}
+
 // Positions: bits sorted by (colour, index).
 static void order_by_colour(const std::vector<int>& colour, int n_colours, std::vector<int32_t>& pos_bit,
                             std::vector<int32_t>& cls_ptr) {
@@ -237,8 +341,10 @@ static int validate_problem(const qb_problem_desc* d) {
 }
 
 // Distance-2 colouring of the bit-ray incidence plus the neighbour graph: two bits
-// share a colour only if they share no ray and are not neighbours. Bits are visited
-// by decreasing conflict weight (their rays' lengths plus neighbours), ties by index.
+// share a colour only if they share no ray and are not neighbours. Conflict weight =
+// the bit's rays' lengths plus its neighbours. Bits with more entries than a sweep
+// workgroup has chain slots (kHeavyEntries) are coloured first, greedily by decreasing
+// weight (ties by index); then every other bit in saturation order.
 // Inside a class the order is free; it is used to group the work: bits with more
 // entries than one chain first, then the others, and last the light bits -- no entry
 // and at most kLightNb neighbours -- which the sweep runs without batches. Each part
@@ -308,11 +414,16 @@ static int colour_problem(const qb_problem_desc* desc, const std::vector<int32_t
             wsum += rt_ptr[size_t(desc->ray[k]) + 1] - rt_ptr[size_t(desc->ray[k])];
         weight[size_t(i)] = wsum;
     }
-    std::vector<int> visit(static_cast<size_t>(n));
-    for (int i = 0; i < n; ++i) visit[size_t(i)] = i;
-    std::stable_sort(visit.begin(), visit.end(), [&](int a, int b) { return weight[size_t(a)] > weight[size_t(b)]; });
+    std::vector<int> by_weight(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) by_weight[size_t(i)] = i;
+    std::stable_sort(by_weight.begin(), by_weight.end(),
+                     [&](int a, int b) { return weight[size_t(a)] > weight[size_t(b)]; });
+    // the multi-pass bits first, so the few of them share few classes
+    std::vector<int> heavy;
+    for (int i : by_weight)
+        if (desc->row_ptr[i + 1] - desc->row_ptr[i] > kHeavyEntries) heavy.push_back(i);
     std::vector<int> colour;
-    const int K = greedy_colour(n, visit, [&](int i, auto mark) {
+    const int K = saturation_colour(n, by_weight, heavy, [&](int i, auto mark) {
         for (int k = desc->row_ptr[i]; k < desc->row_ptr[i + 1]; ++k) {
             const int b = desc->ray[k];
             for (int q = rt_ptr[size_t(b)]; q < rt_ptr[size_t(b) + 1]; ++q) mark(rt_bit[size_t(q)]);

@@ -155,6 +155,9 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
                              hipStream_t st);
 int ray_sweep_blocks_per_cu(size_t table_bytes);
 constexpr int kSweepBlock = 256;   // threads per sweep workgroup
+// Entries of a bit with more chains than a sweep workgroup has slots at group width 2
+// (a multi-pass bit there).
+constexpr int kHeavyEntries = kSeg * (kSweepBlock / 2);
 hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, float* p_void,
                               uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st);
 // FP64 energy of every branch: data and prior partial sums per block, reduced in a

@@ -80,6 +80,47 @@ def test_muon_colouring_is_distance_two():
         assert not np.any(h[1:] & ~h[:-1])
 
 
+def neighbour_problem(n, edges, rays=()):
+    """A problem whose conflicts are the given neighbour pairs and rays (lists of bits)."""
+    nb = [[] for _ in range(n)]
+    for a, b in edges:
+        nb[a].append(b)
+        nb[b].append(a)
+    ent = [[] for _ in range(n)]
+    for r, bits in enumerate(rays):
+        for i in bits:
+            ent[i].append(r)
+    row_ptr = np.cumsum([0] + [len(e) for e in ent])
+    ray = [r for e in ent for r in e]
+    nbr_ptr = np.cumsum([0] + [len(x) for x in nb])
+    nbr = [j for x in nb for j in x]
+    nr = max(1, len(rays))
+    return qa.ProblemArrays(row_ptr, ray, np.full(len(ray), 0.5), np.zeros(nr), np.ones(nr), nbr_ptr, nbr, 0.3, 0.1)
+
+
+def test_saturation_colouring_closed_cases():
+    """DSatur is exact on bipartite graphs (Brelaz 1979): a 2-D grid and an even cycle
+    take 2 classes, an odd cycle 3, and bits of one ray as many classes as the ray has
+    bits. A greedy pass in index order needs 4 classes on the crown graph below (each
+    a_i conflicts with every b_j, j != i); saturation order needs 2."""
+    g = 9
+    grid = [(r * g + c, r * g + c + 1) for r in range(g) for c in range(g - 1)] + \
+           [(r * g + c, (r + 1) * g + c) for r in range(g - 1) for c in range(g)]
+    for n, edges, k in [(g * g, grid, 2), (10, [(i, (i + 1) % 10) for i in range(10)], 2),
+                        (9, [(i, (i + 1) % 9) for i in range(9)], 3)]:
+        P = neighbour_problem(n, edges)
+        order, cp = P.colouring()
+        assert_distance2_colouring(P, order, cp)
+        assert len(cp) - 1 == k
+    crown = [(2 * i, 2 * j + 1) for i in range(4) for j in range(4) if i != j]
+    P = neighbour_problem(8, crown)
+    assert len(P.colouring()[1]) - 1 == 2
+    P = neighbour_problem(12, [], rays=[list(range(7)), [6, 7, 8, 9, 10, 11]])
+    order, cp = P.colouring()
+    assert_distance2_colouring(P, order, cp)
+    assert len(cp) - 1 == 7
+
+
 def test_small_scene_colouring_is_distance_two():
     P = qa.muon_scene(grid=32, sky_az=90, sky_th=20)
     assert_distance2_colouring(P, *P.colouring())
