@@ -759,7 +759,14 @@ static int batches_for(qb_problem* P, int bw, int team_size, const DevBatches** 
     // slot descriptors: one per chain of every packed bit, empty slots pos = -1
     std::vector<SlotDesc> desc(first.size() * size_t(slots), SlotDesc{-1, 0, 0, 0u});
     for (size_t bi = 0; bi < first.size(); ++bi) {
-        if (count[bi] < 0) continue;
+        if (count[bi] < 0) {
+            // a multi-pass batch: its bit in slots 0 and 1, both with pos -1
+            const int q = pos[size_t(first[bi])];
+            desc[bi * size_t(slots)] = SlotDesc{-1, q, P->col_ptr[size_t(q)], uint32_t(P->col_ptr[size_t(q) + 1])};
+            desc[bi * size_t(slots) + 1] = SlotDesc{-1, P->nb_ptr[size_t(q)], P->nb_ptr[size_t(q) + 1],
+                                                    uint32_t(P->seq_update[size_t(q)] ? 1 : 0)};
+            continue;
+        }
         for (int it = 0; it < count[bi]; ++it) {
             const int q = pos[size_t(first[bi] + it)];
             const int k0 = P->col_ptr[size_t(q)], k1 = P->col_ptr[size_t(q) + 1];

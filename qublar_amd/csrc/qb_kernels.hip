@@ -406,13 +406,26 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
     }
 }
 
+// A multi-pass batch's bit: slot descriptors 0 and 1 of the batch carry
+// {-1, pos, k0, k1} and {-1, n0, n1, seq} (pos -1, so both read as empty slots).
+struct HeavyBit {
+    int pos, k0, k1, n0, n1, seq;
+};
+
+__device__ __forceinline__ HeavyBit heavy_bit(const DevBatches& bt, int batch, int slots) {
+    const int4* d = reinterpret_cast<const int4*>(bt.desc + size_t(batch) * size_t(slots));
+    const int4 a = d[0], c = d[1];
+    return HeavyBit{a.y, a.z, a.w, c.y, c.z, c.w};
+}
+
 // A bit with more chains than a workgroup has slots, by a whole workgroup in several
 // passes: the chains are spread over the block's BW-lane slots, their sums meet in
 // LDS, slot 0 adds them in order and decides, and the residual scatter is spread
 // over the block again.
 template <typename T, int BW, int BLOCK>
-__device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState<T>& s, int pos, int g, int sweep,
-                                             double beta) {
+__device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState<T>& s, const HeavyBit& h, int g,
+                                             int sweep, double beta) {
+    const int pos = h.pos;
     constexpr int kSlots = BLOCK / BW;
     __shared__ double part[kSlots][BW];
     __shared__ int accepted[BW];
@@ -421,7 +434,7 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
     T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
     const uint16_t xm = ld_l2(xg + pos);
-    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
+    const int k0 = h.k0, k1 = h.k1;
     const double lin = p.lin[pos];
     const int xb = (xm >> b) & 1;
     const double delta = xb ? -1.0 : 1.0;
@@ -444,7 +457,7 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
         __syncthreads();
     }
     if (slot == 0) {
-        dE = neighbour_terms(p, xg, p.nb_ptr[pos], p.nb_ptr[pos + 1], b, xb, dE);
+        dE = neighbour_terms(p, xg, h.n0, h.n1, b, xb, dE);
         const double u = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(sweep),
                                   uint32_t(p.draw_bit[pos]));
         const bool acc = decide_drawn<T>(s, g * BW + b, u, beta, dE);
@@ -454,7 +467,7 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
     }
     __syncthreads();
     if (accepted[b]) {
-        if (p.seq_update[pos]) {
+        if (h.seq) {
             // a ray repeats in this column: its updates must land in CSR order
             if (slot == 0)
                 for (int k = k0; k < k1; ++k) {
@@ -483,124 +496,147 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
 }
 
 // A bit with more chains than a workgroup has slots, up to kHeavyPasses chains per
-// slot: its column is staged in LDS by the whole workgroup (kStage coalesced loads
-// in flight per thread), then every slot issues the residual loads of all its chains
-// at once, sums each chain, and after the decision scatters them from registers. A
-// step of such a bit costs the staging plus one round of residual loads, where the
+// slot: its column is staged in LDS by the whole workgroup (every load of it in flight
+// at once), then every slot issues the residual loads of all its chains at once, sums
+// each chain, and after the decision scatters them from registers. The neighbour
+// positions and masks are loaded by the deciding slot at the start, beside the column.
+// A step of such a bit costs the staging plus one round of residual loads, where the
 // pass-by-pass form costs two dependent round trips per pass and again per pass to
 // scatter.
 constexpr int kHeavyPasses = 3;
 
 template <typename T, int BW, int BLOCK>
-__device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevState<T>& s, int pos, int k0, int k1,
-                                                 int g, int sweep, double beta) {
+__device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevState<T>& s, const HeavyBit& h, int g,
+                                                 int sweep, double beta) {
     constexpr int kSlots = BLOCK / BW;
     constexpr int kMaxEnt = kHeavyPasses * kSlots * kSeg;
-    constexpr int kStage = 8;
-    __shared__ RayEnt hent[kMaxEnt];
+    constexpr int kNbHeavy = 8;
+    static_assert(kMaxEnt % 64 == 0, "a wave stages 64 entries at a time");
+    __shared__ __align__(16) RayEnt hent[kMaxEnt];
     __shared__ double hpart[kHeavyPasses * kSlots][BW];
     __shared__ int hacc[BW];
     const int tid = int(threadIdx.x);
     const int b = tid % BW, slot = tid / BW;
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
     T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
-    const int deg = k1 - k0;                       // kSlots * kSeg < deg <= kMaxEnt
+    const int pos = h.pos, k0 = h.k0, deg = h.k1 - h.k0;   // kSlots * kSeg < deg <= kMaxEnt
     const int n_chain = (deg + kSeg - 1) / kSeg;
     const uint16_t xm = ld_l2(xg + pos);
-    const int n0 = p.nb_ptr[pos], n1 = p.nb_ptr[pos + 1];
-    // stage the column
-    for (int base = 0; base < deg; base += BLOCK * kStage) {
-        RayEnt tmp[kStage];
-#pragma unroll
-        for (int u = 0; u < kStage; ++u) tmp[u] = p.ent[k0 + min(base + u * BLOCK + tid, deg - 1)];
-#pragma unroll
-        for (int u = 0; u < kStage; ++u)
-            if (base + u * BLOCK + tid < deg) hent[base + u * BLOCK + tid] = tmp[u];
-    }
-    __syncthreads();
-    const int xb = (xm >> b) & 1;
-    const double delta = xb ? -1.0 : 1.0;
-    // the residuals of every chain of this slot, all in flight together
-    T rv[kHeavyPasses][kSeg];
-#pragma unroll
-    for (int q = 0; q < kHeavyPasses; ++q) {
-        const int j = slot + q * kSlots;
-        const int len = j < n_chain ? min(kSeg, deg - j * kSeg) : 0;
-        const RayEnt* e = hent + j * kSeg;
-#pragma unroll
-        for (int u = 0; u < kSeg; ++u) rv[q][u] = T(0);
-        if (len > 0) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) rv[q][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+    // stage the column straight into LDS: wave w copies entries [64 v, 64 v + 64) for
+    // v = w, w + 4, ... (lane i to row 64 v + i; rows past deg repeat the last entry and
+    // stay inside hent, as deg <= kMaxEnt and kMaxEnt is a multiple of 64)
+    {
+        for (int v0 = (tid >> 6) * 64; v0 < deg; v0 += BLOCK) {
+            const RayEnt* src = p.ent + k0 + min(v0 + (tid & 63), deg - 1);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)(hent + v0), 16, 0, 0);
         }
-        if (len > 4) {
+        // the deciding slot's neighbour positions, then their masks (neighbours are in
+        // other classes, final since the barrier)
+        int q[kNbHeavy];
+        uint16_t m[kNbHeavy];
+        if (slot == 0 && h.n1 > h.n0) {
 #pragma unroll
-            for (int u = 4; u < 8; ++u) rv[q][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+            for (int u = 0; u < kNbHeavy; ++u) q[u] = p.nb[min(h.n0 + u, h.n1 - 1)];
+#pragma unroll
+            for (int u = 0; u < kNbHeavy; ++u) m[u] = ld_l2(xg + q[u]);
         }
-        if (len > 8) {
+        __builtin_amdgcn_s_waitcnt(0);   // the LDS copies (and the neighbour loads) have landed
+        __syncthreads();
+        const int xb = (xm >> b) & 1;
+        const double delta = xb ? -1.0 : 1.0;
+        // the residuals of every chain of this slot, all in flight together
+        T rv[kHeavyPasses][kSeg];
 #pragma unroll
-            for (int u = 8; u < 12; ++u) rv[q][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+        for (int qq = 0; qq < kHeavyPasses; ++qq) {
+            const int j = slot + qq * kSlots;
+            const int len = j < n_chain ? min(kSeg, deg - j * kSeg) : 0;
+            const RayEnt* e = hent + j * kSeg;
+#pragma unroll
+            for (int u = 0; u < kSeg; ++u) rv[qq][u] = T(0);
+            if (len > 0) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) rv[qq][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+            }
+            if (len > 4) {
+#pragma unroll
+                for (int u = 4; u < 8; ++u) rv[qq][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+            }
+            if (len > 8) {
+#pragma unroll
+                for (int u = 8; u < 12; ++u) rv[qq][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+            }
+            if (len > 12) {
+#pragma unroll
+                for (int u = 12; u < 16; ++u) rv[qq][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+            }
         }
-        if (len > 12) {
+        double u01 = 0.0;
+        if (slot == 0)
+            u01 = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(sweep),
+                           uint32_t(p.draw_bit[pos]));
+        const double lin = p.lin[pos];
 #pragma unroll
-            for (int u = 12; u < 16; ++u) rv[q][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
+        for (int qq = 0; qq < kHeavyPasses; ++qq) {
+            const int j = slot + qq * kSlots;
+            if (j < n_chain) {
+                const int len = min(kSeg, deg - j * kSeg);
+                RayEnt e[kSeg];
+#pragma unroll
+                for (int u = 0; u < kSeg; ++u) e[u] = hent[j * kSeg + min(u, len - 1)];
+                hpart[j][b] = chain_sum<T>(e, rv[qq], len, delta, lin * delta, j == 0);
+            }
         }
-    }
-    double u01 = 0.0;
-    if (slot == 0)
-        u01 = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(sweep),
-                       uint32_t(p.draw_bit[pos]));
-    const double lin = p.lin[pos];
+        __syncthreads();
+        if (slot == 0) {
+            double dE = hpart[0][b];
+            for (int q0 = 1; q0 < n_chain; q0 += 8) {
+                double v[8];
 #pragma unroll
-    for (int q = 0; q < kHeavyPasses; ++q) {
-        const int j = slot + q * kSlots;
-        if (j < n_chain) {
-            const int len = min(kSeg, deg - j * kSeg);
-            RayEnt e[kSeg];
+                for (int u = 0; u < 8; ++u) v[u] = hpart[min(q0 + u, kHeavyPasses * kSlots - 1)][b];
 #pragma unroll
-            for (int u = 0; u < kSeg; ++u) e[u] = hent[j * kSeg + min(u, len - 1)];
-            hpart[j][b] = chain_sum<T>(e, rv[q], len, delta, lin * delta, j == 0);
+                for (int u = 0; u < 8; ++u) dE = q0 + u < n_chain ? dE + v[u] : dE;
+            }
+            // lambda * (+-1) per neighbour in CSR order
+#pragma unroll
+            for (int u = 0; u < kNbHeavy; ++u) {
+                const int before = ((m[u] >> b) & 1) != xb;
+                const double nxt = dE + p.lambda * (before ? -1.0 : 1.0);
+                dE = h.n0 + u < h.n1 ? nxt : dE;
+            }
+            for (int k = h.n0 + kNbHeavy; k < h.n1; ++k) {
+                const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
+                dE += p.lambda * (before ? -1.0 : 1.0);
+            }
+            const bool acc = decide_drawn<T>(s, g * BW + b, u01, beta, dE);
+            hacc[b] = acc ? 1 : 0;
+            const uint16_t flips = segment_flips<BW>(acc);
+            if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
         }
-    }
-    __syncthreads();
-    if (slot == 0) {
-        double dE = hpart[0][b];
-        for (int q0 = 1; q0 < n_chain; q0 += 8) {
-            double v[8];
+        __syncthreads();
+        if (hacc[b]) {
+            if (h.seq) {
+                // a ray repeats in this column: its updates must land in CSR order
+                if (slot == 0)
+                    for (int k = 0; k < deg; ++k) {
+                        const RayEnt e = hent[k];
+                        T* qp = rg + size_t(e.ray) * BW + b;
+                        *qp = residual_add<T>(ld_l2(qp), e.a, delta);
+                        __builtin_amdgcn_s_waitcnt(0);
+                    }
+            } else {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = hpart[min(q0 + u, kHeavyPasses * kSlots - 1)][b];
+                for (int qq = 0; qq < kHeavyPasses; ++qq) {
+                    const int j = slot + qq * kSlots;
+                    if (j < n_chain) {
+                        const int len = min(kSeg, deg - j * kSeg);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) dE = q0 + u < n_chain ? dE + v[u] : dE;
-        }
-        dE = neighbour_terms(p, xg, n0, n1, b, xb, dE);
-        const bool acc = decide_drawn<T>(s, g * BW + b, u01, beta, dE);
-        hacc[b] = acc ? 1 : 0;
-        const uint16_t flips = segment_flips<BW>(acc);
-        if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
-    }
-    __syncthreads();
-    if (hacc[b]) {
-        if (p.seq_update[pos]) {
-            // a ray repeats in this column: its updates must land in CSR order
-            if (slot == 0)
-                for (int k = 0; k < deg; ++k) {
-                    const RayEnt e = hent[k];
-                    T* q = rg + size_t(e.ray) * BW + b;
-                    *q = residual_add<T>(ld_l2(q), e.a, delta);
-                    __builtin_amdgcn_s_waitcnt(0);
-                }
-        } else {
-#pragma unroll
-            for (int q = 0; q < kHeavyPasses; ++q) {
-                const int j = slot + q * kSlots;
-                if (j < n_chain) {
-                    const int len = min(kSeg, deg - j * kSeg);
-#pragma unroll
-                    for (int u = 0; u < kSeg; ++u)
-                        if (u < len) {
-                            const RayEnt e = hent[j * kSeg + u];
-                            rg[size_t(e.ray) * BW + b] = residual_add<T>(rv[q][u], e.a, delta);
-                        }
+                        for (int u = 0; u < kSeg; ++u)
+                            if (u < len) {
+                                const RayEnt e = hent[j * kSeg + u];
+                                rg[size_t(e.ray) * BW + b] = residual_add<T>(rv[qq][u], e.a, delta);
+                            }
+                    }
                 }
             }
         }
@@ -611,14 +647,14 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
 // Dispatch of a multi-pass batch: staged in LDS when its chains fit kHeavyPasses
 // rounds of the workgroup's slots, else pass by pass.
 template <typename T, int BW, int BLOCK>
-__device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T>& s, int pos, int g, int sweep,
-                                           double beta) {
+__device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T>& s, const DevBatches& bt, int batch,
+                                           int g, int sweep, double beta) {
     constexpr int kSlots = BLOCK / BW;
-    const int k0 = p.col_ptr[pos], k1 = p.col_ptr[pos + 1];
-    if (k1 - k0 <= kHeavyPasses * kSlots * kSeg)
-        heavy_lds_update<T, BW, BLOCK>(p, s, pos, k0, k1, g, sweep, beta);
+    const HeavyBit h = heavy_bit(bt, batch, kSlots);
+    if (h.k1 - h.k0 <= kHeavyPasses * kSlots * kSeg)
+        heavy_lds_update<T, BW, BLOCK>(p, s, h, g, sweep, beta);
     else
-        heavy_update<T, BW, BLOCK>(p, s, pos, g, sweep, beta);
+        heavy_update<T, BW, BLOCK>(p, s, h, g, sweep, beta);
 }
 
 // The light bits of one class, [lb, le): no ray entry, at most kLightNb neighbours,
@@ -861,7 +897,7 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 if (heavy) {
                     if (fetch1) fetch_static(p, nxt);
                     if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
-                    heavy_step<T, BW, BLOCK>(p, s, bt.pos[bt.first[batch]], g, sweep, beta);
+                    heavy_step<T, BW, BLOCK>(p, s, bt, batch, g, sweep, beta);
                 } else {
                     packed_step<T, BW, BLOCK>(
                         p, s, cur, g, sweep, beta,
