@@ -143,6 +143,15 @@ def test_ray_muon_small_fp32_recompute(gpu):
     assert out["report"]["fp32_residual_err"] < 1e-3
 
 
+def test_ray_full_muon_qubo(gpu):
+    """The benchmark problem itself (306,328 bits, 115 classes, multi-pass bits of up to
+    4,353 entries, two classes of ~90k ray-free bits handed out in claimed chunks) for
+    10 sweeps of 16 branches, FP32 with recompute every 4."""
+    P = qa.muon_scene()
+    _, out = check_branches(P, qa.schedule(50.0, 1.0, 8, 2), 16, precision=qa.FP32, recompute_every=4)
+    say(f"kernel_ms {out['report']['kernel_ms']:.2f}")
+
+
 def test_ray_uneven_branch_groups_and_offset(gpu):
     """20 branches (a full group and a partial one) with seed_offset 5."""
     P = small_muon(grid=24, sky_az=60, sky_th=15)
