@@ -161,8 +161,11 @@ static double term(const orc_problem* p, const double* r64, const float* r32, in
 }
 
 /* dE under a mode: the entries summed in one chain (segment == 0, the reference's
-   order) or in chains of `segment` entries whose partial sums are added in order,
-   the first chain seeded with lin * delta; then the neighbour terms in CSR order. */
+   order) or in chains of `segment` entries, the first seeded with lin * delta; the
+   chain sums c_j are added in runs of ORC_CHAIN_GROUP, G_g = ((c_8g + c_8g+1) + ...),
+   and the runs in order, dE = ((G_0 + G_1) + ...) -- with at most ORC_CHAIN_GROUP
+   chains, the chain sums added in order. Then the neighbour terms in CSR order. */
+#define ORC_CHAIN_GROUP 8
 static double mode_dE(const orc_problem* p, const uint8_t* x, const double* r64, const float* r32, int i,
                       int segment) {
     const double delta = x[i] ? -1.0 : 1.0;
@@ -171,12 +174,22 @@ static double mode_dE(const orc_problem* p, const uint8_t* x, const double* r64,
     if (segment <= 0 || k1 - k0 <= segment) {
         for (int k = k0; k < k1; ++k) dE += term(p, r64, r32, k, delta);
     } else {
-        for (int k = k0; k < k0 + segment; ++k) dE += term(p, r64, r32, k, delta);
-        for (int s0 = k0 + segment; s0 < k1; s0 += segment) {
+        const int n_chain = (k1 - k0 + segment - 1) / segment;
+        double g = 0.0;
+        for (int j = 0; j < n_chain; ++j) {
+            const int s0 = k0 + j * segment;
             const int s1 = s0 + segment < k1 ? s0 + segment : k1;
-            double part = term(p, r64, r32, s0, delta);
-            for (int k = s0 + 1; k < s1; ++k) part += term(p, r64, r32, k, delta);
-            dE += part;
+            double c;
+            if (j == 0) {
+                c = dE;
+                for (int k = s0; k < s1; ++k) c += term(p, r64, r32, k, delta);
+            } else {
+                c = term(p, r64, r32, s0, delta);
+                for (int k = s0 + 1; k < s1; ++k) c += term(p, r64, r32, k, delta);
+            }
+            g = j % ORC_CHAIN_GROUP == 0 ? c : g + c;
+            if (j % ORC_CHAIN_GROUP == ORC_CHAIN_GROUP - 1 || j == n_chain - 1)
+                dE = j < ORC_CHAIN_GROUP ? g : dE + g;
         }
     }
     for (int k = p->nbr_ptr[i]; k < p->nbr_ptr[i + 1]; ++k) {

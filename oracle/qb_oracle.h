@@ -68,7 +68,8 @@ typedef struct {
     int recompute_every;   /* >0: FP64 from-scratch residuals after every k-th sweep */
     int segment;           /* 0: dE sums a bit's entries in one sequential chain (the
                               reference); S > 0: in chains of S entries, the first seeded
-                              with lin*delta, the chains then added in order */
+                              with lin*delta, the chain sums added in runs of 8 and the
+                              runs in order */
 } orc_mode;
 
 /* ---- primitives ---- */

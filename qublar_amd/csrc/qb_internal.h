@@ -15,9 +15,11 @@ namespace qb {
 constexpr int kGroup = 16;
 
 // A bit's dE sums its CSR entries in chains of kSeg entries, the first seeded with
-// lin * delta, the chain sums then added in order. Bits with more than kSeg entries
-// ("heavy") are evaluated by a whole workgroup; they come first inside their class.
+// lin * delta; the chain sums are added in runs of kChainRun (each run in order) and
+// the run sums in order -- with at most kChainRun chains, the chain sums in order.
+// Bits with more than kSeg entries come first inside their class.
 constexpr int kSeg = 16;
+constexpr int kChainRun = 8;
 
 // One CSC entry of a bit in colour-class position order: the ray, the bit's share
 // a of it, and the ray's weight w (copied per entry so the sweep streams it).

@@ -140,9 +140,10 @@ __device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int
 }
 
 // The dE of a bit sums its CSR entries in chains of kSeg: the first chain starts
-// from lin * delta, later chains start from their first term, and the chain sums
-// are added in order; the neighbour terms follow in CSR order. For a bit with at
-// most kSeg entries this is local_field_dE's order exactly (ising_recon.hpp:105-118).
+// from lin * delta, later chains start from their first term; the chain sums are
+// added in runs of kChainRun (in order inside a run) and the run sums in order; the
+// neighbour terms follow in CSR order. For a bit with at most kSeg entries this is
+// local_field_dE's order exactly (ising_recon.hpp:105-118).
 
 // Loads of one chain: the entries [ka, kb) (1 <= kb - ka <= kSeg) and the group's
 // residuals on their rays. Every load is unconditional -- past kb the index is
@@ -303,8 +304,8 @@ __device__ __forceinline__ void fetch_static(const DevProblem& p, SlotStatic& S)
 }
 
 // One packed batch for the BW branches of group g: every slot evaluates one chain of
-// one bit; the chain sums meet in LDS, the bit's first slot adds them in order, adds
-// the neighbour terms and decides; each slot then scatters its own chain from
+// one bit; the chain sums meet in LDS, the first slot of each run of kChainRun chains
+// adds the run, the bit's first slot adds the runs and the neighbour terms and decides; each slot then scatters its own chain from
 // registers. `after_issue` runs once the state loads are issued, `after_compute` once
 // this slot's chain sum is in LDS (the caller's prefetch of the next step).
 template <typename T, int BW, int BLOCK, typename F1, typename F2>
@@ -363,15 +364,30 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
     }
     after_compute();
     __syncthreads();
+    // the first slot of every run of kChainRun chains adds the run's sums in order, in
+    // place (no other slot reads them)
+    if (pos >= 0 && j % kChainRun == 0) {
+        const int nch = slot_nch(info);
+        if (j + 1 < nch) {
+            double v[kChainRun];
+#pragma unroll
+            for (int u = 1; u < kChainRun; ++u) v[u] = part[min(slot + u, kSlots - 1)][b];
+            double g = part[slot][b];
+#pragma unroll
+            for (int u = 1; u < kChainRun; ++u) g = j + u < nch ? g + v[u] : g;
+            part[slot][b] = g;
+        }
+    }
+    __syncthreads();
     if (pos >= 0 && j == 0) {
         const int nch = slot_nch(info);
         double dE = part[slot][b];
-        for (int q0 = 1; q0 < nch; q0 += 8) {
+        for (int q0 = kChainRun; q0 < nch; q0 += 8 * kChainRun) {
             double v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = part[min(slot + q0 + u, kSlots - 1)][b];
+            for (int u = 0; u < 8; ++u) v[u] = part[min(slot + q0 + u * kChainRun, kSlots - 1)][b];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) dE = q0 + u < nch ? dE + v[u] : dE;
+            for (int u = 0; u < 8; ++u) dE = q0 + u * kChainRun < nch ? dE + v[u] : dE;
         }
         // lambda * (+-1) per neighbour in CSR order
 #pragma unroll
@@ -420,7 +436,7 @@ __device__ __forceinline__ HeavyBit heavy_bit(const DevBatches& bt, int batch, i
 
 // A bit with more chains than a workgroup has slots, by a whole workgroup in several
 // passes: the chains are spread over the block's BW-lane slots, their sums meet in
-// LDS, slot 0 adds them in order and decides, and the residual scatter is spread
+// LDS, slot 0 adds them (in runs of kChainRun) and decides, and the residual scatter is spread
 // over the block again.
 template <typename T, int BW, int BLOCK>
 __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState<T>& s, const HeavyBit& h, int g,
@@ -439,7 +455,7 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
     const int xb = (xm >> b) & 1;
     const double delta = xb ? -1.0 : 1.0;
     const int n_chain = (k1 - k0 + kSeg - 1) / kSeg;
-    double dE = 0.0;
+    double dE = 0.0, run = 0.0;
     Chain<T> ch;
     for (int j0 = 0; j0 < n_chain; j0 += kSlots) {
         const int j = j0 + slot;
@@ -452,7 +468,11 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
         __syncthreads();
         if (slot == 0) {
             const int m = min(n_chain - j0, kSlots);
-            for (int q = 0; q < m; ++q) dE = (j0 + q == 0) ? part[0][b] : dE + part[q][b];
+            for (int q = 0; q < m; ++q) {
+                const int jq = j0 + q;
+                run = jq % kChainRun == 0 ? part[q][b] : run + part[q][b];
+                if (jq % kChainRun == kChainRun - 1 || jq == n_chain - 1) dE = jq < kChainRun ? run : dE + run;
+            }
         }
         __syncthreads();
     }
@@ -588,14 +608,27 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
             }
         }
         __syncthreads();
+        // run r of kChainRun chain sums, added in order by slot r, in place
+        const int n_run = (n_chain + kChainRun - 1) / kChainRun;   // <= kHeavyPasses * kSlots / kChainRun
+        if (slot < n_run) {
+            const int j0 = slot * kChainRun;
+            double v[kChainRun];
+#pragma unroll
+            for (int u = 1; u < kChainRun; ++u) v[u] = hpart[min(j0 + u, kHeavyPasses * kSlots - 1)][b];
+            double g = hpart[j0][b];
+#pragma unroll
+            for (int u = 1; u < kChainRun; ++u) g = j0 + u < n_chain ? g + v[u] : g;
+            hpart[j0][b] = g;
+        }
+        __syncthreads();
         if (slot == 0) {
             double dE = hpart[0][b];
-            for (int q0 = 1; q0 < n_chain; q0 += 8) {
+            for (int r0 = 1; r0 < n_run; r0 += 8) {
                 double v[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = hpart[min(q0 + u, kHeavyPasses * kSlots - 1)][b];
+                for (int u = 0; u < 8; ++u) v[u] = hpart[min((r0 + u) * kChainRun, kHeavyPasses * kSlots - 1)][b];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) dE = q0 + u < n_chain ? dE + v[u] : dE;
+                for (int u = 0; u < 8; ++u) dE = r0 + u < n_run ? dE + v[u] : dE;
             }
             // lambda * (+-1) per neighbour in CSR order
 #pragma unroll

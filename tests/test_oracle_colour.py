@@ -225,12 +225,13 @@ def test_freeze_bound_holds(orc):
             assert orc.lib().orc_local_field_dE(prob.c, x, r, int(i)) >= 1.0 - 1e-9
 
 
-def long_column_problem(orc, exact, seed=21, n=40, n_rays=120):
-    """Bits with 20-70 rays each, so dE sums run over several chains of 16."""
+def long_column_problem(orc, exact, seed=21, n=40, n_rays=120, lengths=(20, 70)):
+    """Bits with 20-70 rays each, so dE sums run over several chains of 16 (with
+    lengths=(150, 400), over 10-25 chains: several runs of 8 chain sums)."""
     rng = np.random.default_rng(seed)
     row_ptr, ray, a = [0], [], []
     for _ in range(n):
-        rays = np.sort(rng.choice(n_rays, size=rng.integers(20, 70), replace=False))
+        rays = np.sort(rng.choice(n_rays, size=rng.integers(*lengths), replace=False))
         for b in rays:
             ray.append(b)
             a.append(rng.integers(1, 16) / 1024.0 if exact else rng.uniform(0.001, 0.05))
@@ -240,10 +241,15 @@ def long_column_problem(orc, exact, seed=21, n=40, n_rays=120):
     return orc.FloatProblem(row_ptr, ray, a, d, w, np.zeros(n + 1, np.int32), [], 0.0, 0.05)
 
 
-def test_segmented_sum_equals_sequential_when_exact(orc):
-    """With dyadic a, d, w every dE is exact in any order, so chains of 16 and one
-    chain give the same trajectory; a dropped or repeated term would not."""
-    prob = long_column_problem(orc, exact=True)
+LONG = [dict(), dict(n_rays=600, lengths=(150, 400))]
+
+
+@pytest.mark.parametrize("shape", LONG)
+def test_segmented_sum_equals_sequential_when_exact(orc, shape):
+    """With dyadic a, d, w every dE is exact in any order, so chains of 16 (their sums
+    added in runs of 8) and one chain give the same trajectory; a dropped or repeated
+    term, chain or run would not."""
+    prob = long_column_problem(orc, exact=True, **shape)
     order, cp = greedy_colouring(prob.n_vars, ray_conflicts(prob))
     sched = orc.schedule(2.0, 0.05, 40, 5)
     for seed in (1, 2, 3):
@@ -253,11 +259,12 @@ def test_segmented_sum_equals_sequential_when_exact(orc):
             assert np.array_equal(a, b) and 0 < a.sum() < prob.n_vars
 
 
-def test_segmented_sum_is_a_rounding_change_only(orc):
+@pytest.mark.parametrize("shape", LONG)
+def test_segmented_sum_is_a_rounding_change_only(orc, shape):
     """With generic floats the two orders differ only by rounding: over many states the
     segmented dE stays within 1e-12 relative of the sequential one (checked through
     one-sweep decisions at a temperature where every decision is far from the threshold)."""
-    prob = long_column_problem(orc, exact=False)
+    prob = long_column_problem(orc, exact=False, **shape)
     order, cp = greedy_colouring(prob.n_vars, ray_conflicts(prob))
     sched = orc.schedule(1e-6, 1e-6, 1, 0)   # T -> 0: accept iff dE <= 0
     for seed in (1, 2):
