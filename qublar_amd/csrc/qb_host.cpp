@@ -638,12 +638,13 @@ extern "C" int qb_problem_colouring(const qb_problem* p, int32_t* order, int32_t
 }
 
 // Workgroups per XCD team: `requested` total (a multiple of 8), or one per CU.
-static int team_size_for(int requested, size_t table_bytes) {
+static int team_size_for(int requested, int precision, int bw, size_t table_bytes) {
     hipDeviceProp_t prop;
     int rc = device_props(prop);
     if (rc) return -rc;
-    const int per_cu = ray_sweep_blocks_per_cu(table_bytes);
-    if (per_cu <= 0) return -fail(QB_E_HIP, "occupancy query for the sweep kernel failed");
+    const int per_cu = ray_sweep_blocks_per_cu(precision, bw, table_bytes);
+    if (per_cu <= 0)
+        return -fail(QB_E_LIMIT, "the sweep kernel does not fit a CU with %zu bytes of class tables in LDS", table_bytes);
     const int cap = per_cu * prop.multiProcessorCount;
     const int total = requested > 0 ? requested : prop.multiProcessorCount;
     if (total % 8 != 0 || total > cap)
@@ -823,13 +824,13 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     const int bw = group_width_for(R, o.group_width);
     const int n_groups = (R + bw - 1) / bw;
     const size_t tsz = o.precision == QB_FP32 ? sizeof(float) : sizeof(double);
-    const int team_size = team_size_for(o.workgroups, 0);
+    const int team_size = team_size_for(o.workgroups, o.precision, bw, 0);
     if (team_size <= 0) return -team_size;
     const DevBatches* bt = nullptr;
     if ((rc = batches_for(P, bw, team_size, &bt))) return rc;
     {
         // the LDS tables must not lower the residency the team size was chosen for
-        const int checked = team_size_for(o.workgroups, sweep_table_bytes(P->n_classes));
+        const int checked = team_size_for(o.workgroups, o.precision, bw, sweep_table_bytes(P->n_classes));
         if (checked <= 0) return -checked;
     }
     const int n_part = ray_energy_blocks(P->dev, bw);

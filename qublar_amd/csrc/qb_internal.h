@@ -155,7 +155,9 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
                              uint64_t seed_offset, const double* beta, int sweep_begin, int sweep_end,
                              int recompute_every, unsigned* bar, unsigned* tickets, int* err, int team_size,
                              hipStream_t st);
-int ray_sweep_blocks_per_cu(size_t table_bytes);
+// Co-resident sweep workgroups per CU for this precision and group width, with
+// `table_bytes` of dynamic LDS (0 when they do not fit).
+int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes);
 constexpr int kSweepBlock = 256;   // threads per sweep workgroup
 // Entries of a bit with more chains than a sweep workgroup has slots at group width 2
 // (a multi-pass bit there).

@@ -312,39 +312,47 @@ template <typename T, int BW, int BLOCK, typename F1, typename F2>
 __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<T>& s, const SlotStatic& S, int g,
                                             int sweep, double beta, F1&& after_issue, F2&& after_compute) {
     constexpr int kSlots = BLOCK / BW;
+    constexpr int kSpw = 64 / BW;   // slots per wave
     __shared__ double part[kSlots][BW];
     __shared__ int accepted[kSlots][BW];
+    // per wave: its chains' rays [slot][entry], and the loaded residuals [slot][lane b][entry]
+    __shared__ __align__(16) int32_t rrow[kSlots * kSeg];
+    __shared__ __align__(16) T trow[kSlots * BW * kSeg];
     const int tid = int(threadIdx.x);
     const int b = tid % BW, slot = tid / BW;
+    const int lane = tid & 63, wave = tid >> 6;
     const uint32_t info = S.d.info;
     const int pos = S.d.pos, len = slot_len(info), j = slot_j(info), nn = slot_nnb(info);
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
     T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
-    T rv[kSeg];
+    // Residual loads across the wave: load u of lane l is entry q % kSeg of the wave's
+    // chain q / kSeg, q = (64 u + l) / BW, so each load covers whole chains, whose rays
+    // share few lines, instead of one entry of 64 / BW unrelated chains. The rays go
+    // through this wave's LDS rows, and the residuals come back through them in chain
+    // order (a wave's LDS operations complete in order, so no barrier is needed).
+    int32_t* rw = rrow + wave * kSpw * kSeg;
+    T* tw = trow + wave * kSpw * BW * kSeg;
+    const int ls = slot - wave * kSpw;
+    if (b == 0) {
+        int4* row = reinterpret_cast<int4*>(rw + ls * kSeg);
 #pragma unroll
-    for (int u = 0; u < kSeg; ++u) rv[u] = T(0);
+        for (int u = 0; u < kSeg / 4; ++u) {
+            int r4[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) r4[v] = pos >= 0 && 4 * u + v < len ? S.e[4 * u + v].ray : -1;
+            row[u] = int4{r4[0], r4[1], r4[2], r4[3]};
+        }
+    }
+    T rvt[kSeg];
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u) {
+        const int ray = rw[(u * 64 + lane) / BW];
+        rvt[u] = ray >= 0 ? ld_l2(rg + size_t(ray) * BW + b) : T(0);
+    }
     uint16_t m[kNb];
     uint16_t xm = 0;
     if (pos >= 0) {
         xm = ld_l2(xg + pos);
-        // residual loads in quarters of the chain: lanes past their chain's end are
-        // masked off and send no request
-        if (len > 0) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) rv[u] = ld_l2(rg + size_t(S.e[u].ray) * BW + b);
-        }
-        if (len > 4) {
-#pragma unroll
-            for (int u = 4; u < 8; ++u) rv[u] = ld_l2(rg + size_t(S.e[u].ray) * BW + b);
-        }
-        if (len > 8) {
-#pragma unroll
-            for (int u = 8; u < 12; ++u) rv[u] = ld_l2(rg + size_t(S.e[u].ray) * BW + b);
-        }
-        if (len > 12) {
-#pragma unroll
-            for (int u = 12; u < 16; ++u) rv[u] = ld_l2(rg + size_t(S.e[u].ray) * BW + b);
-        }
         if (j == 0 && nn > 0) {
 #pragma unroll
             for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + S.q[u]);
@@ -355,6 +363,15 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
     double u = 0.0;
     if (pos >= 0 && j == 0)
         u = draw_u01(s.seed_offset + S.key + uint64_t(g * BW + b) + 1u, uint32_t(sweep), uint32_t(S.bit));
+    // the residuals back in chain order
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u) {
+        const int q = (u * 64 + lane) / BW;
+        tw[((q / kSeg) * BW + b) * kSeg + q % kSeg] = rvt[u];
+    }
+    T rv[kSeg];
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u) rv[u] = tw[(ls * BW + b) * kSeg + u];
     const int xb = (xm >> b) & 1;
     const double delta = xb ? -1.0 : 1.0;
     if (pos >= 0) {
@@ -1241,14 +1258,14 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
     });
 }
 
-int ray_sweep_blocks_per_cu(size_t table_bytes) {
+int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&ray_team_kernel<double, 16, kSweepBlock>), kSweepBlock,
-            table_bytes) !=
-        hipSuccess)
-        return 0;
-    return per_cu;
+    const hipError_t e = with_bw(bw, [&]<int BW>() {
+        const void* k = precision == 1 ? reinterpret_cast<const void*>(&ray_team_kernel<float, BW, kSweepBlock>)
+                                       : reinterpret_cast<const void*>(&ray_team_kernel<double, BW, kSweepBlock>);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kSweepBlock, table_bytes);
+    });
+    return e == hipSuccess ? per_cu : 0;
 }
 
 hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, float* p_void,
