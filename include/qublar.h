@@ -232,6 +232,63 @@ int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* view);
 int qb_host_problem_truth(const qb_host_problem* h, uint8_t* truth);
 int qb_host_problem_destroy(qb_host_problem* h);
 
+/* ---- batched sparse QUBOs ---------------------------------------------------------
+   No single counterpart in the reference: it solves one QUBO per call (anneal_branch /
+   branch_fractions, src/ising_recon.hpp:121-166, with a QUBO's pair terms encoded as
+   two-bit rays, src/check_ree_qubo.cu:172-192). Here many independent QUBOs
+       E_k(x) = sum_i h_i x_i + sum_{i<j} J_ij x_i x_j,   x in {0,1}^n_k,
+   each with its own replicas and seed, are annealed in one persistent launch. Replica r
+   of instance k anneals from all-zero x with draws keyed by seed_k + r + 1 and counter
+   (bit, sweep), visiting the instance's colour classes in order; a flip is accepted
+   iff dE <= 0 or u < qexp(-dE / T), with dE summed in FP64 over the bit's CSR row in
+   order. Results are identical in every launch shape and to the CPU oracle. */
+typedef struct {
+    int n_instances;
+    const int32_t* n_bits;      /* per instance */
+    const int32_t* row_ptr;     /* per instance n_k + 1 offsets into its own entries, concatenated
+                                   (sum (n_k + 1) values; each instance's list starts at 0) */
+    const int32_t* col;         /* entries of all instances, concatenated; instance-local bit index */
+    const float* J;             /* coupling per entry; J_ij listed in row i and in row j, equal */
+    const float* h;             /* per bit, concatenated */
+    const int32_t* replicas;    /* per instance, 1..32 */
+    const uint64_t* seed;       /* per instance */
+} qb_qubo_batch_desc;
+
+typedef struct qb_qubo_batch qb_qubo_batch;
+
+/* Validates (symmetric rows, no self-coupling, indices in range), colours each instance
+   and uploads the batch. */
+int qb_qubo_batch_create(const qb_qubo_batch_desc* desc, qb_qubo_batch** out);
+int qb_qubo_batch_destroy(qb_qubo_batch* b);
+
+typedef struct {
+    int mode;                 /* 0: one persistent launch for the whole batch; 1: one launch per instance */
+    int replica_block;        /* replicas per job: 4, 8, 16 or 32; 0 = chosen by timing each on a short run */
+    int workgroups;           /* persistent workgroups; 0 = 4 per CU */
+} qb_qubo_opts;
+
+typedef struct {
+    double wall_s;            /* state init to outputs on the host */
+    double kernel_ms;         /* annealing kernels only, HIP events */
+    int launches;
+    int replica_block;        /* the block the run used */
+    int jobs;
+    double spin_updates;      /* sum over instances of replicas x sweeps x n_k */
+} qb_qubo_report;
+
+/* Anneals every replica of every instance. x: sum_k R_k n_k bytes, instance-major then
+   replica-major (nullable); energy: sum_k R_k doubles (nullable), FP64 from scratch. */
+int qb_qubo_batch_anneal(qb_qubo_batch* b, const qb_schedule* s, const qb_qubo_opts* opts, uint8_t* x,
+                         double* energy, qb_qubo_report* rep);
+/* The colouring qb_qubo_batch_create gives an instance, on the host without a device
+   (rows checked as there; class_ptr needs room for n + 1). */
+int qb_colour_qubo(int n, const int32_t* row_ptr, const int32_t* col, const float* J, int32_t* order,
+                   int32_t* class_ptr, int* n_classes);
+/* Instance k's colouring: order[n_k] (bits grouped by class) and class_ptr[n_classes + 1];
+   *n_classes receives the count (class_ptr needs room for n_k + 1). */
+int qb_qubo_batch_colouring(const qb_qubo_batch* b, int instance, int32_t* order, int32_t* class_ptr,
+                            int* n_classes);
+
 /* The Erdos-Renyi max-cut workload: splitmix64 from `seed` over pairs i < j in order;
    a pair is an edge when u < p, and its weight is +1 when the next draw's top bit is
    0, else -1. Fills up to cap edges; *m_out = edge count. */

@@ -108,6 +108,10 @@ def lib():
         L.orc_tts99_bootstrap.argtypes = [_i32p, _f64p, C.c_size_t, C.c_int, C.c_uint64,
                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_sha256_hex.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p]
+        L.orc_qubo_energy.restype = C.c_double
+        L.orc_qubo_energy.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _u8p]
+        L.orc_qubo_anneal.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _i32p, C.POINTER(Schedule),
+                                      C.c_uint64, C.c_int, _u8p]
         _LIB = L
     return _LIB
 
@@ -245,6 +249,26 @@ def anneal_int(g: IntGraph, sched, seed, mode=None):
     e = C.c_int64()
     lib().orc_anneal_int(C.byref(g.c), C.byref(sched), C.c_uint64(seed), C.byref(mode), x, C.byref(e))
     return x, e.value
+
+
+def _qubo_arrays(row_ptr, col, J, h):
+    return (np.ascontiguousarray(row_ptr, np.int32), np.ascontiguousarray(col, np.int32),
+            np.ascontiguousarray(J, np.float32), np.ascontiguousarray(h, np.float32))
+
+
+def qubo_anneal(row_ptr, col, J, h, order, sched, seed, R):
+    """The batch engine's trajectory for one QUBO instance: R replicas, uint8[R, n]."""
+    rp, cl, Jv, hv = _qubo_arrays(row_ptr, col, J, h)
+    n = len(hv)
+    x = np.zeros((R, n), np.uint8)
+    lib().orc_qubo_anneal(n, rp, cl, Jv, hv, np.ascontiguousarray(order, np.int32), C.byref(sched),
+                          C.c_uint64(seed), R, x)
+    return x
+
+
+def qubo_energy(row_ptr, col, J, h, x):
+    rp, cl, Jv, hv = _qubo_arrays(row_ptr, col, J, h)
+    return lib().orc_qubo_energy(len(hv), rp, cl, Jv, hv, np.ascontiguousarray(x, np.uint8))
 
 
 def philox(ctr, key):

@@ -636,3 +636,45 @@ void orc_sha256_hex(const void* data, size_t n, char out[65]) {
         for (int k = 0; k < 8; ++k) out[8 * i + k] = hexd[(h[i] >> (28 - 4 * k)) & 15u];
     out[64] = 0;
 }
+
+/* ---- batched sparse QUBOs -------------------------------------------------------------
+   One instance E(x) = sum_i h_i x_i + sum_{i<j} J_ij x_i x_j (J listed in both rows),
+   replica r annealed from all-zero x with draws keyed seed + r + 1, counter (bit, sweep),
+   classes visited in the given order; dE = (1 - 2 x_i)(h_i + sum_j J_ij x_j) summed in
+   FP64 over the row in CSR order; accept iff dE <= 0 or u < qexp(-dE * beta). No
+   reference counterpart beyond anneal_branch's test and schedule
+   (src/ising_recon.hpp:121-144); the batch engine's trajectory is defined here. */
+double orc_qubo_energy(int n, const int* row_ptr, const int* col, const float* J, const float* h,
+                       const uint8_t* x) {
+    double e = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (!x[i]) continue;
+        double pair = 0.0;
+        for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+            if (x[col[k]]) pair += (double)J[k];
+        e += (double)h[i] + 0.5 * pair;
+    }
+    return e;
+}
+
+void orc_qubo_anneal(int n, const int* row_ptr, const int* col, const float* J, const float* h, const int* order,
+                     const orc_schedule* s, uint64_t seed, int R, uint8_t* x_out) {
+    const int total = s->anneal_sweeps + s->hold_sweeps;
+    for (int r = 0; r < R; ++r) {
+        uint8_t* x = x_out + (size_t)r * (size_t)n;
+        memset(x, 0, (size_t)n);
+        const uint64_t key = seed + (uint64_t)r + 1u;
+        for (int sweep = 0; sweep < total; ++sweep) {
+            const double beta = 1.0 / orc_temperature(s, sweep);
+            for (int t = 0; t < n; ++t) {
+                const int i = order[t];
+                double acc = (double)h[i];
+                for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+                    if (x[col[k]]) acc += (double)J[k];
+                const double dE = x[i] ? -acc : acc;
+                if (dE <= 0.0 || orc_philox_uniform(key, (uint32_t)sweep, (uint32_t)i) < orc_qexp(-dE * beta))
+                    x[i] ^= 1u;
+            }
+        }
+    }
+}

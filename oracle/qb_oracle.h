@@ -126,6 +126,15 @@ void orc_tts99_bootstrap(const int* success, const double* time, size_t r, int r
                          uint64_t seed, double* lo, double* hi);  /* :279-303 */
 void orc_sha256_hex(const void* data, size_t n, char out[65]);   /* :307-379 */
 
+/* ---- batched sparse QUBOs (the batch engine's trajectory) ---- */
+double orc_qubo_energy(int n, const int* row_ptr, const int* col, const float* J, const float* h,
+                       const uint8_t* x);
+/* order: the instance's bits grouped by colour class (a proper colouring of its
+   coupling graph, so bits of a class are independent and any order inside it gives
+   the same result). x_out: R x n. */
+void orc_qubo_anneal(int n, const int* row_ptr, const int* col, const float* J, const float* h, const int* order,
+                     const orc_schedule* s, uint64_t seed, int R, uint8_t* x_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,20 @@ class Plan(C.Structure):
                 ("refused", C.c_char * 64), ("n_classes", C.c_int)]
 
 
+class QuboBatchDesc(C.Structure):
+    _fields_ = [("n_instances", C.c_int), ("n_bits", C.c_void_p), ("row_ptr", C.c_void_p), ("col", C.c_void_p),
+                ("J", C.c_void_p), ("h", C.c_void_p), ("replicas", C.c_void_p), ("seed", C.c_void_p)]
+
+
+class QuboOpts(C.Structure):
+    _fields_ = [("mode", C.c_int), ("replica_block", C.c_int), ("workgroups", C.c_int)]
+
+
+class QuboReport(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
+                ("replica_block", C.c_int), ("jobs", C.c_int), ("spin_updates", C.c_double)]
+
+
 class BatchReport(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
                 ("sweeps_per_launch", C.c_int)]
@@ -132,6 +146,12 @@ def lib():
             "qb_host_problem_truth": (i32, [vp, vp]),
             "qb_host_problem_destroy": (i32, [vp]),
             "qb_scene_er_graph": (i32, [i32, dbl, u64, vp, vp, vp, i64, C.POINTER(i64)]),
+            "qb_qubo_batch_create": (i32, [C.POINTER(QuboBatchDesc), C.POINTER(vp)]),
+            "qb_qubo_batch_destroy": (i32, [vp]),
+            "qb_qubo_batch_anneal": (i32, [vp, C.POINTER(Schedule), C.POINTER(QuboOpts), vp, vp,
+                                           C.POINTER(QuboReport)]),
+            "qb_qubo_batch_colouring": (i32, [vp, i32, vp, vp, C.POINTER(i32)]),
+            "qb_colour_qubo": (i32, [i32, vp, vp, vp, vp, vp, C.POINTER(i32)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -415,3 +435,149 @@ def er_graph(n=800, p=0.06, seed=1):
     m = C.c_int64()
     _check(lib().qb_scene_er_graph(n, float(p), C.c_uint64(seed), _p(eu), _p(ev), _p(ew), cap, C.byref(m)))
     return GraphArrays(n, eu[:m.value], ev[:m.value], ew[:m.value])
+
+
+# ---- batched sparse QUBOs ------------------------------------------------------------------
+
+class QuboInstance:
+    """One sparse QUBO E(x) = sum_i h_i x_i + sum_{i<j} J_ij x_i x_j, as a symmetric CSR
+    (J_ij in row i and row j), with its replica count and seed."""
+
+    def __init__(self, row_ptr, col, J, h, replicas, seed):
+        self.row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+        self.col = np.ascontiguousarray(col, np.int32)
+        self.J = np.ascontiguousarray(J, np.float32)
+        self.h = np.ascontiguousarray(h, np.float32)
+        self.replicas = int(replicas)
+        self.seed = int(seed)
+        self.n = len(self.h)
+
+
+def colour_qubo(q):
+    """The engine's colouring of one instance, computed on the host: (order, class_ptr)."""
+    order = np.zeros(q.n, np.int32)
+    cp = np.zeros(q.n + 1, np.int32)
+    nc = C.c_int()
+    _check(lib().qb_colour_qubo(q.n, _p(q.row_ptr), _p(q.col), _p(q.J), _p(order), _p(cp), C.byref(nc)))
+    return order, cp[:nc.value + 1].copy()
+
+
+def random_regular_qubo(n, degree, seed, replicas=8):
+    """A sparse QUBO on a random graph of the given degree (the union of degree / 2 random
+    2-factors, plus a random perfect matching when the degree is odd; loops and repeated
+    pairs are dropped and the bits left short are paired again at random, a few rounds,
+    so all but a handful of bits have exactly the degree), J ~ N(0, 1) and h ~ N(0, 0.5)
+    as float32, from numpy's PCG64 seeded with `seed`."""
+    rng = np.random.default_rng(seed)
+    us, vs = [], []
+    for _ in range(degree // 2):
+        perm = rng.permutation(n)
+        us.append(np.arange(n)); vs.append(perm)
+    if degree % 2:
+        perm = rng.permutation(n)
+        half = n // 2
+        us.append(perm[:half]); vs.append(perm[half:2 * half])
+    u = np.concatenate(us).astype(np.int64); v = np.concatenate(vs).astype(np.int64)
+    keep = u != v
+    a, b = np.minimum(u[keep], v[keep]), np.maximum(u[keep], v[keep])
+    key = np.unique(a * n + b)
+    for _ in range(30):
+        deg = np.bincount(key // n, minlength=n) + np.bincount(key % n, minlength=n)
+        stubs = np.repeat(np.arange(n), np.maximum(degree - deg, 0))
+        if len(stubs) < 2:
+            break
+        stubs = rng.permutation(stubs)
+        half = len(stubs) // 2
+        u, v = stubs[:half], stubs[half:2 * half]
+        keep = u != v
+        a, b = np.minimum(u[keep], v[keep]), np.maximum(u[keep], v[keep])
+        new = np.unique(a * n + b)
+        new = new[~np.isin(new, key)]
+        key = np.union1d(key, new)
+    a, b = key // n, key % n
+    w = rng.normal(0.0, 1.0, len(key)).astype(np.float32)
+    rows = np.concatenate([a, b]); cols = np.concatenate([b, a]); ws = np.concatenate([w, w])
+    order = np.lexsort((cols, rows))
+    rows, cols, ws = rows[order], cols[order], ws[order]
+    row_ptr = np.zeros(n + 1, np.int64)
+    np.add.at(row_ptr, rows + 1, 1)
+    row_ptr = np.cumsum(row_ptr)
+    h = rng.normal(0.0, 0.5, n).astype(np.float32)
+    return QuboInstance(row_ptr, cols, ws, h, replicas, seed)
+
+
+def qubo_batch_workload(n_instances=256, seed=2026, n_range=(1000, 65536), degrees=(3, 8, 32, 64),
+                        replicas=(8, 32)):
+    """The batched-QUBO workload: n log-uniform in n_range, degree drawn from `degrees`,
+    replicas uniform in `replicas`, every instance from its own seed (derived from `seed`)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_instances):
+        n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        d = int(rng.choice(degrees))
+        r = int(rng.integers(replicas[0], replicas[1] + 1))
+        out.append(random_regular_qubo(n, d, int(rng.integers(1, 2**62)), r))
+    return out
+
+
+class QuboBatch:
+    """Many independent sparse QUBOs resident on the device, annealed together."""
+
+    def __init__(self, instances):
+        self.instances = list(instances)
+        k = len(self.instances)
+        self._n = np.array([q.n for q in self.instances], np.int32)
+        self._row = np.concatenate([q.row_ptr for q in self.instances]).astype(np.int32)
+        self._col = np.concatenate([q.col for q in self.instances]).astype(np.int32)
+        self._J = np.concatenate([q.J for q in self.instances]).astype(np.float32)
+        self._h = np.concatenate([q.h for q in self.instances]).astype(np.float32)
+        self._R = np.array([q.replicas for q in self.instances], np.int32)
+        self._seed = np.array([q.seed for q in self.instances], np.uint64)
+        d = QuboBatchDesc(k, _p(self._n), _p(self._row), _p(self._col), _p(self._J), _p(self._h), _p(self._R),
+                          _p(self._seed))
+        self.h = C.c_void_p()
+        _check(lib().qb_qubo_batch_create(C.byref(d), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qb_qubo_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def colouring(self, k):
+        n = self.instances[k].n
+        order = np.zeros(n, np.int32)
+        cp = np.zeros(n + 1, np.int32)
+        nc = C.c_int()
+        _check(lib().qb_qubo_batch_colouring(self.h, int(k), _p(order), _p(cp), C.byref(nc)))
+        return order, cp[:nc.value + 1].copy()
+
+    def anneal(self, sched, mode=0, replica_block=0, workgroups=0, x=True, energy=True):
+        """mode 0: one persistent launch; 1: one launch per instance. Returns x (per
+        instance uint8[R, n]), energy (per instance float64[R]) and the report."""
+        total_x = int(sum(q.replicas * q.n for q in self.instances))
+        total_r = int(self._R.sum())
+        xb = np.zeros(total_x, np.uint8) if x else None
+        eb = np.zeros(total_r, np.float64) if energy else None
+        rep = QuboReport()
+        o = QuboOpts(int(mode), int(replica_block), int(workgroups))
+        _check(lib().qb_qubo_batch_anneal(self.h, C.byref(sched), C.byref(o), _p(xb), _p(eb), C.byref(rep)))
+        out = {"report": {f: getattr(rep, f) for f, _ in QuboReport._fields_}}
+        if x:
+            xs, off = [], 0
+            for q in self.instances:
+                xs.append(xb[off:off + q.replicas * q.n].reshape(q.replicas, q.n))
+                off += q.replicas * q.n
+            out["x"] = xs
+        if energy:
+            es, off = [], 0
+            for q in self.instances:
+                es.append(eb[off:off + q.replicas])
+                off += q.replicas
+            out["energy"] = es
+        return out

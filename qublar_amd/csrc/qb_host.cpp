@@ -1183,3 +1183,358 @@ extern "C" int qb_device_energies(qb_graph* G, const uint8_t* X, int K, int64_t*
     QB_HIP(hipStreamSynchronize(st));
     return QB_OK;
 }
+
+// ==== batched sparse QUBOs ==============================================================
+
+struct qb_qubo_batch {
+    hipStream_t stream = nullptr;
+    int n_inst = 0;
+    int64_t n_total = 0;
+    std::vector<int32_t> n_bits, replicas, inst_pos, inst_cls, cls_ptr, pos_bit;
+    std::vector<int64_t> entries;     // per instance
+    std::vector<int64_t> x_off;       // per instance: offset of its replicas' bits in the x output
+    std::vector<int32_t> e_off;       // per instance: offset of its replicas in the energy output
+    std::vector<uint64_t> seed;
+    int total_replicas = 0;
+    DevBuf b_inst_pos, b_inst_cls, b_cls, b_row, b_col, b_J, b_h, b_bit, b_seed;
+    struct Jobs {
+        DevBuf lpt, by_inst;          // the same jobs, longest first and in instance order
+        std::vector<QuboJob> host;    // instance order
+        std::vector<int32_t> first;   // per instance: its first job in instance order
+        int64_t mask_words = 0;
+    };
+    std::map<int, std::unique_ptr<Jobs>> jobs;   // by replica block
+    DevBuf w_mask, w_beta, w_queue, w_energy;
+    std::vector<uint32_t> h_mask;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~qb_qubo_batch() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static int check_qubo_rows(int k, int n, const int32_t* row_ptr, const int32_t* col, const float* J);
+
+// A QUBO instance's colouring: saturation order over its coupling graph (ties by
+// degree, then index); pos_bit lists the bits class by class, cp the class bounds.
+static int colour_qubo(int n, const int32_t* rp, const int32_t* cl, std::vector<int32_t>& pos_bit,
+                       std::vector<int32_t>& cp) {
+    std::vector<int> by_weight(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) by_weight[size_t(i)] = i;
+    std::stable_sort(by_weight.begin(), by_weight.end(),
+                     [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+    std::vector<int> colour;
+    const int nc = saturation_colour(n, by_weight, {}, [&](int i, auto mark) {
+        for (int e = rp[i]; e < rp[i + 1]; ++e) mark(cl[e]);
+    }, colour);
+    order_by_colour(colour, nc, pos_bit, cp);
+    return nc;
+}
+
+extern "C" int qb_colour_qubo(int n, const int32_t* row_ptr, const int32_t* col, const float* J, int32_t* order,
+                              int32_t* class_ptr, int* n_classes) {
+    if (n <= 0 || !row_ptr || !order || !class_ptr || !n_classes) return fail(QB_E_ARG, "qb_colour_qubo: bad argument");
+    int rc = check_qubo_rows(0, n, row_ptr, col, J);
+    if (rc) return rc;
+    std::vector<int32_t> pos_bit, cp;
+    const int nc = colour_qubo(n, row_ptr, col, pos_bit, cp);
+    std::memcpy(order, pos_bit.data(), sizeof(int32_t) * size_t(n));
+    std::memcpy(class_ptr, cp.data(), sizeof(int32_t) * cp.size());
+    *n_classes = nc;
+    return QB_OK;
+}
+
+// Instance k's rows: symmetric (every J_ij also listed as J_ji, equal), no self
+// coupling, columns in range. Returns QB_OK or fails naming the first defect.
+static int check_qubo_rows(int k, int n, const int32_t* row_ptr, const int32_t* col, const float* J) {
+    if (row_ptr[0] != 0) return fail(QB_E_ARG, "instance %d: row_ptr must start at 0", k);
+    struct Pair {
+        uint64_t key;
+        float w;
+    };
+    std::vector<Pair> up, down;
+    for (int i = 0; i < n; ++i) {
+        if (row_ptr[i + 1] < row_ptr[i]) return fail(QB_E_ARG, "instance %d: row_ptr decreases at bit %d", k, i);
+        for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+            const int j = col[e];
+            if (j < 0 || j >= n) return fail(QB_E_ARG, "instance %d: bit %d names column %d of %d", k, i, j, n);
+            if (j == i) return fail(QB_E_ARG, "instance %d: bit %d couples to itself", k, i);
+            if (!std::isfinite(J[e])) return fail(QB_E_ARG, "instance %d: coupling of bits %d, %d is not finite", k, i, j);
+            const uint64_t key = (uint64_t(uint32_t(std::min(i, j))) << 32) | uint32_t(std::max(i, j));
+            (i < j ? up : down).push_back(Pair{key, J[e]});
+        }
+    }
+    auto by_key = [](const Pair& a, const Pair& b) { return a.key < b.key; };
+    std::sort(up.begin(), up.end(), by_key);
+    std::sort(down.begin(), down.end(), by_key);
+    if (up.size() != down.size()) return fail(QB_E_ARG, "instance %d: couplings are not symmetric", k);
+    for (size_t t = 0; t < up.size(); ++t) {
+        if (t > 0 && up[t].key == up[t - 1].key)
+            return fail(QB_E_ARG, "instance %d: bits %u and %u are coupled twice", k, unsigned(up[t].key >> 32),
+                        unsigned(up[t].key & 0xffffffffu));
+        if (up[t].key != down[t].key || up[t].w != down[t].w)
+            return fail(QB_E_ARG, "instance %d: coupling of bits %u and %u is not symmetric", k,
+                        unsigned(up[t].key >> 32), unsigned(up[t].key & 0xffffffffu));
+    }
+    return QB_OK;
+}
+
+extern "C" int qb_qubo_batch_create(const qb_qubo_batch_desc* d, qb_qubo_batch** out) {
+    if (!out) return fail(QB_E_ARG, "qb_qubo_batch_create: null output");
+    *out = nullptr;
+    if (!d || d->n_instances <= 0 || !d->n_bits || !d->row_ptr || !d->h || !d->replicas || !d->seed)
+        return fail(QB_E_ARG, "qb_qubo_batch_create: missing arrays or no instances");
+    auto B = std::make_unique<qb_qubo_batch>();
+    const int K = d->n_instances;
+    B->n_inst = K;
+    int64_t rows = 0, ents = 0, xs = 0;
+    for (int k = 0; k < K; ++k) {
+        const int n = d->n_bits[k];
+        if (n <= 0) return fail(QB_E_ARG, "instance %d has %d bits", k, n);
+        if (d->replicas[k] < 1 || d->replicas[k] > 32)
+            return fail(QB_E_ARG, "instance %d: replicas must be 1..32 (got %d)", k, d->replicas[k]);
+        const int32_t* rp = d->row_ptr + rows;
+        int rc = check_qubo_rows(k, n, rp, d->col + ents, d->J + ents);
+        if (rc) return rc;
+        B->n_bits.push_back(n);
+        B->replicas.push_back(d->replicas[k]);
+        B->entries.push_back(rp[n]);
+        B->x_off.push_back(xs);
+        B->e_off.push_back(B->total_replicas);
+        xs += int64_t(n) * d->replicas[k];
+        B->total_replicas += d->replicas[k];
+        rows += n + 1;
+        ents += rp[n];
+    }
+    if (B->n_total + rows > INT32_MAX || ents > INT32_MAX) return fail(QB_E_LIMIT, "the batch exceeds int32 indexing");
+    // colour every instance (saturation order over its coupling graph) and lay it out
+    std::vector<int32_t> row(1, 0), colp;
+    std::vector<float> Jp, hp;
+    std::vector<int32_t> bit;
+    B->inst_pos.push_back(0);
+    B->inst_cls.push_back(0);
+    rows = 0;
+    ents = 0;
+    int64_t hoff = 0;
+    for (int k = 0; k < K; ++k) {
+        const int n = B->n_bits[k];
+        const int32_t* rp = d->row_ptr + rows;
+        const int32_t* cl = d->col + ents;
+        const float* Jk = d->J + ents;
+        std::vector<int32_t> pos_bit, cp;
+        const int nc = colour_qubo(n, rp, cl, pos_bit, cp);
+        std::vector<int32_t> pos_of(static_cast<size_t>(n));
+        for (int p = 0; p < n; ++p) pos_of[size_t(pos_bit[size_t(p)])] = p;
+        const int32_t base = B->inst_pos.back();
+        for (int c = 0; c <= nc; ++c) B->cls_ptr.push_back(base + cp[size_t(c)]);
+        B->inst_cls.push_back(int32_t(B->cls_ptr.size()));
+        for (int p = 0; p < n; ++p) {
+            const int i = pos_bit[size_t(p)];
+            for (int e = rp[i]; e < rp[i + 1]; ++e) {
+                colp.push_back(base + pos_of[size_t(cl[e])]);
+                Jp.push_back(Jk[e]);
+            }
+            row.push_back(int32_t(colp.size()));
+            hp.push_back(d->h[hoff + i]);
+            bit.push_back(i);
+            B->pos_bit.push_back(i);
+        }
+        B->inst_pos.push_back(base + n);
+        B->seed.push_back(d->seed[k]);
+        rows += n + 1;
+        ents += rp[n];
+        hoff += n;
+    }
+    B->n_total = B->inst_pos.back();
+    QB_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    QB_HIP(hipEventCreate(&B->ev0));
+    QB_HIP(hipEventCreate(&B->ev1));
+    hipStream_t st = B->stream;
+    int rc;
+    if ((rc = B->b_inst_pos.upload(B->inst_pos, st)) || (rc = B->b_inst_cls.upload(B->inst_cls, st)) ||
+        (rc = B->b_cls.upload(B->cls_ptr, st)) || (rc = B->b_row.upload(row, st)) || (rc = B->b_col.upload(colp, st)) ||
+        (rc = B->b_J.upload(Jp, st)) || (rc = B->b_h.upload(hp, st)) || (rc = B->b_bit.upload(bit, st)) ||
+        (rc = B->b_seed.upload(B->seed, st)))
+        return rc;
+    QB_HIP(hipStreamSynchronize(st));
+    *out = B.release();
+    return QB_OK;
+}
+
+extern "C" int qb_qubo_batch_destroy(qb_qubo_batch* b) {
+    delete b;
+    return QB_OK;
+}
+
+// Jobs at replica block rb: instance k's replicas in blocks of rb, each block a job with
+// its own spin words. Uploaded in instance order and longest first (cost = rows +
+// entries, times replicas).
+static int qubo_jobs_for(qb_qubo_batch* B, int rb, const qb_qubo_batch::Jobs** out) {
+    auto it = B->jobs.find(rb);
+    if (it != B->jobs.end()) {
+        *out = it->second.get();
+        return QB_OK;
+    }
+    auto Jb = std::make_unique<qb_qubo_batch::Jobs>();
+    int64_t words = 0;
+    for (int k = 0; k < B->n_inst; ++k) {
+        Jb->first.push_back(int32_t(Jb->host.size()));
+        for (int r0 = 0; r0 < B->replicas[size_t(k)]; r0 += rb) {
+            QuboJob j{};
+            j.instance = k;
+            j.replica0 = r0;
+            j.replicas = std::min(rb, B->replicas[size_t(k)] - r0);
+            j.mask_offset = int32_t(words);
+            j.energy_offset = B->e_off[size_t(k)] + r0;
+            words += B->n_bits[size_t(k)];
+            Jb->host.push_back(j);
+        }
+    }
+    Jb->first.push_back(int32_t(Jb->host.size()));
+    if (words > INT32_MAX) return fail(QB_E_LIMIT, "the batch's spin words exceed int32 indexing");
+    Jb->mask_words = words;
+    std::vector<QuboJob> lpt = Jb->host;
+    auto cost = [&](const QuboJob& j) {
+        return (int64_t(B->n_bits[size_t(j.instance)]) + B->entries[size_t(j.instance)]) * j.replicas;
+    };
+    std::stable_sort(lpt.begin(), lpt.end(), [&](const QuboJob& a, const QuboJob& b) { return cost(a) > cost(b); });
+    int rc;
+    if ((rc = Jb->lpt.upload(lpt, B->stream)) || (rc = Jb->by_inst.upload(Jb->host, B->stream))) return rc;
+    QB_HIP(hipStreamSynchronize(B->stream));
+    *out = Jb.get();
+    B->jobs[rb] = std::move(Jb);
+    return QB_OK;
+}
+
+static DevQuboBatch qubo_dev(const qb_qubo_batch* B, const qb_qubo_batch::Jobs& J, bool lpt) {
+    return DevQuboBatch{int(J.host.size()), lpt ? J.lpt.as<QuboJob>() : J.by_inst.as<QuboJob>(),
+                        B->b_inst_pos.as<int32_t>(), B->b_inst_cls.as<int32_t>(), B->b_cls.as<int32_t>(),
+                        B->b_row.as<int32_t>(), B->b_col.as<int32_t>(), B->b_J.as<float>(), B->b_h.as<float>(),
+                        B->b_bit.as<int32_t>(), B->b_seed.as<uint64_t>()};
+}
+
+// One anneal of the whole batch at replica block rb: spin words cleared (all-zero x),
+// then the persistent launch or one launch per instance, timed with events.
+static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int n_sweeps, int mode, int workgroups,
+                    int* launches, float* ms) {
+    hipStream_t st = B->stream;
+    int rc;
+    if ((rc = B->w_mask.reserve(sizeof(uint32_t) * size_t(Jb.mask_words))) ||
+        (rc = B->w_queue.reserve(sizeof(unsigned) * size_t(B->n_inst + 1))))
+        return rc;
+    QB_HIP(hipMemsetAsync(B->w_mask.p, 0, sizeof(uint32_t) * size_t(Jb.mask_words), st));
+    QB_HIP(hipMemsetAsync(B->w_queue.p, 0, sizeof(unsigned) * size_t(B->n_inst + 1), st));
+    QB_HIP(hipEventRecord(B->ev0, st));
+    *launches = 0;
+    if (mode == 0) {
+        const DevQuboBatch q = qubo_dev(B, Jb, true);
+        QB_HIP(launch_qubo_batch(q, rb, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
+                                 B->w_queue.as<unsigned>(), 0, q.n_jobs, std::min(workgroups, q.n_jobs), st));
+        *launches = 1;
+    } else {
+        const DevQuboBatch q = qubo_dev(B, Jb, false);
+        for (int k = 0; k < B->n_inst; ++k) {
+            const int f = Jb.first[size_t(k)], cnt = Jb.first[size_t(k) + 1] - f;
+            QB_HIP(launch_qubo_batch(q, rb, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
+                                     B->w_queue.as<unsigned>() + k, f, cnt, cnt, st));
+            ++*launches;
+        }
+    }
+    QB_HIP(hipEventRecord(B->ev1, st));
+    QB_HIP(hipEventSynchronize(B->ev1));
+    QB_HIP(hipGetLastError());
+    QB_HIP(hipEventElapsedTime(ms, B->ev0, B->ev1));
+    return QB_OK;
+}
+
+extern "C" int qb_qubo_batch_anneal(qb_qubo_batch* B, const qb_schedule* s, const qb_qubo_opts* opts, uint8_t* x,
+                                    double* energy, qb_qubo_report* rep) {
+    if (!B) return fail(QB_E_ARG, "qb_qubo_batch_anneal: null batch");
+    int rc = check_schedule(s);
+    if (rc) return rc;
+    qb_qubo_opts o{0, 0, 0};
+    if (opts) o = *opts;
+    if (o.mode != 0 && o.mode != 1) return fail(QB_E_ARG, "mode must be 0 (persistent) or 1 (per instance)");
+    if (o.replica_block != 0 && o.replica_block != 4 && o.replica_block != 8 && o.replica_block != 16 &&
+        o.replica_block != 32)
+        return fail(QB_E_ARG, "replica_block must be 0, 4, 8, 16 or 32 (got %d)", o.replica_block);
+    if (o.workgroups < 0) return fail(QB_E_ARG, "negative workgroups");
+    hipDeviceProp_t prop;
+    if ((rc = device_props(prop))) return rc;
+    const int workgroups = o.workgroups > 0 ? o.workgroups : 4 * prop.multiProcessorCount;
+    const std::vector<double> beta = betas_of(*s);
+    const int n_sweeps = int(beta.size());
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((rc = B->w_beta.reserve(sizeof(double) * beta.size()))) return rc;
+    QB_HIP(hipMemcpyAsync(B->w_beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice, B->stream));
+    int rb = o.replica_block;
+    if (rb == 0) {
+        // the block shape that anneals a short prefix of the schedule fastest
+        float best = 0.0f;
+        for (int cand : {8, 16, 32}) {
+            const qb_qubo_batch::Jobs* Jb = nullptr;
+            if ((rc = qubo_jobs_for(B, cand, &Jb))) return rc;
+            int l = 0;
+            float ms = 0.0f;
+            if ((rc = qubo_run(B, *Jb, cand, std::min(n_sweeps, 8), o.mode, workgroups, &l, &ms))) return rc;
+            if (rb == 0 || ms < best) {
+                rb = cand;
+                best = ms;
+            }
+        }
+    }
+    const qb_qubo_batch::Jobs* Jb = nullptr;
+    if ((rc = qubo_jobs_for(B, rb, &Jb))) return rc;
+    int launches = 0;
+    float ms = 0.0f;
+    if ((rc = qubo_run(B, *Jb, rb, n_sweeps, o.mode, workgroups, &launches, &ms))) return rc;
+    hipStream_t st = B->stream;
+    if (energy) {
+        if ((rc = B->w_energy.reserve(sizeof(double) * size_t(B->total_replicas)))) return rc;
+        QB_HIP(launch_qubo_energy(qubo_dev(B, *Jb, false), B->w_mask.as<uint32_t>(), B->w_energy.as<double>(), st));
+        QB_HIP(hipMemcpyAsync(energy, B->w_energy.p, sizeof(double) * size_t(B->total_replicas), hipMemcpyDeviceToHost, st));
+    }
+    if (x) {
+        B->h_mask.resize(size_t(Jb->mask_words));
+        QB_HIP(hipMemcpyAsync(B->h_mask.data(), B->w_mask.p, sizeof(uint32_t) * size_t(Jb->mask_words),
+                              hipMemcpyDeviceToHost, st));
+    }
+    QB_HIP(hipStreamSynchronize(st));
+    if (x) {
+        for (const QuboJob& j : Jb->host) {
+            const int n = B->n_bits[size_t(j.instance)];
+            const int32_t base = B->inst_pos[size_t(j.instance)];
+            uint8_t* xk = x + B->x_off[size_t(j.instance)];
+            for (int p = 0; p < n; ++p) {
+                const uint32_t w = B->h_mask[size_t(j.mask_offset) + size_t(p)];
+                const int i = B->pos_bit[size_t(base + p)];
+                for (int r = 0; r < j.replicas; ++r) xk[size_t(j.replica0 + r) * size_t(n) + size_t(i)] = uint8_t((w >> r) & 1u);
+            }
+        }
+    }
+    if (rep) {
+        rep->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rep->kernel_ms = ms;
+        rep->launches = launches;
+        rep->replica_block = rb;
+        rep->jobs = int(Jb->host.size());
+        double upd = 0.0;
+        for (int k = 0; k < B->n_inst; ++k) upd += double(B->replicas[size_t(k)]) * double(B->n_bits[size_t(k)]);
+        rep->spin_updates = upd * n_sweeps;
+    }
+    return QB_OK;
+}
+
+extern "C" int qb_qubo_batch_colouring(const qb_qubo_batch* B, int k, int32_t* order, int32_t* class_ptr,
+                                       int* n_classes) {
+    if (!B || !order || !class_ptr || !n_classes) return fail(QB_E_ARG, "qb_qubo_batch_colouring: null argument");
+    if (k < 0 || k >= B->n_inst) return fail(QB_E_ARG, "instance %d of %d", k, B->n_inst);
+    const int32_t base = B->inst_pos[size_t(k)];
+    const int n = B->n_bits[size_t(k)];
+    std::memcpy(order, B->pos_bit.data() + base, sizeof(int32_t) * size_t(n));
+    const int c0 = B->inst_cls[size_t(k)], c1 = B->inst_cls[size_t(k) + 1];
+    for (int c = c0; c < c1; ++c) class_ptr[c - c0] = B->cls_ptr[size_t(c)] - base;
+    *n_classes = c1 - c0 - 1;
+    return QB_OK;
+}

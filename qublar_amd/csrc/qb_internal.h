@@ -183,4 +183,32 @@ int int_lds_capacity_spins();
 
 hipError_t launch_copy_probe(const void* src, void* dst, size_t n16, hipStream_t st);
 
+// ---- batched sparse QUBOs (qb_qubo.hip) ----
+// Positions: instances one after another, each in colour-major order (instance k's bits
+// are positions [inst_pos[k], inst_pos[k+1]), its classes cls_ptr[inst_cls[k] ..
+// inst_cls[k+1] - 1] as global positions). Rows keep each bit's CSR order.
+struct QuboJob {
+    int32_t instance, replica0, replicas;   // replicas [replica0, replica0 + replicas) of the instance
+    int32_t mask_offset;                    // the job's spin words: masks[mask_offset + local position]
+    int32_t energy_offset;                  // energy[energy_offset + r]
+    int32_t pad[3];
+};
+struct DevQuboBatch {
+    int n_jobs;
+    const QuboJob* jobs;         // longest first
+    const int32_t* inst_pos;     // n_instances + 1
+    const int32_t* inst_cls;     // n_instances + 1 offsets into cls_ptr
+    const int32_t* cls_ptr;      // per instance: its n_classes + 1 class bounds (global positions)
+    const int32_t* row_ptr;      // total bits + 1
+    const int32_t* col;          // neighbour position
+    const float* J;              // coupling per entry
+    const float* h;              // field per position
+    const int32_t* bit_id;       // per position: the bit's index inside its instance (draw counter)
+    const uint64_t* seed;        // per instance
+};
+constexpr int kQuboBlock = 256;
+hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, uint32_t* masks, const double* beta, int n_sweeps,
+                             unsigned* queue, int job_first, int job_count, int workgroups, hipStream_t st);
+hipError_t launch_qubo_energy(const DevQuboBatch& q, const uint32_t* masks, double* energy, hipStream_t st);
+
 }  // namespace qb
