@@ -184,6 +184,111 @@ def run_precision(args):
                       "rows": rows}), flush=True)
 
 
+def qubo_cpu_baseline(inst, sched_args, seconds, threads):
+    """The oracle's batch trajectory on host threads, one (instance, replica) per task, on
+    the first instances of the batch with the sweep count cut so the sample runs about
+    `seconds`: spin updates/s."""
+    import oracle
+    import qublar_amd as qa
+    from concurrent.futures import ThreadPoolExecutor
+
+    t_hot, t_cold = sched_args
+    sample = inst[:32]
+    orders = [qa.colour_qubo(q)[0] for q in sample]
+    probe = oracle.schedule(t_hot, t_cold, 1, 0)
+    t0 = time.perf_counter()
+    oracle.qubo_anneal(sample[0].row_ptr, sample[0].col, sample[0].J, sample[0].h, orders[0], probe, 1, 1)
+    per_upd = (time.perf_counter() - t0) / sample[0].n
+    total_upd = sum(q.n * q.replicas for q in sample)
+    sweeps = int(max(1, min(2000, seconds * threads / max(per_upd * total_upd, 1e-9))))
+    sched = oracle.schedule(t_hot, t_cold, sweeps, 0)
+
+    def task(k):
+        q = sample[k]
+        oracle.qubo_anneal(q.row_ptr, q.col, q.J, q.h, orders[k], sched, q.seed, q.replicas)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(task, range(len(sample))))
+    wall = time.perf_counter() - t0
+    return dict(value=total_upd * sweeps / wall, unit="spin updates/s", cores=threads, kind="port",
+                sample=f"the batch's first {len(sample)} instances ({total_upd} replica-bits), {sweeps} sweeps of "
+                       f"the same schedule's start, the engine's trajectory (oracle), {wall:.1f} s")
+
+
+def run_batched(args):
+    """256 sparse QUBOs (n log-uniform in [1k, 64k], degree 3/8/32/64, J ~ N(0,1),
+    h ~ N(0,0.5), 8-32 replicas), 2000 sweeps T 5 -> 0.05: one persistent launch (replica
+    block picked by timing each on a short run) against one launch per instance."""
+    import qublar_amd as qa
+
+    t0 = time.perf_counter()
+    inst = qa.qubo_batch_workload(args.instances, seed=2026)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    B = qa.QuboBatch(inst)
+    t_up = time.perf_counter() - t0
+    n = np.array([q.n for q in inst]); nnz = np.array([len(q.col) for q in inst]); R = np.array([q.replicas for q in inst])
+    log(f"batch: {len(inst)} instances, {n.sum()} bits, {nnz.sum()} entries, {int((n * R).sum())} replica-bits; "
+        f"generate {t_gen:.1f} s, check+colour+upload {t_up:.1f} s")
+    sweeps = args.batch_sweeps
+    sched = qa.schedule(5.0, 0.05, sweeps, 0)
+    upd = float((n * R).sum()) * sweeps
+    first = B.anneal(sched, mode=0, replica_block=0, x=False, energy=False)
+    rb, blk = first["report"]["replica_block"], first["report"]["block"]
+    log(f"tuned shape: {blk} threads, replica block {rb}: {first['report']['kernel_ms']:.1f} ms")
+    runs = []
+    for s in range(args.steps):
+        out = B.anneal(sched, mode=0, replica_block=rb, block=blk, x=False, energy=(s == args.steps - 1))
+        runs.append(out["report"])
+        log(f"persistent run {s}: kernel {out['report']['kernel_ms']:.1f} ms, wall {out['report']['wall_s'] * 1e3:.1f} ms")
+    # the same comparison at a tenth of the sweeps: one persistent launch against one
+    # launch per instance (each launch as wide as the instance's jobs)
+    short = qa.schedule(5.0, 0.05, max(1, sweeps // 10), 0)
+    pers_short = B.anneal(short, mode=0, replica_block=rb, block=blk, x=False, energy=False)["report"]
+    log(f"persistent, {short.anneal_sweeps} sweeps: kernel {pers_short['kernel_ms']:.1f} ms")
+    per = B.anneal(short, mode=1, replica_block=rb, block=blk, x=False, energy=False)["report"]
+    log(f"one launch per instance, {short.anneal_sweeps} sweeps: kernel {per['kernel_ms']:.1f} ms, "
+        f"{per['launches']} launches")
+    upd_short = float((n * R).sum()) * short.anneal_sweeps
+    kern = float(np.mean([r["kernel_ms"] for r in runs]))
+    wall = float(np.mean([r["wall_s"] for r in runs]))
+    jobs_words = sum(int(q.n) * ((q.replicas + rb - 1) // rb) for q in inst)
+    per_sweep = float(np.sum(4 * (n + 1) + 8 * nnz + 8 * n)) + 8.0 * jobs_words
+    alg = per_sweep * sweeps
+    result = {
+        "workload": "batched",
+        "metric": "spin updates/s, batched sparse QUBOs",
+        "value": upd / wall,
+        "unit": "spin updates/s",
+        "higher_is_better": True,
+        "steps": args.steps,
+        "kernel_ms": kern,
+        "wall_ms": wall * 1e3,
+        "kernel_updates_per_s": upd / (kern * 1e-3),
+        "per_instance_launches": {"sweeps": short.anneal_sweeps, "kernel_ms": per["kernel_ms"],
+                                  "updates_per_s": upd_short / (per["kernel_ms"] * 1e-3), "launches": per["launches"],
+                                  "persistent_kernel_ms": pers_short["kernel_ms"]},
+        "persistent_speedup_vs_per_instance": per["kernel_ms"] / pers_short["kernel_ms"],
+        "replica_block": rb,
+        "block": blk,
+        "jobs": runs[-1]["jobs"],
+        "config": {"workload": f"{len(inst)} sparse QUBOs, n log-uniform in [1k, 64k], degree in {{3, 8, 32, 64}}, "
+                               f"J ~ N(0, 1), h ~ N(0, 0.5), 8-32 replicas, {sweeps} sweeps T 5 -> 0.05",
+                   "bits": int(n.sum()), "entries": int(nnz.sum()), "replica_bits": int((n * R).sum())},
+        "roofline": {"bound": "hbm", "achieved": alg / (kern * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "qubo_batch_kernel", "algorithmic_bytes_per_launch": alg},
+        "dtype": "f64 dE, f32 couplings",
+        "data": "synthetic",
+    }
+    if not args.no_cpu:
+        threads = max(1, min(16, (os.cpu_count() or 2) - 2))
+        log(f"cpu baseline on {threads} threads")
+        result["cpu_baseline"] = qubo_cpu_baseline(inst, (5.0, 0.05), args.cpu_seconds, threads)
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,8 +300,11 @@ def main():
     ap.add_argument("--group-width", type=int, default=0, help="branches per group (2, 4, 8, 16); 0 = auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["muon", "serving", "precision"], default="muon",
-                    help="muon: the headline line (default); serving: the batched serving stream")
+    ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched"], default="muon",
+                    help="muon: the headline line (default); serving: the batched serving stream; "
+                         "batched: many sparse QUBOs in one persistent launch")
+    ap.add_argument("--instances", type=int, default=256, help="batched: instances")
+    ap.add_argument("--batch-sweeps", type=int, default=2000, help="batched: sweeps")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window")
     ap.add_argument("--pool", type=int, default=24, help="serving: distinct request problems")
@@ -217,6 +325,9 @@ def main():
         return
     if args.workload == "precision":
         run_precision(args)
+        return
+    if args.workload == "batched":
+        run_batched(args)
         return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5

@@ -238,10 +238,11 @@ int qb_host_problem_destroy(qb_host_problem* h);
    two-bit rays, src/check_ree_qubo.cu:172-192). Here many independent QUBOs
        E_k(x) = sum_i h_i x_i + sum_{i<j} J_ij x_i x_j,   x in {0,1}^n_k,
    each with its own replicas and seed, are annealed in one persistent launch. Replica r
-   of instance k anneals from all-zero x with draws keyed by seed_k + r + 1 and counter
-   (bit, sweep), visiting the instance's colour classes in order; a flip is accepted
-   iff dE <= 0 or u < qexp(-dE / T), with dE summed in FP64 over the bit's CSR row in
-   order. Results are identical in every launch shape and to the CPU oracle. */
+   of instance k anneals from all-zero x, visiting the instance's colour classes in
+   order; a flip is accepted iff dE <= 0 or u < qexp(-dE / T), with dE summed in FP64
+   over the bit's CSR row in order, and u the (r mod 2)-th 53-bit uniform of one
+   Philox4x32-10 call keyed by seed_k with counter (bit, sweep, r / 2, 0x51B0B47C).
+   Results are identical in every launch shape and to the CPU oracle. */
 typedef struct {
     int n_instances;
     const int32_t* n_bits;      /* per instance */
@@ -263,16 +264,19 @@ int qb_qubo_batch_destroy(qb_qubo_batch* b);
 
 typedef struct {
     int mode;                 /* 0: one persistent launch for the whole batch; 1: one launch per instance */
-    int replica_block;        /* replicas per job: 4, 8, 16 or 32; 0 = chosen by timing each on a short run */
-    int workgroups;           /* persistent workgroups; 0 = 4 per CU */
+    int replica_block;        /* replicas per job: 4, 8, 16 or 32 (0 with block 0: tuned) */
+    int workgroups;           /* persistent workgroups; 0 = as many as fit the device */
+    int block;                /* threads per workgroup: 256 or 1024; 0 with replica_block 0: the
+                                 (block, replica block) shape that runs a short prefix fastest */
 } qb_qubo_opts;
 
 typedef struct {
     double wall_s;            /* state init to outputs on the host */
     double kernel_ms;         /* annealing kernels only, HIP events */
     int launches;
-    int replica_block;        /* the block the run used */
+    int replica_block;        /* the replica block the run used */
     int jobs;
+    int block;                /* the threads per workgroup it used */
     double spin_updates;      /* sum over instances of replicas x sweeps x n_k */
 } qb_qubo_report;
 

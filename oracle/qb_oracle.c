@@ -639,11 +639,22 @@ void orc_sha256_hex(const void* data, size_t n, char out[65]) {
 
 /* ---- batched sparse QUBOs -------------------------------------------------------------
    One instance E(x) = sum_i h_i x_i + sum_{i<j} J_ij x_i x_j (J listed in both rows),
-   replica r annealed from all-zero x with draws keyed seed + r + 1, counter (bit, sweep),
-   classes visited in the given order; dE = (1 - 2 x_i)(h_i + sum_j J_ij x_j) summed in
-   FP64 over the row in CSR order; accept iff dE <= 0 or u < qexp(-dE * beta). No
-   reference counterpart beyond anneal_branch's test and schedule
-   (src/ising_recon.hpp:121-144); the batch engine's trajectory is defined here. */
+   replica r annealed from all-zero x, classes visited in the given order;
+   dE = (1 - 2 x_i)(h_i + sum_j J_ij x_j) summed in FP64 over the row in CSR order; accept
+   iff dE <= 0 or u < qexp(-dE * beta), u = the (r mod 2)-th uniform of the pair drawn by
+   one Philox4x32-10 call with key = seed and counter (bit, sweep, r / 2, 0x51B0B47C):
+   the top 53 bits of (o0 << 32 | o1), then of (o2 << 32 | o3). No reference counterpart
+   beyond anneal_branch's test and schedule (src/ising_recon.hpp:121-144); the batch
+   engine's trajectory is defined here. */
+static double qubo_draw(uint64_t seed, uint32_t sweep, uint32_t bit, int r) {
+    const uint32_t ctr[4] = {bit, sweep, (uint32_t)(r / 2), 0x51B0B47Cu};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    const uint64_t z = r % 2 == 0 ? (((uint64_t)o[0] << 32) | (uint64_t)o[1]) : (((uint64_t)o[2] << 32) | (uint64_t)o[3]);
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
 double orc_qubo_energy(int n, const int* row_ptr, const int* col, const float* J, const float* h,
                        const uint8_t* x) {
     double e = 0.0;
@@ -663,7 +674,6 @@ void orc_qubo_anneal(int n, const int* row_ptr, const int* col, const float* J, 
     for (int r = 0; r < R; ++r) {
         uint8_t* x = x_out + (size_t)r * (size_t)n;
         memset(x, 0, (size_t)n);
-        const uint64_t key = seed + (uint64_t)r + 1u;
         for (int sweep = 0; sweep < total; ++sweep) {
             const double beta = 1.0 / orc_temperature(s, sweep);
             for (int t = 0; t < n; ++t) {
@@ -672,7 +682,7 @@ void orc_qubo_anneal(int n, const int* row_ptr, const int* col, const float* J, 
                 for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
                     if (x[col[k]]) acc += (double)J[k];
                 const double dE = x[i] ? -acc : acc;
-                if (dE <= 0.0 || orc_philox_uniform(key, (uint32_t)sweep, (uint32_t)i) < orc_qexp(-dE * beta))
+                if (dE <= 0.0 || qubo_draw(seed, (uint32_t)sweep, (uint32_t)i, r) < orc_qexp(-dE * beta))
                     x[i] ^= 1u;
             }
         }

@@ -85,12 +85,12 @@ class QuboBatchDesc(C.Structure):
 
 
 class QuboOpts(C.Structure):
-    _fields_ = [("mode", C.c_int), ("replica_block", C.c_int), ("workgroups", C.c_int)]
+    _fields_ = [("mode", C.c_int), ("replica_block", C.c_int), ("workgroups", C.c_int), ("block", C.c_int)]
 
 
 class QuboReport(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
-                ("replica_block", C.c_int), ("jobs", C.c_int), ("spin_updates", C.c_double)]
+                ("replica_block", C.c_int), ("jobs", C.c_int), ("block", C.c_int), ("spin_updates", C.c_double)]
 
 
 class BatchReport(C.Structure):
@@ -557,7 +557,7 @@ class QuboBatch:
         _check(lib().qb_qubo_batch_colouring(self.h, int(k), _p(order), _p(cp), C.byref(nc)))
         return order, cp[:nc.value + 1].copy()
 
-    def anneal(self, sched, mode=0, replica_block=0, workgroups=0, x=True, energy=True):
+    def anneal(self, sched, mode=0, replica_block=0, workgroups=0, x=True, energy=True, block=0):
         """mode 0: one persistent launch; 1: one launch per instance. Returns x (per
         instance uint8[R, n]), energy (per instance float64[R]) and the report."""
         total_x = int(sum(q.replicas * q.n for q in self.instances))
@@ -565,7 +565,10 @@ class QuboBatch:
         xb = np.zeros(total_x, np.uint8) if x else None
         eb = np.zeros(total_r, np.float64) if energy else None
         rep = QuboReport()
-        o = QuboOpts(int(mode), int(replica_block), int(workgroups))
+        if (block == 0) != (replica_block == 0):
+            block = block or 256
+            replica_block = replica_block or 8
+        o = QuboOpts(int(mode), int(replica_block), int(workgroups), int(block))
         _check(lib().qb_qubo_batch_anneal(self.h, C.byref(sched), C.byref(o), _p(xb), _p(eb), C.byref(rep)))
         out = {"report": {f: getattr(rep, f) for f, _ in QuboReport._fields_}}
         if x:

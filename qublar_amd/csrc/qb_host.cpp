@@ -1416,8 +1416,8 @@ static DevQuboBatch qubo_dev(const qb_qubo_batch* B, const qb_qubo_batch::Jobs& 
 
 // One anneal of the whole batch at replica block rb: spin words cleared (all-zero x),
 // then the persistent launch or one launch per instance, timed with events.
-static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int n_sweeps, int mode, int workgroups,
-                    int* launches, float* ms) {
+static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int block, int n_sweeps, int mode,
+                    int workgroups, int* launches, float* ms) {
     hipStream_t st = B->stream;
     int rc;
     if ((rc = B->w_mask.reserve(sizeof(uint32_t) * size_t(Jb.mask_words))) ||
@@ -1429,14 +1429,14 @@ static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int
     *launches = 0;
     if (mode == 0) {
         const DevQuboBatch q = qubo_dev(B, Jb, true);
-        QB_HIP(launch_qubo_batch(q, rb, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
+        QB_HIP(launch_qubo_batch(q, rb, block, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
                                  B->w_queue.as<unsigned>(), 0, q.n_jobs, std::min(workgroups, q.n_jobs), st));
         *launches = 1;
     } else {
         const DevQuboBatch q = qubo_dev(B, Jb, false);
         for (int k = 0; k < B->n_inst; ++k) {
             const int f = Jb.first[size_t(k)], cnt = Jb.first[size_t(k) + 1] - f;
-            QB_HIP(launch_qubo_batch(q, rb, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
+            QB_HIP(launch_qubo_batch(q, rb, block, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
                                      B->w_queue.as<unsigned>() + k, f, cnt, cnt, st));
             ++*launches;
         }
@@ -1453,33 +1453,44 @@ extern "C" int qb_qubo_batch_anneal(qb_qubo_batch* B, const qb_schedule* s, cons
     if (!B) return fail(QB_E_ARG, "qb_qubo_batch_anneal: null batch");
     int rc = check_schedule(s);
     if (rc) return rc;
-    qb_qubo_opts o{0, 0, 0};
+    qb_qubo_opts o{0, 0, 0, 0};
     if (opts) o = *opts;
     if (o.mode != 0 && o.mode != 1) return fail(QB_E_ARG, "mode must be 0 (persistent) or 1 (per instance)");
     if (o.replica_block != 0 && o.replica_block != 4 && o.replica_block != 8 && o.replica_block != 16 &&
         o.replica_block != 32)
         return fail(QB_E_ARG, "replica_block must be 0, 4, 8, 16 or 32 (got %d)", o.replica_block);
     if (o.workgroups < 0) return fail(QB_E_ARG, "negative workgroups");
+    if (o.block != 0 && o.block != 256 && o.block != 1024)
+        return fail(QB_E_ARG, "block must be 0, 256 or 1024 (got %d)", o.block);
+    if ((o.block == 0) != (o.replica_block == 0))
+        return fail(QB_E_ARG, "block and replica_block are tuned together: give both or neither");
     hipDeviceProp_t prop;
     if ((rc = device_props(prop))) return rc;
-    const int workgroups = o.workgroups > 0 ? o.workgroups : 4 * prop.multiProcessorCount;
+    auto grid_for = [&](int rbv, int blk) {
+        const int per_cu = std::max(1, qubo_blocks_per_cu(rbv, blk));
+        return o.workgroups > 0 ? o.workgroups : per_cu * prop.multiProcessorCount;
+    };
     const std::vector<double> beta = betas_of(*s);
     const int n_sweeps = int(beta.size());
     const auto t0 = std::chrono::steady_clock::now();
     if ((rc = B->w_beta.reserve(sizeof(double) * beta.size()))) return rc;
     QB_HIP(hipMemcpyAsync(B->w_beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice, B->stream));
-    int rb = o.replica_block;
+    int rb = o.replica_block, block = o.block;
     if (rb == 0) {
-        // the block shape that anneals a short prefix of the schedule fastest
+        // the (threads per workgroup, replicas per job) shape that anneals a short prefix
+        // of the schedule fastest
+        static const int shapes[][2] = {{256, 8}, {256, 16}, {1024, 4}, {1024, 8}};
         float best = 0.0f;
-        for (int cand : {8, 16, 32}) {
+        for (const auto& sh : shapes) {
             const qb_qubo_batch::Jobs* Jb = nullptr;
-            if ((rc = qubo_jobs_for(B, cand, &Jb))) return rc;
+            if ((rc = qubo_jobs_for(B, sh[1], &Jb))) return rc;
             int l = 0;
             float ms = 0.0f;
-            if ((rc = qubo_run(B, *Jb, cand, std::min(n_sweeps, 8), o.mode, workgroups, &l, &ms))) return rc;
+            if ((rc = qubo_run(B, *Jb, sh[1], sh[0], std::min(n_sweeps, 8), o.mode, grid_for(sh[1], sh[0]), &l, &ms)))
+                return rc;
             if (rb == 0 || ms < best) {
-                rb = cand;
+                rb = sh[1];
+                block = sh[0];
                 best = ms;
             }
         }
@@ -1488,7 +1499,7 @@ extern "C" int qb_qubo_batch_anneal(qb_qubo_batch* B, const qb_schedule* s, cons
     if ((rc = qubo_jobs_for(B, rb, &Jb))) return rc;
     int launches = 0;
     float ms = 0.0f;
-    if ((rc = qubo_run(B, *Jb, rb, n_sweeps, o.mode, workgroups, &launches, &ms))) return rc;
+    if ((rc = qubo_run(B, *Jb, rb, block, n_sweeps, o.mode, grid_for(rb, block), &launches, &ms))) return rc;
     hipStream_t st = B->stream;
     if (energy) {
         if ((rc = B->w_energy.reserve(sizeof(double) * size_t(B->total_replicas)))) return rc;
@@ -1518,6 +1529,7 @@ extern "C" int qb_qubo_batch_anneal(qb_qubo_batch* B, const qb_schedule* s, cons
         rep->kernel_ms = ms;
         rep->launches = launches;
         rep->replica_block = rb;
+        rep->block = block;
         rep->jobs = int(Jb->host.size());
         double upd = 0.0;
         for (int k = 0; k < B->n_inst; ++k) upd += double(B->replicas[size_t(k)]) * double(B->n_bits[size_t(k)]);

@@ -206,9 +206,13 @@ struct DevQuboBatch {
     const int32_t* bit_id;       // per position: the bit's index inside its instance (draw counter)
     const uint64_t* seed;        // per instance
 };
-constexpr int kQuboBlock = 256;
-hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, uint32_t* masks, const double* beta, int n_sweeps,
-                             unsigned* queue, int job_first, int job_count, int workgroups, hipStream_t st);
+constexpr int kQuboBlock = 256;      // threads of the energy kernel
+constexpr int kQuboLdsBytes = 65536;  // LDS spin words of one annealing job
+// block: 256 or 1024 threads per workgroup (one bit per thread in a class pass).
+hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, int block, uint32_t* masks, const double* beta,
+                             int n_sweeps, unsigned* queue, int job_first, int job_count, int workgroups,
+                             hipStream_t st);
+int qubo_blocks_per_cu(int rb, int block);
 hipError_t launch_qubo_energy(const DevQuboBatch& q, const uint32_t* masks, double* energy, hipStream_t st);
 
 }  // namespace qb

@@ -7,29 +7,116 @@
 // workgroup anneals its job alone: every sweep visits the instance's colour classes in
 // order, a class is a set of independent flips, and the workgroup syncs between classes.
 //
-// Inside a class each bit is RB lanes, one per replica: the lanes walk the bit's CSR row
-// together, so a row's columns and couplings are read once for all RB replicas, and each
-// lane picks its replica's bit out of the neighbour's spin word. The spin words of a job
-// (bit r of word i = replica r0 + r's x_i) live in global memory and stay in L2.
+// Inside a class each lane takes one bit for all the job's replicas: it walks the bit's
+// CSR row once, kRowChunk neighbours at a time (columns and couplings, then the
+// neighbours' spin words, all in flight together), and adds each coupling into the
+// accumulator of every replica whose neighbour spin is set -- in row order, per replica.
+// The spin words of a job (bit r of word i = replica r0 + r's x_i) live in the
+// workgroup's LDS while it anneals (global memory only for instances too large for it),
+// so the random reads of neighbours' spins never leave the CU.
 //
-// dE of flipping bit i = (1 - 2 x_i)(h_i + sum_j J_ij x_j), summed in FP64 in CSR order;
-// the draw is Philox4x32-10 keyed by the replica seed (instance seed + replica + 1) with
-// counter (bit, sweep), and a flip is accepted iff dE <= 0 or u < qexp(-dE * beta) --
-// the same test and draws as the ray x voxel engine.
+// dE of flipping bit i = (1 - 2 x_i)(h_i + sum_j J_ij x_j), summed in FP64 in CSR order,
+// and a flip is accepted iff dE <= 0 or u < qexp(-dE * beta) -- the ray x voxel engine's
+// test. Replicas 2m and 2m + 1 share one Philox4x32-10 call keyed by the instance seed with
+// counter (bit, sweep, m, tag), one 53-bit uniform each (draw2_u01).
 #include "qb_internal.h"
 #include "qb_rng.h"
 
 namespace qb {
 
+constexpr int kRowChunk = 16;
+
+// Spin words of a job in LDS: one byte per bit up to 8 replicas per job, 16 or 32 bits
+// beyond; kQuboLdsBytes of them hold instances up to kQuboLdsBytes / sizeof(word) bits.
 template <int RB>
-__global__ void __launch_bounds__(kQuboBlock) qubo_batch_kernel(DevQuboBatch q, uint32_t* masks, const double* beta,
-                                                                 int n_sweeps, unsigned* queue, int job_first,
-                                                                 int job_count) {
-    constexpr int kBitsPerPass = kQuboBlock / RB;
+struct LdsWord {
+    using type = uint32_t;
+};
+template <>
+struct LdsWord<4> {
+    using type = uint8_t;
+};
+template <>
+struct LdsWord<8> {
+    using type = uint8_t;
+};
+template <>
+struct LdsWord<16> {
+    using type = uint16_t;
+};
+
+// One job's sweeps with its spin words at `mk` (LDS or global), local position index.
+template <int RB, int BLOCK, typename W>
+__device__ __forceinline__ void qubo_job(const DevQuboBatch& q, const QuboJob& jb, W* mk, const double* beta,
+                                         int n_sweeps) {
+    const int k = jb.instance, rb = jb.replicas;
+    const int bit0 = q.inst_pos[k];
+    const int c0 = q.inst_cls[k], c1 = q.inst_cls[k + 1];
+    const uint64_t key = q.seed[k];
+    const uint32_t pair0 = uint32_t(jb.replica0) / 2u;   // replica0 is a multiple of RB
+    for (int sweep = 0; sweep < n_sweeps; ++sweep) {
+        const double b = beta[sweep];
+        for (int c = c0; c + 1 < c1; ++c) {
+            const int lo = q.cls_ptr[c], hi = q.cls_ptr[c + 1];
+            for (int pos = lo + int(threadIdx.x); pos < hi; pos += BLOCK) {
+                const uint32_t mi = mk[pos - bit0];
+                const double hp = double(q.h[pos]);
+                double acc[RB];
+#pragma unroll
+                for (int r = 0; r < RB; ++r) acc[r] = hp;
+                const int e0 = q.row_ptr[pos], e1 = q.row_ptr[pos + 1];
+                for (int ea = e0; ea < e1; ea += kRowChunk) {
+                    int cj[kRowChunk];
+                    float wj[kRowChunk];
+#pragma unroll
+                    for (int u = 0; u < kRowChunk; ++u) {
+                        const int e = min(ea + u, e1 - 1);
+                        cj[u] = q.col[e];
+                        wj[u] = q.J[e];
+                    }
+                    uint32_t mj[kRowChunk];
+#pragma unroll
+                    for (int u = 0; u < kRowChunk; ++u) mj[u] = ea + u < e1 ? uint32_t(mk[cj[u] - bit0]) : 0u;
+#pragma unroll
+                    for (int u = 0; u < kRowChunk; ++u) {
+                        const double w = double(wj[u]);
+#pragma unroll
+                        for (int r = 0; r < RB; ++r) {
+                            const double nxt = acc[r] + w;
+                            acc[r] = ((mj[u] >> r) & 1u) ? nxt : acc[r];
+                        }
+                    }
+                }
+                const uint32_t bit = uint32_t(q.bit_id[pos]);
+                uint32_t flips = 0;
+#pragma unroll
+                for (int r = 0; r < RB; r += 2) {
+                    if (r < rb) {
+                        double u[2];
+                        draw2_u01(key, uint32_t(sweep), bit, pair0 + uint32_t(r / 2), u[0], u[1]);
+#pragma unroll
+                        for (int v = 0; v < 2; ++v) {
+                            const double dE = ((mi >> (r + v)) & 1u) ? -acc[r + v] : acc[r + v];
+                            if (r + v < rb && (dE <= 0.0 || u[v] < qexp(-dE * b))) flips |= 1u << (r + v);
+                        }
+                    }
+                }
+                if (flips) mk[pos - bit0] = W(mi ^ flips);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int RB, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) qubo_batch_kernel(DevQuboBatch q, uint32_t* masks, const double* beta,
+                                                            int n_sweeps, unsigned* queue, int job_first,
+                                                            int job_count) {
+    static_assert(RB % 2 == 0, "replicas are drawn in pairs");
+    using W = typename LdsWord<RB>::type;
+    constexpr int kLdsBits = kQuboLdsBytes / int(sizeof(W));
+    __shared__ W lmask[kLdsBits];
     __shared__ int s_job;
-    const int lane_r = int(threadIdx.x) % RB;
-    const int slot = int(threadIdx.x) / RB;
-    const unsigned group = unsigned(threadIdx.x & 63u) & ~unsigned(RB - 1);
     for (;;) {
         if (threadIdx.x == 0) s_job = int(atomicAdd(queue, 1u));
         __syncthreads();
@@ -37,34 +124,17 @@ __global__ void __launch_bounds__(kQuboBlock) qubo_batch_kernel(DevQuboBatch q, 
         __syncthreads();   // s_job is written again by the next take
         if (jn >= job_count) return;
         const QuboJob jb = q.jobs[job_first + jn];
-        const int k = jb.instance, rb = jb.replicas;
-        uint32_t* mk = masks + jb.mask_offset;
-        const int bit0 = q.inst_pos[k];
-        const int c0 = q.inst_cls[k], c1 = q.inst_cls[k + 1];
-        const uint64_t key = q.seed[k] + uint64_t(jb.replica0) + uint64_t(lane_r) + 1u;
-        const bool live = lane_r < rb;
-        for (int sweep = 0; sweep < n_sweeps; ++sweep) {
-            const double b = beta[sweep];
-            for (int c = c0; c + 1 < c1; ++c) {
-                const int lo = q.cls_ptr[c], hi = q.cls_ptr[c + 1];
-                for (int pos = lo + slot; pos < hi; pos += kBitsPerPass) {
-                    const uint32_t mi = mk[pos - bit0];
-                    double acc = double(q.h[pos]);
-                    const int e0 = q.row_ptr[pos], e1 = q.row_ptr[pos + 1];
-                    for (int e = e0; e < e1; ++e) {
-                        const uint32_t mj = mk[q.col[e] - bit0];
-                        const double nxt = acc + double(q.J[e]);
-                        acc = ((mj >> lane_r) & 1u) ? nxt : acc;
-                    }
-                    const double dE = ((mi >> lane_r) & 1u) ? -acc : acc;
-                    const double u = draw_u01(key, uint32_t(sweep), uint32_t(q.bit_id[pos]));
-                    const bool flip = live && (dE <= 0.0 || u < qexp(-dE * b));
-                    const uint64_t bal = __ballot(flip);
-                    const uint32_t flips = uint32_t(bal >> group) & (RB == 32 ? 0xffffffffu : ((1u << RB) - 1u));
-                    if (lane_r == 0 && flips) mk[pos - bit0] = mi ^ flips;
-                }
-                __syncthreads();
-            }
+        const int n = q.inst_pos[jb.instance + 1] - q.inst_pos[jb.instance];
+        uint32_t* gm = masks + jb.mask_offset;
+        if (n <= kLdsBits) {
+            // the job's spins live in LDS from all-zero x; written out once at the end
+            for (int i = int(threadIdx.x); i < n; i += BLOCK) lmask[i] = W(0);
+            __syncthreads();
+            qubo_job<RB, BLOCK, W>(q, jb, lmask, beta, n_sweeps);
+            for (int i = int(threadIdx.x); i < n; i += BLOCK) gm[i] = uint32_t(lmask[i]);
+            __syncthreads();
+        } else {
+            qubo_job<RB, BLOCK, uint32_t>(q, jb, gm, beta, n_sweeps);
         }
     }
 }
@@ -100,16 +170,40 @@ __global__ void __launch_bounds__(kQuboBlock) qubo_energy_kernel(DevQuboBatch q,
     }
 }
 
-hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, uint32_t* masks, const double* beta, int n_sweeps,
-                             unsigned* queue, int job_first, int job_count, int workgroups, hipStream_t st) {
+template <int BLOCK>
+static hipError_t launch_qubo_block(const DevQuboBatch& q, int rb, uint32_t* masks, const double* beta, int n_sweeps,
+                                    unsigned* queue, int job_first, int job_count, int workgroups, hipStream_t st) {
     switch (rb) {
-        case 4: qubo_batch_kernel<4><<<workgroups, kQuboBlock, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
-        case 8: qubo_batch_kernel<8><<<workgroups, kQuboBlock, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
-        case 16: qubo_batch_kernel<16><<<workgroups, kQuboBlock, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
-        case 32: qubo_batch_kernel<32><<<workgroups, kQuboBlock, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
+        case 4: qubo_batch_kernel<4, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
+        case 8: qubo_batch_kernel<8, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
+        case 16: qubo_batch_kernel<16, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
+        case 32: qubo_batch_kernel<32, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, int block, uint32_t* masks, const double* beta,
+                             int n_sweeps, unsigned* queue, int job_first, int job_count, int workgroups,
+                             hipStream_t st) {
+    if (block == 256)
+        return launch_qubo_block<256>(q, rb, masks, beta, n_sweeps, queue, job_first, job_count, workgroups, st);
+    if (block == 1024)
+        return launch_qubo_block<1024>(q, rb, masks, beta, n_sweeps, queue, job_first, job_count, workgroups, st);
+    return hipErrorInvalidValue;
+}
+
+int qubo_blocks_per_cu(int rb, int block) {
+    int per_cu = 0;
+    const void* k = nullptr;
+    if (block == 256)
+        k = rb == 4 ? (const void*)&qubo_batch_kernel<4, 256> : rb == 8 ? (const void*)&qubo_batch_kernel<8, 256>
+          : rb == 16 ? (const void*)&qubo_batch_kernel<16, 256> : (const void*)&qubo_batch_kernel<32, 256>;
+    else
+        k = rb == 4 ? (const void*)&qubo_batch_kernel<4, 1024> : rb == 8 ? (const void*)&qubo_batch_kernel<8, 1024>
+          : rb == 16 ? (const void*)&qubo_batch_kernel<16, 1024> : (const void*)&qubo_batch_kernel<32, 1024>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess) return 0;
+    return per_cu;
 }
 
 hipError_t launch_qubo_energy(const DevQuboBatch& q, const uint32_t* masks, double* energy, hipStream_t st) {

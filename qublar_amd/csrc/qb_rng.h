@@ -46,6 +46,19 @@ QB_HD double draw_u01(uint64_t seed, uint32_t sweep, uint32_t bit) {
     return double(z >> 11) * 0x1.0p-53;
 }
 
+// Two uniforms from one Philox call, for the batched-QUBO engine's replica pairs: key =
+// the instance seed, counter = (bit, sweep, pair, kPairTag); the first is the top 53 bits
+// of (o0 << 32 | o1), the second of (o2 << 32 | o3). The tag keeps these counters apart
+// from the single draws' (bit, sweep, 0, 0).
+constexpr uint32_t kPairTag = 0x51B0B47Cu;
+QB_HD void draw2_u01(uint64_t seed, uint32_t sweep, uint32_t bit, uint32_t pair, double& u0, double& u1) {
+    const u32x4 o = philox4x32_10(u32x4{bit, sweep, pair, kPairTag}, uint32_t(seed), uint32_t(seed >> 32));
+    const uint64_t z0 = (uint64_t(o.x) << 32) | uint64_t(o.y);
+    const uint64_t z1 = (uint64_t(o.z) << 32) | uint64_t(o.w);
+    u0 = double(z0 >> 11) * 0x1.0p-53;
+    u1 = double(z1 >> 11) * 0x1.0p-53;
+}
+
 QB_HD double pow2i(int k) {
     const uint64_t b = uint64_t(k + 1023) << 52;
     double d;

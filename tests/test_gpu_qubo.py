@@ -53,9 +53,9 @@ def test_qubo_launch_shapes_do_not_change_results():
     sched = qa.schedule(3.0, 0.1, 12, 2)
     base = B.anneal(sched, mode=0, replica_block=8)["x"]
     for mode in (0, 1):
-        for rb in (4, 8, 16, 32, 0):
-            say(f"mode {mode} replica block {rb}")
-            got = B.anneal(sched, mode=mode, replica_block=rb, workgroups=37 if mode == 0 else 0)
+        for rb, blk in ((4, 256), (8, 1024), (16, 256), (32, 256), (4, 1024), (0, 0)):
+            say(f"mode {mode} replica block {rb} block {blk}")
+            got = B.anneal(sched, mode=mode, replica_block=rb, block=blk, workgroups=37 if mode == 0 else 0)
             for k in range(len(inst)):
                 assert np.array_equal(got["x"][k], base[k]), (mode, rb, k)
             if mode == 1:
