@@ -1196,7 +1196,7 @@ struct qb_qubo_batch {
     std::vector<int32_t> e_off;       // per instance: offset of its replicas in the energy output
     std::vector<uint64_t> seed;
     int total_replicas = 0;
-    DevBuf b_inst_pos, b_inst_cls, b_cls, b_row, b_col, b_J, b_h, b_bit, b_seed;
+    DevBuf b_inst_pos, b_inst_cls, b_cls, b_row, b_col, b_J, b_h, b_bit, b_seed, b_ctile, b_tptr, b_tcol, b_tJ;
     struct Jobs {
         DevBuf lpt, by_inst;          // the same jobs, longest first and in instance order
         std::vector<QuboJob> host;    // instance order
@@ -1347,6 +1347,35 @@ extern "C" int qb_qubo_batch_create(const qb_qubo_batch_desc* d, qb_qubo_batch**
         hoff += n;
     }
     B->n_total = B->inst_pos.back();
+    // the rows sliced by 64 positions of each class, entry-major (padding repeats a valid entry)
+    std::vector<int32_t> ctile, tptr, tcol;
+    std::vector<float> tJ;
+    for (size_t c = 0; c + 1 < B->cls_ptr.size(); ++c) {
+        ctile.push_back(int32_t(tptr.size()));
+        const int lo = B->cls_ptr[c], hi = B->cls_ptr[c + 1];
+        if (hi < lo) continue;   // the boundary between two instances' class lists
+        for (int t0 = lo; t0 < hi; t0 += 64) {
+            const int t1 = std::min(hi, t0 + 64);
+            int width = 0;
+            for (int p = t0; p < t1; ++p) width = std::max(width, row[size_t(p) + 1] - row[size_t(p)]);
+            if (int64_t(tcol.size()) + int64_t(width) * 64 > INT32_MAX)
+                return fail(QB_E_LIMIT, "the batch's sliced rows exceed int32 indexing");
+            tptr.push_back(int32_t(tcol.size()));
+            const size_t base = tcol.size();
+            tcol.resize(base + size_t(width) * 64, 0);
+            tJ.resize(base + size_t(width) * 64, 0.0f);
+            for (int p = t0; p < t1; ++p) {
+                const int r0 = row[size_t(p)], deg = row[size_t(p) + 1] - r0;
+                for (int u = 0; u < width; ++u) {
+                    const int e = r0 + std::min(u, std::max(deg - 1, 0));
+                    const size_t at = base + size_t(u) * 64 + size_t(p - t0);
+                    tcol[at] = deg > 0 ? colp[size_t(e)] : p;
+                    tJ[at] = deg > 0 ? Jp[size_t(e)] : 0.0f;
+                }
+            }
+        }
+    }
+    ctile.push_back(int32_t(tptr.size()));
     QB_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
     QB_HIP(hipEventCreate(&B->ev0));
     QB_HIP(hipEventCreate(&B->ev1));
@@ -1355,7 +1384,8 @@ extern "C" int qb_qubo_batch_create(const qb_qubo_batch_desc* d, qb_qubo_batch**
     if ((rc = B->b_inst_pos.upload(B->inst_pos, st)) || (rc = B->b_inst_cls.upload(B->inst_cls, st)) ||
         (rc = B->b_cls.upload(B->cls_ptr, st)) || (rc = B->b_row.upload(row, st)) || (rc = B->b_col.upload(colp, st)) ||
         (rc = B->b_J.upload(Jp, st)) || (rc = B->b_h.upload(hp, st)) || (rc = B->b_bit.upload(bit, st)) ||
-        (rc = B->b_seed.upload(B->seed, st)))
+        (rc = B->b_seed.upload(B->seed, st)) || (rc = B->b_ctile.upload(ctile, st)) ||
+        (rc = B->b_tptr.upload(tptr, st)) || (rc = B->b_tcol.upload(tcol, st)) || (rc = B->b_tJ.upload(tJ, st)))
         return rc;
     QB_HIP(hipStreamSynchronize(st));
     *out = B.release();
@@ -1411,7 +1441,8 @@ static DevQuboBatch qubo_dev(const qb_qubo_batch* B, const qb_qubo_batch::Jobs& 
     return DevQuboBatch{int(J.host.size()), lpt ? J.lpt.as<QuboJob>() : J.by_inst.as<QuboJob>(),
                         B->b_inst_pos.as<int32_t>(), B->b_inst_cls.as<int32_t>(), B->b_cls.as<int32_t>(),
                         B->b_row.as<int32_t>(), B->b_col.as<int32_t>(), B->b_J.as<float>(), B->b_h.as<float>(),
-                        B->b_bit.as<int32_t>(), B->b_seed.as<uint64_t>()};
+                        B->b_bit.as<int32_t>(), B->b_seed.as<uint64_t>(), B->b_ctile.as<int32_t>(),
+                        B->b_tptr.as<int32_t>(), B->b_tcol.as<int32_t>(), B->b_tJ.as<float>()};
 }
 
 // One anneal of the whole batch at replica block rb: spin words cleared (all-zero x),

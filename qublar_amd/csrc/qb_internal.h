@@ -205,6 +205,13 @@ struct DevQuboBatch {
     const float* h;              // field per position
     const int32_t* bit_id;       // per position: the bit's index inside its instance (draw counter)
     const uint64_t* seed;        // per instance
+    // The rows again, sliced by 64 positions of a class (a wave's bits in one pass) and
+    // stored entry-major: entry u of tile t's lane l at tile_ptr[t] + 64 u + l, so one load
+    // of a wave reads 64 consecutive words. Tile t of class c is class_tile[c] + (pos - lo) / 64.
+    const int32_t* class_tile;   // per class (global class index)
+    const int32_t* tile_ptr;     // per tile
+    const int32_t* tcol;         // neighbour position
+    const float* tJ;             // coupling
 };
 constexpr int kQuboBlock = 256;      // threads of the energy kernel
 constexpr int kQuboLdsBytes = 65536;  // LDS spin words of one annealing job

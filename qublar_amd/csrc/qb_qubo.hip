@@ -64,19 +64,21 @@ __device__ __forceinline__ void qubo_job(const DevQuboBatch& q, const QuboJob& j
                 double acc[RB];
 #pragma unroll
                 for (int r = 0; r < RB; ++r) acc[r] = hp;
-                const int e0 = q.row_ptr[pos], e1 = q.row_ptr[pos + 1];
-                for (int ea = e0; ea < e1; ea += kRowChunk) {
+                const int deg = q.row_ptr[pos + 1] - q.row_ptr[pos];
+                const int* tc = q.tcol + q.tile_ptr[q.class_tile[c] + (pos - lo) / 64] + ((pos - lo) & 63);
+                const float* tw = q.tJ + (tc - q.tcol);
+                for (int ea = 0; ea < deg; ea += kRowChunk) {
                     int cj[kRowChunk];
                     float wj[kRowChunk];
 #pragma unroll
                     for (int u = 0; u < kRowChunk; ++u) {
-                        const int e = min(ea + u, e1 - 1);
-                        cj[u] = q.col[e];
-                        wj[u] = q.J[e];
+                        const int e = min(ea + u, deg - 1);
+                        cj[u] = tc[64 * e];
+                        wj[u] = tw[64 * e];
                     }
                     uint32_t mj[kRowChunk];
 #pragma unroll
-                    for (int u = 0; u < kRowChunk; ++u) mj[u] = ea + u < e1 ? uint32_t(mk[cj[u] - bit0]) : 0u;
+                    for (int u = 0; u < kRowChunk; ++u) mj[u] = ea + u < deg ? uint32_t(mk[cj[u] - bit0]) : 0u;
 #pragma unroll
                     for (int u = 0; u < kRowChunk; ++u) {
                         const double w = double(wj[u]);
