@@ -293,6 +293,28 @@ int qb_colour_qubo(int n, const int32_t* row_ptr, const int32_t* col, const floa
 int qb_qubo_batch_colouring(const qb_qubo_batch* b, int instance, int32_t* order, int32_t* class_ptr,
                             int* n_classes);
 
+/* ---- dense QUBO field engine (precision study) ------------------------------------
+   No counterpart in the reference. One dense QUBO E(x) = sum_i h_i x_i + sum_{i<j} Q_ij
+   x_i x_j (Q symmetric, zero diagonal), R replicas (seed b + 1 + seed_offset) annealed
+   from all-zero x, bits in index order every sweep, local fields f = h + Q x carried
+   incrementally: FP64, or FP32 (one float add of +-Q_ij per flip) with an FP64
+   recompute every recompute_every sweeps. Accept iff dE = (1 - 2 x_i) f_i <= 0 or
+   dE / T < -qlog(u), u = Philox4x32-10 keyed by the replica seed, counter (bit, sweep). */
+typedef struct qb_dense qb_dense;
+int qb_dense_create(int n, const float* Q, const float* h, qb_dense** out);
+int qb_dense_destroy(qb_dense* d);
+
+typedef struct {
+    double wall_s;            /* state init to outputs */
+    double kernel_ms;         /* the annealing kernel, HIP events */
+    double spin_updates;      /* R x sweeps x n */
+} qb_dense_report;
+
+/* x: R x n (nullable); energy: R FP64 energies from scratch (nullable); field_err: R values
+   of max_i |f_carried - f_scratch(FP64)| at the end (nullable). */
+int qb_dense_anneal(qb_dense* d, const qb_schedule* s, int R, uint64_t seed_offset, int precision,
+                    int recompute_every, uint8_t* x, double* energy, double* field_err, qb_dense_report* rep);
+
 /* The Erdos-Renyi max-cut workload: splitmix64 from `seed` over pairs i < j in order;
    a pair is an edge when u < p, and its weight is +1 when the next draw's top bit is
    0, else -1. Fills up to cap edges; *m_out = edge count. */

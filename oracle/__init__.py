@@ -108,6 +108,12 @@ def lib():
         L.orc_tts99_bootstrap.argtypes = [_i32p, _f64p, C.c_size_t, C.c_int, C.c_uint64,
                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_sha256_hex.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p]
+        L.orc_qlog.restype = C.c_double
+        L.orc_qlog.argtypes = [C.c_double]
+        L.orc_dense_energy.restype = C.c_double
+        L.orc_dense_energy.argtypes = [C.c_int, _f32p, _f32p, _u8p]
+        L.orc_dense_anneal.argtypes = [C.c_int, _f32p, _f32p, C.POINTER(Schedule), C.c_uint64, C.c_int, C.c_int,
+                                       _u8p, C.c_void_p]
         L.orc_qubo_energy.restype = C.c_double
         L.orc_qubo_energy.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _u8p]
         L.orc_qubo_anneal.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _i32p, C.POINTER(Schedule),
@@ -269,6 +275,24 @@ def qubo_anneal(row_ptr, col, J, h, order, sched, seed, R):
 def qubo_energy(row_ptr, col, J, h, x):
     rp, cl, Jv, hv = _qubo_arrays(row_ptr, col, J, h)
     return lib().orc_qubo_energy(len(hv), rp, cl, Jv, hv, np.ascontiguousarray(x, np.uint8))
+
+
+def dense_anneal(Q, h, sched, seed, precision=FP64, recompute_every=0, fields=False):
+    """One replica of the dense field engine's trajectory: x uint8[n] (and the carried fields)."""
+    Qv = np.ascontiguousarray(Q, np.float32)
+    hv = np.ascontiguousarray(h, np.float32)
+    n = len(hv)
+    x = np.zeros(n, np.uint8)
+    f = np.zeros(n, np.float64) if fields else None
+    lib().orc_dense_anneal(n, Qv.reshape(-1), hv, C.byref(sched), C.c_uint64(seed), int(precision),
+                           int(recompute_every), x, _ptr(f))
+    return (x, f) if fields else x
+
+
+def dense_energy(Q, h, x):
+    Qv = np.ascontiguousarray(Q, np.float32)
+    hv = np.ascontiguousarray(h, np.float32)
+    return lib().orc_dense_energy(len(hv), Qv.reshape(-1), hv, np.ascontiguousarray(x, np.uint8))
 
 
 def philox(ctr, key):

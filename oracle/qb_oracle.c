@@ -688,3 +688,89 @@ void orc_qubo_anneal(int n, const int* row_ptr, const int* col, const float* J, 
         }
     }
 }
+
+/* ---- dense QUBO field engine -------------------------------------------------------------
+   qlog: natural log from IEEE operations only (the engine's qlog, restated): x = m 2^e with
+   m in [sqrt(1/2), sqrt(2)), s = (m - 1)/(m + 1), ln m = 2 s P(s^2) with P the series
+   1 + z/3 + ... + z^11/23 in Horner form, then e ln2_hi + (e ln2_lo + ln m). */
+double orc_qlog(double x) {
+    if (!(x > 0.0)) return -HUGE_VAL;
+    uint64_t b;
+    memcpy(&b, &x, sizeof b);
+    int e = (int)((b >> 52) & 0x7ffu) - 1023;
+    const uint64_t mb = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m;
+    memcpy(&m, &mb, sizeof m);
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    static const double inv[12] = {1.0 / 23.0, 1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0, 1.0 / 15.0, 1.0 / 13.0,
+                                   1.0 / 11.0, 1.0 / 9.0, 1.0 / 7.0, 1.0 / 5.0, 1.0 / 3.0, 1.0};
+    double p = inv[0];
+    for (int k = 1; k < 12; ++k) p = p * z + inv[k];
+    const double lnm = 2.0 * s * p;
+    const double ed = (double)e;
+    return ed * 0x1.62e42fee00000p-1 + (ed * 0x1.a39ef35793c76p-33 + lnm);
+}
+
+/* E(x) = sum_i x_i (h_i + 1/2 sum_j Q_ij x_j), Q dense row-major. */
+double orc_dense_energy(int n, const float* Q, const float* h, const uint8_t* x) {
+    double e = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (!x[i]) continue;
+        double pair = 0.0;
+        for (int j = 0; j < n; ++j)
+            if (x[j]) pair += (double)Q[(size_t)i * (size_t)n + (size_t)j];
+        e += (double)h[i] + 0.5 * pair;
+    }
+    return e;
+}
+
+/* One replica of the dense engine: bits in index order every sweep; fields f = h + Q x
+   carried incrementally in FP64 (precision 0) or FP32 (precision 1: f_j = f_j + Q_ij * delta
+   in float on every flip of bit i, and f_j = float(h_j + sum_k x_k Q_kj) summed in FP64,
+   k ascending, after every recompute_every-th sweep). Accept iff dE = (1 - 2 x_i) f_i <= 0
+   or dE * beta < -qlog(u), u = philox_uniform(seed, sweep, i). f_final (nullable) receives
+   the carried fields at the end. */
+void orc_dense_anneal(int n, const float* Q, const float* h, const orc_schedule* s, uint64_t seed, int precision,
+                      int recompute_every, uint8_t* x, double* f_final) {
+    double* f64 = (double*)malloc(sizeof(double) * (size_t)n);
+    float* f32 = (float*)malloc(sizeof(float) * (size_t)n);
+    memset(x, 0, (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        f64[i] = (double)h[i];
+        f32[i] = h[i];
+    }
+    const int total = s->anneal_sweeps + s->hold_sweeps;
+    for (int sweep = 0; sweep < total; ++sweep) {
+        const double beta = 1.0 / orc_temperature(s, sweep);
+        for (int i = 0; i < n; ++i) {
+            const double fi = precision ? (double)f32[i] : f64[i];
+            const double dE = x[i] ? -fi : fi;
+            const double L = -orc_qlog(orc_philox_uniform(seed, (uint32_t)sweep, (uint32_t)i));
+            if (dE <= 0.0 || dE * beta < L) {
+                const float dl = x[i] ? -1.0f : 1.0f;
+                x[i] ^= 1u;
+                const float* row = Q + (size_t)i * (size_t)n;
+                if (precision)
+                    for (int j = 0; j < n; ++j) f32[j] = f32[j] + row[j] * dl;
+                else
+                    for (int j = 0; j < n; ++j) f64[j] = f64[j] + (double)row[j] * (double)dl;
+            }
+        }
+        if (precision && recompute_every > 0 && (sweep + 1) % recompute_every == 0)
+            for (int i = 0; i < n; ++i) {
+                double acc = (double)h[i];
+                for (int k = 0; k < n; ++k)
+                    if (x[k]) acc += (double)Q[(size_t)k * (size_t)n + (size_t)i];
+                f32[i] = (float)acc;
+            }
+    }
+    if (f_final)
+        for (int i = 0; i < n; ++i) f_final[i] = precision ? (double)f32[i] : f64[i];
+    free(f64);
+    free(f32);
+}

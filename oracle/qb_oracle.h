@@ -135,6 +135,12 @@ double orc_qubo_energy(int n, const int* row_ptr, const int* col, const float* J
 void orc_qubo_anneal(int n, const int* row_ptr, const int* col, const float* J, const float* h, const int* order,
                      const orc_schedule* s, uint64_t seed, int R, uint8_t* x_out);
 
+/* ---- dense QUBO field engine (the precision study's dense instance) ---- */
+double orc_qlog(double x);
+double orc_dense_energy(int n, const float* Q, const float* h, const uint8_t* x);
+void orc_dense_anneal(int n, const float* Q, const float* h, const orc_schedule* s, uint64_t seed, int precision,
+                      int recompute_every, uint8_t* x, double* f_final);
+
 #ifdef __cplusplus
 }
 #endif

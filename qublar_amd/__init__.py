@@ -93,6 +93,10 @@ class QuboReport(C.Structure):
                 ("replica_block", C.c_int), ("jobs", C.c_int), ("block", C.c_int), ("spin_updates", C.c_double)]
 
 
+class DenseReport(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double), ("spin_updates", C.c_double)]
+
+
 class BatchReport(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
                 ("sweeps_per_launch", C.c_int)]
@@ -152,6 +156,9 @@ def lib():
                                            C.POINTER(QuboReport)]),
             "qb_qubo_batch_colouring": (i32, [vp, i32, vp, vp, C.POINTER(i32)]),
             "qb_colour_qubo": (i32, [i32, vp, vp, vp, vp, vp, C.POINTER(i32)]),
+            "qb_dense_create": (i32, [i32, vp, vp, C.POINTER(vp)]),
+            "qb_dense_destroy": (i32, [vp]),
+            "qb_dense_anneal": (i32, [vp, C.POINTER(Schedule), i32, u64, i32, i32, vp, vp, vp, C.POINTER(DenseReport)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -584,3 +591,48 @@ class QuboBatch:
                 off += q.replicas
             out["energy"] = es
         return out
+
+
+# ---- dense QUBO field engine -----------------------------------------------------------------
+
+def dense_qubo(n, seed):
+    """A dense symmetric QUBO: Q_ij ~ N(0, 1/sqrt(n)) (standard deviation) for i < j, mirrored,
+    and the diagonal as the linear term h; float32, from numpy's PCG64 seeded with `seed`."""
+    rng = np.random.default_rng(seed)
+    A = rng.normal(0.0, 1.0 / np.sqrt(n), (n, n)).astype(np.float32)
+    Q = np.triu(A, 1)
+    Q = Q + Q.T
+    h = np.ascontiguousarray(np.diag(A))
+    return np.ascontiguousarray(Q, np.float32), h.astype(np.float32)
+
+
+class DenseQubo:
+    """A dense QUBO resident on the device, annealed by the field engine."""
+
+    def __init__(self, Q, h):
+        self.Q = np.ascontiguousarray(Q, np.float32)
+        self.hv = np.ascontiguousarray(h, np.float32)
+        self.n = len(self.hv)
+        self.h = C.c_void_p()
+        _check(lib().qb_dense_create(self.n, _p(self.Q), _p(self.hv), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qb_dense_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def anneal(self, sched, R, seed_offset=0, precision=FP64, recompute_every=0):
+        x = np.zeros((R, self.n), np.uint8)
+        e = np.zeros(R, np.float64)
+        err = np.zeros(R, np.float64)
+        rep = DenseReport()
+        _check(lib().qb_dense_anneal(self.h, C.byref(sched), int(R), C.c_uint64(seed_offset), int(precision),
+                                     int(recompute_every), _p(x), _p(e), _p(err), C.byref(rep)))
+        return {"x": x, "energy": e, "field_err": err,
+                "report": {f: getattr(rep, f) for f, _ in DenseReport._fields_}}

@@ -1581,3 +1581,87 @@ extern "C" int qb_qubo_batch_colouring(const qb_qubo_batch* B, int k, int32_t* o
     *n_classes = c1 - c0 - 1;
     return QB_OK;
 }
+
+// ==== dense QUBO field engine ==============================================================
+
+struct qb_dense {
+    hipStream_t stream = nullptr;
+    int n = 0;
+    DevBuf b_Q, b_h, w_beta, w_x, w_e, w_err;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~qb_dense() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" int qb_dense_create(int n, const float* Q, const float* h, qb_dense** out) {
+    if (!out) return fail(QB_E_ARG, "qb_dense_create: null output");
+    *out = nullptr;
+    if (n <= 0 || !Q || !h) return fail(QB_E_ARG, "qb_dense_create: n must be positive and Q, h given");
+    if (dense_lds_bytes(n, 0) > size_t(150 * 1024))
+        return fail(QB_E_LIMIT, "n = %d: the FP64 fields and spins of one replica must fit 150 KB of LDS", n);
+    for (int i = 0; i < n; ++i) {
+        if (Q[size_t(i) * size_t(n) + size_t(i)] != 0.0f) return fail(QB_E_ARG, "Q[%d][%d] is not zero", i, i);
+        if (!std::isfinite(h[i])) return fail(QB_E_ARG, "h[%d] is not finite", i);
+        for (int j = i + 1; j < n; ++j) {
+            const float a = Q[size_t(i) * size_t(n) + size_t(j)], b = Q[size_t(j) * size_t(n) + size_t(i)];
+            if (a != b || !std::isfinite(a)) return fail(QB_E_ARG, "Q is not symmetric and finite at (%d, %d)", i, j);
+        }
+    }
+    auto D = std::make_unique<qb_dense>();
+    D->n = n;
+    QB_HIP(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
+    QB_HIP(hipEventCreate(&D->ev0));
+    QB_HIP(hipEventCreate(&D->ev1));
+    int rc;
+    if ((rc = D->b_Q.reserve(sizeof(float) * size_t(n) * size_t(n))) || (rc = D->b_h.reserve(sizeof(float) * size_t(n))))
+        return rc;
+    QB_HIP(hipMemcpyAsync(D->b_Q.p, Q, sizeof(float) * size_t(n) * size_t(n), hipMemcpyHostToDevice, D->stream));
+    QB_HIP(hipMemcpyAsync(D->b_h.p, h, sizeof(float) * size_t(n), hipMemcpyHostToDevice, D->stream));
+    QB_HIP(hipStreamSynchronize(D->stream));
+    *out = D.release();
+    return QB_OK;
+}
+
+extern "C" int qb_dense_destroy(qb_dense* d) {
+    delete d;
+    return QB_OK;
+}
+
+extern "C" int qb_dense_anneal(qb_dense* D, const qb_schedule* s, int R, uint64_t seed_offset, int precision,
+                               int recompute_every, uint8_t* x, double* energy, double* field_err,
+                               qb_dense_report* rep) {
+    if (!D) return fail(QB_E_ARG, "qb_dense_anneal: null handle");
+    int rc = check_schedule(s);
+    if (rc) return rc;
+    if (R <= 0) return fail(QB_E_ARG, "R must be positive");
+    if (precision != QB_FP64 && precision != QB_FP32) return fail(QB_E_ARG, "precision must be QB_FP64 or QB_FP32");
+    if (recompute_every < 0) return fail(QB_E_ARG, "negative recompute_every");
+    const std::vector<double> beta = betas_of(*s);
+    const int n = D->n;
+    if ((rc = D->w_beta.reserve(sizeof(double) * beta.size())) || (rc = D->w_x.reserve(size_t(R) * size_t(n))) ||
+        (rc = D->w_e.reserve(sizeof(double) * size_t(R))) || (rc = D->w_err.reserve(sizeof(double) * size_t(R))))
+        return rc;
+    hipStream_t st = D->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    QB_HIP(hipMemcpyAsync(D->w_beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice, st));
+    const DevDense dd{n, D->b_Q.as<float>(), D->b_h.as<float>()};
+    QB_HIP(hipEventRecord(D->ev0, st));
+    QB_HIP(launch_dense(dd, precision, R, D->w_beta.as<double>(), int(beta.size()), precision == QB_FP32 ? recompute_every : 0,
+                        seed_offset, D->w_x.as<uint8_t>(), D->w_e.as<double>(), D->w_err.as<double>(), st));
+    QB_HIP(hipEventRecord(D->ev1, st));
+    if (x) QB_HIP(hipMemcpyAsync(x, D->w_x.p, size_t(R) * size_t(n), hipMemcpyDeviceToHost, st));
+    if (energy) QB_HIP(hipMemcpyAsync(energy, D->w_e.p, sizeof(double) * size_t(R), hipMemcpyDeviceToHost, st));
+    if (field_err) QB_HIP(hipMemcpyAsync(field_err, D->w_err.p, sizeof(double) * size_t(R), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    if (rep) {
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, D->ev0, D->ev1));
+        rep->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rep->kernel_ms = ms;
+        rep->spin_updates = double(R) * double(beta.size()) * double(n);
+    }
+    return QB_OK;
+}

@@ -222,4 +222,18 @@ hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, int block, uint32_t*
 int qubo_blocks_per_cu(int rb, int block);
 hipError_t launch_qubo_energy(const DevQuboBatch& q, const uint32_t* masks, double* energy, hipStream_t st);
 
+// ---- dense QUBO field engine (qb_dense.hip) ----
+struct DevDense {
+    int n;
+    const float* Q;   // n x n, symmetric, zero diagonal
+    const float* h;
+};
+constexpr int kDenseBlock = 1024;
+size_t dense_lds_bytes(int n, int precision);
+// One workgroup per replica (seed b + 1 + seed_offset); x_out R x n, energy and the final
+// max |f_carried - f_scratch| per replica.
+hipError_t launch_dense(const DevDense& d, int precision, int R, const double* beta, int n_sweeps,
+                        int recompute_every, uint64_t seed_offset, uint8_t* x_out, double* energy, double* field_err,
+                        hipStream_t st);
+
 }  // namespace qb

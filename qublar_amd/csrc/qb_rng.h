@@ -94,6 +94,42 @@ QB_HD double qexp(double x) {
     return (q * pow2i(k + 600)) * pow2i(-600);
 }
 
+// Natural log of a positive normal double (0 gives -inf), from IEEE operations only:
+// x = m 2^e with m in [sqrt(1/2), sqrt(2)), s = (m - 1) / (m + 1) (|s| <= 0.1716, m - 1
+// exact), ln m = 2 s (1 + s^2/3 + ... + s^22/23) in Horner form (truncation below 3e-19
+// relative), then e ln2_hi + (e ln2_lo + ln m) with ln2_hi's 32 significant bits making
+// e ln2_hi exact. The dense engine's acceptance test uses -qlog(u).
+QB_HD double qlog(double x) {
+    if (!(x > 0.0)) return -__builtin_huge_val();
+    uint64_t b;
+    __builtin_memcpy(&b, &x, sizeof b);
+    int e = int((b >> 52) & 0x7ffu) - 1023;
+    const uint64_t mb = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m;
+    __builtin_memcpy(&m, &mb, sizeof m);
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 23.0;
+    p = p * z + 1.0 / 21.0;
+    p = p * z + 1.0 / 19.0;
+    p = p * z + 1.0 / 17.0;
+    p = p * z + 1.0 / 15.0;
+    p = p * z + 1.0 / 13.0;
+    p = p * z + 1.0 / 11.0;
+    p = p * z + 1.0 / 9.0;
+    p = p * z + 1.0 / 7.0;
+    p = p * z + 1.0 / 5.0;
+    p = p * z + 1.0 / 3.0;
+    p = p * z + 1.0;
+    const double lnm = 2.0 * s * p;
+    const double ed = double(e);
+    return ed * 0x1.62e42fee00000p-1 + (ed * 0x1.a39ef35793c76p-33 + lnm);
+}
+
 // The Metropolis test for an uphill move (dE > 0) at inverse temperature beta.
 QB_HD bool metropolis_uphill(double dE, double beta, uint64_t seed, uint32_t sweep, uint32_t bit) {
     return draw_u01(seed, sweep, bit) < qexp(-dE * beta);
