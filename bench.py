@@ -289,6 +289,55 @@ def run_batched(args):
     print(json.dumps(result), flush=True)
 
 
+def run_dense_precision(args):
+    """The dense half of the precision study: one dense symmetric QUBO, N = 4096, Q_ij ~
+    N(0, 1/sqrt(N)) float32, 64 replicas, 5000 sweeps T 1 -> 0.01. Full-FP64 fields first,
+    then FP32 fields carried incrementally with an FP64 recompute every k in {1, 10, 100}
+    sweeps (and never): final field error against FP64 from scratch, energies against the
+    FP64 run's, identical replicas, time."""
+    import qublar_amd as qa
+
+    n, R, sweeps = args.dense_n, 64, args.dense_sweeps
+    Q, h = qa.dense_qubo(n, 4096)
+    D = qa.DenseQubo(Q, h)
+    sched = qa.schedule(1.0, 0.01, sweeps, 0)
+    runs = [("fp64", qa.FP64, 0), ("fp32 k=1", qa.FP32, 1), ("fp32 k=10", qa.FP32, 10),
+            ("fp32 k=100", qa.FP32, 100), ("fp32 no recompute", qa.FP32, 0)]
+    base = None
+    rows = []
+    for name, prec, k in runs:
+        out = D.anneal(sched, R, 0, prec, k)
+        rep = out["report"]
+        log(f"{name}: kernel {rep['kernel_ms']:.1f} ms, max field error {out['field_err'].max():.3g}")
+        row = {"run": name, "kernel_ms": rep["kernel_ms"], "spin_updates_per_s": rep["spin_updates"] / (rep["kernel_ms"] * 1e-3),
+               "field_err_max": float(out["field_err"].max()), "field_err_mean": float(out["field_err"].mean()),
+               "energy_mean": float(out["energy"].mean())}
+        if base is None:
+            base = out
+        else:
+            de = (out["energy"] - base["energy"]) / np.abs(base["energy"])
+            row.update({"energy_rel_diff_vs_fp64": {"mean": float(de.mean()), "min": float(de.min()), "max": float(de.max())},
+                        "replicas_identical": int(np.sum(np.all(out["x"] == base["x"], axis=1))),
+                        "spins_equal_fraction": float(np.mean(out["x"] == base["x"]))})
+        rows.append(row)
+    result = {"workload": "dense precision", "problem": f"dense QUBO N={n}, Q ~ N(0, 1/sqrt(N)) float32, {R} replicas, "
+              f"{sweeps} sweeps T 1 -> 0.01", "rows": rows}
+    if not args.no_cpu:
+        import oracle
+        from concurrent.futures import ThreadPoolExecutor
+        threads = max(1, min(16, (os.cpu_count() or 2) - 2))
+        few = max(1, min(sweeps, 400))
+        osch = oracle.schedule(1.0, 0.01, few, 0)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda b: oracle.dense_anneal(Q, h, osch, b + 1, 1, 10), range(threads)))
+        wall = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": threads * few * n / wall, "unit": "spin updates/s", "cores": threads,
+                                  "kind": "port", "sample": f"{threads} replicas x {few} sweeps, FP32 fields, k=10, "
+                                  f"{wall:.1f} s"}
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,7 +349,9 @@ def main():
     ap.add_argument("--group-width", type=int, default=0, help="branches per group (2, 4, 8, 16); 0 = auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched"], default="muon",
+    ap.add_argument("--dense-n", type=int, default=4096, help="dense precision: bits")
+    ap.add_argument("--dense-sweeps", type=int, default=5000, help="dense precision: sweeps")
+    ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched", "dense-precision"], default="muon",
                     help="muon: the headline line (default); serving: the batched serving stream; "
                          "batched: many sparse QUBOs in one persistent launch")
     ap.add_argument("--instances", type=int, default=256, help="batched: instances")
@@ -328,6 +379,9 @@ def main():
         return
     if args.workload == "batched":
         run_batched(args)
+        return
+    if args.workload == "dense-precision":
+        run_dense_precision(args)
         return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5

@@ -25,6 +25,38 @@
 namespace qb {
 
 constexpr int kDenseWin = 64;
+constexpr int kRankChunk = 8;
+constexpr int kFieldGroup = 4;   // fields a thread updates together (i, i + block, ...)
+
+// Fields i = g0 + q kDenseBlock (q < kFieldGroup, i < n) from scratch:
+// h_i + sum_j x_j Q_ji in FP64, j ascending (Q_ji = Q_ij), additions of unset bits
+// skipped; every field's loads of a 16-column chunk in flight together.
+__device__ __forceinline__ void scratch_fields(const DevDense& d, const uint8_t* x, int g0, double (&s)[kFieldGroup]) {
+    constexpr int kChunk = 16;
+    const int n = d.n;
+    int col[kFieldGroup];
+#pragma unroll
+    for (int q = 0; q < kFieldGroup; ++q) {
+        col[q] = min(g0 + q * kDenseBlock, n - 1);
+        s[q] = double(d.h[col[q]]);
+    }
+    for (int j0 = 0; j0 < n; j0 += kChunk) {
+        float v[kFieldGroup][kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int q = 0; q < kFieldGroup; ++q) v[q][u] = d.Q[size_t(min(j0 + u, n - 1)) * size_t(n) + size_t(col[q])];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const bool set = j0 + u < n && x[min(j0 + u, n - 1)];
+#pragma unroll
+            for (int q = 0; q < kFieldGroup; ++q) {
+                const double nxt = s[q] + double(v[q][u]);
+                s[q] = set ? nxt : s[q];
+            }
+        }
+    }
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, const double* beta, int n_sweeps,
@@ -86,40 +118,69 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
             __syncthreads();
             const int k = s_k;
             if (k > 0) {
-                for (int i = tid; i < n; i += kDenseBlock) {
-                    if (i >= i0 && i < i0 + w) continue;
-                    T acc = f[i];
-                    for (int m = 0; m < k; ++m)
-                        acc = acc + T(d.Q[size_t(i0 + s_t[m]) * size_t(n) + size_t(i)]) * T(s_delta[m]);
-                    f[i] = acc;
+                // kRankChunk flipped rows at a time: their loads in flight together, the
+                // additions in flip order
+                for (int g0 = tid; g0 < n; g0 += kFieldGroup * kDenseBlock) {
+                    int col[kFieldGroup];
+                    bool upd[kFieldGroup];
+                    T acc[kFieldGroup];
+#pragma unroll
+                    for (int q = 0; q < kFieldGroup; ++q) {
+                        const int i = g0 + q * kDenseBlock;
+                        upd[q] = i < n && !(i >= i0 && i < i0 + w);
+                        col[q] = min(i, n - 1);
+                        acc[q] = f[col[q]];
+                    }
+                    for (int m0 = 0; m0 < k; m0 += kRankChunk) {
+                        float v[kFieldGroup][kRankChunk];
+#pragma unroll
+                        for (int u = 0; u < kRankChunk; ++u) {
+                            const size_t row = size_t(i0 + s_t[min(m0 + u, k - 1)]) * size_t(n);
+#pragma unroll
+                            for (int q = 0; q < kFieldGroup; ++q) v[q][u] = d.Q[row + size_t(col[q])];
+                        }
+#pragma unroll
+                        for (int u = 0; u < kRankChunk; ++u) {
+                            const T dl = T(s_delta[min(m0 + u, k - 1)]);
+#pragma unroll
+                            for (int q = 0; q < kFieldGroup; ++q) {
+                                const T nxt = acc[q] + T(v[q][u]) * dl;
+                                acc[q] = m0 + u < k ? nxt : acc[q];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < kFieldGroup; ++q)
+                        if (upd[q]) f[g0 + q * kDenseBlock] = acc[q];
                 }
             }
             __syncthreads();
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
             // f = h + Q x from scratch in FP64, j ascending (column i of Q = row i)
-            for (int i = tid; i < n; i += kDenseBlock) {
-                double s = double(d.h[i]);
-                for (int j = 0; j < n; ++j)
-                    if (x[j]) s += double(d.Q[size_t(j) * size_t(n) + size_t(i)]);
-                f[i] = T(s);
+            for (int g0 = tid; g0 < n; g0 += kFieldGroup * kDenseBlock) {
+                double sc[kFieldGroup];
+                scratch_fields(d, x, g0, sc);   // reads x and Q only, so fields may change meanwhile
+#pragma unroll
+                for (int q = 0; q < kFieldGroup; ++q)
+                    if (g0 + q * kDenseBlock < n) f[g0 + q * kDenseBlock] = T(sc[q]);
             }
             __syncthreads();
         }
     }
     // outputs: spins, the FP64 energy and max |f_carried - f_scratch|
     double e = 0.0, err = 0.0;
-    for (int i = tid; i < n; i += kDenseBlock) {
-        double s = double(d.h[i]), pair = 0.0;
-        for (int j = 0; j < n; ++j)
-            if (x[j]) {
-                const double q = double(d.Q[size_t(j) * size_t(n) + size_t(i)]);
-                s += q;
-                pair += q;
-            }
-        err = fmax(err, fabs(double(f[i]) - s));
-        if (x[i]) e += double(d.h[i]) + 0.5 * pair;
-        x_out[size_t(blockIdx.x) * size_t(n) + size_t(i)] = x[i];
+    for (int g0 = tid; g0 < n; g0 += kFieldGroup * kDenseBlock) {
+        double sc[kFieldGroup];
+        scratch_fields(d, x, g0, sc);
+#pragma unroll
+        for (int q = 0; q < kFieldGroup; ++q) {
+            const int i = g0 + q * kDenseBlock;
+            if (i >= n) continue;
+            err = fmax(err, fabs(double(f[i]) - sc[q]));
+            if (x[i]) e += double(d.h[i]) + 0.5 * (sc[q] - double(d.h[i]));
+            x_out[size_t(blockIdx.x) * size_t(n) + size_t(i)] = x[i];
+        }
     }
     for (int o = 32; o > 0; o /= 2) {
         e += __shfl_down(e, o);
