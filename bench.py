@@ -145,18 +145,10 @@ def run_serving(args):
         print(json.dumps(out), flush=True)
 
 
-def run_precision(args):
-    """The FP32 residual carry against full FP64 on the muon QUBO: 64 branches, T 50 -> 1
-    over 400 sweeps + 100 at T = 1, the same keys and order. For FP32 with an FP64
-    recompute every k sweeps (and with none) it reports the final carry error
-    max |r_fp32 - r_fp64(scratch)|, the branches' final energies against the FP64 run's,
-    how many spins and tri-state cells agree, and the time per solve."""
-    import qublar_amd as qa
-
-    P = qa.muon_scene()
-    prob = qa.Problem(P)
-    R = 64
-    sched = qa.schedule(50.0, 1.0, 400, 100)
+def precision_rows(qa, prob, R, sched):
+    """FP64, then FP32 with an FP64 recompute every k in {1, 10, 100} sweeps and with
+    none, the same keys and order: per run the final carry error, the energies against
+    FP64's, the spins and tri-state cells that agree, and the time per solve."""
     runs = [("fp64", qa.FP64, 0), ("fp32 k=1", qa.FP32, 1), ("fp32 k=10", qa.FP32, 10),
             ("fp32 k=100", qa.FP32, 100), ("fp32 no recompute", qa.FP32, 0)]
     base = None
@@ -179,9 +171,59 @@ def run_precision(args):
                         "spins_equal_fraction": float(np.mean(out["branches"] == base["branches"])),
                         "branches_identical": int(np.sum(np.all(out["branches"] == base["branches"], axis=1))),
                         "tri_state_equal_fraction": float(np.mean(out["tri"] == base["tri"]))})
-        rows.append(row)
+        rows.append((row, out))
+    return rows
+
+
+def run_precision(args):
+    """The FP32 residual carry against full FP64 on the muon QUBO: 64 branches, T 50 -> 1
+    over 400 sweeps + 100 at T = 1, the same keys and order. For FP32 with an FP64
+    recompute every k sweeps (and with none) it reports the final carry error
+    max |r_fp32 - r_fp64(scratch)|, the branches' final energies against the FP64 run's,
+    how many spins and tri-state cells agree, and the time per solve."""
+    import qublar_amd as qa
+
+    prob = qa.Problem(qa.muon_scene())
+    rows = precision_rows(qa, prob, 64, qa.schedule(50.0, 1.0, 400, 100))
     print(json.dumps({"workload": "precision", "problem": "muon QUBO 306,328 bits, 64 branches, 500 sweeps",
-                      "rows": rows}), flush=True)
+                      "rows": [r for r, _ in rows]}), flush=True)
+
+
+def run_gravity_precision(args):
+    """The same study on the gravimetry QUBO: 200,000 cells of 5 m under 20,000 surface
+    stations, rows truncated at 50 m (39.1M entries, ~196 stations per bit), 16 branches,
+    T 50 -> 1 over 200 sweeps + 50 at T = 1. Adds the spin-update rate of each run, the
+    builder's and the upload's times, the recovered body, and the host oracle's rate."""
+    import qublar_amd as qa
+
+    t0 = time.perf_counter()
+    P = qa.gravity_scene()
+    t_scene = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    prob = qa.Problem(P)
+    t_upload = time.perf_counter() - t0
+    info = prob.info()
+    log(f"gravity QUBO {P.n_vars} bits, {P.n_rays} stations, {int(P.row_ptr[-1])} entries, {info['n_classes']} "
+        f"classes; scene {t_scene:.2f} s, upload {t_upload:.2f} s")
+    R, A, H = 16, 200, 50
+    rows = precision_rows(qa, prob, R, qa.schedule(50.0, 1.0, A, H))
+    updates = R * (A + H) * P.n_vars
+    for row, out in rows:
+        row["spin_updates_per_s"] = updates / row["solve_s"]
+        row["body_cells_found"] = int(((out["tri"] == 1) & (P.truth == 1)).sum())
+        row["cells_exists"] = int((out["tri"] == 1).sum())
+    result = {"workload": "gravity-precision",
+              "problem": "gravimetry QUBO: 100 x 200 x 10 cells of 5 m, 20,000 stations, rows truncated at 50 m; "
+                         "16 branches, T 50 -> 1 over 200 sweeps + 50 at T = 1",
+              "n_vars": P.n_vars, "n_rays": P.n_rays, "entries": int(P.row_ptr[-1]),
+              "colour_classes": info["n_classes"], "body_cells": int(P.truth.sum()),
+              "scene_s": t_scene, "upload_s": t_upload, "rows": [r for r, _ in rows],
+              "data": "synthetic: the body's truncated anomaly plus N(0, 10 microGal) noise"}
+    if not args.no_cpu:
+        threads = max(1, min(16, (os.cpu_count() or 2) - 2))
+        log(f"cpu baseline on {threads} threads ...")
+        result["cpu_baseline"] = cpu_baseline(P, args.cpu_seconds, threads)
+    print(json.dumps(result), flush=True)
 
 
 def qubo_cpu_baseline(inst, sched_args, seconds, threads):
@@ -351,7 +393,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dense-n", type=int, default=4096, help="dense precision: bits")
     ap.add_argument("--dense-sweeps", type=int, default=5000, help="dense precision: sweeps")
-    ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched", "dense-precision"], default="muon",
+    ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched", "dense-precision",
+                                           "gravity-precision"], default="muon",
                     help="muon: the headline line (default); serving: the batched serving stream; "
                          "batched: many sparse QUBOs in one persistent launch")
     ap.add_argument("--instances", type=int, default=256, help="batched: instances")
@@ -382,6 +425,9 @@ def main():
         return
     if args.workload == "dense-precision":
         run_dense_precision(args)
+        return
+    if args.workload == "gravity-precision":
+        run_gravity_precision(args)
         return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5

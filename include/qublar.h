@@ -232,6 +232,32 @@ int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* view);
 int qb_host_problem_truth(const qb_host_problem* h, uint8_t* truth);
 int qb_host_problem_destroy(qb_host_problem* h);
 
+/* The gravimetry scene: the operator of src/op_gravity.hpp:17-74 (a cell of volume h^3
+   and contrast drho is a point mass at its centre; a_sb = G drho h^3 (z_s - z_b) / r^3 in
+   microGal, one row per station, w = 1 / sigma^2) with every row truncated at `radius`
+   (the reference sums all cells, op_gravity.hpp:59-74). nx x ny x nz cells of edge h lie
+   below the surface z = 0, bits in x-fastest site order, one station at each column
+   centre on the surface; d = the anomaly of the true body (cells whose centre lies in
+   the box body_lo..body_hi) plus N(0, sigma^2) noise; the kappa / lambda prior over
+   6-neighbour cells. The builder also colours the problem: layer k by a lattice of
+   period P_k columns, the smallest P_k >= 2 for which two cells of a class see no
+   common station (qb_host_problem_colour; the view passes it as desc.colour). */
+typedef struct {
+    int nx, ny, nz;             /* cells (100 x 200 x 10) */
+    double h;                   /* cell edge, m (5) */
+    double radius;              /* kernel cut-off, m (50) */
+    double drho;                /* density contrast, kg/m^3 (1000) */
+    double sigma;               /* station noise, microGal (10) */
+    double body_lo[3], body_hi[3];   /* the true body, m (200..230, 400..440, -35..-15) */
+    double lambda, kappa;       /* prior (2, ln 999) */
+    uint64_t seed;              /* noise draws */
+} qb_gravity_scene;
+
+int qb_gravity_scene_defaults(qb_gravity_scene* s);
+int qb_scene_gravity(const qb_gravity_scene* s, qb_host_problem** out);
+/* colour[n_vars]: the builder's colouring; QB_E_ARG when the builder gave none. */
+int qb_host_problem_colour(const qb_host_problem* h, int32_t* colour);
+
 /* ---- batched sparse QUBOs ---------------------------------------------------------
    No single counterpart in the reference: it solves one QUBO per call (anneal_branch /
    branch_fractions, src/ising_recon.hpp:121-166, with a QUBO's pair terms encoded as

@@ -114,6 +114,12 @@ class MuonScene(C.Structure):
                 ("sky_th", C.c_int)]
 
 
+class GravityScene(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("h", C.c_double), ("radius", C.c_double),
+                ("drho", C.c_double), ("sigma", C.c_double), ("body_lo", C.c_double * 3),
+                ("body_hi", C.c_double * 3), ("lam", C.c_double), ("kappa", C.c_double), ("seed", C.c_uint64)]
+
+
 _LIB = None
 
 
@@ -149,6 +155,9 @@ def lib():
             "qb_host_problem_view": (i32, [vp, C.POINTER(ProblemDesc)]),
             "qb_host_problem_truth": (i32, [vp, vp]),
             "qb_host_problem_destroy": (i32, [vp]),
+            "qb_gravity_scene_defaults": (i32, [C.POINTER(GravityScene)]),
+            "qb_scene_gravity": (i32, [C.POINTER(GravityScene), C.POINTER(vp)]),
+            "qb_host_problem_colour": (i32, [vp, vp]),
             "qb_scene_er_graph": (i32, [i32, dbl, u64, vp, vp, vp, i64, C.POINTER(i64)]),
             "qb_qubo_batch_create": (i32, [C.POINTER(QuboBatchDesc), C.POINTER(vp)]),
             "qb_qubo_batch_destroy": (i32, [vp]),
@@ -405,14 +414,9 @@ class Graph:
         return out
 
 
-def muon_scene(**overrides):
-    """The muon-tomography QUBO (host builder); returns ProblemArrays with .truth."""
-    s = MuonScene()
-    _check(lib().qb_muon_scene_defaults(C.byref(s)))
-    for k, v in overrides.items():
-        setattr(s, "lam" if k == "lambda" else k, v)
-    h = C.c_void_p()
-    _check(lib().qb_scene_muon(C.byref(s), C.byref(h)))
+def _host_problem(h, coloured):
+    """ProblemArrays (with .truth, and the builder's colouring when `coloured`) copied out
+    of a host problem, which is destroyed."""
     try:
         v = ProblemDesc()
         _check(lib().qb_host_problem_view(h, C.byref(v)))
@@ -428,11 +432,41 @@ def muon_scene(**overrides):
         nbr_ptr = arr(v.nbr_ptr, n + 1, np.int32)
         truth = np.zeros(n, np.uint8)
         _check(lib().qb_host_problem_truth(h, _p(truth)))
+        colour = None
+        if coloured:
+            colour = np.zeros(n, np.int32)
+            _check(lib().qb_host_problem_colour(h, _p(colour)))
         return ProblemArrays(row_ptr, arr(v.ray, int(row_ptr[-1]), np.int32), arr(v.a, int(row_ptr[-1]), np.float32),
                              arr(v.d, nr, np.float64), arr(v.w, nr, np.float64), nbr_ptr,
-                             arr(v.nbr, int(nbr_ptr[-1]), np.int32), v.lam, v.kappa, None, truth)
+                             arr(v.nbr, int(nbr_ptr[-1]), np.int32), v.lam, v.kappa, None, truth, colour=colour)
     finally:
         lib().qb_host_problem_destroy(h)
+
+
+def muon_scene(**overrides):
+    """The muon-tomography QUBO (host builder); returns ProblemArrays with .truth."""
+    s = MuonScene()
+    _check(lib().qb_muon_scene_defaults(C.byref(s)))
+    for k, v in overrides.items():
+        setattr(s, "lam" if k == "lambda" else k, v)
+    h = C.c_void_p()
+    _check(lib().qb_scene_muon(C.byref(s), C.byref(h)))
+    return _host_problem(h, False)
+
+
+def gravity_scene(**overrides):
+    """The gravimetry QUBO (host builder, stations as rows, rows truncated at `radius`);
+    returns ProblemArrays with .truth and the builder's layer-lattice colouring."""
+    s = GravityScene()
+    _check(lib().qb_gravity_scene_defaults(C.byref(s)))
+    for k, v in overrides.items():
+        if k in ("body_lo", "body_hi"):
+            getattr(s, k)[:] = [float(t) for t in v]
+        else:
+            setattr(s, "lam" if k == "lambda" else k, v)
+    h = C.c_void_p()
+    _check(lib().qb_scene_gravity(C.byref(s), C.byref(h)))
+    return _host_problem(h, True)
 
 
 def er_graph(n=800, p=0.06, seed=1):

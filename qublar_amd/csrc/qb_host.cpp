@@ -644,7 +644,8 @@ static int team_size_for(int requested, int precision, int bw, size_t table_byte
     if (rc) return -rc;
     const int per_cu = ray_sweep_blocks_per_cu(precision, bw, table_bytes);
     if (per_cu <= 0)
-        return -fail(QB_E_LIMIT, "the sweep kernel does not fit a CU with %zu bytes of class tables in LDS", table_bytes);
+        return -fail(QB_E_LIMIT, "the sweep kernel does not fit a CU with %zu bytes of dynamic LDS (class tables, "
+                                 "multi-pass staging)", table_bytes);
     const int cap = per_cu * prop.multiProcessorCount;
     const int total = requested > 0 ? requested : prop.multiProcessorCount;
     if (total % 8 != 0 || total > cap)
@@ -652,12 +653,26 @@ static int team_size_for(int requested, int precision, int bw, size_t table_byte
     return total / 8;
 }
 
+// True when some bit of the problem has more chains than a sweep workgroup has slots
+// at group width bw -- a multi-pass batch of batches_for, whose staging needs LDS.
+static bool has_multipass(const qb_problem* P, int bw) {
+    const int slots = kSweepBlock / bw;
+    for (size_t q = 0; q + 1 < P->col_ptr.size(); ++q)
+        if ((P->col_ptr[q + 1] - P->col_ptr[q] + kSeg - 1) / kSeg > slots) return true;
+    return false;
+}
+
 // Branches per group: the smallest of 2, 4, 8, 16 that spreads the branches over
-// the 8 XCD teams in at most 8 groups, unless the caller fixes it.
-static int group_width_for(int R, int requested) {
+// the 8 XCD teams in at most 8 groups, unless the caller fixes it; wider while the
+// sweep kernel with the problem's dynamic LDS (class tables, multi-pass staging) does
+// not fit a CU.
+static int group_width_for(const qb_problem* P, int R, int requested, int precision) {
     if (requested) return requested;
     int bw = 2;
     while (bw < 16 && (R + bw - 1) / bw > 8) bw *= 2;
+    while (bw < 16 &&
+           ray_sweep_blocks_per_cu(precision, bw, sweep_dynamic_lds(P->n_classes, bw, has_multipass(P, bw))) <= 0)
+        bw *= 2;
     return bw;
 }
 
@@ -797,9 +812,10 @@ static int batches_for(qb_problem* P, int bw, int team_size, const DevBatches** 
         (rc = B->cls_heavy.upload(cls_heavy, P->stream)))
         return rc;
     QB_HIP(hipStreamSynchronize(P->stream));
+    const bool heavy = std::any_of(count.begin(), count.end(), [](int32_t c) { return c < 0; });
     B->dev = DevBatches{B->cls.as<int32_t>(), B->cls_heavy.as<int32_t>(), B->first.as<int32_t>(),
                         B->count.as<int32_t>(), B->pos.as<int32_t>(), B->slot0.as<int32_t>(),
-                        B->desc.as<SlotDesc>(), n_batches};
+                        B->desc.as<SlotDesc>(), n_batches, heavy ? 1 : 0};
     *out = &B->dev;
     P->batches[key] = std::move(B);
     return QB_OK;
@@ -821,7 +837,7 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
         return fail(QB_E_ARG, "group_width must be 0, 2, 4, 8 or 16 (got %d)", o.group_width);
     const int total = s->anneal_sweeps + s->hold_sweeps;
     const int n = P->n, nr = P->n_rays;
-    const int bw = group_width_for(R, o.group_width);
+    const int bw = group_width_for(P, R, o.group_width, o.precision);
     const int n_groups = (R + bw - 1) / bw;
     const size_t tsz = o.precision == QB_FP32 ? sizeof(float) : sizeof(double);
     const int team_size = team_size_for(o.workgroups, o.precision, bw, 0);
@@ -829,8 +845,9 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     const DevBatches* bt = nullptr;
     if ((rc = batches_for(P, bw, team_size, &bt))) return rc;
     {
-        // the LDS tables must not lower the residency the team size was chosen for
-        const int checked = team_size_for(o.workgroups, o.precision, bw, sweep_table_bytes(P->n_classes));
+        // the dynamic LDS must not lower the residency the team size was chosen for
+        const int checked =
+            team_size_for(o.workgroups, o.precision, bw, sweep_dynamic_lds(P->n_classes, bw, bt->heavy_lds != 0));
         if (checked <= 0) return -checked;
     }
     const int n_part = ray_energy_blocks(P->dev, bw);

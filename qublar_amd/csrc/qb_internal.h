@@ -95,11 +95,13 @@ struct DevBatches {
     const int32_t* slot0;       // per item: its first slot in the batch
     const SlotDesc* desc;       // per batch: BLOCK / BW slot descriptors (all empty for count -1)
     int n_batches;
+    int heavy_lds;              // 1 when some batch is multi-pass: the sweep's dynamic LDS then
+                                // holds the staging area after the class tables
 };
 
 // The sweep keeps its class tables (cls_batch, cls_heavy, light_begin, cls_ptr end) in LDS.
 constexpr int kSweepTableLimit = 96 * 1024;
-inline size_t sweep_table_bytes(int n_classes) { return sizeof(int32_t) * size_t(4 * n_classes + 2); }
+__host__ __device__ inline size_t sweep_table_bytes(int n_classes) { return sizeof(int32_t) * size_t(4 * n_classes + 2); }
 
 // Workgroups of a team reserved for a class's multi-pass batches, the packed batches
 // going round-robin over the others; with none reserved every task of the class,
@@ -159,6 +161,20 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
 // `table_bytes` of dynamic LDS (0 when they do not fit).
 int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes);
 constexpr int kSweepBlock = 256;   // threads per sweep workgroup
+// A multi-pass bit whose chains fit kHeavyPasses rounds of a workgroup's slots is staged
+// whole in LDS. The staging area follows the class tables in the sweep's dynamic LDS
+// (16-byte aligned), reserved only when the problem has a multi-pass bit at the group
+// width, so problems without one keep that LDS for their tables.
+constexpr int kHeavyPasses = 3;
+__host__ __device__ inline size_t heavy_stage_offset(int n_classes) {
+    return (sweep_table_bytes(n_classes) + 15) & ~size_t(15);
+}
+__host__ __device__ inline size_t heavy_stage_bytes(int bw) {
+    return sizeof(RayEnt) * size_t(kHeavyPasses) * size_t(kSweepBlock / bw) * size_t(kSeg);
+}
+inline size_t sweep_dynamic_lds(int n_classes, int bw, bool heavy) {
+    return heavy_stage_offset(n_classes) + (heavy ? heavy_stage_bytes(bw) : 0);
+}
 // Entries of a bit with more chains than a sweep workgroup has slots at group width 2
 // (a multi-pass bit there).
 constexpr int kHeavyEntries = kSeg * (kSweepBlock / 2);

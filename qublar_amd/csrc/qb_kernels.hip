@@ -539,17 +539,16 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
 // positions and masks are loaded by the deciding slot at the start, beside the column.
 // A step of such a bit costs the staging plus one round of residual loads, where the
 // pass-by-pass form costs two dependent round trips per pass and again per pass to
-// scatter.
-constexpr int kHeavyPasses = 3;
+// scatter. `hent` is the staging area in the sweep's dynamic LDS (heavy_stage_bytes).
 
 template <typename T, int BW, int BLOCK>
 __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevState<T>& s, const HeavyBit& h, int g,
-                                                 int sweep, double beta) {
+                                                 int sweep, double beta, RayEnt* hent) {
     constexpr int kSlots = BLOCK / BW;
     constexpr int kMaxEnt = kHeavyPasses * kSlots * kSeg;
     constexpr int kNbHeavy = 8;
     static_assert(kMaxEnt % 64 == 0, "a wave stages 64 entries at a time");
-    __shared__ __align__(16) RayEnt hent[kMaxEnt];
+    static_assert(BLOCK == kSweepBlock, "the staging area is sized for the sweep workgroup");
     __shared__ double hpart[kHeavyPasses * kSlots][BW];
     __shared__ int hacc[BW];
     const int tid = int(threadIdx.x);
@@ -695,14 +694,15 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
 }
 
 // Dispatch of a multi-pass batch: staged in LDS when its chains fit kHeavyPasses
-// rounds of the workgroup's slots, else pass by pass.
+// rounds of the workgroup's slots (and the launch reserved the staging area), else
+// pass by pass.
 template <typename T, int BW, int BLOCK>
 __device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T>& s, const DevBatches& bt, int batch,
-                                           int g, int sweep, double beta) {
+                                           int g, int sweep, double beta, RayEnt* hent) {
     constexpr int kSlots = BLOCK / BW;
     const HeavyBit h = heavy_bit(bt, batch, kSlots);
-    if (h.k1 - h.k0 <= kHeavyPasses * kSlots * kSeg)
-        heavy_lds_update<T, BW, BLOCK>(p, s, h, g, sweep, beta);
+    if (bt.heavy_lds && h.k1 - h.k0 <= kHeavyPasses * kSlots * kSeg)
+        heavy_lds_update<T, BW, BLOCK>(p, s, h, g, sweep, beta, hent);
     else
         heavy_update<T, BW, BLOCK>(p, s, h, g, sweep, beta);
 }
@@ -833,7 +833,8 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
     const int my_slot = rank * kSlots + slot;
     if (p.n_classes == 0) return;
     // class -> batch table and heavy-batch bits, copied to LDS once
-    extern __shared__ int32_t tables[];
+    extern __shared__ __align__(16) int32_t tables[];
+    RayEnt* hent = reinterpret_cast<RayEnt*>(reinterpret_cast<unsigned char*>(tables) + heavy_stage_offset(p.n_classes));
     int32_t* cls = tables;                          // n_classes + 1 batch offsets
     int32_t* nheavy = tables + p.n_classes + 1;    // n_classes multi-pass batch counts
     int32_t* lbeg = nheavy + p.n_classes;           // n_classes first light positions
@@ -947,7 +948,7 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 if (heavy) {
                     if (fetch1) fetch_static(p, nxt);
                     if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
-                    heavy_step<T, BW, BLOCK>(p, s, bt, batch, g, sweep, beta);
+                    heavy_step<T, BW, BLOCK>(p, s, bt, batch, g, sweep, beta, hent);
                 } else {
                     packed_step<T, BW, BLOCK>(
                         p, s, cur, g, sweep, beta,
@@ -1252,7 +1253,7 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
                             &team_size};
             return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock>),
                                               dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), args,
-                                              unsigned(sweep_table_bytes(p.n_classes)), st);
+                                              unsigned(sweep_dynamic_lds(p.n_classes, BW, bt.heavy_lds != 0)), st);
         };
         return precision == 1 ? go(0.0f) : go(0.0);
     });

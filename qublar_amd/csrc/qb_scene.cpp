@@ -28,6 +28,7 @@ struct qb_host_problem {
     std::vector<double> d, w;
     double lambda = 0.0, kappa = 0.0;
     std::vector<uint8_t> truth;
+    std::vector<int32_t> colour;   // empty unless the builder colours its problem
 };
 
 namespace {
@@ -255,6 +256,167 @@ extern "C" int qb_scene_muon(const qb_muon_scene* s, qb_host_problem** out) {
     return QB_OK;
 }
 
+// ---- gravimetry ---------------------------------------------------------------------
+// The operator of src/op_gravity.hpp:17-74: a bit is a cell of volume h^3 carrying a
+// density contrast drho, seen from a station as a point mass at the cell's centre, so
+// a_sb = G drho h^3 (z_s - z_b) / r^3 in microGal. Here the rows are truncated at
+// `radius` (the reference sums every cell). Cells lie below a flat surface z = 0, cell
+// (i, j, k) centred at ((i + 1/2) h, (j + 1/2) h, -(k + 1/2) h), bits in x-fastest site
+// order; one station at every column centre on the surface.
+
+constexpr double kGravG = 6.67430e-11;    // m^3 kg^-1 s^-2 (CODATA 2018)
+constexpr double kMicroGal = 1e-8;        // m / s^2
+
+// Station offset (di, dj) columns away sees a cell of layer k iff r <= radius.
+static bool sees(const qb_gravity_scene* s, int di, int dj, int k) {
+    const double dx = di * s->h, dy = dj * s->h, dz = (k + 0.5) * s->h;
+    return dx * dx + dy * dy + dz * dz <= s->radius * s->radius;
+}
+
+// Column reach of the rows: stations farther than the grid spans are never seen.
+static int gravity_reach(const qb_gravity_scene* s) {
+    return int(std::min(std::ceil(s->radius / s->h), double(std::max(s->nx, s->ny))));
+}
+
+// Lattice period of layer k: the smallest P >= 2 such that two cells of the layer whose
+// columns differ by a nonzero multiple of P in x and in y see no common station, i.e.
+// no lattice vector v = (a P, b P) is a difference p - q of two station offsets the
+// layer sees. P = max(nx, ny) makes every cell of the layer a class of its own.
+static int layer_period(const qb_gravity_scene* s, int k) {
+    const int reach = gravity_reach(s);
+    std::vector<std::pair<int, int>> off;
+    for (int dj = -reach; dj <= reach; ++dj)
+        for (int di = -reach; di <= reach; ++di)
+            if (sees(s, di, dj, k)) off.emplace_back(di, dj);
+    const int cap = std::max(2, std::max(s->nx, s->ny));
+    for (int P = 2; P < cap; ++P) {
+        bool ok = true;
+        const int m = 2 * reach / P + 1;
+        for (int b = -m; ok && b <= m; ++b)
+            for (int a = -m; ok && a <= m; ++a) {
+                if (a == 0 && b == 0) continue;
+                for (const auto& p : off)
+                    if (std::abs(p.first - a * P) <= reach && std::abs(p.second - b * P) <= reach &&
+                        sees(s, p.first - a * P, p.second - b * P, k)) {
+                        ok = false;
+                        break;
+                    }
+            }
+        if (ok) return P;
+    }
+    return cap;
+}
+
+extern "C" int qb_gravity_scene_defaults(qb_gravity_scene* s) {
+    if (!s) return QB_E_ARG;
+    *s = qb_gravity_scene{};
+    s->nx = 100;
+    s->ny = 200;
+    s->nz = 10;
+    s->h = 5.0;
+    s->radius = 50.0;
+    s->drho = 1000.0;
+    s->sigma = 10.0;
+    const double lo[3] = {200.0, 400.0, -35.0}, hi[3] = {230.0, 440.0, -15.0};
+    for (int a = 0; a < 3; ++a) {
+        s->body_lo[a] = lo[a];
+        s->body_hi[a] = hi[a];
+    }
+    s->lambda = 2.0;
+    s->kappa = std::log(999.0);
+    s->seed = 20261202;
+    return QB_OK;
+}
+
+extern "C" int qb_scene_gravity(const qb_gravity_scene* s, qb_host_problem** out) {
+    if (!s || !out) return QB_E_ARG;
+    *out = nullptr;
+    if (s->nx < 1 || s->ny < 1 || s->nz < 1 || !(s->h > 0.0) || !(s->radius >= 0.0) || !(s->sigma > 0.0))
+        return QB_E_ARG;
+    if (double(s->nx) * s->ny * s->nz > 2.0e7) return QB_E_LIMIT;
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    const int n = nx * ny * nz, n_st = nx * ny;
+    auto H = std::unique_ptr<qb_host_problem>(new (std::nothrow) qb_host_problem());
+    if (!H) return QB_E_NOMEM;
+    H->n_vars = n;
+    H->n_rays = n_st;
+    H->truth.resize(size_t(n));
+    const double m = s->drho * s->h * s->h * s->h;
+    const int reach = gravity_reach(s);
+    // per layer, the stations a cell sees as column offsets (row-major, so stations come
+    // in ascending index) with their entry value
+    std::vector<std::vector<std::pair<int, int>>> off(static_cast<size_t>(nz));
+    std::vector<std::vector<float>> val(static_cast<size_t>(nz));
+    for (int k = 0; k < nz; ++k)
+        for (int dj = -reach; dj <= reach; ++dj)
+            for (int di = -reach; di <= reach; ++di) {
+                if (!sees(s, di, dj, k)) continue;
+                const double dx = di * s->h, dy = dj * s->h, dz = (k + 0.5) * s->h;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                off[size_t(k)].emplace_back(di, dj);
+                val[size_t(k)].push_back(float(kGravG * m * dz / (r2 * std::sqrt(r2)) / kMicroGal));
+            }
+    H->row_ptr.assign(size_t(n) + 1, 0);
+    for (int b = 0; b < n; ++b) {
+        const int i = b % nx, j = (b / nx) % ny, k = b / n_st;
+        int cnt = 0;
+        for (const auto& o : off[size_t(k)])
+            cnt += i + o.first >= 0 && i + o.first < nx && j + o.second >= 0 && j + o.second < ny;
+        if (int64_t(H->row_ptr[size_t(b)]) + cnt > INT32_MAX) return QB_E_LIMIT;
+        H->row_ptr[size_t(b) + 1] = H->row_ptr[size_t(b)] + cnt;
+    }
+    H->ray.resize(size_t(H->row_ptr[size_t(n)]));
+    H->a.resize(H->ray.size());
+    std::vector<double> model(size_t(n_st), 0.0);
+    for (int b = 0; b < n; ++b) {
+        const int i = b % nx, j = (b / nx) % ny, k = b / n_st;
+        const double c[3] = {(i + 0.5) * s->h, (j + 0.5) * s->h, -(k + 0.5) * s->h};
+        bool body = true;
+        for (int a = 0; a < 3; ++a) body &= c[a] >= s->body_lo[a] && c[a] <= s->body_hi[a];
+        H->truth[size_t(b)] = body;
+        int32_t e = H->row_ptr[size_t(b)];
+        for (size_t q = 0; q < off[size_t(k)].size(); ++q) {
+            const int si = i + off[size_t(k)][q].first, sj = j + off[size_t(k)][q].second;
+            if (si < 0 || si >= nx || sj < 0 || sj >= ny) continue;
+            H->ray[size_t(e)] = sj * nx + si;
+            H->a[size_t(e)] = val[size_t(k)][q];
+            if (body) model[size_t(sj * nx + si)] += double(val[size_t(k)][q]);
+            ++e;
+        }
+    }
+    // d = the true body's anomaly plus Gaussian noise (Box-Muller over splitmix64)
+    H->d.resize(size_t(n_st));
+    H->w.assign(size_t(n_st), 1.0 / (s->sigma * s->sigma));
+    for (int st = 0; st < n_st; ++st) {
+        uint64_t rs = s->seed * 0x9E3779B97F4A7C15ull + uint64_t(st) + 1;
+        const double u1 = 1.0 - uniform(rs), u2 = uniform(rs);
+        H->d[size_t(st)] = model[size_t(st)] + s->sigma * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    }
+    // 6-neighbour adjacency, candidates in the order -x, +x, -y, +y, -z, +z
+    H->nbr_ptr.assign(size_t(n) + 1, 0);
+    for (int b = 0; b < n; ++b) {
+        const int i = b % nx, j = (b / nx) % ny, k = b / n_st;
+        const int cand[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+        for (const auto& q : cand)
+            if (q[0] >= 0 && q[0] < nx && q[1] >= 0 && q[1] < ny && q[2] >= 0 && q[2] < nz)
+                H->nbr.push_back((q[2] * ny + q[1]) * nx + q[0]);
+        H->nbr_ptr[size_t(b) + 1] = int32_t(H->nbr.size());
+    }
+    // colouring: layer k by its lattice of period P_k, layers in separate colour ranges
+    H->colour.resize(size_t(n));
+    int base = 0;
+    for (int k = 0; k < nz; ++k) {
+        const int P = layer_period(s, k);
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) H->colour[size_t((k * ny + j) * nx + i)] = base + i % P + P * (j % P);
+        base += P * P;
+    }
+    H->lambda = s->lambda;
+    H->kappa = s->kappa;
+    *out = H.release();
+    return QB_OK;
+}
+
 extern "C" int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* v) {
     if (!h || !v) return QB_E_ARG;
     v->n_vars = h->n_vars;
@@ -269,12 +431,21 @@ extern "C" int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* v
     v->lambda = h->lambda;
     v->kappa = h->kappa;
     v->field = nullptr;
+    v->draw_bit = nullptr;
+    v->draw_key = nullptr;
+    v->colour = h->colour.empty() ? nullptr : h->colour.data();
     return QB_OK;
 }
 
 extern "C" int qb_host_problem_truth(const qb_host_problem* h, uint8_t* truth) {
     if (!h || !truth) return QB_E_ARG;
     std::memcpy(truth, h->truth.data(), h->truth.size());
+    return QB_OK;
+}
+
+extern "C" int qb_host_problem_colour(const qb_host_problem* h, int32_t* colour) {
+    if (!h || !colour || h->colour.empty()) return QB_E_ARG;
+    std::memcpy(colour, h->colour.data(), sizeof(int32_t) * h->colour.size());
     return QB_OK;
 }
 
