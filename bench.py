@@ -208,16 +208,32 @@ def run_gravity_precision(args):
     R, A, H = 16, 200, 50
     rows = precision_rows(qa, prob, R, qa.schedule(50.0, 1.0, A, H))
     updates = R * (A + H) * P.n_vars
+    # gravity fixes mass well and depth poorly (src/op_gravity.hpp:13-14): count the cells
+    # the branches agree on, those of the body, and those in the body's columns
+    cols = P.truth.reshape(-1, P.n_rays).any(axis=0)
     for row, out in rows:
         row["spin_updates_per_s"] = updates / row["solve_s"]
-        row["body_cells_found"] = int(((out["tri"] == 1) & (P.truth == 1)).sum())
-        row["cells_exists"] = int((out["tri"] == 1).sum())
+        on = (out["tri"] == 1).reshape(-1, P.n_rays)
+        row["cells_exists"] = int(on.sum())
+        row["exists_in_body"] = int((on & (P.truth.reshape(-1, P.n_rays) == 1)).sum())
+        row["exists_in_body_columns"] = int((on & cols[None, :]).sum())
+    # the sweep is bound by its 2,586 dependent class steps, not by their width: more
+    # branches per solve cost little more time
+    wide = {}
+    for R2 in (32, 64):
+        prob.anneal_branches(qa.schedule(50.0, 1.0, 2, 0), R2, 0, qa.FP32, 10, branches=False)   # warm
+        t = time.perf_counter()
+        out = prob.anneal_branches(qa.schedule(50.0, 1.0, A, H), R2, 0, qa.FP32, 10, branches=False)
+        wall = time.perf_counter() - t
+        log(f"fp32 k=10, {R2} branches: {wall:.2f} s")
+        wide[str(R2)] = {"solve_s": wall, "spin_updates_per_s": R2 * (A + H) * P.n_vars / wall}
     result = {"workload": "gravity-precision",
               "problem": "gravimetry QUBO: 100 x 200 x 10 cells of 5 m, 20,000 stations, rows truncated at 50 m; "
                          "16 branches, T 50 -> 1 over 200 sweeps + 50 at T = 1",
               "n_vars": P.n_vars, "n_rays": P.n_rays, "entries": int(P.row_ptr[-1]),
               "colour_classes": info["n_classes"], "body_cells": int(P.truth.sum()),
               "scene_s": t_scene, "upload_s": t_upload, "rows": [r for r, _ in rows],
+              "fp32_k10_more_branches": wide,
               "data": "synthetic: the body's truncated anomaly plus N(0, 10 microGal) noise"}
     if not args.no_cpu:
         threads = max(1, min(16, (os.cpu_count() or 2) - 2))
