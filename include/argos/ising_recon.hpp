@@ -6,13 +6,16 @@
 // with Philox draws keyed by (branch seed, sweep, bit) instead of index order with one
 // splitmix64 stream (src/ising_recon.hpp:121-144). Branch b's seed is still
 // b + 1 + seed_offset. The host helpers binary_energy, local_field_dE,
-// freeze_provable_rock, roi_qubo and classify keep the reference's definitions.
+// freeze_provable_rock, roi_qubo and classify keep the reference's definitions, and
+// muon_rows / build_binary_problem build the muon QUBO array for array as the
+// reference's do, over the caller's medium and view types.
 // Errors of the engine throw argos::EngineError.
 #pragma once
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -305,6 +308,125 @@ inline RoiQubo roi_qubo(const BinaryProblem& p, const std::vector<int>& roi, con
         }
     }
     return q;
+}
+
+// ---- the muon sensor's rows and the QUBO builder -----------------------------------
+// src/op_muon.hpp:22-50 and src/ising_recon.hpp:52-62. The medium and the views are
+// the caller's types -- in the reference, VoxelMedium (src/muon.cuh:38-58) and MuonView /
+// MuonBinnedData (src/muon_recon.hpp:44-70), defined in its CUDA headers -- so both are
+// template parameters here. Members used: medium {lo.x, lo.y, lo.z, voxel, nx, ny, nz,
+// mu_rock, mu[cell]}, view {chamber.x/y/z, data}, data {size(), open[b], det[b],
+// bin_direction(b).x/y/z}.
+
+namespace muon_detail {
+
+// Amanatides & Woo (1987), "A fast voxel traversal algorithm for ray tracing", in float
+// and in the order of operations of the reference's march_impl (src/muon.cuh:60-139), so
+// the segments -- and the problem's entries -- come out bit for bit the same: the grid's
+// slab entry and exit (components with |d| < 1e-12 never divided by), the start cell
+// clamped into the grid, then one cell per step along the axis whose next boundary is
+// nearest (ties to x, then y), visit(cell, segment) summed over every positive segment,
+// at most 3 (nx + ny + nz) + 16 steps. As there, the boundaries are ray parameters from
+// the origin p and the end is the exit parameter less the entry one.
+template <typename Medium, typename Visit>
+inline float march_cells(const Medium& m, const float p[3], const float d[3], Visit&& visit) {
+    constexpr float kFlat = 1e-12f;
+    const float lo[3] = {m.lo.x, m.lo.y, m.lo.z};
+    const int n[3] = {m.nx, m.ny, m.nz};
+    float enter = -1e30f, leave = 1e30f;
+    for (int a = 0; a < 3; ++a) {
+        if (std::fabs(d[a]) < kFlat) {
+            if (p[a] < lo[a] || p[a] > lo[a] + n[a] * m.voxel) return 0.0f;
+            continue;
+        }
+        const float t_lo = (lo[a] - p[a]) / d[a];
+        const float t_hi = (lo[a] + n[a] * m.voxel - p[a]) / d[a];
+        enter = std::max(enter, std::min(t_lo, t_hi));
+        leave = std::min(leave, std::max(t_lo, t_hi));
+    }
+    if (leave <= std::max(enter, 0.0f)) return 0.0f;
+    const float t_start = std::max(enter, 0.0f);
+    const float t_end = leave - t_start;
+    int cell[3];
+    float next[3], step[3];
+    for (int a = 0; a < 3; ++a) {
+        const float q = p[a] + d[a] * t_start;
+        cell[a] = std::min(std::max(static_cast<int>(std::floor((q - lo[a]) / m.voxel)), 0), n[a] - 1);
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (std::fabs(d[a]) < kFlat) {
+            next[a] = 1e30f;
+            step[a] = 0.0f;
+            continue;
+        }
+        const float edge = static_cast<float>(d[a] > 0.0f ? cell[a] + 1 : cell[a]) * m.voxel + lo[a];
+        next[a] = std::max((edge - p[a]) / d[a], 0.0f);
+        step[a] = m.voxel / std::fabs(d[a]);
+    }
+    float total = 0.0f, done = 0.0f;
+    const int max_steps = 3 * (m.nx + m.ny + m.nz) + 16;
+    for (int it = 0; it < max_steps; ++it) {
+        const int a = next[0] <= next[1] ? (next[0] <= next[2] ? 0 : 2) : (next[1] <= next[2] ? 1 : 2);
+        const float t = std::min(next[a], t_end);
+        const float seg = t - done;
+        if (seg > 0.0f) total += visit((cell[2] * m.ny + cell[1]) * m.nx + cell[0], seg);
+        if (t >= t_end) break;
+        done = t;
+        cell[a] += d[a] > 0.0f ? 1 : -1;
+        next[a] += step[a];
+        if (cell[a] < 0 || cell[a] >= n[a]) break;
+    }
+    return total;
+}
+
+}  // namespace muon_detail
+
+// One row per sky bin of each view with at least min_open open-sky muons whose
+// bin-centre ray crosses a bit: d = tau0 - t (the model's optical depth, marched in
+// float, minus the measured -ln(det / open)), w = max(det, 1), and on every bit the ray
+// crosses a = a_per_metre * segment (NaN: the medium's mu_rock, the void path).
+template <typename Medium, typename View>
+inline OperatorRows muon_rows(const Medium& m_model, const std::vector<View>& views, const BitField& bits,
+                              double min_open = 30.0, float a_per_metre = std::numeric_limits<float>::quiet_NaN()) {
+    OperatorRows op;
+    const float a_scale = std::isnan(a_per_metre) ? m_model.mu_rock : a_per_metre;
+    for (const View& view : views) {
+        const auto& data = *view.data;
+        const float p[3] = {view.chamber.x, view.chamber.y, view.chamber.z};
+        for (int b = 0; b < data.size(); ++b) {
+            if (static_cast<double>(data.open[b]) < min_open) continue;
+            const auto dir = data.bin_direction(b);
+            const float d[3] = {dir.x, dir.y, dir.z};
+            const double tau0 = muon_detail::march_cells(m_model, p, d, [&](int c, float seg) {
+                return m_model.mu[c] * seg;
+            });
+            const double det = std::max(static_cast<double>(data.det[b]), 1.0);
+            const double t = -std::log(det / static_cast<double>(data.open[b]));
+            const int row = op.n_rows();
+            const size_t first = op.entries.size();
+            muon_detail::march_cells(m_model, p, d, [&](int c, float seg) {
+                const int bit = bits.site_bit[static_cast<size_t>(c)];
+                if (bit >= 0) op.entries.push_back({bit, row, a_scale * seg});
+                return 0.0f;
+            });
+            if (op.entries.size() == first) continue;
+            op.d.push_back(tau0 - t);
+            op.w.push_back(det);
+        }
+    }
+    return op;
+}
+
+// The QUBO of binned muon data: a grid bit field over the domain, the muon rows, the
+// Ising prior -- assembled into one BinaryProblem.
+template <typename Medium, typename View>
+inline BinaryProblem build_binary_problem(const Medium& m_model, const std::vector<View>& views,
+                                          const std::vector<char>& domain, double lambda, double kappa,
+                                          double min_open = 30.0,
+                                          float a_per_metre = std::numeric_limits<float>::quiet_NaN()) {
+    const BitField bits = BitField::grid(m_model.nx, m_model.ny, m_model.nz, domain);
+    const OperatorRows muons = muon_rows(m_model, views, bits, min_open, a_per_metre);
+    return assemble(bits, {&muons}, IsingPrior{lambda, kappa});
 }
 
 // src/ising_recon.hpp:302-309

@@ -5,10 +5,12 @@
 // invariant, exit code 1 on any failure.
 //   check_dropin            every check (needs the GPU)
 //   check_dropin --host     the host-only checks
+//   check_dropin --assemble / --muon ...   problem dumps for the CPU tests
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -291,7 +293,115 @@ static int assemble_dump(int nx, int ny, int nz, uint64_t seed, int rows) {
     return 0;
 }
 
+// --muon nx ny nz seed bins: build_binary_problem over a medium and views of this
+// driver's own types with the members of the reference's VoxelMedium / MuonView /
+// MuonBinnedData. The grid (voxel 2 m, random lo) holds random mu in a random domain; two
+// views, one chamber inside the grid and one outside it, see `bins` directions each
+// (random, plus axis-parallel ones) with random open and det counts. Prints the inputs
+// and the problem (floats as hex) for a brute-force check.
+struct Vec3 { float x, y, z; };
+struct Medium {
+    Vec3 lo;
+    float voxel;
+    int nx, ny, nz;
+    float mu_rock, mu_air;
+    const float* mu;
+};
+struct Binned {
+    std::vector<Vec3> dirs;
+    std::vector<unsigned long long> open, det;
+    int size() const { return int(dirs.size()); }
+    Vec3 bin_direction(int b) const { return dirs[size_t(b)]; }
+};
+struct View {
+    Vec3 chamber;
+    const Binned* data;
+};
+
+static int muon_dump(int nx, int ny, int nz, uint64_t seed, int bins) {
+    uint64_t s = seed;
+    auto u = [&] { return detail::uniform(s); };
+    Medium m{};
+    m.voxel = 2.0f;
+    m.lo = Vec3{float(-nx + 4.0 * u()), float(-ny + 4.0 * u()), float(-3.0 * u())};
+    m.nx = nx;
+    m.ny = ny;
+    m.nz = nz;
+    m.mu_rock = 0.046f;
+    m.mu_air = 0.00002f;
+    std::vector<float> mu(size_t(nx) * size_t(ny) * size_t(nz));
+    std::vector<char> dom(mu.size());
+    for (size_t c = 0; c < mu.size(); ++c) {
+        dom[c] = u() < 0.7;
+        mu[c] = dom[c] ? float(0.03 + 0.03 * u()) : m.mu_air;
+    }
+    m.mu = mu.data();
+    std::vector<Binned> data(2);
+    std::vector<View> views;
+    // off every cell boundary (an origin on one makes the crossed cells ambiguous)
+    const Vec3 chambers[2] = {Vec3{m.lo.x + float(nx) * 0.9f + 0.37f, m.lo.y + float(ny) * 1.1f + 0.21f, m.lo.z + 1.3f},
+                              Vec3{m.lo.x - 7.5f, m.lo.y + float(ny) * 0.8f + 0.33f, m.lo.z - 2.25f}};
+    for (int v = 0; v < 2; ++v) {
+        Binned& D = data[size_t(v)];
+        for (int b = 0; b < bins; ++b) {
+            Vec3 d;
+            if (b < 6) {   // axis-parallel directions: the flat-component guard
+                d = Vec3{0.0f, 0.0f, 0.0f};
+                (b % 2 ? d.x : (b % 3 ? d.y : d.z)) = b < 3 ? 1.0f : -1.0f;
+            } else {
+                const double th = std::acos(1.0 - 2.0 * u()), az = 6.283185307179586 * u();
+                d = Vec3{float(std::sin(th) * std::cos(az)), float(std::sin(th) * std::sin(az)), float(std::cos(th))};
+                if (v == 1) d.x = std::fabs(d.x);   // the outside chamber looks into the grid
+            }
+            D.dirs.push_back(d);
+            const unsigned long long open = (unsigned long long)(5 + 200 * u());
+            D.open.push_back(open);
+            D.det.push_back((unsigned long long)(double(open) * u() * u()));
+        }
+        views.push_back(View{chambers[v], &D});
+    }
+    const float a_per_metre = seed % 2 ? std::numeric_limits<float>::quiet_NaN() : -0.021f;
+    const BinaryProblem p = build_binary_problem(m, views, dom, 2.0, 6.9, 30.0, a_per_metre);
+    std::printf("grid %d %d %d %a %a %a %a %a\n", nx, ny, nz, double(m.lo.x), double(m.lo.y), double(m.lo.z),
+                double(m.voxel), double(std::isnan(a_per_metre) ? m.mu_rock : a_per_metre));
+    std::printf("mu");
+    for (float x : mu) std::printf(" %a", double(x));
+    std::printf("\ndomain");
+    for (char c : dom) std::printf(" %d", int(c));
+    std::printf("\n");
+    for (int v = 0; v < 2; ++v) {
+        std::printf("view %a %a %a\n", double(chambers[v].x), double(chambers[v].y), double(chambers[v].z));
+        std::printf("dirs");
+        for (const Vec3& d : data[size_t(v)].dirs) std::printf(" %a %a %a", double(d.x), double(d.y), double(d.z));
+        std::printf("\nopen");
+        for (auto o : data[size_t(v)].open) std::printf(" %llu", o);
+        std::printf("\ndet");
+        for (auto o : data[size_t(v)].det) std::printf(" %llu", o);
+        std::printf("\n");
+    }
+    auto ints = [](const char* name, const std::vector<int>& v) {
+        std::printf("%s", name);
+        for (int x : v) std::printf(" %d", x);
+        std::printf("\n");
+    };
+    ints("row_ptr", p.row_ptr);
+    ints("ray", p.ray);
+    std::printf("a");
+    for (float x : p.a) std::printf(" %a", double(x));
+    std::printf("\nd");
+    for (double x : p.d) std::printf(" %a", x);
+    std::printf("\nw");
+    for (double x : p.w) std::printf(" %a", x);
+    std::printf("\n");
+    ints("var_voxel", p.var_voxel);
+    ints("nbr_ptr", p.nbr_ptr);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 7 && std::strcmp(argv[1], "--muon") == 0)
+        return muon_dump(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
+                         std::strtoull(argv[5], nullptr, 10), std::atoi(argv[6]));
     if (argc == 7 && std::strcmp(argv[1], "--assemble") == 0)
         return assemble_dump(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
                              std::strtoull(argv[5], nullptr, 10), std::atoi(argv[6]));

@@ -70,3 +70,109 @@ def test_assemble_matches_reference(orc, driver, shape, seed, rows):
         assert np.array_equal(got[k], ref[k]), k
     mis = R.ref_adjoint_mismatch(n_bits, rows, len(eb), eb, er, ea, 7)
     assert got["adjoint"] == mis
+
+
+def march_segments(p, d, lo, voxel, shape):
+    """Per cell of the grid (x-fastest), the segment the reference's march gives it,
+    from the slab test on every cell in double precision. The march measures cell exits
+    from the origin p but ends at exit - entry of the grid, and its first segment starts
+    at 0 (src/muon.cuh:90, 115-128): for p inside the grid these are the cells' lengths
+    along the ray; for p outside, the first cell's segment includes the gap from p and the
+    march stops early."""
+    nx, ny, nz = shape
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = i.T.reshape(-1), j.T.reshape(-1), k.T.reshape(-1)      # x fastest
+    cell_lo = np.stack([lo[0] + i * voxel, lo[1] + j * voxel, lo[2] + k * voxel], 1)
+    t0 = np.zeros(len(i))
+    t1 = np.full(len(i), np.inf)
+    for a in range(3):
+        if d[a] == 0.0:
+            inside = (p[a] >= cell_lo[:, a]) & (p[a] <= cell_lo[:, a] + voxel)
+            t1 = np.where(inside, t1, -np.inf)
+            continue
+        ta = (cell_lo[:, a] - p[a]) / d[a]
+        tb = (cell_lo[:, a] + voxel - p[a]) / d[a]
+        t0 = np.maximum(t0, np.minimum(ta, tb))
+        t1 = np.minimum(t1, np.maximum(ta, tb))
+    seg = np.zeros(len(i))
+    crossed = np.flatnonzero(t1 > t0)
+    if len(crossed) == 0:
+        return seg
+    crossed = crossed[np.argsort(t1[crossed])]
+    end = t1[crossed].max() - max(t0[crossed].min(), 0.0)
+    done = 0.0
+    for c in crossed:
+        t = min(t1[c], end)
+        seg[c] = max(t - done, 0.0)
+        if t >= end:
+            break
+        done = t
+    return seg
+
+
+def parse_muon_dump(text):
+    lines = [ln.split() for ln in text.strip().splitlines()]
+    out = {"views": []}
+    hexf = lambda v: np.array([float.fromhex(x) for x in v])
+    for key, *vals in lines:
+        if key == "grid":
+            out["shape"] = tuple(int(v) for v in vals[:3])
+            out["lo"] = hexf(vals[3:6])
+            out["voxel"], out["a_scale"] = float.fromhex(vals[6]), float.fromhex(vals[7])
+        elif key == "view":
+            out["views"].append({"chamber": hexf(vals)})
+        elif key == "dirs":
+            out["views"][-1]["dirs"] = hexf(vals).reshape(-1, 3)
+        elif key in ("open", "det"):
+            out["views"][-1][key] = np.array([int(v) for v in vals])
+        elif key in ("mu", "a", "d", "w"):
+            out[key] = hexf(vals)
+        else:
+            out[key] = np.array([int(v) for v in vals])
+    return out
+
+
+@pytest.mark.parametrize("shape,seed,bins", [((6, 5, 4), 3, 12), ((14, 11, 9), 8, 60), ((20, 9, 13), 5, 90)])
+def test_build_binary_problem_muon_rows(driver, shape, seed, bins):
+    """build_binary_problem (src/ising_recon.hpp:52-62) over the muon rows
+    (src/op_muon.hpp:22-50): every populated bin whose ray crosses a bit is one row with
+    d = tau0 - t, w = max(det, 1); every bit the ray crosses carries a = a_per_metre * its
+    segment. Segments, tau0 and the set of crossed cells are pinned by the slab test in
+    double precision on every cell (the float march agrees to ~1e-5 m), for a chamber
+    inside the grid and for one outside it."""
+    P = parse_muon_dump(subprocess.run([driver, "--muon", *map(str, shape), str(seed), str(bins)],
+                                       capture_output=True, text=True, check=True).stdout)
+    dom = P["domain"].astype(bool)
+    assert np.array_equal(P["var_voxel"], np.flatnonzero(dom))
+    site_bit = np.full(dom.size, -1)
+    site_bit[dom] = np.arange(dom.sum())
+    # the problem's entries regrouped by row
+    bit_of = np.repeat(np.arange(len(P["row_ptr"]) - 1), np.diff(P["row_ptr"]))
+    rows = {}
+    for b, r, a in zip(bit_of, P["ray"], P["a"]):
+        rows.setdefault(int(r), {})[int(b)] = a
+    row = 0
+    crossing_rays = 0
+    for v in P["views"]:
+        for b in range(len(v["open"])):
+            if v["open"][b] < 30:
+                continue
+            L = march_segments(v["chamber"], v["dirs"][b], P["lo"], P["voxel"], P["shape"])
+            bits_crossed = (L > 1e-4) & dom
+            ambiguous = (L > 0) & (L <= 1e-4) & dom
+            if not bits_crossed.any():
+                assert not ambiguous.any(), "a ray grazing bits only: pick another seed"
+                continue
+            crossing_rays += 1
+            got = rows.get(row, {})
+            want = {int(site_bit[c]): P["a_scale"] * L[c] for c in np.flatnonzero(bits_crossed)}
+            assert set(want) <= set(got) and set(got) - set(want) <= {int(site_bit[c]) for c in np.flatnonzero(ambiguous)}
+            for bit, a in want.items():
+                assert abs(got[bit] - a) <= 1e-4 * abs(P["a_scale"]), (row, bit, got[bit], a)
+            tau0 = float((P["mu"] * L).sum())
+            det = max(float(v["det"][b]), 1.0)
+            t = -np.log(det / float(v["open"][b]))
+            assert abs(P["d"][row] - (tau0 - t)) < 1e-4, (row, P["d"][row], tau0 - t)
+            assert P["w"][row] == det
+            row += 1
+    assert row == len(P["d"]) == len(P["w"]) and crossing_rays > 5
