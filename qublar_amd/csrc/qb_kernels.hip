@@ -1249,11 +1249,19 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
             s.bar = bar;
             s.tickets = tickets;
             s.err = err;
-            void* args[] = {const_cast<DevProblem*>(&p), const_cast<DevBatches*>(&bt), &s, &sb, &se, &recompute_every,
-                            &team_size};
-            return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock>),
-                                              dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), args,
-                                              unsigned(sweep_dynamic_lds(p.n_classes, BW, bt.heavy_lds != 0)), st);
+            // A plain launch of at most the co-resident workgroup count (the host checks it
+            // against the occupancy with this dynamic LDS). Not a cooperative launch: the
+            // kernel's barriers are its own atomics, and a cooperative launch takes the
+            // device's global wave sync resource, which one process holds at a time, so a
+            // second process on the GPU would be refused. A workgroup that is not resident
+            // leaves its team short or a barrier waiting: both end in the error flag.
+            const size_t lds = sweep_dynamic_lds(p.n_classes, BW, bt.heavy_lds != 0);
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (e != hipSuccess) return e;
+            ray_team_kernel<T, BW, kSweepBlock><<<dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), lds, st>>>(
+                p, bt, s, sb, se, recompute_every, team_size);
+            return hipGetLastError();
         };
         return precision == 1 ? go(0.0f) : go(0.0);
     });

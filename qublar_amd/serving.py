@@ -5,8 +5,9 @@ launch anneals the block-diagonal union of the requests waiting when the GPU com
 admitted in arrival order while the union's device footprint stays within an HBM budget;
 a host thread assembles and uploads the next union while the GPU anneals the current one,
 so host preparation and device work overlap. A request's branches in a union are, spin
-for spin, its branches annealed alone with its own seed (per-bit draw keys), so batching
-changes when a result arrives, never what it is. Sequential mode solves one request per
+for spin, its branches annealed alone with its own seed (per-bit draw keys), and its
+p_void averages its own branches, so batching changes when a result arrives, never what
+it is. Sequential mode solves one request per
 launch in arrival order, for comparison.
 
 This is new around the reference: QuBLAR solves one problem per branch_fractions call
@@ -32,6 +33,7 @@ class Request:
     done: float = -1.0
     batch: int = -1
     p_void: np.ndarray = field(default=None, repr=False)
+    tri: np.ndarray = field(default=None, repr=False)
     colour: object = field(default=None, repr=False)   # a future of the request's colouring
 
 
@@ -80,12 +82,14 @@ class Server:
 
     def _finish(self, batch, prep, t0, index):
         prob, offs, seed, R = prep
-        out = prob.anneal_branches(self.sched, R, seed, self.precision, self.k, branches=False, energy=False)
-        now = time.perf_counter() - t0
+        out = prob.anneal_branches(self.sched, R, seed, self.precision, self.k, branches=True, outputs=False,
+                                   energy=False)
         for q, (b0, _) in zip(batch, offs):
+            q.p_void, q.tri = branch_agreement(out["branches"][:q.branches, b0:b0 + q.problem.n_vars])
+        now = time.perf_counter() - t0
+        for q in batch:
             q.done = now
             q.batch = index
-            q.p_void = out["p_void"][b0:b0 + q.problem.n_vars].copy()
         return out["report"]
 
     def run(self, requests, batched=True):
@@ -137,6 +141,17 @@ class Server:
     def _wait_admitted(batch):
         while any(q.colour is None for q in batch):
             time.sleep(0.0005)
+
+
+def branch_agreement(branches):
+    """p_void and the tri-state map of a request's own branches (R x n spins), as the
+    engine computes them (src/ising_recon.hpp:148-166, 302-309): p = float(count) /
+    float(R); 1 (NotThere) where p >= 0.9, 0 (Exists) where p <= 0.1, else 2. A request
+    in a union averages its own R branches, not the union's."""
+    R = branches.shape[0]
+    pv = np.count_nonzero(branches, axis=0).astype(np.float32) / np.float32(R)
+    tri = np.where(pv >= np.float32(0.9), 1, np.where(pv <= np.float32(0.1), 0, 2)).astype(np.uint8)
+    return pv, tri
 
 
 def latency_summary(requests):

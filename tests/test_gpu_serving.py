@@ -57,3 +57,22 @@ def test_union_is_order_free():
         a = o1[q][0]
         b = o2[n - 1 - q][0]
         assert np.array_equal(m1[:, a:a + P.n_vars], m2[:, b:b + P.n_vars])
+
+
+def test_server_results_equal_alone():
+    """The server's per-request p_void and tri-state maps, from one union of requests
+    with 16, 8 and 12 branches, equal each request's own solve: a request averages its
+    own branches, not the union's 16."""
+    from qublar_amd.serving import Request, Server
+
+    probs, keys = requests()
+    branches = [16, 8, 12]
+    sched = qa.schedule(20.0, 1.0, 14, 4)
+    reqs = [Request(P, R, key, 0.0) for P, R, key in zip(probs, branches, keys)]
+    say("server: one union of 3 requests")
+    reports = Server(sched, qa.FP32, 4).run(reqs, batched=True)
+    assert len(reports) == 1 and all(q.batch == 0 for q in reqs)
+    for q in reqs:
+        alone = qa.Problem(q.problem).anneal_branches(sched, q.branches, q.seed, qa.FP32, 4)
+        assert np.array_equal(q.p_void, alone["p_void"]), f"request with {q.branches} branches"
+        assert np.array_equal(q.tri, alone["tri"])

@@ -58,3 +58,21 @@ def test_union_arrays():
     assert np.array_equal(U.draw_bit[b1:], np.arange(P.n_vars))
     assert np.all(U.draw_key[:b1] == 7) and np.all(U.draw_key[b1:] == 9)
     assert np.allclose(U.field, P.kappa)
+
+
+def test_branch_agreement_counts_the_requests_own_branches():
+    from qublar_amd.serving import branch_agreement
+
+    rng = np.random.default_rng(4)
+    X = (rng.random((12, 500)) < rng.random(500)).astype(np.uint8)
+    X[:, 0] = 1
+    X[:, 1] = 0
+    X[:11, 2] = 1        # 11 of 12: 0.9166... >= 0.9
+    pv, tri = branch_agreement(X)
+    assert pv.dtype == np.float32
+    assert np.array_equal(pv, (X.sum(0) / 12.0).astype(np.float32))
+    assert (pv[0], tri[0], pv[1], tri[1], tri[2]) == (1.0, 1, 0.0, 0, 1)
+    assert np.array_equal(tri, np.where(pv >= 0.9, 1, np.where(pv <= 0.1, 0, 2)))
+    # a request of 8 branches in a union of 12 sees only its 8
+    pv8, _ = branch_agreement(X[:8])
+    assert np.array_equal(pv8, (X[:8].sum(0) / 8.0).astype(np.float32))
