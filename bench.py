@@ -242,6 +242,43 @@ def run_gravity_precision(args):
     print(json.dumps(result), flush=True)
 
 
+def run_roi(args):
+    """The exact posterior of a region's QUBO over all 2^n states (the enumeration of
+    tools/branches_blaze.py:45-53 over roi_qubo's h and J): n = --roi-n bits with h ~ N(0, 2),
+    J_ij ~ N(0, 1); both device passes (minimum, then Z and the marginals) per step. The
+    host baseline is the oracle's brute force (energy from scratch per state) at n = 21,
+    as states per second."""
+    import qublar_amd as qa
+    import oracle
+
+    n = args.roi_n
+    rng = np.random.default_rng(31)
+    h = rng.normal(0, 2, n)
+    J = np.triu(rng.normal(0, 1, (n, n)), 1)
+    qa.roi_enumerate(h[:16], J[:16, :16])    # warm
+    walls, kms = [], []
+    for s in range(args.steps):
+        t = time.perf_counter()
+        out = qa.roi_enumerate(h, J, 1.0)
+        walls.append(time.perf_counter() - t)
+        kms.append(out["kernel_ms"])
+        log(f"step {s}: {walls[-1] * 1e3:.1f} ms (kernels {kms[-1]:.1f} ms)")
+    states = 2.0 ** n
+    result = {"workload": "roi-enumeration", "n": n, "states": states, "steps": args.steps,
+              "ms_per_step": float(np.mean(walls)) * 1e3, "kernel_ms": float(np.mean(kms)),
+              "states_per_s": states / float(np.mean(walls)), "kernel_states_per_s": states / (float(np.mean(kms)) / 1e3),
+              "e_min": out["e_min"], "log_z": out["log_z"], "p_map": out["p_map"],
+              "data": "synthetic h ~ N(0, 2), J ~ N(0, 1)"}
+    if not args.no_cpu:
+        m = 21
+        t = time.perf_counter()
+        oracle.roi_enumerate(h[:m], J[:m, :m], 1.0)
+        wall = time.perf_counter() - t
+        result["cpu_baseline"] = {"value": 2.0 ** m / wall, "unit": "states/s", "cores": 1, "kind": "port",
+                                  "sample": f"the oracle's brute force at n = {m} ({wall:.1f} s)"}
+    print(json.dumps(result), flush=True)
+
+
 def qubo_cpu_baseline(inst, sched_args, seconds, threads):
     """The oracle's batch trajectory on host threads, one (instance, replica) per task, on
     the first instances of the batch with the sweep count cut so the sample runs about
@@ -410,10 +447,11 @@ def main():
     ap.add_argument("--dense-n", type=int, default=4096, help="dense precision: bits")
     ap.add_argument("--dense-sweeps", type=int, default=5000, help="dense precision: sweeps")
     ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched", "dense-precision",
-                                           "gravity-precision"], default="muon",
+                                           "gravity-precision", "roi"], default="muon",
                     help="muon: the headline line (default); serving: the batched serving stream; "
                          "batched: many sparse QUBOs in one persistent launch")
     ap.add_argument("--instances", type=int, default=256, help="batched: instances")
+    ap.add_argument("--roi-n", type=int, default=32, help="roi: bits enumerated (2^n states)")
     ap.add_argument("--batch-sweeps", type=int, default=2000, help="batched: sweeps")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window")
@@ -444,6 +482,9 @@ def main():
         return
     if args.workload == "gravity-precision":
         run_gravity_precision(args)
+        return
+    if args.workload == "roi":
+        run_roi(args)
         return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5

@@ -203,6 +203,25 @@ int qb_run_batch(qb_graph* g, const qb_schedule* b, uint64_t seed_base, int R, d
 /* device_energies (ising_gpu.cuh:461-475): -cut of K states (K x n bytes, 0/1). */
 int qb_device_energies(qb_graph* g, const uint8_t* X, int K, int64_t* energy_out);
 
+/* ---- exact region-of-interest posterior ------------------------------------------------
+   The conditional QUBO of a region (roi_qubo, src/ising_recon.hpp:245-300: h[n], J[n x n]
+   row-major with the upper triangle used) enumerated over all 2^n states on the device, as
+   tools/branches_blaze.py:45-53, 80-81 does on the host: p(x) = exp(-beta E(x)) / Z with
+   E(x) = sum_i h_i x_i + sum_{i<j} J_ij x_i x_j. marginal[i] = p(x_i = 1) (may be NULL);
+   map_state[i] = bit i of the minimum-energy state, ties to the smallest state index
+   (bit i of the index = x_i; may be NULL). 1 <= n <= 36, beta >= 0 finite. */
+typedef struct {
+    double e_min;           /* minimum energy */
+    double log_z;           /* log sum_x exp(-beta E(x)) */
+    double p_map;           /* probability of the minimum-energy state */
+    uint64_t map_index;     /* that state, bit i = x_i */
+    double states;          /* 2^n */
+    double kernel_ms;       /* both enumeration passes, device time */
+} qb_roi_result;
+
+int qb_roi_enumerate(int n, const double* h, const double* J, double beta, double* marginal, uint8_t* map_state,
+                     qb_roi_result* out);
+
 /* ==== problem builders (host) ======================================================== */
 
 typedef struct qb_host_problem qb_host_problem;

@@ -118,6 +118,9 @@ def lib():
         L.orc_qubo_energy.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _u8p]
         L.orc_qubo_anneal.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _i32p, C.POINTER(Schedule),
                                       C.c_uint64, C.c_int, _u8p]
+        L.orc_roi_enumerate.restype = C.c_double
+        L.orc_roi_enumerate.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         _LIB = L
     return _LIB
 
@@ -293,6 +296,20 @@ def dense_energy(Q, h, x):
     Qv = np.ascontiguousarray(Q, np.float32)
     hv = np.ascontiguousarray(h, np.float32)
     return lib().orc_dense_energy(len(hv), Qv.reshape(-1), hv, np.ascontiguousarray(x, np.uint8))
+
+
+def roi_enumerate(h, J, beta=1.0):
+    """Exact posterior of a small QUBO by brute force: dict(e_min, log_z, map_index,
+    marginal)."""
+    h = np.ascontiguousarray(h, np.float64)
+    J = np.ascontiguousarray(J, np.float64)
+    n = len(h)
+    m = np.zeros(n, np.float64)
+    lz = C.c_double()
+    idx = C.c_uint64()
+    e = lib().orc_roi_enumerate(n, h.ctypes.data, J.ctypes.data, float(beta), m.ctypes.data, C.byref(lz),
+                                C.byref(idx))
+    return dict(e_min=e, log_z=lz.value, map_index=idx.value, marginal=m)
 
 
 def philox(ctr, key):

@@ -774,3 +774,52 @@ void orc_dense_anneal(int n, const float* Q, const float* h, const orc_schedule*
     free(f64);
     free(f32);
 }
+
+/* ---- exact ROI posterior ---------------------------------------------------------------
+   tools/branches_blaze.py:45-53: every state's energy from scratch, p = exp(-(E - E_min))
+   normalised (beta = 1 there), marginals sum_x p(x) x_i (:80-81). */
+static double roi_energy(int n, const double* h, const double* J, uint64_t s) {
+    double e = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (!((s >> i) & 1u)) continue;
+        e += h[i];
+        for (int j = i + 1; j < n; ++j)
+            if ((s >> j) & 1u) e += J[(size_t)i * (size_t)n + (size_t)j];
+    }
+    return e;
+}
+
+double orc_roi_enumerate(int n, const double* h, const double* J, double beta, double* marginal, double* log_z,
+                         uint64_t* map_index) {
+    const uint64_t states = (uint64_t)1 << n;
+    double e_min = HUGE_VAL;
+    uint64_t s_min = 0;
+    for (uint64_t s = 0; s < states; ++s) {
+        const double e = roi_energy(n, h, J, s);
+        if (e < e_min) {
+            e_min = e;
+            s_min = s;
+        }
+    }
+    /* Kahan-compensated sums (Kahan 1965, "Further remarks on reducing truncation
+       errors"): over 2^n terms a plain running sum would lose up to 2^n ulp */
+    double z = 0.0, zc = 0.0;
+    double mc[64];
+    for (int i = 0; i < n; ++i) marginal[i] = mc[i] = 0.0;
+    for (uint64_t s = 0; s < states; ++s) {
+        const double w = exp(-beta * (roi_energy(n, h, J, s) - e_min));
+        const double y = w - zc, t = z + y;
+        zc = (t - z) - y;
+        z = t;
+        for (int i = 0; i < n; ++i)
+            if ((s >> i) & 1u) {
+                const double yi = w - mc[i], ti = marginal[i] + yi;
+                mc[i] = (ti - marginal[i]) - yi;
+                marginal[i] = ti;
+            }
+    }
+    for (int i = 0; i < n; ++i) marginal[i] /= z;
+    *log_z = -beta * e_min + log(z);
+    *map_index = s_min;
+    return e_min;
+}

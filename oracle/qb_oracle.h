@@ -141,6 +141,14 @@ double orc_dense_energy(int n, const float* Q, const float* h, const uint8_t* x)
 void orc_dense_anneal(int n, const float* Q, const float* h, const orc_schedule* s, uint64_t seed, int precision,
                       int recompute_every, uint8_t* x, double* f_final);
 
+/* The exact posterior of a small QUBO over all 2^n states, by brute force in state index
+   order (tools/branches_blaze.py:45-53 exact_posterior, 80-81 marginals; the QUBO is
+   roi_qubo's, src/ising_recon.hpp:245-300): E(x) = sum h_i x_i + sum_{i<j} J_ij x_i x_j
+   (J n x n, upper triangle), p ~ exp(-beta (E - E_min)). Returns E_min; marginal[i],
+   *log_z, *map_index (first state of minimum energy, bit i = x_i). n <= 30. */
+double orc_roi_enumerate(int n, const double* h, const double* J, double beta, double* marginal, double* log_z,
+                         uint64_t* map_index);
+
 #ifdef __cplusplus
 }
 #endif

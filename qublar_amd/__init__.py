@@ -102,6 +102,11 @@ class BatchReport(C.Structure):
                 ("sweeps_per_launch", C.c_int)]
 
 
+class RoiResult(C.Structure):
+    _fields_ = [("e_min", C.c_double), ("log_z", C.c_double), ("p_map", C.c_double), ("map_index", C.c_uint64),
+                ("states", C.c_double), ("kernel_ms", C.c_double)]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int),
                 ("xcds", C.c_int), ("hbm_bytes", C.c_int64), ("lds_bytes_per_cu", C.c_int)]
@@ -167,6 +172,7 @@ def lib():
             "qb_colour_qubo": (i32, [i32, vp, vp, vp, vp, vp, C.POINTER(i32)]),
             "qb_dense_create": (i32, [i32, vp, vp, C.POINTER(vp)]),
             "qb_dense_destroy": (i32, [vp]),
+            "qb_roi_enumerate": (i32, [i32, vp, vp, dbl, vp, vp, C.POINTER(RoiResult)]),
             "qb_dense_anneal": (i32, [vp, C.POINTER(Schedule), i32, u64, i32, i32, vp, vp, vp, C.POINTER(DenseReport)]),
         }
         for name, (res, args) in sig.items():
@@ -670,3 +676,19 @@ class DenseQubo:
                                      int(recompute_every), _p(x), _p(e), _p(err), C.byref(rep)))
         return {"x": x, "energy": e, "field_err": err,
                 "report": {f: getattr(rep, f) for f, _ in DenseReport._fields_}}
+
+
+def roi_enumerate(h, J, beta=1.0):
+    """The exact posterior of a region's QUBO (roi_qubo's h and J, J's upper triangle
+    used) over all 2^n states on the device: dict(marginal, map_state, e_min, log_z,
+    p_map, map_index, kernel_ms)."""
+    h = np.ascontiguousarray(h, np.float64)
+    n = len(h)
+    J = np.ascontiguousarray(J, np.float64).reshape(n, n)
+    m = np.zeros(n, np.float64)
+    ms = np.zeros(n, np.uint8)
+    r = RoiResult()
+    _check(lib().qb_roi_enumerate(n, _p(h), _p(J), float(beta), _p(m), _p(ms), C.byref(r)))
+    out = {f: getattr(r, f) for f, _ in RoiResult._fields_}
+    out.update(marginal=m, map_state=ms)
+    return out

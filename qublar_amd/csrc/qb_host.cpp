@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1680,5 +1681,102 @@ extern "C" int qb_dense_anneal(qb_dense* D, const qb_schedule* s, int R, uint64_
         rep->kernel_ms = ms;
         rep->spin_updates = double(R) * double(beta.size()) * double(n);
     }
+    return QB_OK;
+}
+
+// ---- exact region-of-interest posterior ------------------------------------------------
+
+extern "C" int qb_roi_enumerate(int n, const double* h, const double* J, double beta, double* marginal,
+                                uint8_t* map_state, qb_roi_result* out) {
+    if (n < 1 || n > 36) return fail(QB_E_ARG, "qb_roi_enumerate: n = %d, must be 1..36", n);
+    if (!h || !J || !out) return fail(QB_E_ARG, "qb_roi_enumerate: null argument");
+    if (!(beta >= 0.0) || !std::isfinite(beta)) return fail(QB_E_ARG, "beta must be finite and >= 0");
+    std::vector<double> Js(size_t(n) * size_t(n), 0.0);
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(h[i])) return fail(QB_E_ARG, "h[%d] is not finite", i);
+        for (int j = i + 1; j < n; ++j) {
+            const double v = J[size_t(i) * size_t(n) + size_t(j)];
+            if (!std::isfinite(v)) return fail(QB_E_ARG, "J[%d][%d] is not finite", i, j);
+            Js[size_t(i) * size_t(n) + size_t(j)] = Js[size_t(j) * size_t(n) + size_t(i)] = v;
+        }
+    }
+    // 2^k states per thread (k <= 12 keeps the carried energy near a scratch sum), the
+    // chunks split over launches of at most 2^22 threads
+    const int k = std::max(0, std::min(n - 14, 12));
+    const uint64_t chunks = uint64_t(1) << (n - k);
+    const uint64_t per_launch = uint64_t(1) << 22;
+    const int nmax = roi_nmax(n);
+    const size_t max_blocks = size_t((std::min(chunks, per_launch) + 255) / 256);
+    DevBuf d_h, d_J, d_e, d_s, d_m;
+    int rc;
+    if ((rc = d_h.reserve(sizeof(double) * size_t(n))) || (rc = d_J.reserve(sizeof(double) * Js.size())) ||
+        (rc = d_e.reserve(sizeof(double) * max_blocks)) || (rc = d_s.reserve(sizeof(uint64_t) * max_blocks)) ||
+        (rc = d_m.reserve(sizeof(double) * max_blocks * size_t(nmax + 1))))
+        return rc;
+    QB_HIP(hipMemcpy(d_h.p, h, sizeof(double) * size_t(n), hipMemcpyHostToDevice));
+    QB_HIP(hipMemcpy(d_J.p, Js.data(), sizeof(double) * Js.size(), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    QB_HIP(hipEventCreate(&e0));
+    QB_HIP(hipEventCreate(&e1));
+    struct Events {
+        hipEvent_t a, b;
+        ~Events() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ev{e0, e1};
+    double kernel_ms = 0.0;
+    // pass 1: the minimum energy and its state (ties to the smallest index)
+    double e_min = std::numeric_limits<double>::infinity();
+    uint64_t s_min = 0;
+    std::vector<double> be(max_blocks);
+    std::vector<uint64_t> bs(max_blocks);
+    for (uint64_t c0 = 0; c0 < chunks; c0 += per_launch) {
+        const uint64_t cn = std::min(per_launch, chunks - c0);
+        const size_t blocks = size_t((cn + 255) / 256);
+        QB_HIP(hipEventRecord(e0, nullptr));
+        QB_HIP(launch_roi_min(n, k, d_h.as<double>(), d_J.as<double>(), c0, cn, d_e.as<double>(), d_s.as<uint64_t>(),
+                              nullptr));
+        QB_HIP(hipEventRecord(e1, nullptr));
+        QB_HIP(hipMemcpy(be.data(), d_e.p, sizeof(double) * blocks, hipMemcpyDeviceToHost));
+        QB_HIP(hipMemcpy(bs.data(), d_s.p, sizeof(uint64_t) * blocks, hipMemcpyDeviceToHost));
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        kernel_ms += ms;
+        for (size_t b = 0; b < blocks; ++b)
+            if (be[b] < e_min || (be[b] == e_min && bs[b] < s_min)) {
+                e_min = be[b];
+                s_min = bs[b];
+            }
+    }
+    // pass 2: Z and the weighted bit sums relative to the minimum, reduced in block order
+    double z = 0.0;
+    std::vector<double> msum(size_t(n), 0.0), bm(max_blocks * size_t(nmax + 1));
+    for (uint64_t c0 = 0; c0 < chunks; c0 += per_launch) {
+        const uint64_t cn = std::min(per_launch, chunks - c0);
+        const size_t blocks = size_t((cn + 255) / 256);
+        QB_HIP(hipEventRecord(e0, nullptr));
+        QB_HIP(launch_roi_sum(n, k, d_h.as<double>(), d_J.as<double>(), beta, e_min, c0, cn, d_m.as<double>(), nullptr));
+        QB_HIP(hipEventRecord(e1, nullptr));
+        QB_HIP(hipMemcpy(bm.data(), d_m.p, sizeof(double) * blocks * size_t(nmax + 1), hipMemcpyDeviceToHost));
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        kernel_ms += ms;
+        for (size_t b = 0; b < blocks; ++b) {
+            const double* row = bm.data() + b * size_t(nmax + 1);
+            for (int i = 0; i < n; ++i) msum[size_t(i)] += row[i];
+            z += row[nmax];
+        }
+    }
+    if (marginal)
+        for (int i = 0; i < n; ++i) marginal[i] = msum[size_t(i)] / z;
+    if (map_state)
+        for (int i = 0; i < n; ++i) map_state[i] = uint8_t((s_min >> i) & 1u);
+    out->e_min = e_min;
+    out->log_z = -beta * e_min + std::log(z);
+    out->p_map = 1.0 / z;
+    out->map_index = s_min;
+    out->states = std::ldexp(1.0, n);
+    out->kernel_ms = kernel_ms;
     return QB_OK;
 }

@@ -252,4 +252,14 @@ hipError_t launch_dense(const DevDense& d, int precision, int R, const double* b
                         int recompute_every, uint64_t seed_offset, uint8_t* x_out, double* energy, double* field_err,
                         hipStream_t st);
 
+
+// ---- exact region-of-interest posterior (qb_roi.hip) -----------------------------------
+// Couplings symmetric with a zero diagonal (Js, n x n), chunks of 2^k states per thread.
+int roi_nmax(int n);
+hipError_t launch_roi_min(int n, int k, const double* h, const double* Js, uint64_t chunk0, uint64_t chunks,
+                          double* blk_e, uint64_t* blk_s, hipStream_t st);
+// blk: per block NMAX weighted bit sums and Z (roi_nmax(n) + 1 doubles).
+hipError_t launch_roi_sum(int n, int k, const double* h, const double* Js, double beta, double e_ref, uint64_t chunk0,
+                          uint64_t chunks, double* blk, hipStream_t st);
+
 }  // namespace qb
