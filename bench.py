@@ -242,6 +242,62 @@ def run_gravity_precision(args):
     print(json.dumps(result), flush=True)
 
 
+def run_maxcut(args):
+    """The reference's GPU integer-annealer benchmark shape (tools/gpu_anneal.cu --bench,
+    experiments/p2_gpu): a G1-like graph -- Erdos-Renyi n = 800, p = 0.06, all weights +1
+    (m ~ 19,176) -- 16,384 replicas with seeds 1.., 4000 + 200 sweeps under scaled_budget
+    (T_hot = sigma = sqrt(2 sum w^2 / n), T_cold = 0.1 w_min; src/ising_gpu.cuh:104-124),
+    one run_batch per step; spin updates/s = R x sweeps x n / wall (state init to results
+    on the host), against the reference's published 1.114e10 on an RTX 5060 Ti."""
+    import qublar_amd as qa
+    import oracle
+
+    g = qa.er_graph(800, 0.06, 1)
+    g = qa.GraphArrays(g.n, g.eu, g.ev, np.abs(g.ew))
+    sigma = float(np.sqrt(2.0 * float(np.sum(g.ew.astype(np.float64) ** 2)) / g.n))
+    A, H, R = 4000, 200, args.maxcut_replicas
+    sched = qa.schedule(sigma, 0.1, A, H)
+    G = qa.Graph(g)
+    plan = G.plan()
+    log(f"G1-like graph n={g.n} m={g.m}; {plan['n_classes']} colour classes; sigma {sigma:.3f}")
+    G.run_batch(qa.schedule(sigma, 0.1, 2, 0), 1, R)   # warm
+    walls, kms = [], []
+    for st in range(args.steps):
+        t = time.perf_counter()
+        s, e, rep = G.run_batch(sched, 1, R)
+        walls.append(time.perf_counter() - t)
+        kms.append(rep["kernel_ms"])
+        log(f"step {st}: {walls[-1]:.3f} s (kernels {rep['kernel_ms']:.1f} ms, {rep['launches']} launches); best cut {-e.min()}")
+    updates = R * (A + H) * g.n
+    value = updates / float(np.mean(walls))
+    result = {"workload": "maxcut-g1", "metric": "spin updates/s, G1-like max-cut, 16,384 replicas x 4200 sweeps",
+              "value": value, "unit": "spin updates/s", "steps": args.steps, "ms_per_step": float(np.mean(walls)) * 1e3,
+              "kernel_ms": float(np.mean(kms)), "vs_reference_gpu": value / 1.114e10,
+              "reference": "1.114e10 spin updates/s, RTX 5060 Ti (experiments/p2_gpu/summary.csv:2)",
+              "config": {"n": g.n, "m": g.m, "replicas": R, "sweeps": A + H, "t_hot": sigma, "t_cold": 0.1,
+                         "colour_classes": plan["n_classes"]},
+              "best_cut": int(-e.min()), "mean_cut": float(-e.mean()), "dtype": "int32 fields, f64 accept",
+              "data": "synthetic Erdos-Renyi graph (the G-set files are not available)"}
+    if not args.no_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+
+        threads = max(1, min(16, (os.cpu_count() or 2) - 2))
+        og = oracle.IntGraph(g.n, g.eu, g.ev, g.ew)
+        osched = oracle.schedule(sigma, 0.1, A, H)
+        t = time.perf_counter()
+        oracle.anneal_int(og, osched, 1)
+        one = time.perf_counter() - t
+        runs = max(threads, int(args.cpu_seconds / one) * threads // threads)
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda k: oracle.anneal_int(og, osched, k + 1), range(runs)))
+        wall = time.perf_counter() - t
+        result["cpu_baseline"] = {"value": runs * (A + H) * g.n / wall, "unit": "spin updates/s", "cores": threads,
+                                  "kind": "port", "sample": f"{runs} replicas of the same schedule on {threads} "
+                                  f"threads, the oracle's index-order splitmix trajectory, {wall:.1f} s"}
+    print(json.dumps(result), flush=True)
+
+
 def run_roi(args):
     """The exact posterior of a region's QUBO over all 2^n states (the enumeration of
     tools/branches_blaze.py:45-53 over roi_qubo's h and J): n = --roi-n bits with h ~ N(0, 2),
@@ -447,11 +503,12 @@ def main():
     ap.add_argument("--dense-n", type=int, default=4096, help="dense precision: bits")
     ap.add_argument("--dense-sweeps", type=int, default=5000, help="dense precision: sweeps")
     ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched", "dense-precision",
-                                           "gravity-precision", "roi"], default="muon",
+                                           "gravity-precision", "roi", "maxcut"], default="muon",
                     help="muon: the headline line (default); serving: the batched serving stream; "
                          "batched: many sparse QUBOs in one persistent launch")
     ap.add_argument("--instances", type=int, default=256, help="batched: instances")
     ap.add_argument("--roi-n", type=int, default=32, help="roi: bits enumerated (2^n states)")
+    ap.add_argument("--maxcut-replicas", type=int, default=16384, help="maxcut: replicas per batch")
     ap.add_argument("--batch-sweeps", type=int, default=2000, help="batched: sweeps")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window")
@@ -485,6 +542,9 @@ def main():
         return
     if args.workload == "roi":
         run_roi(args)
+        return
+    if args.workload == "maxcut":
+        run_maxcut(args)
         return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5
