@@ -17,7 +17,7 @@
 //
 // dE of flipping bit i = (1 - 2 x_i)(h_i + sum_j J_ij x_j), summed in FP64 in CSR order,
 // and a flip is accepted iff dE <= 0 or u < qexp(-dE * beta) -- the ray x voxel engine's
-// test. Replicas 2m and 2m + 1 share one Philox4x32-10 call keyed by the instance seed with
+// test, decided through a float exponential with a guard band (uphill_accept). Replicas 2m and 2m + 1 share one Philox4x32-10 call keyed by the instance seed with
 // counter (bit, sweep, m, tag), one 53-bit uniform each (draw2_u01).
 #include "qb_internal.h"
 #include "qb_rng.h"
@@ -25,6 +25,22 @@
 namespace qb {
 
 constexpr int kRowChunk = 16;
+
+// The Metropolis decision u < qexp(-dE * beta) for dE > 0, decided by the hardware float
+// exponential where that is unambiguous. For -80 < x = -dE beta <= 0, e_f = 2^(float(x)
+// log2e) (v_exp_f32) is within 1.5e-5 of e^x (float(x): 4.8e-6 absolute; the product with
+// log2e: 2.1e-5 in the exponent; v_exp_f32: a few ulp), so u < e_f (1 - 2^-14) implies
+// u < qexp(x) and u > e_f (1 + 2^-14) implies u > qexp(x); a draw inside the band takes
+// qexp itself. For x <= -80, qexp(x) < 2e-35 lies below every nonzero 53-bit draw, so
+// only u = 0 can pass. Every decision is therefore qexp's, bit for bit.
+__device__ __forceinline__ bool uphill_accept(double x, double u) {
+    if (!(x > -80.0)) return u == 0.0 && qexp(x) > 0.0;
+    constexpr double kBand = 0x1.0p-14;
+    const double e = double(__builtin_amdgcn_exp2f(float(x) * 1.44269504f));
+    if (u < e * (1.0 - kBand)) return true;
+    if (u > e * (1.0 + kBand)) return false;
+    return u < qexp(x);
+}
 
 // Spin words of a job in LDS: one byte per bit up to 8 replicas per job, 16 or 32 bits
 // beyond; kQuboLdsBytes of them hold instances up to kQuboLdsBytes / sizeof(word) bits.
@@ -79,14 +95,15 @@ __device__ __forceinline__ void qubo_job(const DevQuboBatch& q, const QuboJob& j
                     uint32_t mj[kRowChunk];
 #pragma unroll
                     for (int u = 0; u < kRowChunk; ++u) mj[u] = ea + u < deg ? uint32_t(mk[cj[u] - bit0]) : 0u;
+                    // acc += w where the neighbour's spin is set, as fma(w, x_j, acc): w * 1 and
+                    // w * 0 are exact, so each addition rounds once as acc + w does, and an
+                    // unset neighbour leaves acc (up to the sign of an exact zero, which no
+                    // decision sees) -- three instructions per (entry, replica), no select
 #pragma unroll
                     for (int u = 0; u < kRowChunk; ++u) {
                         const double w = double(wj[u]);
 #pragma unroll
-                        for (int r = 0; r < RB; ++r) {
-                            const double nxt = acc[r] + w;
-                            acc[r] = ((mj[u] >> r) & 1u) ? nxt : acc[r];
-                        }
+                        for (int r = 0; r < RB; ++r) acc[r] = fma(w, double((mj[u] >> r) & 1u), acc[r]);
                     }
                 }
                 const uint32_t bit = uint32_t(q.bit_id[pos]);
@@ -99,7 +116,7 @@ __device__ __forceinline__ void qubo_job(const DevQuboBatch& q, const QuboJob& j
 #pragma unroll
                         for (int v = 0; v < 2; ++v) {
                             const double dE = ((mi >> (r + v)) & 1u) ? -acc[r + v] : acc[r + v];
-                            if (r + v < rb && (dE <= 0.0 || u[v] < qexp(-dE * b))) flips |= 1u << (r + v);
+                            if (r + v < rb && (dE <= 0.0 || uphill_accept(-dE * b, u[v]))) flips |= 1u << (r + v);
                         }
                     }
                 }
