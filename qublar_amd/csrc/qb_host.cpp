@@ -1216,7 +1216,8 @@ struct qb_qubo_batch {
     int total_replicas = 0;
     DevBuf b_inst_pos, b_inst_cls, b_cls, b_row, b_col, b_J, b_h, b_bit, b_seed, b_ctile, b_tptr, b_tcol, b_tJ;
     struct Jobs {
-        DevBuf lpt, by_inst;          // the same jobs, longest first and in instance order
+        DevBuf by_xcd, by_inst;       // the same jobs, grouped by XCD queue and in instance order
+        QuboQueues segs{};            // the XCD queues' ranges in by_xcd
         std::vector<QuboJob> host;    // instance order
         std::vector<int32_t> first;   // per instance: its first job in instance order
         int64_t mask_words = 0;
@@ -1416,8 +1417,10 @@ extern "C" int qb_qubo_batch_destroy(qb_qubo_batch* b) {
 }
 
 // Jobs at replica block rb: instance k's replicas in blocks of rb, each block a job with
-// its own spin words. Uploaded in instance order and longest first (cost = rows +
-// entries, times replicas).
+// its own spin words. Uploaded in instance order, and grouped by XCD queue: instances in
+// decreasing cost (rows + entries, times replicas) each go to the least loaded of the
+// eight queues, and a queue lists its instances in that order with each instance's jobs
+// side by side.
 static int qubo_jobs_for(qb_qubo_batch* B, int rb, const qb_qubo_batch::Jobs** out) {
     auto it = B->jobs.find(rb);
     if (it != B->jobs.end()) {
@@ -1442,21 +1445,37 @@ static int qubo_jobs_for(qb_qubo_batch* B, int rb, const qb_qubo_batch::Jobs** o
     Jb->first.push_back(int32_t(Jb->host.size()));
     if (words > INT32_MAX) return fail(QB_E_LIMIT, "the batch's spin words exceed int32 indexing");
     Jb->mask_words = words;
-    std::vector<QuboJob> lpt = Jb->host;
-    auto cost = [&](const QuboJob& j) {
-        return (int64_t(B->n_bits[size_t(j.instance)]) + B->entries[size_t(j.instance)]) * j.replicas;
+    auto cost = [&](int k) {
+        return (int64_t(B->n_bits[size_t(k)]) + B->entries[size_t(k)]) * int64_t(B->replicas[size_t(k)]);
     };
-    std::stable_sort(lpt.begin(), lpt.end(), [&](const QuboJob& a, const QuboJob& b) { return cost(a) > cost(b); });
+    std::vector<int> order(static_cast<size_t>(B->n_inst));
+    for (int k = 0; k < B->n_inst; ++k) order[size_t(k)] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<std::vector<int>> queue_inst(kXcds);
+    std::vector<int64_t> load(kXcds, 0);
+    for (int k : order) {
+        const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+        queue_inst[size_t(x)].push_back(k);
+        load[size_t(x)] += cost(k);
+    }
+    std::vector<QuboJob> by_xcd;
+    by_xcd.reserve(Jb->host.size());
+    for (int x = 0; x < kXcds; ++x) {
+        Jb->segs.begin[x] = int32_t(by_xcd.size());
+        for (int k : queue_inst[size_t(x)])
+            for (int j = Jb->first[size_t(k)]; j < Jb->first[size_t(k) + 1]; ++j) by_xcd.push_back(Jb->host[size_t(j)]);
+    }
+    Jb->segs.begin[kXcds] = int32_t(by_xcd.size());
     int rc;
-    if ((rc = Jb->lpt.upload(lpt, B->stream)) || (rc = Jb->by_inst.upload(Jb->host, B->stream))) return rc;
+    if ((rc = Jb->by_xcd.upload(by_xcd, B->stream)) || (rc = Jb->by_inst.upload(Jb->host, B->stream))) return rc;
     QB_HIP(hipStreamSynchronize(B->stream));
     *out = Jb.get();
     B->jobs[rb] = std::move(Jb);
     return QB_OK;
 }
 
-static DevQuboBatch qubo_dev(const qb_qubo_batch* B, const qb_qubo_batch::Jobs& J, bool lpt) {
-    return DevQuboBatch{int(J.host.size()), lpt ? J.lpt.as<QuboJob>() : J.by_inst.as<QuboJob>(),
+static DevQuboBatch qubo_dev(const qb_qubo_batch* B, const qb_qubo_batch::Jobs& J, bool by_xcd) {
+    return DevQuboBatch{int(J.host.size()), by_xcd ? J.by_xcd.as<QuboJob>() : J.by_inst.as<QuboJob>(),
                         B->b_inst_pos.as<int32_t>(), B->b_inst_cls.as<int32_t>(), B->b_cls.as<int32_t>(),
                         B->b_row.as<int32_t>(), B->b_col.as<int32_t>(), B->b_J.as<float>(), B->b_h.as<float>(),
                         B->b_bit.as<int32_t>(), B->b_seed.as<uint64_t>(), B->b_ctile.as<int32_t>(),
@@ -1468,25 +1487,31 @@ static DevQuboBatch qubo_dev(const qb_qubo_batch* B, const qb_qubo_batch::Jobs& 
 static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int block, int n_sweeps, int mode,
                     int workgroups, int* launches, float* ms) {
     hipStream_t st = B->stream;
+    // one set of XCD queue counters per launch (kXcds lines each)
+    const size_t set = size_t(kXcds) * size_t(kQueueStride);
+    const size_t queue_bytes = sizeof(unsigned) * set * size_t(mode == 0 ? 1 : B->n_inst);
     int rc;
     if ((rc = B->w_mask.reserve(sizeof(uint32_t) * size_t(Jb.mask_words))) ||
-        (rc = B->w_queue.reserve(sizeof(unsigned) * size_t(B->n_inst + 1))))
+        (rc = B->w_queue.reserve(queue_bytes)))
         return rc;
     QB_HIP(hipMemsetAsync(B->w_mask.p, 0, sizeof(uint32_t) * size_t(Jb.mask_words), st));
-    QB_HIP(hipMemsetAsync(B->w_queue.p, 0, sizeof(unsigned) * size_t(B->n_inst + 1), st));
+    QB_HIP(hipMemsetAsync(B->w_queue.p, 0, queue_bytes, st));
     QB_HIP(hipEventRecord(B->ev0, st));
     *launches = 0;
     if (mode == 0) {
         const DevQuboBatch q = qubo_dev(B, Jb, true);
         QB_HIP(launch_qubo_batch(q, rb, block, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
-                                 B->w_queue.as<unsigned>(), 0, q.n_jobs, std::min(workgroups, q.n_jobs), st));
+                                 B->w_queue.as<unsigned>(), Jb.segs, std::min(workgroups, q.n_jobs), st));
         *launches = 1;
     } else {
         const DevQuboBatch q = qubo_dev(B, Jb, false);
         for (int k = 0; k < B->n_inst; ++k) {
             const int f = Jb.first[size_t(k)], cnt = Jb.first[size_t(k) + 1] - f;
+            QuboQueues one{};   // the instance's jobs in queue 0, the others empty
+            one.begin[0] = f;
+            for (int x = 1; x <= kXcds; ++x) one.begin[x] = f + cnt;
             QB_HIP(launch_qubo_batch(q, rb, block, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
-                                     B->w_queue.as<unsigned>() + k, f, cnt, cnt, st));
+                                     B->w_queue.as<unsigned>() + set * size_t(k), one, cnt, st));
             ++*launches;
         }
     }

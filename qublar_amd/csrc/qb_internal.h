@@ -211,7 +211,7 @@ struct QuboJob {
 };
 struct DevQuboBatch {
     int n_jobs;
-    const QuboJob* jobs;         // longest first
+    const QuboJob* jobs;         // grouped by XCD queue (or in instance order)
     const int32_t* inst_pos;     // n_instances + 1
     const int32_t* inst_cls;     // n_instances + 1 offsets into cls_ptr
     const int32_t* cls_ptr;      // per instance: its n_classes + 1 class bounds (global positions)
@@ -232,9 +232,16 @@ struct DevQuboBatch {
 constexpr int kQuboBlock = 256;      // threads of the energy kernel
 constexpr int kQuboLdsBytes = 65536;  // LDS spin words of one annealing job
 // block: 256 or 1024 threads per workgroup (one bit per thread in a class pass).
+// The persistent kernel's job queues, one per XCD: jobs [begin[x], begin[x + 1]) of
+// q.jobs are XCD x's, counted by queue[kQueueStride * x]. A workgroup takes from its own
+// XCD's queue, then from the others in turn.
+constexpr int kXcds = 8;
+constexpr int kQueueStride = 16;   // one 64-byte line per counter
+struct QuboQueues {
+    int32_t begin[kXcds + 1];
+};
 hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, int block, uint32_t* masks, const double* beta,
-                             int n_sweeps, unsigned* queue, int job_first, int job_count, int workgroups,
-                             hipStream_t st);
+                             int n_sweeps, unsigned* queue, const QuboQueues& segs, int workgroups, hipStream_t st);
 int qubo_blocks_per_cu(int rb, int block);
 hipError_t launch_qubo_energy(const DevQuboBatch& q, const uint32_t* masks, double* energy, hipStream_t st);
 
@@ -252,6 +259,11 @@ hipError_t launch_dense(const DevDense& d, int precision, int R, const double* b
                         int recompute_every, uint64_t seed_offset, uint8_t* x_out, double* energy, double* field_err,
                         hipStream_t st);
 
+
+#if defined(__HIPCC__)
+// The XCD (accelerator complex die) the calling wave runs on: HW_REG_XCC_ID bits 0-3.
+__device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
+#endif
 
 // ---- exact region-of-interest posterior (qb_roi.hip) -----------------------------------
 // Couplings symmetric with a zero diagonal (Js, n x n), chunks of 2^k states per thread.

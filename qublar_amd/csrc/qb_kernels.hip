@@ -84,7 +84,6 @@ __device__ __forceinline__ float residual_add<float>(float r, float a, double de
 // that state with loads that skip L1 and hit L2 (scope bit sc1). No cache is written
 // back or invalidated between steps, and teams never touch each other's state.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
 
 template <typename V>
 __device__ __forceinline__ V ld_l2(const V* p) {

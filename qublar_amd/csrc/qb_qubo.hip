@@ -3,8 +3,11 @@
 // A batch holds many independent QUBOs, E(x) = sum_i h_i x_i + sum_{i<j} J_ij x_i x_j
 // over x in {0,1}^n, each with its own replicas. The work unit is a job: one instance
 // and a block of RB of its replicas. Jobs share nothing, so the persistent kernel takes
-// them from a queue (longest first) with no inter-workgroup synchronisation, and a
-// workgroup anneals its job alone: every sweep visits the instance's colour classes in
+// them from queues with no inter-workgroup synchronisation, and a workgroup anneals its
+// job alone. Each instance belongs to one XCD's queue (instances spread over the eight
+// by decreasing cost), its jobs side by side, so the jobs of an instance run together on
+// one XCD and read its rows through that XCD's L2; a workgroup whose queue is empty takes
+// from the others. Inside a job: every sweep visits the instance's colour classes in
 // order, a class is a set of independent flips, and the workgroup syncs between classes.
 //
 // Inside a class each lane takes one bit for all the job's replicas: it walks the bit's
@@ -127,22 +130,36 @@ __device__ __forceinline__ void qubo_job(const DevQuboBatch& q, const QuboJob& j
     }
 }
 
+// The next job for this workgroup: from its own XCD's queue, else from the others in turn
+// (a queue already read as exhausted costs no atomic); -1 when every queue is empty.
+__device__ __forceinline__ int take_job(unsigned* queue, const QuboQueues& segs, int home) {
+    for (int t = 0; t < kXcds; ++t) {
+        const int x = (home + t) % kXcds;
+        const int len = segs.begin[x + 1] - segs.begin[x];
+        unsigned* ctr = queue + kQueueStride * x;
+        if (len <= 0 || __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= unsigned(len)) continue;
+        const unsigned k = atomicAdd(ctr, 1u);
+        if (k < unsigned(len)) return segs.begin[x] + int(k);
+    }
+    return -1;
+}
+
 template <int RB, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) qubo_batch_kernel(DevQuboBatch q, uint32_t* masks, const double* beta,
-                                                            int n_sweeps, unsigned* queue, int job_first,
-                                                            int job_count) {
+                                                            int n_sweeps, unsigned* queue, QuboQueues segs) {
     static_assert(RB % 2 == 0, "replicas are drawn in pairs");
     using W = typename LdsWord<RB>::type;
     constexpr int kLdsBits = kQuboLdsBytes / int(sizeof(W));
     __shared__ W lmask[kLdsBits];
     __shared__ int s_job;
+    const int home = int(xcc_id() % kXcds);
     for (;;) {
-        if (threadIdx.x == 0) s_job = int(atomicAdd(queue, 1u));
+        if (threadIdx.x == 0) s_job = take_job(queue, segs, home);
         __syncthreads();
         const int jn = s_job;
         __syncthreads();   // s_job is written again by the next take
-        if (jn >= job_count) return;
-        const QuboJob jb = q.jobs[job_first + jn];
+        if (jn < 0) return;
+        const QuboJob jb = q.jobs[jn];
         const int n = q.inst_pos[jb.instance + 1] - q.inst_pos[jb.instance];
         uint32_t* gm = masks + jb.mask_offset;
         if (n <= kLdsBits) {
@@ -191,24 +208,24 @@ __global__ void __launch_bounds__(kQuboBlock) qubo_energy_kernel(DevQuboBatch q,
 
 template <int BLOCK>
 static hipError_t launch_qubo_block(const DevQuboBatch& q, int rb, uint32_t* masks, const double* beta, int n_sweeps,
-                                    unsigned* queue, int job_first, int job_count, int workgroups, hipStream_t st) {
+                                    unsigned* queue, const QuboQueues& segs, int workgroups, hipStream_t st) {
     switch (rb) {
-        case 4: qubo_batch_kernel<4, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
-        case 8: qubo_batch_kernel<8, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
-        case 16: qubo_batch_kernel<16, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
-        case 32: qubo_batch_kernel<32, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, job_first, job_count); break;
+        case 4: qubo_batch_kernel<4, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, segs); break;
+        case 8: qubo_batch_kernel<8, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, segs); break;
+        case 16: qubo_batch_kernel<16, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, segs); break;
+        case 32: qubo_batch_kernel<32, BLOCK><<<workgroups, BLOCK, 0, st>>>(q, masks, beta, n_sweeps, queue, segs); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, int block, uint32_t* masks, const double* beta,
-                             int n_sweeps, unsigned* queue, int job_first, int job_count, int workgroups,
-                             hipStream_t st) {
-    if (block == 256)
-        return launch_qubo_block<256>(q, rb, masks, beta, n_sweeps, queue, job_first, job_count, workgroups, st);
-    if (block == 1024)
-        return launch_qubo_block<1024>(q, rb, masks, beta, n_sweeps, queue, job_first, job_count, workgroups, st);
+                             int n_sweeps, unsigned* queue, const QuboQueues& segs, int workgroups, hipStream_t st) {
+    for (int x = 0; x < kXcds; ++x)
+        if (segs.begin[x] < 0 || segs.begin[x + 1] < segs.begin[x] || segs.begin[x + 1] > q.n_jobs)
+            return hipErrorInvalidValue;
+    if (block == 256) return launch_qubo_block<256>(q, rb, masks, beta, n_sweeps, queue, segs, workgroups, st);
+    if (block == 1024) return launch_qubo_block<1024>(q, rb, masks, beta, n_sweeps, queue, segs, workgroups, st);
     return hipErrorInvalidValue;
 }
 
