@@ -4,9 +4,10 @@
 //
 // Plan/make_plan, scaled_budget and temperatures keep the reference's rules.
 // DeviceGraph owns the graph on the device; run_batch anneals replicas with seeds
-// seed_base .. seed_base + R - 1 (one workgroup per replica, state in LDS when it fits,
-// colour-class sweeps, Philox draws keyed by (seed, sweep, vertex)); device_energies is
-// the independent edge-list energy. Engine errors throw wising::EngineError (the
+// seed_base .. seed_base + R - 1 in the reference's own trajectory (one wave per replica,
+// its spins and fields in LDS: vertices in index order, one splitmix64 stream drawn on
+// uphill proposals), so replica r is wising::anneal(g, b, seed_base + r), as the
+// reference's run_batch is; device_energies is the independent edge-list energy. Engine errors throw wising::EngineError (the
 // reference printed them and called exit(3), src/ising_gpu.cuh:44-52).
 #pragma once
 
@@ -142,7 +143,8 @@ inline BatchOut run_batch(const DeviceGraph& dg, const Budget& b, uint64_t seed_
     std::vector<int64_t> e(static_cast<size_t>(R));
     const qb_schedule qs{b.t_hot, b.t_cold, b.anneal_sweeps, b.hold_sweeps};
     qb_batch_report rep{};
-    const int rc = qb_run_batch(dg.handle(), &qs, seed_base, R, updates_per_launch, out.s.data(), e.data(), &rep);
+    const int rc =
+        qb_run_batch_reference(dg.handle(), &qs, seed_base, R, updates_per_launch, out.s.data(), e.data(), &rep);
     if (rc != QB_OK) throw wising::EngineError(rc, std::string("qublar: ") + qb_last_error());
     out.energy.assign(e.begin(), e.end());
     out.wall_s = rep.wall_s;

@@ -5,9 +5,9 @@
 //   f_i = sum_j w_ij s_j, dE_i = -s_i f_i
 //
 // anneal() runs on the GPU through libqublar_hip.so (one replica; run_batch in
-// argos/ising_gpu.cuh runs many); its trajectory visits vertices by colour class with
-// Philox draws keyed by (seed, sweep, vertex) instead of index order with one
-// splitmix64 stream (src/ising_weighted.hpp:250-267). The loader, exact energies,
+// argos/ising_gpu.cuh runs many) in the reference's trajectory: vertices in index order,
+// one splitmix64 stream drawn on uphill proposals (src/ising_weighted.hpp:250-267), so
+// its assignment and tracked energy are the reference's. The loader, exact energies,
 // State, TTS99 and SHA-256 are host code with the reference's behaviour.
 #pragma once
 
@@ -276,7 +276,7 @@ inline RunOut anneal(const WeightedGraph& g, const Budget& b, uint64_t seed) {
     std::vector<int8_t> s(static_cast<size_t>(plan.n_pad));
     int64_t e = 0;
     const qb_schedule qs{b.t_hot, b.t_cold, b.anneal_sweeps, b.hold_sweeps};
-    const int rc = qb_run_batch(h, &qs, seed, 1, 1e18, s.data(), &e, nullptr);
+    const int rc = qb_run_batch_reference(h, &qs, seed, 1, 1e18, s.data(), &e, nullptr);
     const std::string msg = rc == QB_OK ? "" : qb_last_error();
     qb_graph_destroy(h);
     if (rc != QB_OK) throw EngineError(rc, "qublar: " + msg);

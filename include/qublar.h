@@ -200,6 +200,17 @@ typedef struct {
 int qb_run_batch(qb_graph* g, const qb_schedule* b, uint64_t seed_base, int R, double updates_per_launch,
                  int8_t* s_out, int64_t* energy_out, qb_batch_report* rep);
 
+/* run_batch in the reference's own trajectory (src/ising_gpu.cuh:396-458 with the semantics it
+   shares with wising::anneal, src/ising_weighted.hpp:250-267): replica r anneals from s = -1
+   with seed seed_base + r, bits in index order, one splitmix64 stream per replica drawn only
+   for uphill proposals, a flip iff dE <= 0 or u < exp(-dE / T). Each replica's assignment and
+   tracked energy are those of wising::anneal(g, b, seed_base + r), except where a draw falls
+   within an ulp or two of exp(-dE / T) (the device's exponential against the host's libm).
+   Same outputs and launch split as qb_run_batch; n_pad * 5 bytes of a replica's state must fit
+   160 KB of LDS (QB_E_LIMIT otherwise). */
+int qb_run_batch_reference(qb_graph* g, const qb_schedule* b, uint64_t seed_base, int R, double updates_per_launch,
+                           int8_t* s_out, int64_t* energy_out, qb_batch_report* rep);
+
 /* device_energies (ising_gpu.cuh:461-475): -cut of K states (K x n bytes, 0/1). */
 int qb_device_energies(qb_graph* g, const uint8_t* X, int K, int64_t* energy_out);
 

@@ -155,6 +155,7 @@ def lib():
             "qb_colour_graph": (i32, [C.POINTER(GraphDesc), vp, vp, C.POINTER(i32)]),
             "qb_run_batch": (i32, [vp, C.POINTER(Schedule), u64, i32, dbl, vp, vp, C.POINTER(BatchReport)]),
             "qb_device_energies": (i32, [vp, vp, i32, vp]),
+            "qb_run_batch_reference": (i32, [vp, C.POINTER(Schedule), u64, i32, dbl, vp, vp, C.POINTER(BatchReport)]),
             "qb_muon_scene_defaults": (i32, [C.POINTER(MuonScene)]),
             "qb_scene_muon": (i32, [C.POINTER(MuonScene), C.POINTER(vp)]),
             "qb_host_problem_view": (i32, [vp, C.POINTER(ProblemDesc)]),
@@ -411,6 +412,17 @@ class Graph:
         rep = BatchReport()
         _check(lib().qb_run_batch(self.h, C.byref(sched), C.c_uint64(seed_base), R, float(updates_per_launch),
                                   _p(s), _p(e), C.byref(rep)))
+        return s, e, {f: getattr(rep, f) for f, _ in BatchReport._fields_}
+
+    def run_batch_reference(self, sched, seed_base, R, updates_per_launch=1e9):
+        """run_batch in the reference's own trajectory: replica r is wising::anneal with seed
+        seed_base + r (index order, splitmix64 on uphill proposals)."""
+        n_pad = self.plan()["n_pad"]
+        s = np.zeros((R, n_pad), np.int8)
+        e = np.zeros(R, np.int64)
+        rep = BatchReport()
+        _check(lib().qb_run_batch_reference(self.h, C.byref(sched), C.c_uint64(seed_base), R, float(updates_per_launch),
+                                            _p(s), _p(e), C.byref(rep)))
         return s, e, {f: getattr(rep, f) for f, _ in BatchReport._fields_}
 
     def device_energies(self, X):

@@ -964,8 +964,8 @@ struct qb_graph {
     qb_plan plan{};
     bool lds_state = false;
     std::vector<int32_t> pos_bit, cls_ptr;
-    DevBuf b_cls, b_pos, b_nbp, b_nb, b_nw, b_f0, b_eu, b_ev, b_ew;
-    DevBuf w_s, w_f, w_e, w_sout, w_beta, w_X, w_out;
+    DevBuf b_cls, b_pos, b_nbp, b_nb, b_nw, b_f0, b_eu, b_ev, b_ew, b_bitpos;
+    DevBuf w_s, w_f, w_e, w_sout, w_beta, w_X, w_out, w_rng;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~qb_graph() {
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1105,7 +1105,7 @@ extern "C" int qb_graph_create(const qb_graph_desc* d, int force_global, qb_grap
     if ((rc = G->b_cls.upload(G->cls_ptr, st)) || (rc = G->b_pos.upload(G->pos_bit, st)) ||
         (rc = G->b_nbp.upload(nb_ptr, st)) || (rc = G->b_nb.upload(nb, st)) || (rc = G->b_nw.upload(nw, st)) ||
         (rc = G->b_f0.upload(f0, st)) || (rc = G->b_eu.upload(eu, st)) || (rc = G->b_ev.upload(ev, st)) ||
-        (rc = G->b_ew.upload(ewv, st)))
+        (rc = G->b_ew.upload(ewv, st)) || (rc = G->b_bitpos.upload(pos_of, st)))
         return rc;
     QB_HIP(hipStreamSynchronize(st));
     DevGraph& D = G->dev;
@@ -1119,6 +1119,7 @@ extern "C" int qb_graph_create(const qb_graph_desc* d, int force_global, qb_grap
     D.nb = G->b_nb.as<int32_t>();
     D.nw = G->b_nw.as<int32_t>();
     D.f0 = G->b_f0.as<int32_t>();
+    D.bit_pos = G->b_bitpos.as<int32_t>();
     D.eu = G->b_eu.as<int32_t>();
     D.ev = G->b_ev.as<int32_t>();
     D.ew = G->b_ew.as<int32_t>();
@@ -1170,6 +1171,60 @@ extern "C" int qb_run_batch(qb_graph* G, const qb_schedule* b, uint64_t seed_bas
         QB_HIP(launch_int_sweeps(G->dev, R, seed_base, G->w_beta.as<double>(), sb, std::min(total, sb + chunk),
                                  sb == 0 ? 1 : 0, G->w_s.as<int8_t>(), G->w_f.as<int32_t>(), G->w_e.as<long long>(),
                                  G->lds_state, st));
+        ++launches;
+    }
+    QB_HIP(hipEventRecord(G->ev1, st));
+    QB_HIP(launch_int_unpermute(G->dev, R, G->w_s.as<int8_t>(), G->w_sout.as<int8_t>(), st));
+    if (s_out) QB_HIP(hipMemcpyAsync(s_out, G->w_sout.p, cells, hipMemcpyDeviceToHost, st));
+    if (energy_out) QB_HIP(hipMemcpyAsync(energy_out, G->w_e.p, sizeof(int64_t) * size_t(R), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    if (rep) {
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, G->ev0, G->ev1));
+        rep->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rep->kernel_ms = ms;
+        rep->launches = launches;
+        rep->sweeps_per_launch = chunk;
+    }
+    return QB_OK;
+}
+
+extern "C" int qb_run_batch_reference(qb_graph* G, const qb_schedule* b, uint64_t seed_base, int R,
+                                      double updates_per_launch, int8_t* s_out, int64_t* energy_out,
+                                      qb_batch_report* rep) {
+    if (!G) return fail(QB_E_ARG, "qb_run_batch_reference: null graph");
+    int rc = check_schedule(b);
+    if (rc) return rc;
+    if (R <= 0) return fail(QB_E_ARG, "R must be positive (got %d)", R);
+    if (G->plan.refused[0]) return fail(QB_E_REFUSED, "plan refused: %s", G->plan.refused);
+    if (int_ref_lds_bytes(G->dev) > kRefLdsLimit)
+        return fail(QB_E_LIMIT, "n = %d: one replica's spins and fields (5 bytes a bit) must fit %zu bytes of LDS",
+                    G->plan.n, kRefLdsLimit);
+    const int total = b->anneal_sweeps + b->hold_sweeps;
+    const int np = G->plan.n_pad, n = G->plan.n;
+    const size_t cells = size_t(R) * size_t(np);
+    if ((rc = G->w_s.reserve(cells)) || (rc = G->w_f.reserve(cells * 4)) ||
+        (rc = G->w_e.reserve(sizeof(long long) * size_t(R))) || (rc = G->w_rng.reserve(sizeof(uint64_t) * size_t(R))) ||
+        (rc = G->w_sout.reserve(cells)) || (rc = G->w_beta.reserve(sizeof(double) * size_t(total))))
+        return rc;
+    // the temperature of every sweep by the reference's expression (src/ising_weighted.hpp:255-258)
+    std::vector<double> temp(static_cast<size_t>(total));
+    for (int sweep = 0; sweep < total; ++sweep) {
+        const double frac = std::min(1.0, double(sweep) / double(std::max(1, b->anneal_sweeps - 1)));
+        temp[size_t(sweep)] = sweep < b->anneal_sweeps ? b->t_hot * std::pow(b->t_cold / b->t_hot, frac) : b->t_cold;
+    }
+    hipStream_t st = G->stream;
+    int chunk = int(std::floor(updates_per_launch / (double(R) * double(n))));
+    chunk = std::max(1, std::min(total, chunk));
+    QB_HIP(hipStreamSynchronize(st));
+    const auto t0 = std::chrono::steady_clock::now();
+    QB_HIP(hipMemcpyAsync(G->w_beta.p, temp.data(), sizeof(double) * temp.size(), hipMemcpyHostToDevice, st));
+    QB_HIP(hipEventRecord(G->ev0, st));
+    int launches = 0;
+    for (int sb = 0; sb < total; sb += chunk) {
+        QB_HIP(launch_int_ref(G->dev, R, seed_base, G->w_beta.as<double>(), sb, std::min(total, sb + chunk),
+                              sb == 0 ? 1 : 0, G->w_s.as<int8_t>(), G->w_f.as<int32_t>(), G->w_rng.as<uint64_t>(),
+                              G->w_e.as<long long>(), st));
         ++launches;
     }
     QB_HIP(hipEventRecord(G->ev1, st));

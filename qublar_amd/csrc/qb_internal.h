@@ -143,6 +143,7 @@ struct DevGraph {
     const int32_t* nb;
     const int32_t* nw;
     const int32_t* f0;      // fields of the all-zero assignment
+    const int32_t* bit_pos; // original bit -> position
     const int32_t* eu;      // edge list, original indices
     const int32_t* ev;
     const int32_t* ew;
@@ -192,6 +193,14 @@ hipError_t launch_pack_states(const uint8_t* X, int K, int n, const int32_t* pos
 hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const double* beta,
                              int sweep_begin, int sweep_end, int init, int8_t* s_state, int32_t* f_state,
                              long long* energy, bool lds_state, hipStream_t st);
+// The reference's own integer trajectory (wising::anneal), one wave per replica with its
+// state in LDS (n_pad * 5 bytes; four replicas per workgroup while that fits): temp[sweep]
+// is the temperature, rng[r] the replica's splitmix64 state between launches.
+constexpr size_t kRefLdsLimit = 160 * 1024;
+size_t int_ref_lds_bytes(const DevGraph& g);
+hipError_t launch_int_ref(const DevGraph& g, int R, uint64_t seed_base, const double* temp, int sweep_begin,
+                          int sweep_end, int init, int8_t* s_state, int32_t* f_state, uint64_t* rng,
+                          long long* energy, hipStream_t st);
 hipError_t launch_int_unpermute(const DevGraph& g, int R, const int8_t* s_state, int8_t* s_out,
                                 hipStream_t st);
 hipError_t launch_int_energies(const DevGraph& g, const uint8_t* X, int K, long long* out, hipStream_t st);

@@ -1343,6 +1343,141 @@ hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const
     return hipGetLastError();
 }
 
+// ---- the reference's own trajectory (wising::anneal, src/ising_weighted.hpp:250-267) ----
+// Replica r anneals from s = -1 with seed_base + r: bits in index order, one splitmix64
+// stream (state seed * 0x2545F4914F6CDD1D + 1) drawn only for uphill proposals, a flip
+// taken iff dE <= 0 or u < exp(-dE / T) with T per sweep (the host's expression). One wave
+// per replica, its spins and integer fields in LDS, positions of the colour layout:
+// the wave evaluates the next 64 bits at once against the current state. The k-th uphill
+// proposal of the window draws from state + k * gamma (splitmix64 is a counter: the n-th
+// draw mixes state_0 + n * gamma), so every lane decides its own bit as the sequential
+// loop would if no earlier bit of the window flipped. The first accepted bit is the one
+// the loop flips: the draws up to it are consumed, it flips (its row's fields updated
+// by the wave's lanes), and the next window starts after it. With none accepted the
+// whole window is consumed.
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// u < exp(-dE / T) for u = u53 2^-53 and dE > 0. A float exponential decides outside a
+// band of 2^-14 around it (its error is below 1.5e-5 for -80 < x <= 0); inside the band,
+// and for the rare u = 0, the FP64 qexp decides. qexp and the host's libm exp differ by
+// at most an ulp or two, so a decision can differ from the host's only for a u within
+// that distance of the exponential.
+__device__ __forceinline__ bool ref_accept(uint64_t u53, long long dE, double T) {
+    const double x = -double(dE) / T;
+    const double u = double(u53) * 0x1.0p-53;
+    if (!(x > -80.0)) return u < qexp(x);   // only u = 0 can pass below e^-80
+    constexpr double kBand = 0x1.0p-14;
+    const double e = double(__builtin_amdgcn_exp2f(float(x) * 1.44269504f));
+    if (u < e * (1.0 - kBand)) return true;
+    if (u > e * (1.0 + kBand)) return false;
+    return u < qexp(x);
+}
+
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, uint64_t seed_base, const double* temp,
+                                                              int sweep_begin, int sweep_end, int init,
+                                                              int8_t* s_state, int32_t* f_state, uint64_t* rng,
+                                                              long long* energy) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
+    const int r = int(blockIdx.x) * WAVES + w;
+    if (r >= R) return;   // the whole wave leaves; waves never sync with each other
+    const int n = g.n, np = g.n_pad;
+    int32_t* f = reinterpret_cast<int32_t*>(smem + size_t(w) * size_t(np) * 5);
+    int8_t* s = reinterpret_cast<int8_t*>(f + np);
+    int8_t* gs = s_state + size_t(r) * size_t(np);
+    int32_t* gf = f_state + size_t(r) * size_t(np);
+    uint64_t st;
+    long long e;
+    if (init) {
+        for (int p = lane; p < np; p += 64) {
+            s[p] = -1;
+            f[p] = p < n ? g.f0[p] : 0;
+        }
+        st = (seed_base + uint64_t(r)) * 0x2545F4914F6CDD1Dull + 1;
+        e = 0;
+    } else {
+        for (int p = lane; p < np; p += 64) {
+            s[p] = gs[p];
+            f[p] = gf[p];
+        }
+        st = rng[r];
+        e = energy[r];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t below = (uint64_t(1) << lane) - 1u;
+    for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
+        const double T = temp[sweep];
+        int i0 = 0;
+        while (i0 < n) {
+            const int i = i0 + lane;
+            const bool valid = i < n;
+            const int p = valid ? g.bit_pos[i] : 0;
+            const long long dE = valid ? -(long long)s[p] * (long long)f[p] : 0;
+            const bool up = valid && dE > 0;
+            const uint64_t m_up = __ballot(up);
+            bool acc = valid && dE <= 0;
+            if (up) acc = ref_accept(mix64(st + uint64_t(__popcll(m_up & below) + 1) * kGamma) >> 11, dE, T);
+            const uint64_t m_acc = __ballot(acc);
+            if (m_acc == 0) {
+                st += uint64_t(__popcll(m_up)) * kGamma;
+                i0 += 64;
+                continue;
+            }
+            const int js = __builtin_ctzll(m_acc);
+            const uint64_t upto = js == 63 ? ~uint64_t(0) : (uint64_t(2) << js) - 1u;
+            st += uint64_t(__popcll(m_up & upto)) * kGamma;
+            const int pf = __shfl(p, js);
+            e += __shfl(dE, js);
+            const int si = s[pf];
+            for (int q = g.nb_ptr[pf] + lane; q < g.nb_ptr[pf + 1]; q += 64) f[g.nb[q]] -= 2 * g.nw[q] * si;
+            if (lane == 0) s[pf] = int8_t(-si);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            i0 += js + 1;
+        }
+    }
+    for (int p = lane; p < np; p += 64) {
+        gs[p] = s[p];
+        gf[p] = f[p];
+    }
+    if (lane == 0) {
+        rng[r] = st;
+        energy[r] = e;
+    }
+}
+
+// Replicas per workgroup: four while their states fit the LDS limit, else one.
+static int int_ref_waves(const DevGraph& g) { return size_t(4) * size_t(g.n_pad) * 5 <= kRefLdsLimit ? 4 : 1; }
+
+size_t int_ref_lds_bytes(const DevGraph& g) { return size_t(int_ref_waves(g)) * size_t(g.n_pad) * 5; }
+
+hipError_t launch_int_ref(const DevGraph& g, int R, uint64_t seed_base, const double* temp, int sweep_begin,
+                          int sweep_end, int init, int8_t* s_state, int32_t* f_state, uint64_t* rng,
+                          long long* energy, hipStream_t st) {
+    const size_t lds = int_ref_lds_bytes(g);
+    if (lds > kRefLdsLimit) return hipErrorInvalidValue;
+    const int waves = int_ref_waves(g);
+    const void* k = waves == 4 ? reinterpret_cast<const void*>(&int_ref_kernel<4>)
+                               : reinterpret_cast<const void*>(&int_ref_kernel<1>);
+    const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    if (waves == 4)
+        int_ref_kernel<4><<<unsigned((R + 3) / 4), 256, lds, st>>>(g, R, seed_base, temp, sweep_begin, sweep_end, init,
+                                                                   s_state, f_state, rng, energy);
+    else
+        int_ref_kernel<1><<<unsigned(R), 64, lds, st>>>(g, R, seed_base, temp, sweep_begin, sweep_end, init, s_state,
+                                                        f_state, rng, energy);
+    return hipGetLastError();
+}
+
 hipError_t launch_int_unpermute(const DevGraph& g, int R, const int8_t* s_state, int8_t* s_out, hipStream_t st) {
     const size_t total = size_t(R) * size_t(g.n_pad);
     int_unpermute_kernel<<<unsigned((total + 255) / 256), 256, 0, st>>>(g, R, s_state, s_out);

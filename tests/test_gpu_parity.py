@@ -267,3 +267,70 @@ def test_errors_are_reported_not_fatal(gpu):
     assert e.value.code == qa.QB_E_ARG
     with pytest.raises(qa.QublarError):
         prob.anneal_branches(qa.schedule(5.0, 1.0, 10, 0), 0)
+
+
+# ---- the reference's own integer trajectory ---------------------------------------------
+
+def check_int_reference(g, sched, R, seed_base=1, updates_per_launch=1e9):
+    """qb_run_batch_reference against the oracle's reference mode (wising::anneal: index order,
+    splitmix64 drawn on uphill proposals), replica by replica."""
+    G = qa.Graph(g)
+    say(f"run_batch_reference n={g.n} m={g.m} R={R} upl={updates_per_launch:g}")
+    s, e, rep = G.run_batch_reference(sched, seed_base, R, updates_per_launch)
+    og = oracle_graph(g)
+    for r in range(R):
+        x, en = orc_mod.anneal_int(og, osched(sched), seed_base + r)
+        xg = (s[r, :g.n] > 0).astype(np.uint8)
+        assert np.array_equal(xg, x), f"replica {r}: {int((xg != x).sum())} spins differ"
+        assert e[r] == en, f"replica {r}: energy {e[r]} vs {en}"
+    X = (s[:, :g.n] > 0).astype(np.uint8)
+    assert np.array_equal(G.device_energies(X), e)
+    return s, e, rep
+
+
+def test_int_reference_trajectory_maxcut_800(gpu):
+    """The max-cut workload in the reference's trajectory: each replica equals
+    wising::anneal with its seed (src/ising_weighted.hpp:250-267)."""
+    g = qa.er_graph(800, 0.06, 1)
+    s, e, rep = check_int_reference(g, qa.schedule(3.0, 0.05, 1000, 0), 16)
+    say(f"kernel_ms {rep['kernel_ms']:.2f} best cut {-e.min()}")
+
+
+def test_int_reference_trajectory_split_launches_and_weights(gpu):
+    """Signed weights, dense K16 and a launch split every few sweeps (the splitmix64 state
+    carries across launches); results do not depend on the split."""
+    rng = np.random.default_rng(3)
+    iu = np.triu_indices(16, 1)
+    k16 = qa.GraphArrays(16, iu[0], iu[1], rng.integers(-1000, 1001, len(iu[0])))
+    check_int_reference(k16, qa.schedule(1000.0, 0.1, 300, 20), 8, 5, updates_per_launch=16 * 8 * 7)
+    g = qa.er_graph(300, 0.1, 4)
+    sched = qa.schedule(3.0, 0.05, 120, 10)
+    s0, e0, _ = check_int_reference(g, sched, 12, 9)
+    s1, e1, rep = check_int_reference(g, sched, 12, 9, updates_per_launch=12 * 300 * 11)
+    assert np.array_equal(s0, s1) and np.array_equal(e0, e1) and rep["launches"] > 1
+
+
+def test_int_reference_trajectory_one_replica_per_workgroup(gpu):
+    """A graph whose replica state takes more than a quarter of the LDS limit runs one
+    replica per workgroup."""
+    g = qa.er_graph(9000, 0.0005, 6)
+    check_int_reference(g, qa.schedule(2.0, 0.05, 6, 2), 3, 11)
+
+
+def test_int_reference_trajectory_reproduces_published_k16_cuts(gpu):
+    """gset_bench --check B on the device: the best of 8 replicas (seeds 1..8) of the three
+    random K16 of gset_bench.cu:252-264 at Budget(5, 1e-3, 4000, 200) cut 10622, 12434 and
+    5050 -- the numbers the reference recorded (experiments/p1_bench/check_log.txt:211)."""
+    import ctypes as C
+
+    for seed, want in zip((1, 2, 3), (10622, 12434, 5050)):
+        s = C.c_uint64(seed)
+        eu, ev, ew = [], [], []
+        for i in range(16):
+            for j in range(i + 1, 16):
+                eu.append(i); ev.append(j)
+                ew.append(int(orc_mod.lib().orc_splitmix(C.byref(s)) % 2001) - 1000)
+        G = qa.Graph(qa.GraphArrays(16, eu, ev, ew))
+        _, e, _ = G.run_batch_reference(qa.schedule(5.0, 1e-3, 4000, 200), 1, 8)
+        say(f"K16 seed {seed}: best cut {-e.min()} (recorded {want})")
+        assert -e.min() == want
