@@ -146,6 +146,18 @@ int qb_anneal_branches(qb_problem* p, const qb_schedule* s, int n_branches, uint
                        const qb_anneal_opts* opts, uint8_t* branches, float* p_void, uint8_t* tri,
                        float* certainty, double* energy, qb_report* rep);
 
+/* branch_fractions in the reference's own trajectory (src/ising_recon.hpp:119-166): branch b
+   anneals from x = 0, r = -d with seed b + 1 + seed_offset, bits in index order, dE by
+   local_field_dE's arithmetic and order (:105-118), one splitmix64 stream drawn only for
+   uphill proposals, a flip iff dE <= 0 or u < exp(-dE / T). Each branch is anneal_branch's,
+   except where a draw falls within an ulp or two of exp(-dE / T) (the device's exponential
+   against the host's libm). One wave per branch: for many-branch or small problems; the
+   colour-class engine above is the fast path. draw_bit and draw_key do not apply. Outputs as
+   qb_anneal_branches (energies FP64 from scratch); rep.fp32_residual_err is 0. */
+int qb_anneal_branches_reference(qb_problem* p, const qb_schedule* s, int n_branches, uint64_t seed_offset,
+                                 uint8_t* branches, float* p_void, uint8_t* tri, float* certainty, double* energy,
+                                 qb_report* rep);
+
 /* binary_energy (src/ising_recon.hpp:65-79) of K states (K x n_vars bytes, 0/1),
    computed on the device in FP64; data and prior parts separately (each nullable). */
 int qb_energy(qb_problem* p, const uint8_t* X, int K, double* total, double* data, double* prior);

@@ -148,6 +148,8 @@ def lib():
             "qb_anneal_branches": (i32, [vp, C.POINTER(Schedule), i32, u64, C.POINTER(AnnealOpts), vp, vp, vp, vp,
                                          vp, C.POINTER(Report)]),
             "qb_energy": (i32, [vp, vp, i32, vp, vp, vp]),
+            "qb_anneal_branches_reference": (i32, [vp, C.POINTER(Schedule), i32, u64, vp, vp, vp, vp, vp,
+                                                   C.POINTER(Report)]),
             "qb_graph_create": (i32, [C.POINTER(GraphDesc), i32, C.POINTER(vp)]),
             "qb_graph_destroy": (i32, [vp]),
             "qb_graph_get_plan": (i32, [vp, C.POINTER(Plan)]),
@@ -347,6 +349,20 @@ class Problem:
         rep = Report()
         _check(lib().qb_anneal_branches(self.h, C.byref(sched), n_branches, C.c_uint64(seed_offset), C.byref(opts),
                                         _p(br), _p(pv), _p(tri), _p(cert), _p(en), C.byref(rep)))
+        return dict(branches=br, p_void=pv, tri=tri, certainty=cert, energy=en,
+                    report={f: getattr(rep, f) for f, _ in Report._fields_})
+
+    def anneal_branches_reference(self, sched, n_branches, seed_offset=0, branches=True):
+        """branch_fractions in the reference's own trajectory (index order, splitmix64)."""
+        n = self.arrays.n_vars
+        br = np.zeros((n_branches, n), np.uint8) if branches else None
+        pv = np.zeros(n, np.float32)
+        tri = np.zeros(n, np.uint8)
+        cert = np.zeros(n, np.float32)
+        en = np.zeros(n_branches, np.float64)
+        rep = Report()
+        _check(lib().qb_anneal_branches_reference(self.h, C.byref(sched), n_branches, C.c_uint64(seed_offset), _p(br),
+                                                  _p(pv), _p(tri), _p(cert), _p(en), C.byref(rep)))
         return dict(branches=br, p_void=pv, tri=tri, certainty=cert, energy=en,
                     report={f: getattr(rep, f) for f, _ in Report._fields_})
 

@@ -299,7 +299,7 @@ struct qb_problem {
     int64_t device_bytes = 0;
     std::vector<int32_t> pos_bit, cls_ptr;
     DevBuf b_cls, b_pos, b_col, b_ent, b_nbp, b_nb, b_nbbit, b_lin, b_d, b_w, b_rtp, b_rt, b_seq, b_dbit, b_dkey,
-        b_lbeg, b_lnb;
+        b_lbeg, b_lnb, b_bitpos;
     std::vector<int32_t> light_begin;   // host copy: each class's first light position
     std::vector<int32_t> col_ptr, nb_ptr;   // host copies, position space
     std::vector<uint8_t> seq_update;
@@ -309,7 +309,7 @@ struct qb_problem {
     };
     std::map<int, std::unique_ptr<Batches>> batches;   // by group width and team size
     // workspace
-    DevBuf w_x, w_r, w_beta, w_bar, w_err, w_part, w_pv, w_tri, w_cert, w_br, w_data, w_prior, w_rerr, w_X;
+    DevBuf w_x, w_r, w_beta, w_bar, w_err, w_part, w_pv, w_tri, w_cert, w_br, w_data, w_prior, w_rerr, w_X, w_rref;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~qb_problem() {
         if (ev0) (void)hipEventDestroy(ev0);
@@ -572,7 +572,7 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
         (rc = P->b_col.upload(col_ptr, st)) || (rc = P->b_ent.upload(ent, st)) || (rc = P->b_nbp.upload(nb_ptr, st)) ||
         (rc = P->b_nb.upload(nb, st)) || (rc = P->b_nbbit.upload(nb_bit, st)) || (rc = P->b_lin.upload(lin, st)) ||
         (rc = P->b_d.upload(dv, st)) || (rc = P->b_w.upload(wv, st)) || (rc = P->b_rtp.upload(rt_ptr, st)) ||
-        (rc = P->b_rt.upload(rt, st)))
+        (rc = P->b_rt.upload(rt, st)) || (rc = P->b_bitpos.upload(pos_of, st)))
         return rc;
     QB_HIP(hipStreamSynchronize(st));
     P->col_ptr = col_ptr;
@@ -596,6 +596,7 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     D.light_begin = P->b_lbeg.as<int32_t>();
     D.light_nb = P->b_lnb.as<int32_t>();
     D.pos_bit = P->b_pos.as<int32_t>();
+    D.bit_pos = P->b_bitpos.as<int32_t>();
     D.col_ptr = P->b_col.as<int32_t>();
     D.ent = P->b_ent.as<RayEnt>();
     D.nb_ptr = P->b_nbp.as<int32_t>();
@@ -923,6 +924,76 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
         rep->launches = launches;
         rep->spin_updates = double(R) * double(total) * double(n);
         rep->fp32_residual_err = rerr;
+    }
+    return QB_OK;
+}
+
+extern "C" int qb_anneal_branches_reference(qb_problem* P, const qb_schedule* s, int n_branches, uint64_t seed_offset,
+                                           uint8_t* branches, float* p_void, uint8_t* tri, float* certainty,
+                                           double* energy, qb_report* rep) {
+    if (!P) return fail(QB_E_ARG, "qb_anneal_branches_reference: null problem");
+    int rc = check_schedule(s);
+    if (rc) return rc;
+    const int R = n_branches;
+    if (R <= 0) return fail(QB_E_ARG, "n_branches must be positive (got %d)", R);
+    const int total = s->anneal_sweeps + s->hold_sweeps;
+    const int n = P->n, nr = P->n_rays;
+    const int n_groups = (R + kGroup - 1) / kGroup;
+    const int n_part = ray_energy_blocks(P->dev, kGroup);
+    if ((rc = P->w_X.reserve(size_t(R) * size_t(std::max(n, 1)))) ||
+        (rc = P->w_rref.reserve(sizeof(double) * size_t(R) * size_t(std::max(nr, 1)))) ||
+        (rc = P->w_beta.reserve(sizeof(double) * size_t(total))) ||
+        (rc = P->w_x.reserve(sizeof(uint16_t) * size_t(n_groups) * size_t(std::max(n, 1)))) ||
+        (rc = P->w_part.reserve(sizeof(double) * size_t(n_part) * size_t(n_groups) * kGroup)) ||
+        (rc = P->w_pv.reserve(sizeof(float) * size_t(std::max(n, 1)))) || (rc = P->w_tri.reserve(size_t(std::max(n, 1)))) ||
+        (rc = P->w_cert.reserve(sizeof(float) * size_t(std::max(n, 1)))) ||
+        (rc = P->w_br.reserve(branches ? size_t(R) * size_t(std::max(n, 1)) : 1)) ||
+        (rc = P->w_data.reserve(sizeof(double) * size_t(R))) || (rc = P->w_prior.reserve(sizeof(double) * size_t(R))))
+        return rc;
+    // the temperature of every sweep by the reference's expression (src/ising_recon.hpp:128-130)
+    std::vector<double> temp(static_cast<size_t>(total));
+    for (int sweep = 0; sweep < total; ++sweep) {
+        const double frac = std::min(1.0, double(sweep) / double(std::max(1, s->anneal_sweeps - 1)));
+        temp[size_t(sweep)] = sweep < s->anneal_sweeps ? s->t_hot * std::pow(s->t_cold / s->t_hot, frac) : s->t_cold;
+    }
+    hipStream_t st = P->stream;
+    QB_HIP(hipStreamSynchronize(st));
+    const auto t0 = std::chrono::steady_clock::now();
+    QB_HIP(hipMemcpyAsync(P->w_beta.p, temp.data(), sizeof(double) * temp.size(), hipMemcpyHostToDevice, st));
+    QB_HIP(hipEventRecord(P->ev0, st));
+    QB_HIP(launch_ray_ref(P->dev, R, seed_offset, P->w_beta.as<double>(), total, P->w_X.as<uint8_t>(),
+                          P->w_rref.as<double>(), st));
+    QB_HIP(hipEventRecord(P->ev1, st));
+    // the branches in the group masks of the read-out and energy kernels
+    if (n > 0) QB_HIP(launch_pack_states(P->w_X.as<uint8_t>(), R, n, P->dev.pos_bit, P->w_x.as<uint16_t>(), st));
+    if (p_void || tri || certainty || branches)
+        QB_HIP(launch_ray_outputs(P->dev, P->w_x.as<uint16_t>(), n_groups, kGroup, R, P->w_pv.as<float>(),
+                                  P->w_tri.as<uint8_t>(), P->w_cert.as<float>(),
+                                  branches ? P->w_br.as<uint8_t>() : nullptr, st));
+    const bool want_energy = energy != nullptr;
+    if (want_energy)
+        QB_HIP(launch_ray_energy(P->dev, P->w_x.as<uint16_t>(), n_groups, kGroup, R, nullptr, 0, P->w_part.as<double>(),
+                                 n_part, P->w_data.as<double>(), P->w_prior.as<double>(), nullptr, st));
+    if (p_void) QB_HIP(hipMemcpyAsync(p_void, P->w_pv.p, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, st));
+    if (tri) QB_HIP(hipMemcpyAsync(tri, P->w_tri.p, size_t(n), hipMemcpyDeviceToHost, st));
+    if (certainty) QB_HIP(hipMemcpyAsync(certainty, P->w_cert.p, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, st));
+    if (branches) QB_HIP(hipMemcpyAsync(branches, P->w_br.p, size_t(R) * size_t(n), hipMemcpyDeviceToHost, st));
+    std::vector<double> dat(static_cast<size_t>(R)), pri(static_cast<size_t>(R));
+    if (want_energy) {
+        QB_HIP(hipMemcpyAsync(dat.data(), P->w_data.p, sizeof(double) * size_t(R), hipMemcpyDeviceToHost, st));
+        QB_HIP(hipMemcpyAsync(pri.data(), P->w_prior.p, sizeof(double) * size_t(R), hipMemcpyDeviceToHost, st));
+    }
+    QB_HIP(hipStreamSynchronize(st));
+    if (energy)
+        for (int b = 0; b < R; ++b) energy[b] = dat[size_t(b)] + pri[size_t(b)];
+    if (rep) {
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+        rep->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rep->kernel_ms = ms;
+        rep->launches = 1;
+        rep->spin_updates = double(R) * double(total) * double(n);
+        rep->fp32_residual_err = 0.0;
     }
     return QB_OK;
 }

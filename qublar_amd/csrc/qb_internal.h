@@ -43,6 +43,7 @@ struct DevProblem {
     int64_t nnz, n_nbr;
     const int32_t* cls_ptr;
     const int32_t* pos_bit;
+    const int32_t* bit_pos;     // original bit -> position
     const int32_t* col_ptr;
     const RayEnt* ent;
     const int32_t* nb_ptr;
@@ -193,6 +194,10 @@ hipError_t launch_pack_states(const uint8_t* X, int K, int n, const int32_t* pos
 hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const double* beta,
                              int sweep_begin, int sweep_end, int init, int8_t* s_state, int32_t* f_state,
                              long long* energy, bool lds_state, hipStream_t st);
+// The reference's own float trajectory (anneal_branch): branch b's spins xs[b][bit order]
+// (bytes) and residuals rs[b][ray] (FP64) in global memory, all sweeps in one launch.
+hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, const double* temp, int n_sweeps,
+                          uint8_t* xs, double* rs, hipStream_t st);
 // The reference's own integer trajectory (wising::anneal), one wave per replica with its
 // state in LDS (n_pad * 5 bytes; four replicas per workgroup while that fits): temp[sweep]
 // is the temperature, rng[r] the replica's splitmix64 state between launches.

@@ -1478,6 +1478,97 @@ hipError_t launch_int_ref(const DevGraph& g, int R, uint64_t seed_base, const do
     return hipGetLastError();
 }
 
+// ---- the reference's own float trajectory (anneal_branch, src/ising_recon.hpp:119-143) ----
+// Branch b anneals from x = 0, r = -d with seed b + 1 + seed_offset: bits in index order,
+// dE = local_field_dE (lin delta, then w (r a + a^2 / 2) per entry with a = a_k delta in CSR
+// order, then lambda (+-1) per neighbour in CSR order), one splitmix64 stream drawn only
+// for uphill proposals, a flip iff dE <= 0 or u < exp(-dE / T), and on a flip r += a_k delta
+// per entry. One wave per branch; x (bytes, bit order) and r (FP64) of a branch live in
+// global memory, read through the CU's L1 (a wave sees its own completed stores). As in the
+// integer trajectory, the wave evaluates the next 64 bits at once -- each lane sums its
+// bit's dE alone, in the reference's order -- and the first accepted bit is the loop's next
+// flip. Draw keys and counters of unions (draw_bit, draw_key) do not apply here.
+__device__ __forceinline__ bool ref_accept_f(uint64_t u53, double dE, double T) {
+    const double x = -dE / T;
+    const double u = double(u53) * 0x1.0p-53;
+    if (!(x > -80.0)) return u < qexp(x);
+    constexpr double kBand = 0x1.0p-14;
+    const double e = double(__builtin_amdgcn_exp2f(float(x) * 1.44269504f));
+    if (u < e * (1.0 - kBand)) return true;
+    if (u > e * (1.0 + kBand)) return false;
+    return u < qexp(x);
+}
+
+__global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint64_t seed_offset, const double* temp,
+                                                       int n_sweeps, uint8_t* xs, double* rs) {
+    const int w = int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
+    const int b = int(blockIdx.x) * 4 + w;
+    if (b >= R) return;   // the whole wave leaves; waves never sync with each other
+    const int n = p.n;
+    uint8_t* x = xs + size_t(b) * size_t(n);
+    double* r = rs + size_t(b) * size_t(p.n_rays);
+    for (int i = lane; i < n; i += 64) x[i] = 0;
+    for (int k = lane; k < p.n_rays; k += 64) r[k] = -p.d[k];
+    uint64_t st = (uint64_t(b) + 1u + seed_offset) * 0x2545F4914F6CDD1Dull + 1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t below = (uint64_t(1) << lane) - 1u;
+    for (int sweep = 0; sweep < n_sweeps; ++sweep) {
+        const double T = temp[sweep];
+        int i0 = 0;
+        while (i0 < n) {
+            const int i = i0 + lane;
+            const bool valid = i < n;
+            const int pos = valid ? p.bit_pos[i] : 0;
+            double dE = 0.0;
+            if (valid) {
+                const int xi = x[i];
+                const double delta = xi ? -1.0 : 1.0;
+                dE = p.lin[pos] * delta;
+                for (int k = p.col_ptr[pos]; k < p.col_ptr[pos + 1]; ++k) {
+                    const RayEnt e = p.ent[k];
+                    const double a = double(e.a) * delta;
+                    dE += e.w * (r[e.ray] * a + 0.5 * a * a);
+                }
+                for (int q = p.nb_ptr[pos]; q < p.nb_ptr[pos + 1]; ++q)
+                    dE += p.lambda * (int(x[p.nb_bit[q]]) != xi ? -1.0 : 1.0);
+            }
+            const bool up = valid && dE > 0.0;
+            const uint64_t m_up = __ballot(up);
+            bool acc = valid && dE <= 0.0;
+            if (up) acc = ref_accept_f(mix64(st + uint64_t(__popcll(m_up & below) + 1) * kGamma) >> 11, dE, T);
+            const uint64_t m_acc = __ballot(acc);
+            if (m_acc == 0) {
+                st += uint64_t(__popcll(m_up)) * kGamma;
+                i0 += 64;
+                continue;
+            }
+            const int js = __builtin_ctzll(m_acc);
+            const uint64_t upto = js == 63 ? ~uint64_t(0) : (uint64_t(2) << js) - 1u;
+            st += uint64_t(__popcll(m_up & upto)) * kGamma;
+            const int fi = i0 + js, fp = __shfl(pos, js);
+            const int xf = x[fi];
+            const double delta = xf ? -1.0 : 1.0;
+            const int k0 = p.col_ptr[fp], k1 = p.col_ptr[fp + 1];
+            if (!p.seq_update[fp]) {
+                for (int k = k0 + lane; k < k1; k += 64) r[p.ent[k].ray] += double(p.ent[k].a) * delta;
+            } else if (lane == 0) {   // a ray repeats in the column: its additions in CSR order
+                for (int k = k0; k < k1; ++k) r[p.ent[k].ray] += double(p.ent[k].a) * delta;
+            }
+            if (lane == 0) x[fi] = uint8_t(xf ^ 1);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            i0 = fi + 1;
+        }
+    }
+}
+
+hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, const double* temp, int n_sweeps,
+                          uint8_t* xs, double* rs, hipStream_t st) {
+    ray_ref_kernel<<<unsigned((R + 3) / 4), 256, 0, st>>>(p, R, seed_offset, temp, n_sweeps, xs, rs);
+    return hipGetLastError();
+}
+
 hipError_t launch_int_unpermute(const DevGraph& g, int R, const int8_t* s_state, int8_t* s_out, hipStream_t st) {
     const size_t total = size_t(R) * size_t(g.n_pad);
     int_unpermute_kernel<<<unsigned((total + 255) / 256), 256, 0, st>>>(g, R, s_state, s_out);

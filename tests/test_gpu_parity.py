@@ -334,3 +334,38 @@ def test_int_reference_trajectory_reproduces_published_k16_cuts(gpu):
         _, e, _ = G.run_batch_reference(qa.schedule(5.0, 1e-3, 4000, 200), 1, 8)
         say(f"K16 seed {seed}: best cut {-e.min()} (recorded {want})")
         assert -e.min() == want
+
+
+# ---- the reference's own float trajectory -------------------------------------------------
+
+def check_branches_reference(P, sched, R, seed_offset=0):
+    """qb_anneal_branches_reference against the oracle's reference mode (anneal_branch:
+    index order, splitmix64 on uphill proposals, local_field_dE's order), branch by branch,
+    with p_void, tri-state and FP64 energies."""
+    prob = qa.Problem(P)
+    say(f"anneal_branches_reference n={P.n_vars} rays={P.n_rays} R={R} offset={seed_offset}")
+    out = prob.anneal_branches_reference(sched, R, seed_offset)
+    op = oracle_problem(P)
+    pv, br = orc_mod.branch_fractions(op, osched(sched), R, 16, seed_offset)
+    diff = (out["branches"] != br).sum(axis=1)
+    assert not diff.any(), f"branches differ at {np.flatnonzero(diff)[:8]}: counts {diff[diff > 0][:8]}"
+    assert np.array_equal(out["p_void"], pv)
+    tri = np.where(pv >= np.float32(0.9), 1, np.where(pv <= np.float32(0.1), 0, 2))
+    assert np.array_equal(out["tri"], tri.astype(np.uint8))
+    e_ref = np.array([orc_mod.binary_energy(op, br[b]) for b in range(R)])
+    assert np.allclose(out["energy"], e_ref, rtol=ENERGY_RTOL, atol=1e-12 * max(1.0, np.abs(e_ref).max()))
+    return out
+
+
+def test_float_reference_trajectory_muon(gpu):
+    """A small muon QUBO, 16 branches: each branch is anneal_branch with seed b + 1."""
+    out = check_branches_reference(small_muon(), qa.schedule(50.0, 1.0, 30, 10), 16)
+    assert 0 < out["branches"].sum() < out["branches"].size
+    say(f"kernel_ms {out['report']['kernel_ms']:.2f}")
+
+
+def test_float_reference_trajectory_fields_duplicates_and_offset(gpu):
+    """Per-bit fields, repeated rays in a column (residual additions in CSR order), 13
+    branches with seed_offset 4, and the very heavy columns."""
+    check_branches_reference(synthetic_problem(dup=True), qa.schedule(8.0, 0.3, 40, 10), 13, seed_offset=4)
+    check_branches_reference(very_heavy_problem(), qa.schedule(6.0, 0.2, 12, 3), 5, seed_offset=1)
