@@ -1522,16 +1522,38 @@ __global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint6
             const int pos = valid ? p.bit_pos[i] : 0;
             double dE = 0.0;
             if (valid) {
+                // eight entries' loads, then their residuals, in flight together; the sum
+                // itself stays in CSR order
+                constexpr int kChunk = 8;
                 const int xi = x[i];
                 const double delta = xi ? -1.0 : 1.0;
                 dE = p.lin[pos] * delta;
-                for (int k = p.col_ptr[pos]; k < p.col_ptr[pos + 1]; ++k) {
-                    const RayEnt e = p.ent[k];
-                    const double a = double(e.a) * delta;
-                    dE += e.w * (r[e.ray] * a + 0.5 * a * a);
+                const int k1 = p.col_ptr[pos + 1];
+                for (int kb = p.col_ptr[pos]; kb < k1; kb += kChunk) {
+                    RayEnt e[kChunk];
+                    double rv[kChunk];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) e[u] = p.ent[min(kb + u, k1 - 1)];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) rv[u] = r[e[u].ray];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) {
+                        if (kb + u >= k1) break;
+                        const double a = double(e[u].a) * delta;
+                        dE += e[u].w * (rv[u] * a + 0.5 * a * a);
+                    }
                 }
-                for (int q = p.nb_ptr[pos]; q < p.nb_ptr[pos + 1]; ++q)
-                    dE += p.lambda * (int(x[p.nb_bit[q]]) != xi ? -1.0 : 1.0);
+                const int q1 = p.nb_ptr[pos + 1];
+                for (int qb = p.nb_ptr[pos]; qb < q1; qb += kChunk) {
+                    int xv[kChunk];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) xv[u] = x[p.nb_bit[min(qb + u, q1 - 1)]];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) {
+                        if (qb + u >= q1) break;
+                        dE += p.lambda * (xv[u] != xi ? -1.0 : 1.0);
+                    }
+                }
             }
             const bool up = valid && dE > 0.0;
             const uint64_t m_up = __ballot(up);
