@@ -158,6 +158,15 @@ int qb_anneal_branches_reference(qb_problem* p, const qb_schedule* s, int n_bran
                                  uint8_t* branches, float* p_void, uint8_t* tri, float* certainty, double* energy,
                                  qb_report* rep);
 
+/* The frozen set of freeze_provable_rock (src/ising_recon.hpp:187-205) on the device:
+   frozen[i] = 1 (original bit order) where bit i is provably rock at `margin` nats. The
+   bounds are the reference's, each bit's data bound summed in CSR order; the rounds test
+   every bit against the previous round's frozen set, which for lambda >= 0 reaches the same
+   fixpoint as the reference's in-order rounds (QB_E_ARG for lambda < 0). *rounds (nullable):
+   the rounds run, the last one freezing nothing. The reduced problem is built on the host
+   (argos::freeze_provable_rock). */
+int qb_freeze(qb_problem* p, double margin, uint8_t* frozen, int* rounds);
+
 /* binary_energy (src/ising_recon.hpp:65-79) of K states (K x n_vars bytes, 0/1),
    computed on the device in FP64; data and prior parts separately (each nullable). */
 int qb_energy(qb_problem* p, const uint8_t* X, int K, double* total, double* data, double* prior);

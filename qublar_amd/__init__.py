@@ -148,6 +148,7 @@ def lib():
             "qb_anneal_branches": (i32, [vp, C.POINTER(Schedule), i32, u64, C.POINTER(AnnealOpts), vp, vp, vp, vp,
                                          vp, C.POINTER(Report)]),
             "qb_energy": (i32, [vp, vp, i32, vp, vp, vp]),
+            "qb_freeze": (i32, [vp, dbl, vp, C.POINTER(i32)]),
             "qb_anneal_branches_reference": (i32, [vp, C.POINTER(Schedule), i32, u64, vp, vp, vp, vp, vp,
                                                    C.POINTER(Report)]),
             "qb_graph_create": (i32, [C.POINTER(GraphDesc), i32, C.POINTER(vp)]),
@@ -351,6 +352,13 @@ class Problem:
                                         _p(br), _p(pv), _p(tri), _p(cert), _p(en), C.byref(rep)))
         return dict(branches=br, p_void=pv, tri=tri, certainty=cert, energy=en,
                     report={f: getattr(rep, f) for f, _ in Report._fields_})
+
+    def freeze(self, margin):
+        """freeze_provable_rock's frozen set on the device: (frozen uint8[n], rounds)."""
+        fr = np.zeros(self.arrays.n_vars, np.uint8)
+        rounds = C.c_int()
+        _check(lib().qb_freeze(self.h, float(margin), _p(fr), C.byref(rounds)))
+        return fr, rounds.value
 
     def anneal_branches_reference(self, sched, n_branches, seed_offset=0, branches=True):
         """branch_fractions in the reference's own trajectory (index order, splitmix64)."""

@@ -998,6 +998,43 @@ extern "C" int qb_anneal_branches_reference(qb_problem* P, const qb_schedule* s,
     return QB_OK;
 }
 
+extern "C" int qb_freeze(qb_problem* P, double margin, uint8_t* frozen, int* rounds) {
+    if (!P || !frozen) return fail(QB_E_ARG, "qb_freeze: null argument");
+    if (!std::isfinite(margin)) return fail(QB_E_ARG, "margin must be finite");
+    if (P->dev.lambda < 0.0)
+        return fail(QB_E_ARG, "freeze needs lambda >= 0 (then its fixpoint does not depend on the visit order)");
+    const int n = P->n;
+    int rc;
+    // workspace: the two frozen masks, then the change counter on its own 64-byte line
+    const size_t mask = size_t(std::max(n, 1));
+    const size_t counter_at = (2 * mask + 63) / 64 * 64;
+    if ((rc = P->w_part.reserve(sizeof(double) * mask)) || (rc = P->w_X.reserve(counter_at + 64))) return rc;
+    hipStream_t st = P->stream;
+    double* lb = P->w_part.as<double>();
+    uint8_t* a = P->w_X.as<uint8_t>();
+    uint8_t* b = a + mask;
+    unsigned* changed = reinterpret_cast<unsigned*>(P->w_X.as<uint8_t>() + counter_at);
+    QB_HIP(hipMemsetAsync(a, 0, mask, st));
+    QB_HIP(launch_freeze_bound(P->dev, lb, st));
+    int r = 0;
+    for (;;) {
+        ++r;
+        QB_HIP(hipMemsetAsync(changed, 0, sizeof(unsigned), st));
+        QB_HIP(launch_freeze_round(P->dev, lb, margin, a, b, changed, st));
+        unsigned nc = 0;
+        QB_HIP(hipMemcpyAsync(&nc, changed, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        QB_HIP(hipStreamSynchronize(st));
+        std::swap(a, b);
+        if (nc == 0 || r > n + 1) break;   // at most n rounds can freeze a bit
+    }
+    std::vector<uint8_t> byp(static_cast<size_t>(n));
+    if (n > 0) QB_HIP(hipMemcpyAsync(byp.data(), a, size_t(n), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    for (int pos = 0; pos < n; ++pos) frozen[P->pos_bit[size_t(pos)]] = byp[size_t(pos)];
+    if (rounds) *rounds = r;
+    return QB_OK;
+}
+
 extern "C" int qb_energy(qb_problem* P, const uint8_t* X, int K, double* total, double* data, double* prior) {
     if (!P || (!X && K > 0)) return fail(QB_E_ARG, "qb_energy: null argument");
     if (K <= 0) return fail(QB_E_ARG, "qb_energy: K must be positive");

@@ -194,6 +194,11 @@ hipError_t launch_pack_states(const uint8_t* X, int K, int n, const int32_t* pos
 hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const double* beta,
                              int sweep_begin, int sweep_end, int init, int8_t* s_state, int32_t* f_state,
                              long long* energy, bool lds_state, hipStream_t st);
+// freeze_provable_rock on the device: per-position data bounds, then one fixpoint round
+// (frozen -> next, *changed counts the bits newly frozen).
+hipError_t launch_freeze_bound(const DevProblem& p, double* lb, hipStream_t st);
+hipError_t launch_freeze_round(const DevProblem& p, const double* lb, double margin, const uint8_t* frozen,
+                               uint8_t* next, unsigned* changed, hipStream_t st);
 // The reference's own float trajectory (anneal_branch): branch b's spins xs[b][bit order]
 // (bytes) and residuals rs[b][ray] (FP64) in global memory, all sweeps in one launch.
 hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, const double* temp, int n_sweeps,

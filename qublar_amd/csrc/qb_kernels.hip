@@ -1591,6 +1591,48 @@ hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, cons
     return hipGetLastError();
 }
 
+// ---- freeze_provable_rock's fixpoint (src/ising_recon.hpp:187-205) ------------------------
+// The data bound of a 0 -> 1 flip, sum_k w_b a_k (a_k / 2 - d_b) in CSR order, one thread per
+// position; then rounds in which every unfrozen bit freezes iff lb + lin + lambda (rock - free)
+// >= margin against the previous round's frozen set. With lambda >= 0 a bound only grows as
+// neighbours freeze, so these rounds reach the reference's in-order fixpoint.
+__global__ void freeze_bound_kernel(DevProblem p, double* lb) {
+    const int pos = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (pos >= p.n) return;
+    double s = 0.0;
+    for (int k = p.col_ptr[pos]; k < p.col_ptr[pos + 1]; ++k) {
+        const RayEnt e = p.ent[k];
+        s += e.w * double(e.a) * (0.5 * double(e.a) - p.d[e.ray]);
+    }
+    lb[pos] = s;
+}
+
+__global__ void freeze_round_kernel(DevProblem p, const double* lb, double margin, const uint8_t* frozen,
+                                    uint8_t* next, unsigned* changed) {
+    const int pos = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (pos >= p.n) return;
+    if (frozen[pos]) {
+        next[pos] = 1;
+        return;
+    }
+    int rock = 0, free_nb = 0;
+    for (int q = p.nb_ptr[pos]; q < p.nb_ptr[pos + 1]; ++q) (frozen[p.nb[q]] ? rock : free_nb)++;
+    const bool f = lb[pos] + p.lin[pos] + p.lambda * (rock - free_nb) >= margin;
+    next[pos] = f ? 1 : 0;
+    if (f) atomicAdd(changed, 1u);
+}
+
+hipError_t launch_freeze_bound(const DevProblem& p, double* lb, hipStream_t st) {
+    if (p.n > 0) freeze_bound_kernel<<<unsigned((p.n + 255) / 256), 256, 0, st>>>(p, lb);
+    return hipGetLastError();
+}
+
+hipError_t launch_freeze_round(const DevProblem& p, const double* lb, double margin, const uint8_t* frozen,
+                               uint8_t* next, unsigned* changed, hipStream_t st) {
+    if (p.n > 0) freeze_round_kernel<<<unsigned((p.n + 255) / 256), 256, 0, st>>>(p, lb, margin, frozen, next, changed);
+    return hipGetLastError();
+}
+
 hipError_t launch_int_unpermute(const DevGraph& g, int R, const int8_t* s_state, int8_t* s_out, hipStream_t st) {
     const size_t total = size_t(R) * size_t(g.n_pad);
     int_unpermute_kernel<<<unsigned((total + 255) / 256), 256, 0, st>>>(g, R, s_state, s_out);

@@ -369,3 +369,24 @@ def test_float_reference_trajectory_fields_duplicates_and_offset(gpu):
     branches with seed_offset 4, and the very heavy columns."""
     check_branches_reference(synthetic_problem(dup=True), qa.schedule(8.0, 0.3, 40, 10), 13, seed_offset=4)
     check_branches_reference(very_heavy_problem(), qa.schedule(6.0, 0.2, 12, 3), 5, seed_offset=1)
+
+
+# ---- freeze_provable_rock's fixpoint on the device ------------------------------------------
+
+def test_freeze_equals_reference_fixpoint(gpu):
+    """qb_freeze's frozen set equals the oracle's in-order freeze_provable_rock
+    (src/ising_recon.hpp:187-205) for lambda >= 0, on a muon scene at several margins and on
+    the full muon QUBO; lambda < 0 is refused."""
+    for P, margins in ((small_muon(), (0.0, 2.0, 10.0)), (qa.muon_scene(), (10.0,))):
+        prob = qa.Problem(P)
+        op = oracle_problem(P)
+        for margin in margins:
+            got, rounds = prob.freeze(margin)
+            want = np.zeros(P.n_vars, np.uint8)
+            orc_rounds = orc_mod.lib().orc_freeze_mask(op.c, margin, want)
+            say(f"freeze n={P.n_vars} margin {margin}: {int(got.sum())} frozen in {rounds} rounds "
+                f"(in order: {orc_rounds})")
+            assert np.array_equal(got, want)
+    neg = small_muon(**{"lambda": -0.5})
+    with pytest.raises(qa.QublarError):
+        qa.Problem(neg).freeze(1.0)
