@@ -309,6 +309,41 @@ def run_maxcut(args):
     print(json.dumps(result), flush=True)
 
 
+def run_reference_float(args):
+    """The float path in the reference's own trajectory (anneal_branch: index order,
+    splitmix64) on a small muon QUBO (32^3 grid, 90 x 20 sky bins: ~5k bits) with
+    --ref-branches branches, T 50 -> 1 over 200 sweeps + 50 at T = 1, against the host
+    oracle running the same trajectory (branch_fractions restated) on min(16, nproc - 2)
+    threads: like for like, bit for bit the same branches."""
+    import qublar_amd as qa
+    import oracle
+
+    P = qa.muon_scene(grid=32, sky_az=90, sky_th=20, log2_muons=22)
+    prob = qa.Problem(P)
+    R, A, H = args.ref_branches, 200, 50
+    sched = qa.schedule(50.0, 1.0, A, H)
+    prob.anneal_branches_reference(qa.schedule(50.0, 1.0, 2, 0), R, 0, branches=False)   # warm
+    t = time.perf_counter()
+    out = prob.anneal_branches_reference(sched, R, 0, branches=False)
+    wall = time.perf_counter() - t
+    updates = R * (A + H) * P.n_vars
+    log(f"reference trajectory: {R} branches x {P.n_vars} bits: {wall:.2f} s (kernel {out['report']['kernel_ms']:.0f} ms)")
+    result = {"workload": "reference-float", "n_vars": P.n_vars, "branches": R, "sweeps": A + H,
+              "solve_s": wall, "spin_updates_per_s": updates / wall,
+              "trajectory": "anneal_branch per branch (index order, splitmix64 on uphill proposals)"}
+    if not args.no_cpu:
+        threads = max(1, min(16, (os.cpu_count() or 2) - 2))
+        op = oracle.FloatProblem(P.row_ptr, P.ray, P.a, P.d, P.w, P.nbr_ptr, P.nbr, P.lam, P.kappa)
+        rc = 4 * threads
+        t = time.perf_counter()
+        oracle.branch_fractions(op, oracle.schedule(50.0, 1.0, A, H), rc, threads)
+        cw = time.perf_counter() - t
+        result["cpu_baseline"] = {"value": rc * (A + H) * P.n_vars / cw, "unit": "spin updates/s", "cores": threads,
+                                  "kind": "port", "sample": f"{rc} branches of the same trajectory, {cw:.1f} s"}
+        result["vs_cpu"] = result["spin_updates_per_s"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
 def run_roi(args):
     """The exact posterior of a region's QUBO over all 2^n states (the enumeration of
     tools/branches_blaze.py:45-53 over roi_qubo's h and J): n = --roi-n bits with h ~ N(0, 2),
@@ -514,12 +549,14 @@ def main():
     ap.add_argument("--dense-n", type=int, default=4096, help="dense precision: bits")
     ap.add_argument("--dense-sweeps", type=int, default=5000, help="dense precision: sweeps")
     ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched", "dense-precision",
-                                           "gravity-precision", "roi", "maxcut"], default="muon",
+                                           "gravity-precision", "roi", "maxcut", "reference-float"],
+                    default="muon",
                     help="muon: the headline line (default); serving: the batched serving stream; "
                          "batched: many sparse QUBOs in one persistent launch")
     ap.add_argument("--instances", type=int, default=256, help="batched: instances")
     ap.add_argument("--roi-n", type=int, default=32, help="roi: bits enumerated (2^n states)")
     ap.add_argument("--maxcut-replicas", type=int, default=16384, help="maxcut: replicas per batch")
+    ap.add_argument("--ref-branches", type=int, default=4096, help="reference-float: branches")
     ap.add_argument("--batch-sweeps", type=int, default=2000, help="batched: sweeps")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window")
@@ -556,6 +593,9 @@ def main():
         return
     if args.workload == "maxcut":
         run_maxcut(args)
+        return
+    if args.workload == "reference-float":
+        run_reference_float(args)
         return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5
