@@ -1353,7 +1353,8 @@ hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const
 // draw mixes state_0 + n * gamma), so every lane decides its own bit as the sequential
 // loop would if no earlier bit of the window flipped. The first accepted bit is the one
 // the loop flips: the draws up to it are consumed, it flips (its row's fields updated
-// by the wave's lanes), and the next window starts after it. With none accepted the
+// by the wave's lanes), and the later lanes keep their decisions up to the first lane
+// whose field the flip changed, where the next window starts. With none accepted the
 // whole window is consumed.
 constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
 
@@ -1363,20 +1364,22 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// u < exp(-dE / T) for u = u53 2^-53 and dE > 0. A float exponential decides outside a
-// band of 2^-14 around it (its error is below 1.5e-5 for -80 < x <= 0); inside the band,
-// and for the rare u = 0, the FP64 qexp decides. qexp and the host's libm exp differ by
-// at most an ulp or two, so a decision can differ from the host's only for a u within
-// that distance of the exponential.
-__device__ __forceinline__ bool ref_accept(uint64_t u53, long long dE, double T) {
-    const double x = -double(dE) / T;
+// u < exp(-dE / T) for u = u53 2^-53 and dE > 0, with x = -dE / T the host's quotient.
+// A float exponential of xa = dE * (-1 / T) decides outside a band of 2^-14 around it:
+// xa is within 1e-14 of x, and the float exponential's error, the rounding of xa to float
+// included, is below 2e-5 for -79 < xa <= 0. Inside the band, and for the rare u = 0,
+// the exact quotient and the FP64 qexp decide. qexp and the host's libm exp differ by at
+// most an ulp or two, so a decision can differ from the host's only for a u within that
+// distance of the exponential.
+__device__ __forceinline__ bool ref_accept(uint64_t u53, long long dE, double T, double nrT) {
+    const double xa = double(dE) * nrT;
     const double u = double(u53) * 0x1.0p-53;
-    if (!(x > -80.0)) return u < qexp(x);   // only u = 0 can pass below e^-80
+    if (!(xa > -79.0)) return u < qexp(-double(dE) / T);   // only u = 0 can pass below e^-79
     constexpr double kBand = 0x1.0p-14;
-    const double e = double(__builtin_amdgcn_exp2f(float(x) * 1.44269504f));
+    const double e = double(__builtin_amdgcn_exp2f(float(xa) * 1.44269504f));
     if (u < e * (1.0 - kBand)) return true;
     if (u > e * (1.0 + kBand)) return false;
-    return u < qexp(x);
+    return u < qexp(-double(dE) / T);
 }
 
 template <int WAVES>
@@ -1415,33 +1418,46 @@ __global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, 
     const uint64_t below = (uint64_t(1) << lane) - 1u;
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         const double T = temp[sweep];
+        const double nrT = -1.0 / T;
         int i0 = 0;
         while (i0 < n) {
             const int i = i0 + lane;
             const bool valid = i < n;
             const int p = valid ? g.bit_pos[i] : 0;
-            const long long dE = valid ? -(long long)s[p] * (long long)f[p] : 0;
+            const int32_t fp = f[p];
+            const long long dE = valid ? -(long long)s[p] * (long long)fp : 0;
             const bool up = valid && dE > 0;
             const uint64_t m_up = __ballot(up);
             bool acc = valid && dE <= 0;
-            if (up) acc = ref_accept(mix64(st + uint64_t(__popcll(m_up & below) + 1) * kGamma) >> 11, dE, T);
+            if (up) acc = ref_accept(mix64(st + uint64_t(__popcll(m_up & below) + 1) * kGamma) >> 11, dE, T, nrT);
             const uint64_t m_acc = __ballot(acc);
-            if (m_acc == 0) {
-                st += uint64_t(__popcll(m_up)) * kGamma;
-                i0 += 64;
-                continue;
+            // Lanes in [lo, lim) hold decisions made on the current state: a flip changes
+            // no s but its own and only its neighbours' f, so a later lane whose f is
+            // unchanged keeps its dE, its uphill status and (all earlier lanes being
+            // unchanged too) its draw. The first lane whose f changed bounds lim.
+            uint64_t todo = ~uint64_t(0);   // lanes >= lo
+            uint64_t ok = ~uint64_t(0);     // lanes < lim
+            for (;;) {
+                const uint64_t cand = m_acc & todo & ok;
+                if (cand == 0) {
+                    st += uint64_t(__popcll(m_up & todo & ok)) * kGamma;
+                    i0 += ok == ~uint64_t(0) ? 64 : __builtin_ctzll(~ok);
+                    break;
+                }
+                const int js = __builtin_ctzll(cand);
+                const uint64_t upto = js == 63 ? ~uint64_t(0) : (uint64_t(2) << js) - 1u;
+                st += uint64_t(__popcll(m_up & todo & upto)) * kGamma;
+                const int pf = __shfl(p, js);
+                e += __shfl(dE, js);
+                const int si = s[pf];
+                for (int q = g.nb_ptr[pf] + lane; q < g.nb_ptr[pf + 1]; q += 64) f[g.nb[q]] -= 2 * g.nw[q] * si;
+                if (lane == 0) s[pf] = int8_t(-si);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                todo = ~upto;
+                const uint64_t m_dirty = __ballot(valid && f[p] != fp) & todo;
+                if (m_dirty) ok &= (uint64_t(1) << __builtin_ctzll(m_dirty)) - 1u;
             }
-            const int js = __builtin_ctzll(m_acc);
-            const uint64_t upto = js == 63 ? ~uint64_t(0) : (uint64_t(2) << js) - 1u;
-            st += uint64_t(__popcll(m_up & upto)) * kGamma;
-            const int pf = __shfl(p, js);
-            e += __shfl(dE, js);
-            const int si = s[pf];
-            for (int q = g.nb_ptr[pf] + lane; q < g.nb_ptr[pf + 1]; q += 64) f[g.nb[q]] -= 2 * g.nw[q] * si;
-            if (lane == 0) s[pf] = int8_t(-si);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            i0 += js + 1;
         }
     }
     for (int p = lane; p < np; p += 64) {
