@@ -5,7 +5,9 @@
 // residuals, the reference's arithmetic); their trajectory visits bits by colour class
 // with Philox draws keyed by (branch seed, sweep, bit) instead of index order with one
 // splitmix64 stream (src/ising_recon.hpp:121-144). Branch b's seed is still
-// b + 1 + seed_offset. The host helpers binary_energy, local_field_dE,
+// b + 1 + seed_offset. Defining QUBLAR_REFERENCE_TRAJECTORY to 1 before including this
+// header makes both follow the reference's own trajectory bit for bit instead
+// (qb_anneal_branches_reference: one wave per branch, slow on large problems). The host helpers binary_energy, local_field_dE,
 // freeze_provable_rock, roi_qubo and classify keep the reference's definitions, and
 // muon_rows / build_binary_problem build the muon QUBO array for array as the
 // reference's do, over the caller's medium and view types.
@@ -23,6 +25,10 @@
 
 #include "../qublar.h"
 #include "engine.hpp"
+
+#ifndef QUBLAR_REFERENCE_TRAJECTORY
+#define QUBLAR_REFERENCE_TRAJECTORY 0
+#endif
 
 namespace argos {
 
@@ -125,7 +131,7 @@ public:
     void anneal(const Schedule& s, int R, uint64_t seed_offset, std::vector<std::vector<uint8_t>>* branches,
                 std::vector<float>* p_void, std::vector<uint8_t>* tri = nullptr, std::vector<float>* certainty = nullptr,
                 std::vector<double>* energy = nullptr, qb_report* rep = nullptr,
-                const qb_anneal_opts* opts = nullptr) const {
+                const qb_anneal_opts* opts = nullptr, bool reference_trajectory = false) const {
         if (R <= 0) throw EngineError(QB_E_ARG, "qublar: n_branches must be positive");
         const size_t n = static_cast<size_t>(n_);
         if (n_ == 0) {
@@ -143,10 +149,16 @@ public:
         if (tri) tri->assign(n, 0);
         if (certainty) certainty->assign(n, 0.0f);
         if (energy) energy->assign(static_cast<size_t>(R), 0.0);
-        engine_detail::check(qb_anneal_branches(h_, &qs, R, seed_offset, &o, branches ? flat.data() : nullptr,
-                                                p_void ? p_void->data() : nullptr, tri ? tri->data() : nullptr,
-                                                certainty ? certainty->data() : nullptr,
-                                                energy ? energy->data() : nullptr, rep));
+        if (reference_trajectory)
+            engine_detail::check(qb_anneal_branches_reference(
+                h_, &qs, R, seed_offset, branches ? flat.data() : nullptr, p_void ? p_void->data() : nullptr,
+                tri ? tri->data() : nullptr, certainty ? certainty->data() : nullptr,
+                energy ? energy->data() : nullptr, rep));
+        else
+            engine_detail::check(qb_anneal_branches(h_, &qs, R, seed_offset, &o, branches ? flat.data() : nullptr,
+                                                    p_void ? p_void->data() : nullptr, tri ? tri->data() : nullptr,
+                                                    certainty ? certainty->data() : nullptr,
+                                                    energy ? energy->data() : nullptr, rep));
         if (branches) {
             branches->resize(static_cast<size_t>(R));
             for (size_t b = 0; b < static_cast<size_t>(R); ++b)
@@ -163,7 +175,9 @@ private:
 // One branch from all-zero x (src/ising_recon.hpp:121-144), on the GPU.
 inline std::vector<uint8_t> anneal_branch(const BinaryProblem& p, const Schedule& s, uint64_t seed) {
     std::vector<std::vector<uint8_t>> br;
-    DeviceProblem(p).anneal(s, 1, seed - 1, &br, nullptr);   // branch 0 has seed 0 + 1 + (seed - 1)
+    // branch 0 has seed 0 + 1 + (seed - 1)
+    DeviceProblem(p).anneal(s, 1, seed - 1, &br, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            QUBLAR_REFERENCE_TRAJECTORY != 0);
     return br.empty() ? std::vector<uint8_t>() : std::move(br[0]);
 }
 
@@ -174,7 +188,8 @@ inline std::vector<float> branch_fractions(const BinaryProblem& p, const Schedul
                                            uint64_t seed_offset = 0) {
     (void)n_threads;
     std::vector<float> frac;
-    DeviceProblem(p).anneal(s, n_branches, seed_offset, keep, &frac);
+    DeviceProblem(p).anneal(s, n_branches, seed_offset, keep, &frac, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            QUBLAR_REFERENCE_TRAJECTORY != 0);
     if (p.n_vars() == 0) frac.clear();
     return frac;
 }
