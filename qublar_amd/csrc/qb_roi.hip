@@ -14,6 +14,11 @@
 // within ~1e-13 of a scratch sum. Per-block results go to global memory; the host reduces
 // them in block order, so a result does not depend on anything but n, k and the launch
 // split (both fixed by n).
+//
+// The step loop is unrolled by 2^kRoiUnrollLog: within a group, step i > 0 flips bit
+// ctz(i), known at compile time, so only a group's first step selects its field and
+// coupling row by a runtime index. Pass 2 sums a group's weights first and adds that sum
+// to the bits the group does not change (chunks shorter than a group run the plain loop).
 #include "qb_internal.h"
 
 namespace qb {
@@ -67,6 +72,9 @@ __device__ __forceinline__ double roi_step(uint64_t& x, int b, double e, const d
     return e;
 }
 
+constexpr int kRoiUnrollLog = 3;   // unrolled steps per group: 2^kRoiUnrollLog
+constexpr int kRoiUnroll = 1 << kRoiUnrollLog;
+
 template <int NMAX>
 __global__ void __launch_bounds__(kRoiBlock) roi_min_kernel(int n, int k, const double* h, const double* Js,
                                                              uint64_t chunk0, uint64_t chunks, double* blk_e,
@@ -84,11 +92,28 @@ __global__ void __launch_bounds__(kRoiBlock) roi_min_kernel(int n, int k, const 
     double best = e;
     uint64_t best_s = x;
     const uint64_t steps = uint64_t(1) << k;
-    for (uint64_t t = 1; t < steps; ++t) {
-        e = roi_step<NMAX>(x, __builtin_ctzll(t), e, sJ, f);
+    auto keep = [&]() {
         if (e < best || (e == best && x < best_s)) {
             best = e;
             best_s = x;
+        }
+    };
+    if (k >= kRoiUnrollLog) {
+        for (uint64_t tb = 0; tb < (steps >> kRoiUnrollLog); ++tb) {
+            if (tb > 0) {
+                e = roi_step<NMAX>(x, kRoiUnrollLog + __builtin_ctzll(tb), e, sJ, f);
+                keep();
+            }
+#pragma unroll
+            for (int i = 1; i < kRoiUnroll; ++i) {
+                e = roi_step<NMAX>(x, __builtin_ctz(unsigned(i)), e, sJ, f);
+                keep();
+            }
+        }
+    } else {
+        for (uint64_t t = 1; t < steps; ++t) {
+            e = roi_step<NMAX>(x, __builtin_ctzll(t), e, sJ, f);
+            keep();
         }
     }
     re[threadIdx.x] = live ? best : __builtin_huge_val();
@@ -128,13 +153,40 @@ __global__ void __launch_bounds__(kRoiBlock) roi_sum_kernel(int n, int k, const 
     double m[NMAX];
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) m[j] = 0.0;
-    const uint64_t steps = uint64_t(1) << k;
-    for (uint64_t t = 0; t < steps; ++t) {
-        if (t > 0) e = roi_step<NMAX>(x, __builtin_ctzll(t), e, sJ, f);
+    auto add = [&]() {
         const double w = exp(-beta * (e - e_ref));
         z += w;
 #pragma unroll
         for (int j = 0; j < NMAX; ++j) m[j] += ((x >> j) & 1u) ? w : 0.0;
+    };
+    const uint64_t steps = uint64_t(1) << k;
+    if (k >= kRoiUnrollLog) {
+        // Bits kRoiUnrollLog.. stay fixed over a group (its first step flips the last of
+        // them to change): the group's weight sum goes to them once; only the low bits are
+        // summed state by state.
+        for (uint64_t tb = 0; tb < (steps >> kRoiUnrollLog); ++tb) {
+            if (tb > 0) e = roi_step<NMAX>(x, kRoiUnrollLog + __builtin_ctzll(tb), e, sJ, f);
+            double wg = 0.0, lo[kRoiUnrollLog];
+#pragma unroll
+            for (int l = 0; l < kRoiUnrollLog; ++l) lo[l] = 0.0;
+#pragma unroll
+            for (int i = 0; i < kRoiUnroll; ++i) {
+                if (i > 0) e = roi_step<NMAX>(x, __builtin_ctz(unsigned(i)), e, sJ, f);
+                const double w = exp(-beta * (e - e_ref));
+                wg += w;
+#pragma unroll
+                for (int l = 0; l < kRoiUnrollLog; ++l) lo[l] += ((x >> l) & 1u) ? w : 0.0;
+            }
+            z += wg;
+#pragma unroll
+            for (int j = 0; j < NMAX; ++j) m[j] += j < kRoiUnrollLog ? lo[j < kRoiUnrollLog ? j : 0]
+                                                                    : (((x >> j) & 1u) ? wg : 0.0);
+        }
+    } else {
+        for (uint64_t t = 0; t < steps; ++t) {
+            if (t > 0) e = roi_step<NMAX>(x, __builtin_ctzll(t), e, sJ, f);
+            add();
+        }
     }
     double* out = blk + size_t(blockIdx.x) * (NMAX + 1);
     for (int q = 0; q <= NMAX; ++q) {
