@@ -176,3 +176,23 @@ def test_build_binary_problem_muon_rows(driver, shape, seed, bins):
             assert P["w"][row] == det
             row += 1
     assert row == len(P["d"]) == len(P["w"]) and crossing_rays > 5
+
+
+def test_reference_drivers_link_and_fail_loudly_without_a_device():
+    """The reference's own tools/gpu_anneal.cu and tools/gset_bench.cu, compiled unchanged
+    against include/argos (make -C tests/cpp drivers), load the engine library. gset_bench's
+    host-only checks (the rudy loader's refusals, tools/gset_bench.cu:313-333) pass here;
+    its first anneal then stops with the engine's error, as there is no device and no
+    CPU fallback. The device half runs in tests/test_gpu_ref_drivers.py."""
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    if not os.path.exists(os.path.join(ref, "gset_bench")):
+        pytest.skip("the reference's drivers are built only where the reference tree exists")
+    r = subprocess.run([os.path.join(ref, "gpu_anneal")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage: gpu_anneal" in r.stderr
+    # line-buffered, so the lines printed before the abort are not lost with its stdio buffer
+    r = subprocess.run(["stdbuf", "-oL", os.path.join(ref, "gset_bench"), "--check"], capture_output=True, text=True,
+                       timeout=60)
+    lines = [ln for ln in r.stdout.splitlines() if " PASS " in ln]
+    assert any("refuses 5 malformed inputs" in ln for ln in lines)
+    assert any("missing weight is +1" in ln for ln in lines)
+    assert r.returncode != 0 and "EngineError" in r.stderr
