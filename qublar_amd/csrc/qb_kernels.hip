@@ -1503,7 +1503,9 @@ hipError_t launch_int_ref(const DevGraph& g, int R, uint64_t seed_base, const do
 // global memory, read through the CU's L1 (a wave sees its own completed stores). As in the
 // integer trajectory, the wave evaluates the next 64 bits at once -- each lane sums its
 // bit's dE alone, in the reference's order -- and the first accepted bit is the loop's next
-// flip. Draw keys and counters of unions (draw_bit, draw_key) do not apply here.
+// flip. A flip of a bit with rays ends the window; after a ray-free bit's flip the later
+// lanes only sum their neighbour terms again (the last part of the reference's sum) and
+// decide anew. Draw keys and counters of unions (draw_bit, draw_key) do not apply here.
 __device__ __forceinline__ bool ref_accept_f(uint64_t u53, double dE, double T) {
     const double x = -dE / T;
     const double u = double(u53) * 0x1.0p-53;
@@ -1536,12 +1538,12 @@ __global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint6
             const int i = i0 + lane;
             const bool valid = i < n;
             const int pos = valid ? p.bit_pos[i] : 0;
-            double dE = 0.0;
+            double dE = 0.0, base = 0.0;   // base: the sum up to the neighbour terms
+            const int xi = valid ? int(x[i]) : 0;
+            constexpr int kChunk = 8;
             if (valid) {
                 // eight entries' loads, then their residuals, in flight together; the sum
                 // itself stays in CSR order
-                constexpr int kChunk = 8;
-                const int xi = x[i];
                 const double delta = xi ? -1.0 : 1.0;
                 dE = p.lin[pos] * delta;
                 const int k1 = p.col_ptr[pos + 1];
@@ -1559,6 +1561,7 @@ __global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint6
                         dE += e[u].w * (rv[u] * a + 0.5 * a * a);
                     }
                 }
+                base = dE;
                 const int q1 = p.nb_ptr[pos + 1];
                 for (int qb = p.nb_ptr[pos]; qb < q1; qb += kChunk) {
                     int xv[kChunk];
@@ -1571,32 +1574,62 @@ __global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint6
                     }
                 }
             }
-            const bool up = valid && dE > 0.0;
-            const uint64_t m_up = __ballot(up);
-            bool acc = valid && dE <= 0.0;
-            if (up) acc = ref_accept_f(mix64(st + uint64_t(__popcll(m_up & below) + 1) * kGamma) >> 11, dE, T);
-            const uint64_t m_acc = __ballot(acc);
-            if (m_acc == 0) {
-                st += uint64_t(__popcll(m_up)) * kGamma;
-                i0 += 64;
-                continue;
+            // decisions of the lanes in todo (lanes >= lo) on the current state, the k-th
+            // uphill one drawing st + k gamma
+            uint64_t todo = ~uint64_t(0), m_up = 0, m_acc = 0;
+            auto decide = [&]() {
+                const bool mine = valid && ((todo >> lane) & 1u);
+                const bool up = mine && dE > 0.0;
+                m_up = __ballot(up) & todo;
+                bool acc = mine && dE <= 0.0;
+                if (up) acc = ref_accept_f(mix64(st + uint64_t(__popcll(m_up & below) + 1) * kGamma) >> 11, dE, T);
+                m_acc = __ballot(acc) & todo;
+            };
+            decide();
+            for (;;) {
+                if (m_acc == 0) {
+                    st += uint64_t(__popcll(m_up)) * kGamma;
+                    i0 += 64;
+                    break;
+                }
+                const int js = __builtin_ctzll(m_acc);
+                const uint64_t upto = js == 63 ? ~uint64_t(0) : (uint64_t(2) << js) - 1u;
+                st += uint64_t(__popcll(m_up & upto)) * kGamma;
+                const int fi = i0 + js, fp = __shfl(pos, js);
+                const int xf = x[fi];
+                const double delta = xf ? -1.0 : 1.0;
+                const int k0 = p.col_ptr[fp], k1 = p.col_ptr[fp + 1];
+                if (!p.seq_update[fp]) {
+                    for (int k = k0 + lane; k < k1; k += 64) r[p.ent[k].ray] += double(p.ent[k].a) * delta;
+                } else if (lane == 0) {   // a ray repeats in the column: its additions in CSR order
+                    for (int k = k0; k < k1; ++k) r[p.ent[k].ray] += double(p.ent[k].a) * delta;
+                }
+                if (lane == 0) x[fi] = uint8_t(xf ^ 1);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                todo = ~upto;
+                if (k1 > k0 || todo == 0) {   // residuals changed (or the window is done)
+                    i0 = fi + 1;
+                    break;
+                }
+                // A ray-free bit flipped: no residual changed, so a later lane's sum up to its
+                // neighbour terms stands; only those terms are summed again, in order.
+                if (valid && ((todo >> lane) & 1u)) {
+                    dE = base;
+                    const int q1 = p.nb_ptr[pos + 1];
+                    for (int qb = p.nb_ptr[pos]; qb < q1; qb += kChunk) {
+                        int xv[kChunk];
+#pragma unroll
+                        for (int u = 0; u < kChunk; ++u) xv[u] = x[p.nb_bit[min(qb + u, q1 - 1)]];
+#pragma unroll
+                        for (int u = 0; u < kChunk; ++u) {
+                            if (qb + u >= q1) break;
+                            dE += p.lambda * (xv[u] != xi ? -1.0 : 1.0);
+                        }
+                    }
+                }
+                decide();
             }
-            const int js = __builtin_ctzll(m_acc);
-            const uint64_t upto = js == 63 ? ~uint64_t(0) : (uint64_t(2) << js) - 1u;
-            st += uint64_t(__popcll(m_up & upto)) * kGamma;
-            const int fi = i0 + js, fp = __shfl(pos, js);
-            const int xf = x[fi];
-            const double delta = xf ? -1.0 : 1.0;
-            const int k0 = p.col_ptr[fp], k1 = p.col_ptr[fp + 1];
-            if (!p.seq_update[fp]) {
-                for (int k = k0 + lane; k < k1; k += 64) r[p.ent[k].ray] += double(p.ent[k].a) * delta;
-            } else if (lane == 0) {   // a ray repeats in the column: its additions in CSR order
-                for (int k = k0; k < k1; ++k) r[p.ent[k].ray] += double(p.ent[k].a) * delta;
-            }
-            if (lane == 0) x[fi] = uint8_t(xf ^ 1);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            i0 = fi + 1;
         }
     }
 }
