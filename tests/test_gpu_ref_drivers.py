@@ -124,3 +124,31 @@ def test_gpu_anneal_bench_suite_csvs(tmp_path):
     tail = thr_rows[1].split(",")[-10:]
     assert int(tail[-2]) == 16384 and int(tail[-1]) == int(summ[1]["runs"])
     assert abs(float(tail[1]) - rate) <= 1e-6 * rate   # rate_gpu_per_s, as in summary.csv
+
+
+def test_gset_bench_suite_csvs(tmp_path):
+    """The P1 suite of tools/gset_bench.cu --suite (:417-541) on the G1-like graph: cells B1,
+    B10 and B100 (64, 64 and 16 runs of wising::anneal) and its runs, summary and load CSVs,
+    read back. Through the drop-in headers every run is a one-replica device call."""
+    exe = driver("gset_bench")
+    g = qa.er_graph(800, 0.06, 1)
+    g = qa.GraphArrays(g.n, g.eu, g.ev, np.abs(g.ew))
+    graph = tmp_path / "G1like.rudy"
+    write_rudy(graph, g)
+    sha = hashlib.sha256(graph.read_bytes()).hexdigest()
+    (tmp_path / "list.txt").write_text(f"G1like {graph} 11400 {sha} {g.n} {g.m} {g.m} 0\n")
+    r = run(["stdbuf", "-oL", exe, "--suite", "list.txt", ".", "16", "60"], cwd=tmp_path)
+    assert r.returncode == 0
+    f = lambda name: read_csv(tmp_path / (".\\" + name))
+    load, summ, runs = f("load.csv"), f("summary.csv"), f("runs.csv")
+    assert len(load) == 1 and load[0]["sha_ok"] == "1" and load[0]["expect_ok"] == "1"
+    assert [row["budget"] for row in summ] == ["B1", "B10", "B100"]
+    planned = {"B1": 64, "B10": 64, "B100": 16}
+    for row in summ:
+        assert int(row["runs_done"]) == int(row["runs_planned"]) == planned[row["budget"]]
+        assert row["all_consistent"] == "1"
+    assert len(runs) == 144 and all(row["consistent"] == "1" for row in runs)
+    # Budget(5, 1e-3, 4000, 200) on a graph whose best cuts are near 11,495
+    b10 = [int(row["cut"]) for row in runs if row["budget"] == "B10"]
+    assert max(b10) > 11000
+    print("suite:", {row["budget"]: (row["best_cut"], row["p_success"], row["t_mean_s"]) for row in summ}, flush=True)
