@@ -1424,6 +1424,9 @@ __global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, 
             const int i = i0 + lane;
             const bool valid = i < n;
             const int p = valid ? g.bit_pos[i] : 0;
+            // each lane's adjacency row bounds, loaded with the window so that a flip does not
+            // wait on them
+            const int row0 = valid ? g.nb_ptr[p] : 0, row1 = valid ? g.nb_ptr[p + 1] : 0;
             const int32_t fp = f[p];
             const long long dE = valid ? -(long long)s[p] * (long long)fp : 0;
             const bool up = valid && dE > 0;
@@ -1447,10 +1450,10 @@ __global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, 
                 const int js = __builtin_ctzll(cand);
                 const uint64_t upto = js == 63 ? ~uint64_t(0) : (uint64_t(2) << js) - 1u;
                 st += uint64_t(__popcll(m_up & todo & upto)) * kGamma;
-                const int pf = __shfl(p, js);
+                const int pf = __shfl(p, js), q0 = __shfl(row0, js), q1 = __shfl(row1, js);
                 e += __shfl(dE, js);
                 const int si = s[pf];
-                for (int q = g.nb_ptr[pf] + lane; q < g.nb_ptr[pf + 1]; q += 64) f[g.nb[q]] -= 2 * g.nw[q] * si;
+                for (int q = q0 + lane; q < q1; q += 64) f[g.nb[q]] -= 2 * g.nw[q] * si;
                 if (lane == 0) s[pf] = int8_t(-si);
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
