@@ -1543,14 +1543,17 @@ __global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint6
             const int pos = valid ? p.bit_pos[i] : 0;
             double dE = 0.0, base = 0.0;   // base: the sum up to the neighbour terms
             const int xi = valid ? int(x[i]) : 0;
+            // each lane's column bounds and update order, kept for the flip
+            const int c0 = valid ? p.col_ptr[pos] : 0, c1 = valid ? p.col_ptr[pos + 1] : 0;
+            const int seq = valid ? int(p.seq_update[pos]) : 0;
             constexpr int kChunk = 8;
             if (valid) {
                 // eight entries' loads, then their residuals, in flight together; the sum
                 // itself stays in CSR order
                 const double delta = xi ? -1.0 : 1.0;
                 dE = p.lin[pos] * delta;
-                const int k1 = p.col_ptr[pos + 1];
-                for (int kb = p.col_ptr[pos]; kb < k1; kb += kChunk) {
+                const int k1 = c1;
+                for (int kb = c0; kb < k1; kb += kChunk) {
                     RayEnt e[kChunk];
                     double rv[kChunk];
 #pragma unroll
@@ -1598,11 +1601,11 @@ __global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint6
                 const int js = __builtin_ctzll(m_acc);
                 const uint64_t upto = js == 63 ? ~uint64_t(0) : (uint64_t(2) << js) - 1u;
                 st += uint64_t(__popcll(m_up & upto)) * kGamma;
-                const int fi = i0 + js, fp = __shfl(pos, js);
-                const int xf = x[fi];
+                const int fi = i0 + js;
+                const int k0 = __shfl(c0, js), k1 = __shfl(c1, js), seq_f = __shfl(seq, js);
+                const int xf = __shfl(xi, js);
                 const double delta = xf ? -1.0 : 1.0;
-                const int k0 = p.col_ptr[fp], k1 = p.col_ptr[fp + 1];
-                if (!p.seq_update[fp]) {
+                if (!seq_f) {
                     for (int k = k0 + lane; k < k1; k += 64) r[p.ent[k].ray] += double(p.ent[k].a) * delta;
                 } else if (lane == 0) {   // a ray repeats in the column: its additions in CSR order
                     for (int k = k0; k < k1; ++k) r[p.ent[k].ray] += double(p.ent[k].a) * delta;
