@@ -19,6 +19,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../qublar.h"
@@ -36,136 +37,153 @@ struct WeightedGraph {
     int64_t sum_w = 0, n_pos = 0, n_neg = 0, n_other = 0;
 };
 
-namespace loader {
-inline bool blank(char c) { return c == ' ' || c == '\t' || c == '\r'; }
+namespace rudy {
+// The text is read line by line. A line's tokens are its maximal runs of characters
+// other than ' ', '\t', '\r' (a line ends at '\n'); lines without tokens are skipped.
+// The first tokenful line opens with "n m", and whatever follows m on that line is read
+// as an edge line of its own; every later tokenful line is an edge "u v" or "u v w".
+using Tokens = std::vector<std::pair<const char*, const char*>>;
 
-// A signed decimal integer with magnitude at most 2^40; anything else is refused.
-inline bool to_int(const char* s, size_t len, int64_t& out) {
-    size_t i = 0;
-    bool neg = false;
-    if (len > 0 && (s[0] == '+' || s[0] == '-')) {
-        neg = s[0] == '-';
-        i = 1;
+inline bool is_separator(char c) { return c == ' ' || c == '\t' || c == '\r'; }
+
+// Splits [b, e) into tokens; returns the number found.
+inline size_t split(const char* b, const char* e, Tokens& out) {
+    out.clear();
+    const char* p = b;
+    while (p != e) {
+        if (is_separator(*p)) { ++p; continue; }
+        const char* q = p;
+        while (q != e && !is_separator(*q)) ++q;
+        out.emplace_back(p, q);
+        p = q;
     }
-    if (i == len) return false;
-    int64_t v = 0;
-    for (; i < len; ++i) {
-        if (s[i] < '0' || s[i] > '9') return false;
-        v = v * 10 + (s[i] - '0');
-        if (v > (int64_t(1) << 40)) return false;
+    return out.size();
+}
+
+// An optionally signed run of decimal digits whose magnitude is at most 2^40.
+inline bool integer(const std::pair<const char*, const char*>& tok, int64_t& value) {
+    const char* p = tok.first;
+    int sign = 1;
+    if (p != tok.second && (*p == '-' || *p == '+')) sign = (*p++ == '-') ? -1 : 1;
+    if (p == tok.second) return false;
+    constexpr uint64_t kCap = uint64_t(1) << 40;
+    uint64_t mag = 0;
+    for (; p != tok.second; ++p) {
+        const unsigned digit = unsigned(static_cast<unsigned char>(*p)) - unsigned('0');
+        if (digit > 9u) return false;
+        mag = mag * 10u + digit;
+        if (mag > kCap) return false;   // checked per digit, so mag never overflows
     }
-    out = neg ? -v : v;
+    value = sign * int64_t(mag);
     return true;
 }
 
-// Tokens of one line at a time; a token ends at a blank or a newline.
-struct Reader {
-    const char* at;
-    const char* end;
-    void skip_blank() { while (at < end && blank(*at)) ++at; }
-    void skip_space() { while (at < end && (blank(*at) || *at == '\n')) ++at; }
-    bool next(const char*& s, size_t& len) {
-        skip_blank();
-        if (at >= end || *at == '\n') return false;
-        s = at;
-        while (at < end && !blank(*at) && *at != '\n') ++at;
-        len = static_cast<size_t>(at - s);
-        return true;
-    }
-    bool line_done() { skip_blank(); return at >= end || *at == '\n'; }
-};
-}  // namespace loader
+inline std::string text_of(const std::pair<const char*, const char*>& tok) {
+    return std::string(tok.first, tok.second);
+}
+}  // namespace rudy
 
-// rudy / Gset text: "n m", then m lines "u v [w]" (1-indexed, missing w = +1).
-// Refuses a non-integer weight, a self-loop, a duplicate edge (either orientation),
-// an index out of range, or an edge count that differs from the header
-// (src/ising_weighted.hpp:83-149).
+// rudy / Gset text (src/ising_weighted.hpp:83-149): "n m", then m edges "u v [w]",
+// 1-indexed, a missing w is +1. A file is refused (false, err says why) when a weight is
+// not an integer or leaves int32, an edge line has one token or more than three, an edge
+// is a self-loop or repeats another in either orientation, an index leaves [1, n], or the
+// edge count differs from m.
 inline bool parse_rudy(const char* data, size_t size, WeightedGraph& g, std::string& err) {
     g = WeightedGraph();
-    loader::Reader rd{data, data + size};
-    const char* t = nullptr;
-    size_t tl = 0;
-    int64_t n = 0, m = 0;
-    rd.skip_space();
-    if (!rd.next(t, tl) || !loader::to_int(t, tl, n) || n <= 0 || n > (int64_t(1) << 30)) {
-        err = "bad header: n";
+    const char* const end = data + size;
+    rudy::Tokens tok;
+    int64_t n = -1, m = -1;
+    int64_t line_no = 0;
+    auto refuse = [&](const std::string& why) {
+        err = "rudy line " + std::to_string(line_no) + ": " + why;
         return false;
-    }
-    if (!rd.next(t, tl) || !loader::to_int(t, tl, m) || m < 0) {
-        err = "bad header: m";
-        return false;
-    }
-    g.n = static_cast<int>(n);
-    g.m = m;
-    int64_t count = 0;
-    for (;;) {
-        rd.skip_space();
-        if (rd.at >= rd.end) break;
+    };
+    // one edge from tokens [first, tok.size())
+    auto edge = [&](size_t first) {
+        const size_t have = tok.size() - first;
+        if (have < 2) return refuse("an edge needs two vertex indices");
+        if (have > 3) return refuse("an edge line holds at most u, v and a weight");
         int64_t u = 0, v = 0, w = 1;
-        if (!rd.next(t, tl) || !loader::to_int(t, tl, u) || !rd.next(t, tl) || !loader::to_int(t, tl, v)) {
-            err = "bad vertex token '" + std::string(t ? t : "", t ? tl : 0) + "'";
-            return false;
+        if (!rudy::integer(tok[first], u) || !rudy::integer(tok[first + 1], v))
+            return refuse("vertex indices must be integers");
+        if (have == 3) {
+            if (!rudy::integer(tok[first + 2], w))
+                return refuse("weight '" + rudy::text_of(tok[first + 2]) + "' is not an integer");
+            if (w < int64_t(std::numeric_limits<int32_t>::min()) || w > int64_t(std::numeric_limits<int32_t>::max()))
+                return refuse("weight does not fit int32");
         }
-        if (rd.next(t, tl)) {
-            if (!loader::to_int(t, tl, w)) {
-                err = "weight '" + std::string(t, tl) + "' is not an integer; the float path is BinaryProblem";
+        if (std::min(u, v) < 1 || std::max(u, v) > n) return refuse("vertex index outside 1.." + std::to_string(n));
+        if (u == v) return refuse("edge joins a vertex to itself");
+        g.eu.push_back(int32_t(u - 1));
+        g.ev.push_back(int32_t(v - 1));
+        g.ew.push_back(int32_t(w));
+        return true;
+    };
+    for (const char* b = data; b < end;) {
+        const char* e = static_cast<const char*>(std::memchr(b, '\n', size_t(end - b)));
+        if (!e) e = end;
+        ++line_no;
+        if (rudy::split(b, e, tok) > 0) {
+            if (n < 0) {
+                if (!rudy::integer(tok[0], n) || n < 1 || n > (int64_t(1) << 30))
+                    return refuse("the vertex count must be an integer in 1..2^30");
+                if (tok.size() < 2 || !rudy::integer(tok[1], m) || m < 0)
+                    return refuse("the edge count must follow the vertex count as an integer >= 0");
+                if (tok.size() > 2 && !edge(2)) return false;
+            } else if (!edge(0)) {
                 return false;
             }
-            if (w < std::numeric_limits<int32_t>::min() || w > std::numeric_limits<int32_t>::max()) {
-                err = "weight out of int32 range";
-                return false;
-            }
-            if (!rd.line_done()) {
-                err = "trailing token on an edge line";
-                return false;
-            }
         }
-        if (u < 1 || v < 1 || u > n || v > n) {
-            err = "vertex index out of range";
-            return false;
-        }
-        if (u == v) {
-            err = "self-loop";
-            return false;
-        }
-        g.eu.push_back(static_cast<int32_t>(u - 1));
-        g.ev.push_back(static_cast<int32_t>(v - 1));
-        g.ew.push_back(static_cast<int32_t>(w));
-        ++count;
+        b = e + 1;
     }
-    if (count != m) {
-        err = "header says " + std::to_string(m) + " edges, file has " + std::to_string(count);
+    if (n < 0) {
+        err = "rudy: no header line";
         return false;
     }
-    std::vector<uint64_t> keys(static_cast<size_t>(m));
-    for (size_t e = 0; e < keys.size(); ++e) {
-        const uint64_t lo = static_cast<uint64_t>(std::min(g.eu[e], g.ev[e]));
-        const uint64_t hi = static_cast<uint64_t>(std::max(g.eu[e], g.ev[e]));
-        keys[e] = (lo << 32) | hi;
-    }
-    std::sort(keys.begin(), keys.end());
-    if (std::adjacent_find(keys.begin(), keys.end()) != keys.end()) {
-        err = "duplicate edge";
+    const int64_t got = int64_t(g.eu.size());
+    if (got != m) {
+        err = "rudy: " + std::to_string(got) + " edges listed, the header announced " + std::to_string(m);
         return false;
     }
-    g.row.assign(static_cast<size_t>(n) + 1, 0);
-    for (size_t e = 0; e < keys.size(); ++e) {
-        ++g.row[static_cast<size_t>(g.eu[e]) + 1];
-        ++g.row[static_cast<size_t>(g.ev[e]) + 1];
+    g.n = int(n);
+    g.m = m;
+    // Both directions of every edge, rows filled in edge order.
+    g.row.assign(size_t(n) + 1, 0);
+    for (int64_t k = 0; k < m; ++k) {
+        ++g.row[size_t(g.eu[size_t(k)]) + 1];
+        ++g.row[size_t(g.ev[size_t(k)]) + 1];
     }
-    for (size_t i = 0; i < static_cast<size_t>(n); ++i) g.row[i + 1] += g.row[i];
-    g.col.resize(static_cast<size_t>(2 * m));
-    g.w.resize(static_cast<size_t>(2 * m));
-    std::vector<int64_t> next(g.row.begin(), g.row.end() - 1);
-    for (size_t e = 0; e < keys.size(); ++e) {
-        const int32_t a = g.eu[e], b = g.ev[e], wt = g.ew[e];
-        g.col[static_cast<size_t>(next[static_cast<size_t>(a)])] = b;
-        g.w[static_cast<size_t>(next[static_cast<size_t>(a)]++)] = wt;
-        g.col[static_cast<size_t>(next[static_cast<size_t>(b)])] = a;
-        g.w[static_cast<size_t>(next[static_cast<size_t>(b)]++)] = wt;
-        g.sum_w += wt;
-        if (wt == 1) ++g.n_pos;
-        else if (wt == -1) ++g.n_neg;
+    for (size_t i = 1; i < g.row.size(); ++i) g.row[i] += g.row[i - 1];
+    g.col.resize(size_t(2 * m));
+    g.w.resize(size_t(2 * m));
+    std::vector<int64_t> fill(g.row.begin(), g.row.end() - 1);
+    for (int64_t k = 0; k < m; ++k) {
+        const int32_t u = g.eu[size_t(k)], v = g.ev[size_t(k)], w = g.ew[size_t(k)];
+        int64_t& fu = fill[size_t(u)];
+        g.col[size_t(fu)] = v;
+        g.w[size_t(fu++)] = w;
+        int64_t& fv = fill[size_t(v)];
+        g.col[size_t(fv)] = u;
+        g.w[size_t(fv++)] = w;
+    }
+    // A repeated edge (either orientation) is a neighbour listed twice in a row: one
+    // pass over the rows with a stamp per vertex.
+    std::vector<int32_t> stamp(size_t(n), -1);
+    for (int32_t i = 0; i < int32_t(n); ++i) {
+        for (int64_t k = g.row[size_t(i)]; k < g.row[size_t(i) + 1]; ++k) {
+            const int32_t j = g.col[size_t(k)];
+            if (stamp[size_t(j)] == i) {
+                g = WeightedGraph();
+                err = "rudy: edge " + std::to_string(i + 1) + "-" + std::to_string(j + 1) + " is listed twice";
+                return false;
+            }
+            stamp[size_t(j)] = i;
+        }
+    }
+    for (int32_t w : g.ew) {
+        g.sum_w += w;
+        if (w == 1) ++g.n_pos;
+        else if (w == -1) ++g.n_neg;
         else ++g.n_other;
     }
     return true;

@@ -157,6 +157,14 @@ int qb_anneal_branches(qb_problem* p, const qb_schedule* s, int n_branches, uint
 int qb_anneal_branches_reference(qb_problem* p, const qb_schedule* s, int n_branches, uint64_t seed_offset,
                                  uint8_t* branches, float* p_void, uint8_t* tri, float* certainty, double* energy,
                                  qb_report* rep);
+/* The same with the launch split stated: each launch runs at most sweeps_per_launch sweeps
+   (0: the library bounds a launch at about 2^25 proposals), and the next continues from the
+   spins, residuals and splitmix64 state the last one left, so the split never changes
+   results. rep.launches counts the launches. No counterpart in the reference, whose CPU
+   loop has no launches (src/ising_recon.hpp:121-144). */
+int qb_anneal_branches_reference_split(qb_problem* p, const qb_schedule* s, int n_branches, uint64_t seed_offset,
+                                       int sweeps_per_launch, uint8_t* branches, float* p_void, uint8_t* tri,
+                                       float* certainty, double* energy, qb_report* rep);
 
 /* The frozen set of freeze_provable_rock (src/ising_recon.hpp:187-205) on the device:
    frozen[i] = 1 (original bit order) where bit i is provably rock at `margin` nats. The

@@ -92,6 +92,7 @@ def lib():
         L.orc_classify.argtypes = [C.c_float, C.c_float, C.c_float]
         L.orc_freeze_mask.restype = C.c_int
         L.orc_freeze_mask.argtypes = [C.POINTER(Problem), C.c_double, _u8p]
+        L.orc_freeze_bounds.argtypes = [C.POINTER(Problem), _u8p, _f64p]
         L.orc_roi_qubo.argtypes = [C.POINTER(Problem), _i32p, C.c_int, _u8p, _f64p, _f64p]
         L.orc_graph_csr.argtypes = [C.c_int, C.c_int64, _i32p, _i32p, _i32p, _i64p, _i32p, _i32p]
         L.orc_parse_rudy.restype = C.c_int

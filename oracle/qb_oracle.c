@@ -298,29 +298,52 @@ int orc_classify(float p, float lo, float hi) {
     return 2;
 }
 
+/* The data part of freeze_provable_rock's bound (src/ising_recon.hpp:189-192): each
+   entry's w a (a / 2 - d), summed in CSR order. */
+static void freeze_data_bounds(const orc_problem* p, double* lb) {
+    for (int i = 0; i < p->n_vars; ++i) {
+        lb[i] = 0.0;
+        for (int k = p->row_ptr[i]; k < p->row_ptr[i + 1]; ++k)
+            lb[i] += p->w[p->ray[k]] * (double)p->a[k] * (0.5 * (double)p->a[k] - p->d[p->ray[k]]);
+    }
+}
+
+/* Bit i's bound given the frozen set (src/ising_recon.hpp:198-202): data + lin +
+   lambda (rock - free) over its neighbours. */
+static double freeze_bound_of(const orc_problem* p, const double* lb, const uint8_t* frozen, int i) {
+    int rock = 0, fr = 0;
+    for (int k = p->nbr_ptr[i]; k < p->nbr_ptr[i + 1]; ++k) {
+        if (frozen[p->nbr[k]]) ++rock; else ++fr;
+    }
+    return lb[i] + lin_of(p, i) + p->lambda * (double)(rock - fr);
+}
+
 /* freeze_provable_rock's fixpoint (src/ising_recon.hpp:187-205). */
 int orc_freeze_mask(const orc_problem* p, double margin, uint8_t* frozen) {
     const int n = p->n_vars;
     double* lb = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
-    for (int i = 0; i < n; ++i)
-        for (int k = p->row_ptr[i]; k < p->row_ptr[i + 1]; ++k)
-            lb[i] += p->w[p->ray[k]] * (double)p->a[k] * (0.5 * (double)p->a[k] - p->d[p->ray[k]]);
+    freeze_data_bounds(p, lb);
     memset(frozen, 0, (size_t)n);
     int rounds = 0;
     for (int changed = 1; changed; ++rounds) {
         changed = 0;
         for (int i = 0; i < n; ++i) {
             if (frozen[i]) continue;
-            int rock = 0, fr = 0;
-            for (int k = p->nbr_ptr[i]; k < p->nbr_ptr[i + 1]; ++k) {
-                if (frozen[p->nbr[k]]) ++rock; else ++fr;
-            }
-            const double bound = lb[i] + lin_of(p, i) + p->lambda * (double)(rock - fr);
-            if (bound >= margin) { frozen[i] = 1; changed = 1; }
+            if (freeze_bound_of(p, lb, frozen, i) >= margin) { frozen[i] = 1; changed = 1; }
         }
     }
     free(lb);
     return rounds;
+}
+
+/* Every bit's bound with `frozen` held fixed (the quantity the rounds above compare with
+   the margin), for checking it against a brute-force minimum of dE(0 -> 1). */
+void orc_freeze_bounds(const orc_problem* p, const uint8_t* frozen, double* bound) {
+    const int n = p->n_vars;
+    double* lb = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    freeze_data_bounds(p, lb);
+    for (int i = 0; i < n; ++i) bound[i] = freeze_bound_of(p, lb, frozen, i);
+    free(lb);
 }
 
 /* roi_qubo (src/ising_recon.hpp:251-300). Pairs sharing a ray are visited ray by ray,

@@ -96,6 +96,8 @@ void orc_branch_fractions(const orc_problem* p, const orc_schedule* s, int n_bra
 int orc_classify(float p, float lo, float hi);             /* ising_recon.hpp:302-309 */
 /* freeze_provable_rock (ising_recon.hpp:187-234): frozen[i] = 1 for frozen bits; returns rounds. */
 int orc_freeze_mask(const orc_problem* p, double margin, uint8_t* frozen);
+/* Each bit's freeze bound with the frozen set held fixed (ising_recon.hpp:198-202). */
+void orc_freeze_bounds(const orc_problem* p, const uint8_t* frozen, double* bound);
 /* roi_qubo (ising_recon.hpp:251-300): h[n], J[n*n] (upper triangle). */
 void orc_roi_qubo(const orc_problem* p, const int* roi, int n, const uint8_t* x_bar,
                   double* h, double* J);

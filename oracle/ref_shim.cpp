@@ -56,6 +56,30 @@ int ref_parse_rudy(const char* text, size_t size, int* n, int64_t* m, int64_t* s
     return 0;
 }
 
+// parse_rudy's whole result as one FNV-1a 64 digest over n, m, the edge lists, the CSR
+// and the weight counts (0 when refused); returns 0 when accepted.
+int ref_parse_rudy_digest(const char* text, size_t size, uint64_t* digest) {
+    wi::WeightedGraph g;
+    std::string err;
+    *digest = 0;
+    if (!wi::parse_rudy(text, size, g, err)) return 1;
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void* p, size_t bytes) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < bytes; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    };
+    const int64_t head[6] = {g.n, g.m, g.sum_w, g.n_pos, g.n_neg, g.n_other};
+    mix(head, sizeof(head));
+    mix(g.eu.data(), 4 * g.eu.size());
+    mix(g.ev.data(), 4 * g.ev.size());
+    mix(g.ew.data(), 4 * g.ew.size());
+    mix(g.row.data(), 8 * g.row.size());
+    mix(g.col.data(), 4 * g.col.size());
+    mix(g.w.data(), 4 * g.w.size());
+    *digest = h;
+    return 0;
+}
+
 int64_t ref_cut_of(int n, int64_t m, const int32_t* eu, const int32_t* ev, const int32_t* ew,
                    const uint8_t* x) {
     const wi::WeightedGraph g = make_graph(n, m, eu, ev, ew);

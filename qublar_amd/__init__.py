@@ -151,6 +151,8 @@ def lib():
             "qb_freeze": (i32, [vp, dbl, vp, C.POINTER(i32)]),
             "qb_anneal_branches_reference": (i32, [vp, C.POINTER(Schedule), i32, u64, vp, vp, vp, vp, vp,
                                                    C.POINTER(Report)]),
+            "qb_anneal_branches_reference_split": (i32, [vp, C.POINTER(Schedule), i32, u64, i32, vp, vp, vp, vp, vp,
+                                                         C.POINTER(Report)]),
             "qb_graph_create": (i32, [C.POINTER(GraphDesc), i32, C.POINTER(vp)]),
             "qb_graph_destroy": (i32, [vp]),
             "qb_graph_get_plan": (i32, [vp, C.POINTER(Plan)]),
@@ -360,8 +362,9 @@ class Problem:
         _check(lib().qb_freeze(self.h, float(margin), _p(fr), C.byref(rounds)))
         return fr, rounds.value
 
-    def anneal_branches_reference(self, sched, n_branches, seed_offset=0, branches=True):
-        """branch_fractions in the reference's own trajectory (index order, splitmix64)."""
+    def anneal_branches_reference(self, sched, n_branches, seed_offset=0, branches=True, sweeps_per_launch=0):
+        """branch_fractions in the reference's own trajectory (index order, splitmix64);
+        sweeps_per_launch 0 lets the library split the launches."""
         n = self.arrays.n_vars
         br = np.zeros((n_branches, n), np.uint8) if branches else None
         pv = np.zeros(n, np.float32)
@@ -369,8 +372,9 @@ class Problem:
         cert = np.zeros(n, np.float32)
         en = np.zeros(n_branches, np.float64)
         rep = Report()
-        _check(lib().qb_anneal_branches_reference(self.h, C.byref(sched), n_branches, C.c_uint64(seed_offset), _p(br),
-                                                  _p(pv), _p(tri), _p(cert), _p(en), C.byref(rep)))
+        _check(lib().qb_anneal_branches_reference_split(self.h, C.byref(sched), n_branches, C.c_uint64(seed_offset),
+                                                        int(sweeps_per_launch), _p(br), _p(pv), _p(tri), _p(cert),
+                                                        _p(en), C.byref(rep)))
         return dict(branches=br, p_void=pv, tri=tri, certainty=cert, energy=en,
                     report={f: getattr(rep, f) for f, _ in Report._fields_})
 
@@ -590,17 +594,23 @@ def random_regular_qubo(n, degree, seed, replicas=8):
 
 
 def qubo_batch_workload(n_instances=256, seed=2026, n_range=(1000, 65536), degrees=(3, 8, 32, 64),
-                        replicas=(8, 32)):
+                        replicas=(8, 32), threads=8):
     """The batched-QUBO workload: n log-uniform in n_range, degree drawn from `degrees`,
-    replicas uniform in `replicas`, every instance from its own seed (derived from `seed`)."""
+    replicas uniform in `replicas`, every instance from its own seed (derived from `seed`).
+    The specs are drawn in order from one generator, then the instances are generated on
+    `threads` host threads; each depends only on its spec, so the batch is the same for
+    any thread count."""
+    from concurrent.futures import ThreadPoolExecutor
+
     rng = np.random.default_rng(seed)
-    out = []
+    specs = []
     for k in range(n_instances):
         n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
         d = int(rng.choice(degrees))
         r = int(rng.integers(replicas[0], replicas[1] + 1))
-        out.append(random_regular_qubo(n, d, int(rng.integers(1, 2**62)), r))
-    return out
+        specs.append((n, d, int(rng.integers(1, 2**62)), r))
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(lambda s: random_regular_qubo(*s), specs))
 
 
 class QuboBatch:

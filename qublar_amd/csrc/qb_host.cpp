@@ -309,7 +309,8 @@ struct qb_problem {
     };
     std::map<int, std::unique_ptr<Batches>> batches;   // by group width and team size
     // workspace
-    DevBuf w_x, w_r, w_beta, w_bar, w_err, w_part, w_pv, w_tri, w_cert, w_br, w_data, w_prior, w_rerr, w_X, w_rref;
+    DevBuf w_x, w_r, w_beta, w_bar, w_err, w_part, w_pv, w_tri, w_cert, w_br, w_data, w_prior, w_rerr, w_X, w_rref,
+        w_rng;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~qb_problem() {
         if (ev0) (void)hipEventDestroy(ev0);
@@ -928,16 +929,33 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     return QB_OK;
 }
 
+// Proposals one launch of the reference float trajectory makes at most when the caller
+// leaves the split to the library (about a second at a few branches of the muon QUBO).
+constexpr double kRefFloatUpdatesPerLaunch = 33554432.0;
+
 extern "C" int qb_anneal_branches_reference(qb_problem* P, const qb_schedule* s, int n_branches, uint64_t seed_offset,
                                            uint8_t* branches, float* p_void, uint8_t* tri, float* certainty,
                                            double* energy, qb_report* rep) {
+    return qb_anneal_branches_reference_split(P, s, n_branches, seed_offset, 0, branches, p_void, tri, certainty,
+                                              energy, rep);
+}
+
+extern "C" int qb_anneal_branches_reference_split(qb_problem* P, const qb_schedule* s, int n_branches,
+                                                 uint64_t seed_offset, int sweeps_per_launch, uint8_t* branches,
+                                                 float* p_void, uint8_t* tri, float* certainty, double* energy,
+                                                 qb_report* rep) {
     if (!P) return fail(QB_E_ARG, "qb_anneal_branches_reference: null problem");
     int rc = check_schedule(s);
     if (rc) return rc;
     const int R = n_branches;
     if (R <= 0) return fail(QB_E_ARG, "n_branches must be positive (got %d)", R);
+    if (sweeps_per_launch < 0) return fail(QB_E_ARG, "sweeps_per_launch must be >= 0 (got %d)", sweeps_per_launch);
     const int total = s->anneal_sweeps + s->hold_sweeps;
     const int n = P->n, nr = P->n_rays;
+    int per_launch = sweeps_per_launch;
+    if (per_launch == 0)
+        per_launch = int(std::clamp(kRefFloatUpdatesPerLaunch / (double(R) * double(std::max(n, 1))), 1.0,
+                                    double(std::max(total, 1))));
     const int n_groups = (R + kGroup - 1) / kGroup;
     const int n_part = ray_energy_blocks(P->dev, kGroup);
     if ((rc = P->w_X.reserve(size_t(R) * size_t(std::max(n, 1)))) ||
@@ -948,7 +966,8 @@ extern "C" int qb_anneal_branches_reference(qb_problem* P, const qb_schedule* s,
         (rc = P->w_pv.reserve(sizeof(float) * size_t(std::max(n, 1)))) || (rc = P->w_tri.reserve(size_t(std::max(n, 1)))) ||
         (rc = P->w_cert.reserve(sizeof(float) * size_t(std::max(n, 1)))) ||
         (rc = P->w_br.reserve(branches ? size_t(R) * size_t(std::max(n, 1)) : 1)) ||
-        (rc = P->w_data.reserve(sizeof(double) * size_t(R))) || (rc = P->w_prior.reserve(sizeof(double) * size_t(R))))
+        (rc = P->w_data.reserve(sizeof(double) * size_t(R))) || (rc = P->w_prior.reserve(sizeof(double) * size_t(R))) ||
+        (rc = P->w_rng.reserve(sizeof(uint64_t) * size_t(R))))
         return rc;
     // the temperature of every sweep by the reference's expression (src/ising_recon.hpp:128-130)
     std::vector<double> temp(static_cast<size_t>(total));
@@ -961,8 +980,10 @@ extern "C" int qb_anneal_branches_reference(qb_problem* P, const qb_schedule* s,
     const auto t0 = std::chrono::steady_clock::now();
     QB_HIP(hipMemcpyAsync(P->w_beta.p, temp.data(), sizeof(double) * temp.size(), hipMemcpyHostToDevice, st));
     QB_HIP(hipEventRecord(P->ev0, st));
-    QB_HIP(launch_ray_ref(P->dev, R, seed_offset, P->w_beta.as<double>(), total, P->w_X.as<uint8_t>(),
-                          P->w_rref.as<double>(), st));
+    int launches = 0;
+    for (int s0 = 0; s0 < total || launches == 0; s0 += per_launch, ++launches)
+        QB_HIP(launch_ray_ref(P->dev, R, seed_offset, P->w_beta.as<double>(), s0, std::min(total, s0 + per_launch),
+                              P->w_X.as<uint8_t>(), P->w_rref.as<double>(), P->w_rng.as<uint64_t>(), st));
     QB_HIP(hipEventRecord(P->ev1, st));
     // the branches in the group masks of the read-out and energy kernels
     if (n > 0) QB_HIP(launch_pack_states(P->w_X.as<uint8_t>(), R, n, P->dev.pos_bit, P->w_x.as<uint16_t>(), st));
@@ -991,7 +1012,7 @@ extern "C" int qb_anneal_branches_reference(qb_problem* P, const qb_schedule* s,
         QB_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
         rep->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         rep->kernel_ms = ms;
-        rep->launches = 1;
+        rep->launches = launches;
         rep->spin_updates = double(R) * double(total) * double(n);
         rep->fp32_residual_err = 0.0;
     }
@@ -1153,6 +1174,28 @@ static int validate_graph(const qb_graph_desc* d) {
             return fail(QB_E_ARG, "edge %lld names a vertex out of range", (long long)e);
         if (d->eu[e] == d->ev[e]) return fail(QB_E_ARG, "edge %lld is a self-loop", (long long)e);
     }
+    // A repeated edge (either orientation) would list a neighbour twice in one adjacency row,
+    // and the kernels update each neighbour of a flipped vertex once per listing, from
+    // different lanes. Bucket the edges by their lower endpoint, then stamp each bucket's
+    // upper endpoints.
+    std::vector<int64_t> start(size_t(d->n) + 1, 0);
+    for (int64_t e = 0; e < d->m; ++e) ++start[size_t(std::min(d->eu[e], d->ev[e])) + 1];
+    for (int i = 0; i < d->n; ++i) start[size_t(i) + 1] += start[size_t(i)];
+    std::vector<int32_t> upper(size_t(d->m)), edge_of(size_t(d->m));
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int64_t e = 0; e < d->m; ++e) {
+        const size_t at = size_t(fill[size_t(std::min(d->eu[e], d->ev[e]))]++);
+        upper[at] = std::max(d->eu[e], d->ev[e]);
+        edge_of[at] = int32_t(e);
+    }
+    std::vector<int32_t> stamp(size_t(d->n), -1);
+    for (int i = 0; i < d->n; ++i)
+        for (int64_t k = start[size_t(i)]; k < start[size_t(i) + 1]; ++k) {
+            if (stamp[size_t(upper[size_t(k)])] == i)
+                return fail(QB_E_ARG, "edge %d repeats an earlier edge between vertices %d and %d", edge_of[size_t(k)],
+                            i, upper[size_t(k)]);
+            stamp[size_t(upper[size_t(k)])] = i;
+        }
     return QB_OK;
 }
 

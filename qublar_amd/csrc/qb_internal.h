@@ -200,9 +200,10 @@ hipError_t launch_freeze_bound(const DevProblem& p, double* lb, hipStream_t st);
 hipError_t launch_freeze_round(const DevProblem& p, const double* lb, double margin, const uint8_t* frozen,
                                uint8_t* next, unsigned* changed, hipStream_t st);
 // The reference's own float trajectory (anneal_branch): branch b's spins xs[b][bit order]
-// (bytes) and residuals rs[b][ray] (FP64) in global memory, all sweeps in one launch.
-hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, const double* temp, int n_sweeps,
-                          uint8_t* xs, double* rs, hipStream_t st);
+// (bytes), residuals rs[b][ray] (FP64) and splitmix64 state sm[b] in global memory; one
+// launch runs sweeps [sweep_begin, sweep_end), the first starting from x = 0.
+hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, const double* temp, int sweep_begin,
+                          int sweep_end, uint8_t* xs, double* rs, uint64_t* sm, hipStream_t st);
 // The reference's own integer trajectory (wising::anneal), one wave per replica with its
 // state in LDS (n_pad * 5 bytes; four replicas per workgroup while that fits): temp[sweep]
 // is the temperature, rng[r] the replica's splitmix64 state between launches.

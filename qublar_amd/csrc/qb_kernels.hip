@@ -1520,21 +1520,30 @@ __device__ __forceinline__ bool ref_accept_f(uint64_t u53, double dE, double T) 
     return u < qexp(x);
 }
 
+// A launch runs sweeps [sweep_begin, sweep_end); the first one starts every branch from
+// x = 0, r = -d and its seed's splitmix64 state, a later one continues from the x, r and
+// state (sm[b]) the previous launch left, so the split never changes the trajectory.
 __global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint64_t seed_offset, const double* temp,
-                                                       int n_sweeps, uint8_t* xs, double* rs) {
+                                                       int sweep_begin, int sweep_end, uint8_t* xs, double* rs,
+                                                       uint64_t* sm) {
     const int w = int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
     const int b = int(blockIdx.x) * 4 + w;
     if (b >= R) return;   // the whole wave leaves; waves never sync with each other
     const int n = p.n;
     uint8_t* x = xs + size_t(b) * size_t(n);
     double* r = rs + size_t(b) * size_t(p.n_rays);
-    for (int i = lane; i < n; i += 64) x[i] = 0;
-    for (int k = lane; k < p.n_rays; k += 64) r[k] = -p.d[k];
-    uint64_t st = (uint64_t(b) + 1u + seed_offset) * 0x2545F4914F6CDD1Dull + 1;
+    uint64_t st;
+    if (sweep_begin == 0) {
+        for (int i = lane; i < n; i += 64) x[i] = 0;
+        for (int k = lane; k < p.n_rays; k += 64) r[k] = -p.d[k];
+        st = (uint64_t(b) + 1u + seed_offset) * 0x2545F4914F6CDD1Dull + 1;
+    } else {
+        st = sm[b];
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const uint64_t below = (uint64_t(1) << lane) - 1u;
-    for (int sweep = 0; sweep < n_sweeps; ++sweep) {
+    for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         const double T = temp[sweep];
         int i0 = 0;
         while (i0 < n) {
@@ -1638,11 +1647,13 @@ __global__ void __launch_bounds__(256) ray_ref_kernel(DevProblem p, int R, uint6
             }
         }
     }
+    if (lane == 0) sm[b] = st;
 }
 
-hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, const double* temp, int n_sweeps,
-                          uint8_t* xs, double* rs, hipStream_t st) {
-    ray_ref_kernel<<<unsigned((R + 3) / 4), 256, 0, st>>>(p, R, seed_offset, temp, n_sweeps, xs, rs);
+hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, const double* temp, int sweep_begin,
+                          int sweep_end, uint8_t* xs, double* rs, uint64_t* sm, hipStream_t st) {
+    ray_ref_kernel<<<unsigned((R + 3) / 4), 256, 0, st>>>(p, R, seed_offset, temp, sweep_begin, sweep_end, xs, rs,
+                                                          sm);
     return hipGetLastError();
 }
 

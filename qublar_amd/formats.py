@@ -165,15 +165,17 @@ def cloud_u8(n_voxels, var_voxel, p):
 
 
 def write_cloud(path, shape, voxel, lo, var_voxel, p, truth_voxels=()):
-    """The ledger's cloud and its JSON companion (src/check_ising.cu:596-606)."""
+    """The ledger's cloud and its JSON companion (src/check_ising.cu:596-606). The JSON is
+    the reference's stream output byte for byte: keys in its order, numbers as a default
+    ostream prints them (voxel 2 -> "2", lo -128 -> "-128"), no spaces. A non-cubic grid
+    adds "ny" and "nz" after "nx" (the reference's grid is cubic)."""
     nx, ny, nz = shape
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     cloud_u8(nx * ny * nz, var_voxel, p).tofile(path)
-    meta = {"nx": nx, "voxel_m": voxel, "lo_m": list(lo),
-            "encoding": "u8: 255 outside the domain, round(254 p_void) inside",
-            "classify": {"exists_max": CLASSIFY_LO, "not_there_min": CLASSIFY_HI},
-            "truth_voxels": [int(v) for v in truth_voxels]}
-    if not (nx == ny == nz):
-        meta.update(ny=ny, nz=nz)
+    dims = f'"nx":{nx}' + ("" if nx == ny == nz else f',"ny":{ny},"nz":{nz}')
+    text = ("{" + dims + f',"voxel_m":{_g6(voxel)},"lo_m":[' + ",".join(_g6(v) for v in lo) + "],"
+            '"encoding":"u8: 255 outside the domain, round(254 p_void) inside",'
+            f'"classify":{{"exists_max":{_g6(CLASSIFY_LO)},"not_there_min":{_g6(CLASSIFY_HI)}}},'
+            '"truth_voxels":[' + ",".join(str(int(v)) for v in truth_voxels) + "]}\n")
     with open(os.path.splitext(path)[0] + ".json", "w") as f:
-        f.write(json.dumps(meta, separators=(",", ":")) + "\n")
+        f.write(text)

@@ -399,6 +399,33 @@ static int muon_dump(int nx, int ny, int nz, uint64_t seed, int bins) {
 }
 
 int main(int argc, char** argv) {
+    if (argc >= 2 && std::strcmp(argv[1], "--rudy-digest") == 0) {
+        // one line per file: "refused" or "ok <FNV-1a 64 of the parsed graph>"
+        for (int f = 2; f < argc; ++f) {
+            std::vector<char> buf;
+            wi::WeightedGraph g;
+            std::string err;
+            if (!wi::read_file(argv[f], buf) || !wi::parse_rudy(buf.data(), buf.size(), g, err)) {
+                std::printf("refused\n");
+                continue;
+            }
+            uint64_t h = 1469598103934665603ull;
+            auto mix = [&h](const void* p, size_t bytes) {
+                const unsigned char* c = static_cast<const unsigned char*>(p);
+                for (size_t i = 0; i < bytes; ++i) h = (h ^ c[i]) * 1099511628211ull;
+            };
+            const int64_t head[6] = {g.n, g.m, g.sum_w, g.n_pos, g.n_neg, g.n_other};
+            mix(head, sizeof(head));
+            mix(g.eu.data(), 4 * g.eu.size());
+            mix(g.ev.data(), 4 * g.ev.size());
+            mix(g.ew.data(), 4 * g.ew.size());
+            mix(g.row.data(), 8 * g.row.size());
+            mix(g.col.data(), 4 * g.col.size());
+            mix(g.w.data(), 4 * g.w.size());
+            std::printf("ok %llu\n", static_cast<unsigned long long>(h));
+        }
+        return 0;
+    }
     if (argc == 7 && std::strcmp(argv[1], "--muon") == 0)
         return muon_dump(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
                          std::strtoull(argv[5], nullptr, 10), std::atoi(argv[6]));

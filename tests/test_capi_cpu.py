@@ -1,5 +1,6 @@
 """CPU checks of libqublar_hip.so that need no GPU: the library loads and exports every
 function include/qublar.h declares; the host problem builders and the colourings."""
+import ctypes as C
 import os
 import re
 
@@ -145,3 +146,17 @@ def test_graph_colouring_is_proper():
     col = colours_of(order, cp, g.n)
     assert np.all(col[g.eu] != col[g.ev])
     assert len(cp) - 1 < 40
+
+
+@pytest.mark.parametrize("eu,ev", [([0, 1, 2], [1, 2, 1]), ([0, 1, 0], [1, 2, 1]), ([0, 3, 2, 3], [3, 0, 1, 1])])
+def test_graph_refuses_repeated_edges(eu, ev):
+    """An edge listed twice, in either orientation, is refused (src/ising_weighted.hpp:124-133
+    refuses it in the loader): the kernels would update the repeated neighbour twice."""
+    g = qa.GraphArrays(4, eu, ev, np.ones(len(eu), np.int32))
+    order = np.zeros(4, np.int32)
+    cp = np.zeros(5, np.int32)
+    rc = qa.lib().qb_colour_graph(C.byref(g.desc()), order.ctypes.data, cp.ctypes.data, C.byref(C.c_int()))
+    assert rc == qa.QB_E_ARG
+    assert "repeats an earlier edge" in qa.lib().qb_last_error().decode()
+    ok = qa.GraphArrays(4, [0, 1, 2], [1, 2, 3], np.ones(3, np.int32))
+    assert qa.lib().qb_colour_graph(C.byref(ok.desc()), order.ctypes.data, cp.ctypes.data, C.byref(C.c_int())) == 0

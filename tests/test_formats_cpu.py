@@ -100,3 +100,13 @@ def test_cloud_u8_rounding_and_json(tmp_path):
     meta = json.loads((tmp_path / "ledger" / "cloud.json").read_text())
     assert meta["encoding"] == "u8: 255 outside the domain, round(254 p_void) inside"
     assert meta["classify"] == {"exists_max": 0.1, "not_there_min": 0.9} and meta["truth_voxels"] == [3, 4]
+
+
+def test_cloud_json_bytes_for_the_reference_scene(tmp_path):
+    """The reference scene's companion (kN 128, voxel 2.0f, lo (-128, -128, -10); check_ising.cu:
+    597-604) byte for byte: ostream prints the floats 2 and -128 without a decimal point."""
+    fm.write_cloud(str(tmp_path / "cloud_pyramid.u8"), (128, 128, 128), 2.0, (-128.0, -128.0, -10.0), np.arange(3),
+                   np.zeros(3), [7, 4099])
+    want = ('{"nx":128,"voxel_m":2,"lo_m":[-128,-128,-10],"encoding":"u8: 255 outside the domain, '
+            'round(254 p_void) inside","classify":{"exists_max":0.1,"not_there_min":0.9},"truth_voxels":[7,4099]}\n')
+    assert (tmp_path / "cloud_pyramid.json").read_text() == want

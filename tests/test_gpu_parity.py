@@ -338,13 +338,14 @@ def test_int_reference_trajectory_reproduces_published_k16_cuts(gpu):
 
 # ---- the reference's own float trajectory -------------------------------------------------
 
-def check_branches_reference(P, sched, R, seed_offset=0):
+def check_branches_reference(P, sched, R, seed_offset=0, sweeps_per_launch=0):
     """qb_anneal_branches_reference against the oracle's reference mode (anneal_branch:
     index order, splitmix64 on uphill proposals, local_field_dE's order), branch by branch,
     with p_void, tri-state and FP64 energies."""
     prob = qa.Problem(P)
-    say(f"anneal_branches_reference n={P.n_vars} rays={P.n_rays} R={R} offset={seed_offset}")
-    out = prob.anneal_branches_reference(sched, R, seed_offset)
+    say(f"anneal_branches_reference n={P.n_vars} rays={P.n_rays} R={R} offset={seed_offset} "
+        f"sweeps/launch={sweeps_per_launch}")
+    out = prob.anneal_branches_reference(sched, R, seed_offset, sweeps_per_launch=sweeps_per_launch)
     op = oracle_problem(P)
     pv, br = orc_mod.branch_fractions(op, osched(sched), R, 16, seed_offset)
     diff = (out["branches"] != br).sum(axis=1)
@@ -362,6 +363,19 @@ def test_float_reference_trajectory_muon(gpu):
     out = check_branches_reference(small_muon(), qa.schedule(50.0, 1.0, 30, 10), 16)
     assert 0 < out["branches"].sum() < out["branches"].size
     say(f"kernel_ms {out['report']['kernel_ms']:.2f}")
+
+
+def test_float_reference_trajectory_split_launches(gpu):
+    """Launches of 3 and of 7 sweeps continue from the spins, residuals and splitmix64 state
+    the last launch left: every branch is still anneal_branch's, and the same as one launch."""
+    P = small_muon(grid=24, sky_az=60, sky_th=15)
+    sched = qa.schedule(30.0, 1.0, 16, 4)
+    one = check_branches_reference(P, sched, 9, seed_offset=2, sweeps_per_launch=20)
+    assert one["report"]["launches"] == 1
+    for per in (3, 7):
+        out = check_branches_reference(P, sched, 9, seed_offset=2, sweeps_per_launch=per)
+        assert out["report"]["launches"] == (20 + per - 1) // per
+        assert np.array_equal(out["branches"], one["branches"])
 
 
 def test_float_reference_trajectory_fields_duplicates_and_offset(gpu):

@@ -8,7 +8,10 @@ reproduces), which has no counterpart in the reference, by invariants:
 * a from-scratch recompute after the last sweep returns float(scratch residuals);
 * local_field_dE = E(flip) - E(x) (check_local_field.cu:362-378);
 * the ROI QUBO expansion is exact (check_ising.cu:536-559);
-* frozen bits satisfy the freeze bound in every sampled context.
+* frozen bits satisfy the freeze bound in every sampled context;
+* where every ray is private to one bit the bound is the exact minimum of dE(0 -> 1)
+  over contexts (brute force over all of them), and the frozen set equals the one built
+  round by round from those minima, including a case that needs more than one round.
 """
 import numpy as np
 import pytest
@@ -223,6 +226,112 @@ def test_freeze_bound_holds(orc):
         r = scratch_residuals(prob, x)
         for i in np.flatnonzero(frozen):
             assert orc.lib().orc_local_field_dE(prob.c, x, r, int(i)) >= 1.0 - 1e-9
+
+
+def private_ray_problem(orc, n, seed, lam=1.0, edges=None):
+    """n bits, each on 1-3 rays no other bit crosses, a per-bit field, and a random
+    neighbour graph (or `edges`). With private rays a bit's data term does not depend on
+    the other bits, so freeze_provable_rock's bound (src/ising_recon.hpp:189-202) is the
+    exact minimum of dE(0 -> 1) over every context that holds the frozen bits at 0."""
+    rng = np.random.default_rng(seed)
+    row_ptr, ray, a, d, w = [0], [], [], [], []
+    for i in range(n):
+        for _ in range(int(rng.integers(1, 4))):
+            ray.append(len(d))
+            a.append(rng.uniform(0.05, 1.5))
+            d.append(rng.uniform(0.0, 1.2))
+            w.append(rng.uniform(0.5, 4.0))
+        row_ptr.append(len(ray))
+    if edges is None:
+        edges = {(int(u), int(v)) for u, v in rng.integers(0, n, (2 * n, 2)) if u != v}
+        edges = {(min(u, v), max(u, v)) for u, v in edges}
+    adj = [[] for _ in range(n)]
+    for u, v in sorted(edges):
+        adj[u].append(v)
+        adj[v].append(u)
+    nbr_ptr = np.concatenate([[0], np.cumsum([len(x) for x in adj])])
+    nbr = [j for x in adj for j in x]
+    field = rng.uniform(-1.0, 6.0, n)
+    return orc.FloatProblem(row_ptr, ray, a, d, w, nbr_ptr, nbr, lam, 0.0, field)
+
+
+def brute_min_dE(orc, prob, frozen):
+    """min over contexts x (frozen bits 0, the bit itself 0, every other bit free) of
+    E(x + e_i) - E(x), from binary_energy alone."""
+    n = prob.n_vars
+    free = np.flatnonzero(frozen == 0)
+    best = np.full(n, np.inf)
+    for mask in range(1 << len(free)):
+        x = np.zeros(n, np.uint8)
+        x[free] = (mask >> np.arange(len(free))) & 1
+        e0 = None
+        for i in range(n):
+            if x[i]:
+                continue
+            if e0 is None:
+                e0 = orc.binary_energy(prob, x)
+            y = x.copy()
+            y[i] = 1
+            best[i] = min(best[i], orc.binary_energy(prob, y) - e0)
+    return best
+
+
+def fixpoint_from_minima(orc, prob, margin):
+    """The frozen set built round by round from brute-force minima: a bit freezes once its
+    minimum dE over contexts consistent with the frozen set reaches the margin."""
+    frozen = np.zeros(prob.n_vars, np.uint8)
+    sets = []
+    while True:
+        m = brute_min_dE(orc, prob, frozen)
+        new = (frozen == 0) & (m >= margin)
+        if not new.any():
+            return frozen, sets
+        frozen = frozen | new.astype(np.uint8)
+        sets.append(frozen.copy())
+
+
+@pytest.mark.parametrize("seed,n,margin", [(1, 9, 0.5), (2, 11, 1.0), (3, 12, 2.0), (4, 10, 0.0)])
+def test_freeze_bound_is_exact_with_private_rays(orc, seed, n, margin):
+    """Both sides of the freeze: with private rays the oracle's bound equals the brute-force
+    minimum of dE(0 -> 1) (a dropped lin term, a lost data term or lambda (rock - free)
+    written as -lambda free all break the equality), and its frozen set equals the one built
+    from those minima round by round (so it neither over- nor under-freezes)."""
+    prob = private_ray_problem(orc, n, seed)
+    got = np.zeros(n, np.uint8)
+    rounds = orc.lib().orc_freeze_mask(prob.c, margin, got)
+    want, sets = fixpoint_from_minima(orc, prob, margin)
+    assert np.array_equal(got, want), (got, want)
+    assert rounds >= 1
+    for frozen in [np.zeros(n, np.uint8)] + sets:
+        bound = np.zeros(n, np.float64)
+        orc.lib().orc_freeze_bounds(prob.c, frozen, bound)
+        m = brute_min_dE(orc, prob, frozen)
+        live = frozen == 0
+        assert np.allclose(bound[live], m[live], rtol=1e-12, atol=1e-12), (bound[live], m[live])
+    assert 0 < want.sum() < n or margin == 0.0
+
+
+def test_freeze_needs_more_than_one_round(orc):
+    """A bit whose bound reaches the margin only once a later-indexed neighbour is frozen:
+    one in-order round freezes the neighbour alone, the fixpoint freezes both, and the
+    oracle runs the extra rounds (the last freezing nothing)."""
+    prob = private_ray_problem(orc, 4, 7, lam=1.5, edges={(0, 1), (2, 3)})
+    fld = prob.field.copy()
+    bound0 = np.zeros(4, np.float64)
+    orc.lib().orc_freeze_bounds(prob.c, np.zeros(4, np.uint8), bound0)
+    data = bound0 - fld + 1.5     # each bit has one free neighbour at the start
+    margin = 2.0
+    fld[1] = margin + 1.5 + 1.0 - data[1]      # bit 1 clears the margin with its neighbour free
+    fld[0] = margin - data[0]                  # bit 0 only with bit 1 frozen: lb - 1.5 < margin <= lb + 1.5
+    fld[2] = fld[3] = -5.0
+    prob = orc.FloatProblem(prob.row_ptr, prob.ray, prob.a, prob.d, prob.w, prob.nbr_ptr, prob.nbr, 1.5, 0.0, fld)
+    got = np.zeros(4, np.uint8)
+    rounds = orc.lib().orc_freeze_mask(prob.c, margin, got)
+    want, sets = fixpoint_from_minima(orc, prob, margin)
+    assert sets[0].tolist() == [0, 1, 0, 0]
+    assert want.tolist() == [1, 1, 0, 0]
+    assert np.array_equal(got, want)
+    assert rounds == 3
 
 
 def long_column_problem(orc, exact, seed=21, n=40, n_rays=120, lengths=(20, 70)):
