@@ -5,10 +5,13 @@
 // Plan/make_plan, scaled_budget and temperatures keep the reference's rules.
 // DeviceGraph owns the graph on the device; run_batch anneals replicas with seeds
 // seed_base .. seed_base + R - 1 in the reference's own trajectory (one wave per replica,
-// its spins and fields in LDS: vertices in index order, one splitmix64 stream drawn on
-// uphill proposals), so replica r is wising::anneal(g, b, seed_base + r), as the
-// reference's run_batch is; device_energies is the independent edge-list energy. Engine errors throw wising::EngineError (the
-// reference printed them and called exit(3), src/ising_gpu.cuh:44-52).
+// its spins and fields in LDS, or in global memory for graphs past 32k vertices: vertices
+// in index order, one splitmix64 stream drawn on uphill proposals), so replica r is
+// wising::anneal(g, b, seed_base + r), as the reference's run_batch is; wising::anneal
+// itself stays the host annealer, the independent side of that comparison.
+// device_energies is the independent edge-list energy. Engine errors throw
+// wising::EngineError (the reference printed them and called exit(3),
+// src/ising_gpu.cuh:44-52).
 #pragma once
 
 #include <chrono>

@@ -17,7 +17,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <limits>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -172,12 +175,83 @@ private:
     qb_problem* h_ = nullptr;
 };
 
+namespace engine_detail {
+// A 64-bit digest of every array and scalar of a problem (a multiply-xorshift over 8-byte
+// words), so a problem passed again, at any address, is recognised by its content.
+inline uint64_t digest(const BinaryProblem& p) {
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+    auto mix = [&h](const void* data, size_t bytes) {
+        const unsigned char* c = static_cast<const unsigned char*>(data);
+        size_t i = 0;
+        for (; i + 8 <= bytes; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, c + i, 8);
+            h = (h ^ w) * 0xD6E8FEB86659FD93ull;
+            h ^= h >> 32;
+        }
+        uint64_t tail = bytes;
+        for (; i < bytes; ++i) tail = (tail << 8) | c[i];
+        h = (h ^ tail) * 0xD6E8FEB86659FD93ull;
+        h ^= h >> 29;
+    };
+    const int64_t sizes[6] = {p.n_vars(), p.n_rays(), int64_t(p.ray.size()), int64_t(p.nbr.size()),
+                              int64_t(p.field.size()), int64_t(p.a.size())};
+    mix(sizes, sizeof(sizes));
+    mix(&p.lambda, sizeof(double));
+    mix(&p.kappa, sizeof(double));
+    mix(p.row_ptr.data(), 4 * p.row_ptr.size());
+    mix(p.ray.data(), 4 * p.ray.size());
+    mix(p.a.data(), 4 * p.a.size());
+    mix(p.d.data(), 8 * p.d.size());
+    mix(p.w.data(), 8 * p.w.size());
+    mix(p.nbr_ptr.data(), 4 * p.nbr_ptr.size());
+    mix(p.nbr.data(), 4 * p.nbr.size());
+    mix(p.field.data(), 8 * p.field.size());
+    return h;
+}
+
+// A problem resident on the device, with the lock its anneals take (one caller at a time
+// per handle).
+struct Resident {
+    std::mutex busy;
+    DeviceProblem dev;
+    explicit Resident(const BinaryProblem& p) : dev(p) {}
+};
+
+// The last kResidentProblems problems annealed through anneal_branch / branch_fractions stay
+// resident, so calling again on the same problem (check_ising.cu calls branch_fractions six
+// times) skips the colouring and the upload. The cache is never destroyed: its handles are
+// released with the process rather than in static destructors that run after the HIP
+// runtime may already be gone.
+constexpr size_t kResidentProblems = 4;
+
+inline std::shared_ptr<Resident> resident(const BinaryProblem& p) {
+    static std::mutex lock;
+    static auto* cache = new std::vector<std::pair<uint64_t, std::shared_ptr<Resident>>>();
+    const uint64_t key = digest(p);
+    std::lock_guard<std::mutex> hold(lock);
+    for (size_t k = 0; k < cache->size(); ++k)
+        if ((*cache)[k].first == key) {
+            auto hit = (*cache)[k];
+            cache->erase(cache->begin() + static_cast<std::ptrdiff_t>(k));
+            cache->push_back(hit);   // most recent last
+            return hit.second;
+        }
+    auto made = std::make_shared<Resident>(p);
+    if (cache->size() == kResidentProblems) cache->erase(cache->begin());
+    cache->emplace_back(key, made);
+    return made;
+}
+}  // namespace engine_detail
+
 // One branch from all-zero x (src/ising_recon.hpp:121-144), on the GPU.
 inline std::vector<uint8_t> anneal_branch(const BinaryProblem& p, const Schedule& s, uint64_t seed) {
     std::vector<std::vector<uint8_t>> br;
+    const auto r = engine_detail::resident(p);
+    std::lock_guard<std::mutex> hold(r->busy);
     // branch 0 has seed 0 + 1 + (seed - 1)
-    DeviceProblem(p).anneal(s, 1, seed - 1, &br, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                            QUBLAR_REFERENCE_TRAJECTORY != 0);
+    r->dev.anneal(s, 1, seed - 1, &br, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                  QUBLAR_REFERENCE_TRAJECTORY != 0);
     return br.empty() ? std::vector<uint8_t>() : std::move(br[0]);
 }
 
@@ -188,8 +262,10 @@ inline std::vector<float> branch_fractions(const BinaryProblem& p, const Schedul
                                            uint64_t seed_offset = 0) {
     (void)n_threads;
     std::vector<float> frac;
-    DeviceProblem(p).anneal(s, n_branches, seed_offset, keep, &frac, nullptr, nullptr, nullptr, nullptr, nullptr,
-                            QUBLAR_REFERENCE_TRAJECTORY != 0);
+    const auto r = engine_detail::resident(p);
+    std::lock_guard<std::mutex> hold(r->busy);
+    r->dev.anneal(s, n_branches, seed_offset, keep, &frac, nullptr, nullptr, nullptr, nullptr, nullptr,
+                  QUBLAR_REFERENCE_TRAJECTORY != 0);
     if (p.n_vars() == 0) frac.clear();
     return frac;
 }

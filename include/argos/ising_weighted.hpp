@@ -4,11 +4,10 @@
 //   x in {0,1}^n, s = 2x - 1, cut(x) = sum_edges w_uv [x_u != x_v], E = -cut
 //   f_i = sum_j w_ij s_j, dE_i = -s_i f_i
 //
-// anneal() runs on the GPU through libqublar_hip.so (one replica; run_batch in
-// argos/ising_gpu.cuh runs many) in the reference's trajectory: vertices in index order,
-// one splitmix64 stream drawn on uphill proposals (src/ising_weighted.hpp:250-267), so
-// its assignment and tracked energy are the reference's. The loader, exact energies,
-// State, TTS99 and SHA-256 are host code with the reference's behaviour.
+// anneal() is the host annealer, as the reference's (src/ising_weighted.hpp:250-267):
+// the device engine is gising::run_batch (argos/ising_gpu.cuh), and the reference's
+// harnesses check one against the other. The loader, exact energies, State, TTS99 and
+// SHA-256 are host code with the reference's behaviour.
 #pragma once
 
 #include <algorithm>
@@ -286,22 +285,30 @@ inline qb_graph* upload_graph(const WeightedGraph& g, bool force_global) {
     return h;
 }
 
-// One replica with seed `seed` on the GPU (src/ising_weighted.hpp:250-267).
+// One replica with seed `seed` (src/ising_weighted.hpp:250-267), on the host, as the
+// reference's: the CPU annealer is the independent trajectory the reference's harnesses
+// compare the device engine with (gpu_anneal's run_batch == anneal check, runs_cpu.csv,
+// agreement.csv), so it is not routed to the device. Vertices in index order; one
+// splitmix64 stream seeded seed * 0x2545F4914F6CDD1D + 1 is drawn only for uphill
+// proposals; a flip iff dE <= 0 or u < exp(-dE / T); T(s) is the schedule's temperature.
 inline RunOut anneal(const WeightedGraph& g, const Budget& b, uint64_t seed) {
-    qb_graph* h = upload_graph(g, false);
-    qb_plan plan{};
-    qb_graph_get_plan(h, &plan);
-    std::vector<int8_t> s(static_cast<size_t>(plan.n_pad));
-    int64_t e = 0;
-    const qb_schedule qs{b.t_hot, b.t_cold, b.anneal_sweeps, b.hold_sweeps};
-    const int rc = qb_run_batch_reference(h, &qs, seed, 1, 1e18, s.data(), &e, nullptr);
-    const std::string msg = rc == QB_OK ? "" : qb_last_error();
-    qb_graph_destroy(h);
-    if (rc != QB_OK) throw EngineError(rc, "qublar: " + msg);
+    const int sweeps = b.anneal_sweeps + b.hold_sweeps;
+    std::vector<double> temperature(static_cast<size_t>(std::max(sweeps, 0)));
+    const double span = static_cast<double>(std::max(1, b.anneal_sweeps - 1));
+    for (int s = 0; s < sweeps; ++s)
+        temperature[static_cast<size_t>(s)] =
+            s < b.anneal_sweeps ? b.t_hot * std::pow(b.t_cold / b.t_hot, std::min(1.0, double(s) / span)) : b.t_cold;
+    State state(g);
+    uint64_t draws = seed * 0x2545F4914F6CDD1Dull + 1;
+    for (const double t : temperature)
+        for (int i = 0; i < g.n; ++i) {
+            const int64_t d = state.dE(i);
+            if (d > 0 && !(detail::uniform(draws) < std::exp(-static_cast<double>(d) / t))) continue;
+            state.flip(i);
+        }
     RunOut out;
-    out.x.resize(static_cast<size_t>(g.n));
-    for (int i = 0; i < g.n; ++i) out.x[static_cast<size_t>(i)] = s[static_cast<size_t>(i)] > 0 ? 1u : 0u;
-    out.energy_tracked = e;
+    out.x = state.x();
+    out.energy_tracked = state.energy;
     return out;
 }
 

@@ -235,8 +235,8 @@ int qb_run_batch(qb_graph* g, const qb_schedule* b, uint64_t seed_base, int R, d
    for uphill proposals, a flip iff dE <= 0 or u < exp(-dE / T). Each replica's assignment and
    tracked energy are those of wising::anneal(g, b, seed_base + r), except where a draw falls
    within an ulp or two of exp(-dE / T) (the device's exponential against the host's libm).
-   Same outputs and launch split as qb_run_batch; n_pad * 5 bytes of a replica's state must fit
-   160 KB of LDS (QB_E_LIMIT otherwise). */
+   Same outputs and launch split as qb_run_batch. A replica's state (n_pad * 5 bytes) is held
+   in LDS up to 160 KB and in global memory beyond; the trajectory is the same either way. */
 int qb_run_batch_reference(qb_graph* g, const qb_schedule* b, uint64_t seed_base, int R, double updates_per_launch,
                            int8_t* s_out, int64_t* energy_out, qb_batch_report* rep);
 

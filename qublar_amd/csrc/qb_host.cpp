@@ -1348,9 +1348,6 @@ extern "C" int qb_run_batch_reference(qb_graph* G, const qb_schedule* b, uint64_
     if (rc) return rc;
     if (R <= 0) return fail(QB_E_ARG, "R must be positive (got %d)", R);
     if (G->plan.refused[0]) return fail(QB_E_REFUSED, "plan refused: %s", G->plan.refused);
-    if (int_ref_lds_bytes(G->dev) > kRefLdsLimit)
-        return fail(QB_E_LIMIT, "n = %d: one replica's spins and fields (5 bytes a bit) must fit %zu bytes of LDS",
-                    G->plan.n, kRefLdsLimit);
     const int total = b->anneal_sweeps + b->hold_sweeps;
     const int np = G->plan.n_pad, n = G->plan.n;
     const size_t cells = size_t(R) * size_t(np);

@@ -205,10 +205,12 @@ hipError_t launch_freeze_round(const DevProblem& p, const double* lb, double mar
 hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, const double* temp, int sweep_begin,
                           int sweep_end, uint8_t* xs, double* rs, uint64_t* sm, hipStream_t st);
 // The reference's own integer trajectory (wising::anneal), one wave per replica with its
-// state in LDS (n_pad * 5 bytes; four replicas per workgroup while that fits): temp[sweep]
-// is the temperature, rng[r] the replica's splitmix64 state between launches.
+// state in LDS (n_pad * 5 bytes; four replicas per workgroup while that fits), or in global
+// memory when one replica's state exceeds kRefLdsLimit: temp[sweep] is the temperature,
+// rng[r] the replica's splitmix64 state between launches.
 constexpr size_t kRefLdsLimit = 160 * 1024;
 size_t int_ref_lds_bytes(const DevGraph& g);
+bool int_ref_in_lds(const DevGraph& g);
 hipError_t launch_int_ref(const DevGraph& g, int R, uint64_t seed_base, const double* temp, int sweep_begin,
                           int sweep_end, int init, int8_t* s_state, int32_t* f_state, uint64_t* rng,
                           long long* energy, hipStream_t st);

@@ -1382,7 +1382,17 @@ __device__ __forceinline__ bool ref_accept(uint64_t u53, long long dE, double T,
     return u < qexp(-double(dE) / T);
 }
 
-template <int WAVES>
+// IN_LDS: a replica's spins and fields are copied into LDS for the launch (n_pad * 5 bytes
+// a replica); otherwise they are worked on where they lie in global memory, read through
+// the CU's L1, which a wave's own completed stores update (larger graphs).
+template <bool IN_LDS>
+__device__ __forceinline__ void int_ref_sync() {
+    if constexpr (IN_LDS) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int WAVES, bool IN_LDS>
 __global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, uint64_t seed_base, const double* temp,
                                                               int sweep_begin, int sweep_end, int init,
                                                               int8_t* s_state, int32_t* f_state, uint64_t* rng,
@@ -1392,10 +1402,10 @@ __global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, 
     const int r = int(blockIdx.x) * WAVES + w;
     if (r >= R) return;   // the whole wave leaves; waves never sync with each other
     const int n = g.n, np = g.n_pad;
-    int32_t* f = reinterpret_cast<int32_t*>(smem + size_t(w) * size_t(np) * 5);
-    int8_t* s = reinterpret_cast<int8_t*>(f + np);
     int8_t* gs = s_state + size_t(r) * size_t(np);
     int32_t* gf = f_state + size_t(r) * size_t(np);
+    int32_t* f = IN_LDS ? reinterpret_cast<int32_t*>(smem + size_t(w) * size_t(np) * 5) : gf;
+    int8_t* s = IN_LDS ? reinterpret_cast<int8_t*>(f + np) : gs;
     uint64_t st;
     long long e;
     if (init) {
@@ -1406,15 +1416,16 @@ __global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, 
         st = (seed_base + uint64_t(r)) * 0x2545F4914F6CDD1Dull + 1;
         e = 0;
     } else {
-        for (int p = lane; p < np; p += 64) {
-            s[p] = gs[p];
-            f[p] = gf[p];
+        if constexpr (IN_LDS) {
+            for (int p = lane; p < np; p += 64) {
+                s[p] = gs[p];
+                f[p] = gf[p];
+            }
         }
         st = rng[r];
         e = energy[r];
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    int_ref_sync<IN_LDS>();
     const uint64_t below = (uint64_t(1) << lane) - 1u;
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         const double T = temp[sweep];
@@ -1455,17 +1466,18 @@ __global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, 
                 const int si = s[pf];
                 for (int q = q0 + lane; q < q1; q += 64) f[g.nb[q]] -= 2 * g.nw[q] * si;
                 if (lane == 0) s[pf] = int8_t(-si);
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
+                int_ref_sync<IN_LDS>();
                 todo = ~upto;
                 const uint64_t m_dirty = __ballot(valid && f[p] != fp) & todo;
                 if (m_dirty) ok &= (uint64_t(1) << __builtin_ctzll(m_dirty)) - 1u;
             }
         }
     }
-    for (int p = lane; p < np; p += 64) {
-        gs[p] = s[p];
-        gf[p] = f[p];
+    if constexpr (IN_LDS) {
+        for (int p = lane; p < np; p += 64) {
+            gs[p] = s[p];
+            gf[p] = f[p];
+        }
     }
     if (lane == 0) {
         rng[r] = st;
@@ -1473,27 +1485,36 @@ __global__ void __launch_bounds__(64 * WAVES) int_ref_kernel(DevGraph g, int R, 
     }
 }
 
-// Replicas per workgroup: four while their states fit the LDS limit, else one.
+// Where a replica's state lives: LDS, four replicas per workgroup while their states fit
+// the LDS limit, else one; global memory beyond one replica's fit (four per workgroup).
 static int int_ref_waves(const DevGraph& g) { return size_t(4) * size_t(g.n_pad) * 5 <= kRefLdsLimit ? 4 : 1; }
 
-size_t int_ref_lds_bytes(const DevGraph& g) { return size_t(int_ref_waves(g)) * size_t(g.n_pad) * 5; }
+bool int_ref_in_lds(const DevGraph& g) { return size_t(g.n_pad) * 5 <= kRefLdsLimit; }
+
+size_t int_ref_lds_bytes(const DevGraph& g) {
+    return int_ref_in_lds(g) ? size_t(int_ref_waves(g)) * size_t(g.n_pad) * 5 : 0;
+}
 
 hipError_t launch_int_ref(const DevGraph& g, int R, uint64_t seed_base, const double* temp, int sweep_begin,
                           int sweep_end, int init, int8_t* s_state, int32_t* f_state, uint64_t* rng,
                           long long* energy, hipStream_t st) {
+    if (!int_ref_in_lds(g)) {
+        int_ref_kernel<4, false><<<unsigned((R + 3) / 4), 256, 0, st>>>(g, R, seed_base, temp, sweep_begin, sweep_end,
+                                                                         init, s_state, f_state, rng, energy);
+        return hipGetLastError();
+    }
     const size_t lds = int_ref_lds_bytes(g);
-    if (lds > kRefLdsLimit) return hipErrorInvalidValue;
     const int waves = int_ref_waves(g);
-    const void* k = waves == 4 ? reinterpret_cast<const void*>(&int_ref_kernel<4>)
-                               : reinterpret_cast<const void*>(&int_ref_kernel<1>);
+    const void* k = waves == 4 ? reinterpret_cast<const void*>(&int_ref_kernel<4, true>)
+                               : reinterpret_cast<const void*>(&int_ref_kernel<1, true>);
     const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
     if (waves == 4)
-        int_ref_kernel<4><<<unsigned((R + 3) / 4), 256, lds, st>>>(g, R, seed_base, temp, sweep_begin, sweep_end, init,
-                                                                   s_state, f_state, rng, energy);
+        int_ref_kernel<4, true><<<unsigned((R + 3) / 4), 256, lds, st>>>(g, R, seed_base, temp, sweep_begin, sweep_end,
+                                                                         init, s_state, f_state, rng, energy);
     else
-        int_ref_kernel<1><<<unsigned(R), 64, lds, st>>>(g, R, seed_base, temp, sweep_begin, sweep_end, init, s_state,
-                                                        f_state, rng, energy);
+        int_ref_kernel<1, true><<<unsigned(R), 64, lds, st>>>(g, R, seed_base, temp, sweep_begin, sweep_end, init,
+                                                              s_state, f_state, rng, energy);
     return hipGetLastError();
 }
 

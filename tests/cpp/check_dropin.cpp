@@ -255,6 +255,17 @@ static void gpu_checks() {
              "annealed " + std::to_string(best) + ", exact " + std::to_string(exact));
         line("anneal_branch(seed 3) = branch 2 of branch_fractions", std::fabs(one - binary_energy(p, br[2])) == 0.0,
              "");
+        // The problem stays resident between calls: a second call gives the same branches, and
+        // a copy with one datum changed is a different problem, not the resident one.
+        std::vector<std::vector<uint8_t>> again;
+        branch_fractions(p, map, 8, 1, &again);
+        BinaryProblem q = p;
+        q.d[0] += 40.0;
+        q.kappa = -3.0;
+        std::vector<std::vector<uint8_t>> other;
+        branch_fractions(q, map, 8, 1, &other);
+        line("a resident problem anneals as a fresh one; a changed copy is not served stale", again == br && other != br,
+             "");
     }
 }
 
@@ -399,6 +410,26 @@ static int muon_dump(int nx, int ny, int nz, uint64_t seed, int bins) {
 }
 
 int main(int argc, char** argv) {
+    if (argc == 8 && std::strcmp(argv[1], "--anneal-int") == 0) {
+        // wising::anneal on a rudy file: "energy <tracked> x <0/1 string>"
+        std::vector<char> buf;
+        wi::WeightedGraph g;
+        std::string err;
+        if (!wi::read_file(argv[2], buf) || !wi::parse_rudy(buf.data(), buf.size(), g, err)) {
+            std::fprintf(stderr, "cannot load %s: %s\n", argv[2], err.c_str());
+            return 2;
+        }
+        wi::Budget b;
+        b.t_hot = std::atof(argv[3]);
+        b.t_cold = std::atof(argv[4]);
+        b.anneal_sweeps = std::atoi(argv[5]);
+        b.hold_sweeps = std::atoi(argv[6]);
+        const wi::RunOut r = wi::anneal(g, b, std::strtoull(argv[7], nullptr, 10));
+        std::printf("energy %lld x ", static_cast<long long>(r.energy_tracked));
+        for (uint8_t v : r.x) std::putchar(v ? '1' : '0');
+        std::printf("\n");
+        return 0;
+    }
     if (argc >= 2 && std::strcmp(argv[1], "--rudy-digest") == 0) {
         // one line per file: "refused" or "ok <FNV-1a 64 of the parsed graph>"
         for (int f = 2; f < argc; ++f) {

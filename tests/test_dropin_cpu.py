@@ -74,6 +74,35 @@ def test_parse_rudy_equals_reference_loader(orc, driver, tmp_path):
     assert sum(w.startswith("ok") for w in want) == 10
 
 
+def test_host_anneal_equals_reference_code(orc, driver, tmp_path):
+    """The drop-in wising::anneal is the host annealer, as the reference's: seed for seed it
+    equals the reference's own src/ising_weighted.hpp:250-267 compiled into oracle/_ref
+    (assignment and tracked energy), on a signed K16 and a sparse +-1 graph."""
+    R = orc.ref()
+    if R is None:
+        pytest.skip("reference shim not built")
+    rng = np.random.default_rng(5)
+    iu = np.triu_indices(16, 1)
+    graphs = [(16, iu[0], iu[1], rng.integers(-1000, 1001, len(iu[0])), (5.0, 1e-3, 300, 20))]
+    pairs = sorted({(int(min(u, v)), int(max(u, v))) for u, v in rng.integers(0, 200, (1500, 2)) if u != v})
+    eu, ev = np.array(pairs).T
+    graphs.append((200, eu, ev, rng.choice([-1, 1], len(eu)), (3.0, 0.05, 200, 10)))
+    for k, (n, eu, ev, ew, b) in enumerate(graphs):
+        path = tmp_path / f"g{k}.rudy"
+        path.write_text(f"{n} {len(eu)}\n" + "".join(f"{u + 1} {v + 1} {w}\n" for u, v, w in zip(eu, ev, ew)))
+        for seed in (1, 2, 7):
+            out = subprocess.run([driver, "--anneal-int", str(path), *map(str, b), str(seed)], capture_output=True,
+                                 text=True, timeout=60)
+            assert out.returncode == 0, out.stderr
+            _, e, _, xs = out.stdout.split()
+            x = np.zeros(n, np.uint8)
+            e_ref = C.c_int64()
+            R.ref_anneal_int(n, len(eu), np.ascontiguousarray(eu, np.int32), np.ascontiguousarray(ev, np.int32),
+                             np.ascontiguousarray(ew, np.int32), b[0], b[1], b[2], b[3], seed, x, C.byref(e_ref))
+            assert int(e) == e_ref.value
+            assert xs == "".join("1" if v else "0" for v in x)
+
+
 def parse_dump(text):
     out = {}
     for line in text.strip().splitlines():

@@ -317,6 +317,19 @@ def test_int_reference_trajectory_one_replica_per_workgroup(gpu):
     check_int_reference(g, qa.schedule(2.0, 0.05, 6, 2), 3, 11)
 
 
+def test_int_reference_trajectory_past_the_lds_limit(gpu):
+    """A graph whose replica state (5 bytes a vertex) exceeds 160 KB of LDS runs with its
+    spins and fields in global memory, replica by replica equal to wising::anneal, also
+    across a launch split (the reference's run_batch has no size limit either,
+    src/ising_gpu.cuh:396-458)."""
+    g = qa.er_graph(33000, 0.0002, 8)
+    sched = qa.schedule(2.5, 0.05, 14, 3)
+    s0, e0, _ = check_int_reference(g, sched, 5, 21)
+    s1, e1, rep = check_int_reference(g, sched, 5, 21, updates_per_launch=5 * 33000 * 4)
+    assert rep["launches"] == 5
+    assert np.array_equal(s0, s1) and np.array_equal(e0, e1)
+
+
 def test_int_reference_trajectory_reproduces_published_k16_cuts(gpu):
     """gset_bench --check B on the device: the best of 8 replicas (seeds 1..8) of the three
     random K16 of gset_bench.cu:252-264 at Budget(5, 1e-3, 4000, 200) cut 10622, 12434 and
