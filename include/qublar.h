@@ -179,6 +179,17 @@ int qb_freeze(qb_problem* p, double margin, uint8_t* frozen, int* rounds);
    computed on the device in FP64; data and prior parts separately (each nullable). */
 int qb_energy(qb_problem* p, const uint8_t* X, int K, double* total, double* data, double* prior);
 
+/* No counterpart in the reference. qb_energy for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+   same kernels and results, bit for bit, with every device buffer of the call (the staged
+   states, their packed masks, the partial sums and the three energy vectors) carved from a
+   static device scratch of 32 MiB that each device holds until the process ends. The call
+   makes no hipMalloc or hipFree. It needs about K * n_vars * 1.125 bytes plus the partial
+   sums; a call that needs more returns QB_E_LIMIT and does nothing. Every call on one device
+   shares that device's scratch, so calls that overlap on one device (from several threads
+   or on several problems) race: serialise them. */
+int qb_energy_noalloc(qb_problem* p, const uint8_t* X, int K, double* total, double* data, double* prior);
+
 /* ==== integer path: weighted MaxCut / Ising ========================================== */
 
 /* WeightedGraph edge list (src/ising_weighted.hpp:35-42), 0-indexed. */

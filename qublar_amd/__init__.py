@@ -148,11 +148,7 @@ def lib():
             "qb_anneal_branches": (i32, [vp, C.POINTER(Schedule), i32, u64, C.POINTER(AnnealOpts), vp, vp, vp, vp,
                                          vp, C.POINTER(Report)]),
             "qb_energy": (i32, [vp, vp, i32, vp, vp, vp]),
-            "qb_freeze": (i32, [vp, dbl, vp, C.POINTER(i32)]),
-            "qb_anneal_branches_reference": (i32, [vp, C.POINTER(Schedule), i32, u64, vp, vp, vp, vp, vp,
-                                                   C.POINTER(Report)]),
-            "qb_anneal_branches_reference_split": (i32, [vp, C.POINTER(Schedule), i32, u64, i32, vp, vp, vp, vp, vp,
-                                                         C.POINTER(Report)]),
+            "qb_energy_noalloc": (i32, [vp, vp, i32, vp, vp, vp]),
             "qb_graph_create": (i32, [C.POINTER(GraphDesc), i32, C.POINTER(vp)]),
             "qb_graph_destroy": (i32, [vp]),
             "qb_graph_get_plan": (i32, [vp, C.POINTER(Plan)]),
@@ -383,6 +379,15 @@ class Problem:
         K = X.shape[0]
         tot, dat, pri = np.zeros(K), np.zeros(K), np.zeros(K)
         _check(lib().qb_energy(self.h, _p(X), K, _p(tot), _p(dat), _p(pri)))
+        return tot, dat, pri
+
+    def energy_noalloc(self, X):
+        """energy() through qb_energy_noalloc: the library's static device scratch, no
+        device allocation in the call."""
+        X = np.ascontiguousarray(X, np.uint8)
+        K = X.shape[0]
+        tot, dat, pri = np.zeros(K), np.zeros(K), np.zeros(K)
+        _check(lib().qb_energy_noalloc(self.h, _p(X), K, _p(tot), _p(dat), _p(pri)))
         return tot, dat, pri
 
 

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -310,7 +311,8 @@ struct qb_problem {
     std::map<int, std::unique_ptr<Batches>> batches;   // by group width and team size
     // workspace
     DevBuf w_x, w_r, w_beta, w_bar, w_err, w_part, w_pv, w_tri, w_cert, w_br, w_data, w_prior, w_rerr, w_X, w_rref,
-        w_rng;
+        w_rng, w_energy;
+    int device = 0;   // the device current at creation: its stream, buffers and scratch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~qb_problem() {
         if (ev0) (void)hipEventDestroy(ev0);
@@ -562,6 +564,7 @@ extern "C" int qb_problem_create(const qb_problem_desc* desc, qb_problem** out) 
     for (size_t q = 0; q < rt.size(); ++q) rt[q] = BitEnt{pos_of[size_t(rt_bit[q])], rt_a[q]};
     std::vector<double> dv(desc->d, desc->d + nr), wv(desc->w, desc->w + nr);
 
+    QB_HIP(hipGetDevice(&P->device));
     QB_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     QB_HIP(hipEventCreate(&P->ev0));
     QB_HIP(hipEventCreate(&P->ev1));
@@ -1056,33 +1059,104 @@ extern "C" int qb_freeze(qb_problem* P, double margin, uint8_t* frozen, int* rou
     return QB_OK;
 }
 
+// Workspace of one energy evaluation of K states: the states as given (K x n bytes), the
+// same packed into 16-bit group masks in position order, the per-block partial sums, and
+// the data, prior and total energies (K doubles each). Each part starts on a 256-byte
+// boundary.
+struct EnergyWorkspace {
+    size_t X = 0, x = 0, part = 0, data = 0, prior = 0, total = 0, bytes = 0;
+    int n_groups = 0, n_part = 0;
+};
+
+static EnergyWorkspace energy_workspace(const qb_problem* P, int K) {
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    EnergyWorkspace L;
+    const size_t n = size_t(std::max(P->n, 1));
+    L.n_groups = (K + kGroup - 1) / kGroup;
+    L.n_part = ray_energy_blocks(P->dev, kGroup);
+    L.X = 0;
+    L.x = L.X + up(size_t(K) * n);
+    L.part = L.x + up(sizeof(uint16_t) * size_t(L.n_groups) * n);
+    L.data = L.part + up(sizeof(double) * size_t(L.n_part) * size_t(L.n_groups) * kGroup);
+    L.prior = L.data + up(sizeof(double) * size_t(K));
+    L.total = L.prior + up(sizeof(double) * size_t(K));
+    L.bytes = L.total + up(sizeof(double) * size_t(K));
+    return L;
+}
+
+// binary_energy of K states into the caller's outputs, all device work inside `ws`
+// (laid out by energy_workspace). Allocates nothing.
+static int energy_into(qb_problem* P, const uint8_t* X, int K, unsigned char* ws, const EnergyWorkspace& L,
+                       double* total, double* data, double* prior) {
+    const int n = P->n;
+    hipStream_t st = P->stream;
+    auto at = [ws](size_t off) { return ws + off; };
+    if (n > 0) {
+        QB_HIP(hipMemcpyAsync(at(L.X), X, size_t(K) * size_t(n), hipMemcpyHostToDevice, st));
+        QB_HIP(launch_pack_states(at(L.X), K, n, P->dev.pos_bit, reinterpret_cast<uint16_t*>(at(L.x)), st));
+    }
+    QB_HIP(launch_ray_energy(P->dev, reinterpret_cast<const uint16_t*>(at(L.x)), L.n_groups, kGroup, K, nullptr, 0,
+                             reinterpret_cast<double*>(at(L.part)), L.n_part, reinterpret_cast<double*>(at(L.data)),
+                             reinterpret_cast<double*>(at(L.prior)), nullptr, st,
+                             reinterpret_cast<double*>(at(L.total))));
+    const size_t kb = sizeof(double) * size_t(K);
+    if (data) QB_HIP(hipMemcpyAsync(data, at(L.data), kb, hipMemcpyDeviceToHost, st));
+    if (prior) QB_HIP(hipMemcpyAsync(prior, at(L.prior), kb, hipMemcpyDeviceToHost, st));
+    if (total) QB_HIP(hipMemcpyAsync(total, at(L.total), kb, hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    return QB_OK;
+}
+
 extern "C" int qb_energy(qb_problem* P, const uint8_t* X, int K, double* total, double* data, double* prior) {
     if (!P || (!X && K > 0)) return fail(QB_E_ARG, "qb_energy: null argument");
     if (K <= 0) return fail(QB_E_ARG, "qb_energy: K must be positive");
-    const int n = P->n;
-    const int n_groups = (K + kGroup - 1) / kGroup;
-    const int n_part = ray_energy_blocks(P->dev, kGroup);
-    int rc;
-    if ((rc = P->w_X.reserve(size_t(K) * size_t(std::max(n, 1)))) ||
-        (rc = P->w_x.reserve(sizeof(uint16_t) * size_t(n_groups) * size_t(std::max(n, 1)))) ||
-        (rc = P->w_part.reserve(sizeof(double) * size_t(n_part) * size_t(n_groups) * kGroup)) ||
-        (rc = P->w_data.reserve(sizeof(double) * size_t(K))) || (rc = P->w_prior.reserve(sizeof(double) * size_t(K))))
-        return rc;
-    hipStream_t st = P->stream;
-    QB_HIP(hipMemcpyAsync(P->w_X.p, X, size_t(K) * size_t(n), hipMemcpyHostToDevice, st));
-    if (n > 0) QB_HIP(launch_pack_states(P->w_X.as<uint8_t>(), K, n, P->dev.pos_bit, P->w_x.as<uint16_t>(), st));
-    QB_HIP(launch_ray_energy(P->dev, P->w_x.as<uint16_t>(), n_groups, kGroup, K, nullptr, 0, P->w_part.as<double>(), n_part,
-                             P->w_data.as<double>(), P->w_prior.as<double>(), nullptr, st));
-    std::vector<double> dat(static_cast<size_t>(K)), pri(static_cast<size_t>(K));
-    QB_HIP(hipMemcpyAsync(dat.data(), P->w_data.p, sizeof(double) * size_t(K), hipMemcpyDeviceToHost, st));
-    QB_HIP(hipMemcpyAsync(pri.data(), P->w_prior.p, sizeof(double) * size_t(K), hipMemcpyDeviceToHost, st));
-    QB_HIP(hipStreamSynchronize(st));
-    for (int k = 0; k < K; ++k) {
-        if (total) total[k] = dat[size_t(k)] + pri[size_t(k)];
-        if (data) data[k] = dat[size_t(k)];
-        if (prior) prior[k] = pri[size_t(k)];
+    const EnergyWorkspace L = energy_workspace(P, K);
+    if (int rc = P->w_energy.reserve(L.bytes)) return rc;
+    return energy_into(P, X, K, P->w_energy.as<unsigned char>(), L, total, data, prior);
+}
+
+// ---- the allocation-free energy entry point ---------------------------------------------
+// `scratch` is a file-scope device array: every device the library runs on holds its own
+// copy until the process ends. In host code its name is only a host-side stand-in, so no
+// kernel argument, memset or copy ever names it: its device address is looked up with
+// hipGetSymbolAddress once per device, with that device current, and that address is used.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_addr[kMaxDevices] = {};
+
+static int scratch_on(int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(QB_E_LIMIT, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        int cur = 0;
+        QB_HIP(hipGetDevice(&cur));
+        if (cur != device) QB_HIP(hipSetDevice(device));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) QB_HIP(hipSetDevice(cur));
+        if (e != hipSuccess || !ptr)
+            return fail(QB_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(e));
+        g_scratch_addr[device] = static_cast<unsigned char*>(ptr);
     }
+    *out = g_scratch_addr[device];
     return QB_OK;
+}
+
+extern "C" int qb_energy_noalloc(qb_problem* P, const uint8_t* X, int K, double* total, double* data,
+                                 double* prior) {
+    if (!P || (!X && K > 0)) return fail(QB_E_ARG, "qb_energy_noalloc: null argument");
+    if (K <= 0) return fail(QB_E_ARG, "qb_energy_noalloc: K must be positive");
+    const EnergyWorkspace L = energy_workspace(P, K);
+    if (L.bytes > kNoAllocScratchBytes)
+        return fail(QB_E_LIMIT,
+                    "qb_energy_noalloc: %d states of %d bits need %zu bytes of scratch; the static scratch holds %zu "
+                    "(call qb_energy, or pass fewer states)",
+                    K, P->n, L.bytes, kNoAllocScratchBytes);
+    unsigned char* ws = nullptr;
+    if (int rc = scratch_on(P->device, &ws)) return rc;
+    return energy_into(P, X, K, ws, L, total, data, prior);
 }
 
 // ==== integer path =====================================================================

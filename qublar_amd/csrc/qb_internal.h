@@ -183,10 +183,12 @@ constexpr int kHeavyEntries = kSeg * (kSweepBlock / 2);
 hipError_t launch_ray_outputs(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, float* p_void,
                               uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st);
 // FP64 energy of every branch: data and prior partial sums per block, reduced in a
-// fixed order; resid_err receives max |r_carried - r_scratch| when r_carried != null.
+// fixed order; resid_err receives max |r_carried - r_scratch| when r_carried != null;
+// total (nullable) receives data + prior.
 hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R,
                              const void* r_carried, int precision, double* part, int n_part_blocks,
-                             double* data, double* prior, float* resid_err, hipStream_t st);
+                             double* data, double* prior, float* resid_err, hipStream_t st,
+                             double* total = nullptr);
 int ray_energy_blocks(const DevProblem& p, int bw);
 hipError_t launch_pack_states(const uint8_t* X, int K, int n, const int32_t* pos_bit, uint16_t* x,
                               hipStream_t st);

@@ -1076,7 +1076,8 @@ ray_energy_prior_kernel(DevProblem p, const uint16_t* x, double* part, int part_
 
 // Sums the partials of each branch in block order (deterministic).
 __global__ void ray_energy_final_kernel(const double* part_data, int blocks_data, const double* part_prior,
-                                        int blocks_prior, int stride, int R, double* data, double* prior) {
+                                        int blocks_prior, int stride, int R, double* data, double* prior,
+                                        double* total) {
     const int br = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (br >= R) return;
     double d = 0.0, q = 0.0;
@@ -1084,6 +1085,7 @@ __global__ void ray_energy_final_kernel(const double* part_data, int blocks_data
     for (int k = 0; k < blocks_prior; ++k) q += part_prior[size_t(k) * stride + br];
     data[br] = d;
     prior[br] = q;
+    if (total) total[br] = d + q;
 }
 
 // K states (original bit order, one byte per bit) into 16-bit group masks.
@@ -1291,7 +1293,7 @@ int ray_energy_blocks(const DevProblem& p, int bw) {
 
 hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_groups, int bw, int R, const void* r_carried,
                              int precision, double* part, int n_part_blocks, double* data, double* prior,
-                             float* resid_err, hipStream_t st) {
+                             float* resid_err, hipStream_t st, double* total) {
     const int items = kEnergyBlock / bw;
     const int bd = (p.n_rays + items - 1) / items;
     const int bp = (p.n + items - 1) / items;
@@ -1311,7 +1313,8 @@ hipError_t launch_ray_energy(const DevProblem& p, const uint16_t* x, int n_group
         if (bp > 0)
             ray_energy_prior_kernel<BW><<<dim3(unsigned(bp), unsigned(n_groups)), kEnergyBlock, 0, st>>>(p, x, part_prior,
                                                                                                       stride);
-        ray_energy_final_kernel<<<unsigned((R + 63) / 64), 64, 0, st>>>(part, bd, part_prior, bp, stride, R, data, prior);
+        ray_energy_final_kernel<<<unsigned((R + 63) / 64), 64, 0, st>>>(part, bd, part_prior, bp, stride, R, data, prior,
+                                                                              total);
         return hipGetLastError();
     });
 }
