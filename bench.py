@@ -109,39 +109,84 @@ def serving_pool(n, seed):
     return pool
 
 
+def serving_stream(pool, rate, seconds, seed):
+    """Poisson arrivals at `rate` per second over `seconds`: each request a pool problem
+    drawn at random, 8-16 branches, its own seed."""
+    from qublar_amd.serving import Request
+
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        P = pool[int(rng.integers(len(pool)))]
+        reqs.append(Request(P, int(rng.integers(8, 17)), 100 + len(reqs), t, keep_branches=True))
+    return reqs
+
+
+def serve(mode, pool, sched, rate, seconds, seed=11, stage_sweeps=25, k=25):
+    """Serves one stream in `mode` (continuous, unions or sequential); returns the summary
+    and the requests (each with its results)."""
+    import qublar_amd as qa
+    from qublar_amd.serving import ContinuousServer, Server, latency_summary
+
+    reqs = serving_stream(pool, rate, seconds, seed)
+    total = sched.anneal_sweeps + sched.hold_sweeps
+    t0 = time.perf_counter()
+    if mode == "continuous":
+        reports = ContinuousServer(sched, qa.FP32, k, stage_sweeps=stage_sweeps).run(reqs)
+    else:
+        reports = Server(sched, qa.FP32, k).run(reqs, batched=mode == "unions")
+    wall = time.perf_counter() - t0
+    updates = sum(q.branches * total * q.problem.n_vars for q in reqs)
+    kms = np.array([r["kernel_ms"] for r in reports])
+    out = {"mode": {"continuous": "continuous batching (stage unions, state kept on the device)",
+                    "unions": "a union per launch, formed when the GPU comes free",
+                    "sequential": "one solve per launch"}[mode],
+           "offered_req_per_s": rate, "arrival_window_s": seconds, "requests": len(reqs),
+           "launches": len(reports), "sweep_kernel_ms": {"mean": float(kms.mean()), "max": float(kms.max())},
+           "gpu_busy_fraction": float(kms.sum() / 1e3 / wall), "wall_s": wall,
+           "spin_updates_per_s": updates / wall, "latency_ms": latency_summary(reqs)}
+    if mode == "continuous":
+        out["stage_sweeps"] = stage_sweeps
+        out["unions"] = len({r["union"] for r in reports})
+    return out, reqs
+
+
+def final_energies(pool, reqs, limit=64):
+    """FP64 energies of the first `limit` requests' branches, each on its own problem."""
+    import qublar_amd as qa
+
+    probs = {}
+    e = []
+    for q in reqs[:limit]:
+        if id(q.problem) not in probs:
+            probs[id(q.problem)] = qa.Problem(q.problem)
+        e.append(probs[id(q.problem)].energy(q.branches_out)[0])
+    for p in probs.values():
+        p.close()
+    return e
+
+
 def run_serving(args):
     """Poisson arrivals of ray x voxel requests (8-16 branches, annealed to T = 1) for
-    --serve-seconds, served batched (unions) and then one solve per launch; prints one
-    JSON line per mode with p50 / p99 latency and the served spin-update rate."""
+    --serve-seconds, served by the continuous server, then by unions formed at launch
+    boundaries, then one solve per launch; prints one JSON line per mode with p50 / p99
+    latency and the served spin-update rate, and checks that the modes end at the same
+    final energies (every request's branches are its solve alone in each mode)."""
     import qublar_amd as qa
-    from qublar_amd.serving import Request, Server, latency_summary
 
     pool = serving_pool(args.pool, 7)
     log("pool sizes:", sorted(P.n_vars for P in pool))
     sched = qa.schedule(50.0, 1.0, 200, 50)
-    rng = np.random.default_rng(11)
-    for batched in (True, False):
-        t, reqs = 0.0, []
-        while True:
-            t += rng.exponential(1.0 / args.rate)
-            if t > args.serve_seconds:
-                break
-            P = pool[int(rng.integers(len(pool)))]
-            reqs.append(Request(P, int(rng.integers(8, 17)), 100 + len(reqs), t))
-        srv = Server(sched, qa.FP32, 50)
-        t0 = time.perf_counter()
-        reports = srv.run(reqs, batched=batched)
-        wall = time.perf_counter() - t0
-        updates = sum(q.branches * 250 * q.problem.n_vars for q in reqs)
-        sizes = np.bincount([q.batch for q in reqs])
-        kms = np.array([r["kernel_ms"] for r in reports])
-        out = {"workload": "serving", "mode": "batched unions" if batched else "one solve per launch",
-               "offered_req_per_s": args.rate, "arrival_window_s": args.serve_seconds, "requests": len(reqs),
-               "batches": len(reports), "requests_per_batch": {"mean": float(sizes.mean()), "max": int(sizes.max())},
-               "sweep_kernel_ms": {"mean": float(kms.mean()), "max": float(kms.max())},
-               "gpu_busy_fraction": float(kms.sum() / 1e3 / wall),
-               "wall_s": wall, "spin_updates_per_s": updates / wall, "latency_ms": latency_summary(reqs),
-               "data": "synthetic muon-tomography QUBOs; each request coloured on arrival on a host thread"}
+    energies = {}
+    for mode in ("continuous", "unions", "sequential"):
+        out, reqs = serve(mode, pool, sched, args.rate, args.serve_seconds, stage_sweeps=args.stage_sweeps)
+        energies[mode] = final_energies(pool, reqs)
+        same = all(np.array_equal(a, b) for a, b in zip(energies[mode], energies["continuous"]))
+        out.update({"workload": "serving", "same_final_energies_as_continuous": same,
+                    "data": "synthetic muon-tomography QUBOs; each request coloured on arrival on a host thread"})
         print(json.dumps(out), flush=True)
 
 
@@ -235,6 +280,27 @@ def run_gravity_precision(args):
               "scene_s": t_scene, "upload_s": t_upload, "rows": [r for r, _ in rows],
               "fp32_k10_more_branches": wide,
               "data": "synthetic: the body's truncated anomaly plus N(0, 10 microGal) noise"}
+    if args.headline_serve_seconds > 0:
+        # the metric's batched solve latency: the serving stream (50 requests/s, 2k-300k
+        # voxels, 8-16 branches, the same schedule) under continuous batching, beside one
+        # solve per launch of the same requests; both end at the same final energies
+        pool = serving_pool(args.pool, 7)
+        lat = {}
+        for mode in ("continuous", "sequential"):
+            o, reqs = serve(mode, pool, sched, args.rate, args.headline_serve_seconds, stage_sweeps=args.stage_sweeps)
+            lat[mode] = (o, final_energies(pool, reqs, 16))
+            log(f"serving {mode}: p50 {o['latency_ms']['p50_ms']:.0f} ms, p99 {o['latency_ms']['p99_ms']:.0f} ms")
+        result["batched_latency_ms"] = {
+            "p50": lat["continuous"][0]["latency_ms"]["p50_ms"], "p99": lat["continuous"][0]["latency_ms"]["p99_ms"],
+            "sequential_p50": lat["sequential"][0]["latency_ms"]["p50_ms"],
+            "sequential_p99": lat["sequential"][0]["latency_ms"]["p99_ms"],
+            "requests": lat["continuous"][0]["requests"], "arrival_window_s": args.headline_serve_seconds,
+            "offered_req_per_s": args.rate,
+            "served_spin_updates_per_s": lat["continuous"][0]["spin_updates_per_s"],
+            "sequential_spin_updates_per_s": lat["sequential"][0]["spin_updates_per_s"],
+            "same_final_energies": all(np.array_equal(a, b) for a, b in zip(lat["continuous"][1], lat["sequential"][1])),
+            "kind": "serving stream, continuous batching vs one solve per launch",
+        }
     if not args.no_cpu:
         threads = max(1, min(16, (os.cpu_count() or 2) - 2))
         log(f"cpu baseline on {threads} threads ...")
@@ -535,6 +601,33 @@ def run_dense_precision(args):
     print(json.dumps(result), flush=True)
 
 
+def run_builders(args):
+    """The problem builders on the device against the host builders: the muon QUBO (the
+    reference's candidate exposure, 2^27 muons per chamber, rows, assembly, neighbour
+    lists) and the gravimetry QUBO (39M-entry rows), build wall time each, best of 3."""
+    import qublar_amd as qa
+
+    out = {"workload": "builders"}
+    for name, dev, host in (("muon 128^3, 3 x 2^27 candidates", lambda: qa.muon_scene_device(), qa.muon_scene),
+                            ("gravity 100 x 200 x 10", lambda: qa.gravity_scene_device(), qa.gravity_scene)):
+        dev()   # warm: code objects, allocator
+        best, best_wall = None, None
+        for _ in range(3):
+            t = time.perf_counter()
+            P, ms = dev()
+            wall = time.perf_counter() - t
+            if best is None or ms < best:
+                best, best_wall = ms, wall
+        t = time.perf_counter()
+        H = host()
+        host_s = time.perf_counter() - t
+        out[name] = {"device_build_ms": best, "device_wall_incl_python_copy_s": best_wall, "host_build_s": host_s,
+                     "n_vars": P.n_vars, "n_rays": P.n_rays, "entries": int(P.row_ptr[-1]),
+                     "host_entries": int(H.row_ptr[-1])}
+        log(name, out[name])
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -549,7 +642,7 @@ def main():
     ap.add_argument("--dense-n", type=int, default=4096, help="dense precision: bits")
     ap.add_argument("--dense-sweeps", type=int, default=5000, help="dense precision: sweeps")
     ap.add_argument("--workload", choices=["muon", "serving", "precision", "batched", "dense-precision",
-                                           "gravity-precision", "roi", "maxcut", "reference-float"],
+                                           "gravity-precision", "roi", "maxcut", "reference-float", "builders"],
                     default="muon",
                     help="muon: the headline line (default); serving: the batched serving stream; "
                          "batched: many sparse QUBOs in one persistent launch")
@@ -561,6 +654,9 @@ def main():
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window")
     ap.add_argument("--pool", type=int, default=24, help="serving: distinct request problems")
+    ap.add_argument("--stage-sweeps", type=int, default=25, help="serving: sweeps per continuous-batching stage")
+    ap.add_argument("--headline-serve-seconds", type=float, default=8.0,
+                    help="muon: arrival window of the batched-serving latency beside the headline (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain 1 GiB device copy (HBM counter calibration) and print its rate")
     args = ap.parse_args()
@@ -596,6 +692,9 @@ def main():
         return
     if args.workload == "reference-float":
         run_reference_float(args)
+        return
+    if args.workload == "builders":
+        run_builders(args)
         return
     if args.copy_probe:
         nbytes, reps = 1 << 30, 5
@@ -665,7 +764,7 @@ def main():
             if prec == qa.FP32 else "FP64 residuals, dE and energy",
         },
         "latency_ms": {"p50": float(np.percentile(walls, 50) * 1e3), "p99": float(np.percentile(walls, 99) * 1e3),
-                       "kind": "one solve per call, sequential"},
+                       "kind": "the headline solve, one per call, sequential"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "ray_team_kernel", "launch_ms": launch_s * 1e3,
@@ -674,6 +773,27 @@ def main():
                          "fp32_residual_err": out["report"]["fp32_residual_err"],
                          "energy_mean": float(np.mean(out["energy"]))},
     }
+    if args.headline_serve_seconds > 0:
+        # the metric's batched solve latency: the serving stream (50 requests/s, 2k-300k
+        # voxels, 8-16 branches, the same schedule) under continuous batching, beside one
+        # solve per launch of the same requests; both end at the same final energies
+        pool = serving_pool(args.pool, 7)
+        lat = {}
+        for mode in ("continuous", "sequential"):
+            o, reqs = serve(mode, pool, sched, args.rate, args.headline_serve_seconds, stage_sweeps=args.stage_sweeps)
+            lat[mode] = (o, final_energies(pool, reqs, 16))
+            log(f"serving {mode}: p50 {o['latency_ms']['p50_ms']:.0f} ms, p99 {o['latency_ms']['p99_ms']:.0f} ms")
+        result["batched_latency_ms"] = {
+            "p50": lat["continuous"][0]["latency_ms"]["p50_ms"], "p99": lat["continuous"][0]["latency_ms"]["p99_ms"],
+            "sequential_p50": lat["sequential"][0]["latency_ms"]["p50_ms"],
+            "sequential_p99": lat["sequential"][0]["latency_ms"]["p99_ms"],
+            "requests": lat["continuous"][0]["requests"], "arrival_window_s": args.headline_serve_seconds,
+            "offered_req_per_s": args.rate,
+            "served_spin_updates_per_s": lat["continuous"][0]["spin_updates_per_s"],
+            "sequential_spin_updates_per_s": lat["sequential"][0]["spin_updates_per_s"],
+            "same_final_energies": all(np.array_equal(a, b) for a, b in zip(lat["continuous"][1], lat["sequential"][1])),
+            "kind": "serving stream, continuous batching vs one solve per launch",
+        }
     if not args.no_cpu:
         threads = max(1, min(16, (os.cpu_count() or 2) - 2))
         log(f"cpu baseline on {threads} threads ...")

@@ -190,6 +190,47 @@ int qb_energy(qb_problem* p, const uint8_t* X, int K, double* total, double* dat
    or on several problems) race: serialise them. */
 int qb_energy_noalloc(qb_problem* p, const uint8_t* X, int K, double* total, double* data, double* prior);
 
+/* ---- staged solves (continuous batching) ----------------------------------------------
+   No counterpart in the reference, which anneals one problem per branch_fractions call
+   (src/ising_recon.hpp:148-166). A stage keeps the branch state (spins and residuals) of a
+   problem -- in practice a union of independent requests built with draw_bit / draw_key /
+   colour -- on the device between calls, so the union can be annealed a few sweeps at a time
+   while requests join and leave. Every bit carries a clock: its own sweep is the stage's sweep
+   count plus its span's offset, its temperature and Philox counter are those of its own sweep,
+   and once its own sweep reaches the schedule's end it stays put; a span may also be limited to
+   its request's first n_branches branches. A request that joins a running union with its state
+   imported therefore follows, spin for spin, the trajectory of qb_anneal_branches on it alone
+   (with seed_offset = its key), whatever it shares the union with and whenever it joins. With
+   QB_FP32 and a recompute period k, spans are set and moved only at multiples of k sweeps, so the
+   FP64 recompute falls on every request's own k-th sweeps. Calls on one stage are serialised by
+   its problem's stream; the problem must outlive its stages. */
+typedef struct qb_stage qb_stage;
+/* A stage of n_branches branches (seed offset 0: keys come from draw_key), every bit at own
+   sweep 0 with all branches, from x = 0, r = -d. opts: precision, recompute_every, workgroups,
+   group_width as for qb_anneal_branches (sweeps_per_launch unused). */
+int qb_stage_create(qb_problem* p, const qb_schedule* s, int n_branches, const qb_anneal_opts* opts, qb_stage** out);
+int qb_stage_destroy(qb_stage* st);
+/* Before the stage's first run: bits [bit0, bit0 + n_bits) (original index) start at own sweep
+   sweeps_done, annealing their first n_branches branches (0: all; at most 255). */
+int qb_stage_set_span(qb_stage* st, int bit0, int n_bits, int sweeps_done, int n_branches);
+/* Before dst's first run: the state of src bits [src_bit0, + n_bits) and rays [src_ray0, + n_rays)
+   -- spins, clocks (own sweeps kept) and residuals of the groups both stages hold -- into dst
+   bits [dst_bit0, ...) and rays [dst_ray0, ...). The stages must agree in precision, group width,
+   schedule length and recompute period, and src must be at a multiple of that period. */
+int qb_stage_import(qb_stage* dst, const qb_stage* src, int src_bit0, int dst_bit0, int n_bits, int src_ray0,
+                    int dst_ray0, int n_rays);
+/* Runs the stage's next n_sweeps sweeps (one launch). rep.spin_updates counts every proposal
+   slot of the launch, live or not. */
+int qb_stage_run(qb_stage* st, int n_sweeps, qb_report* rep);
+/* The outputs of bits [bit0, + n_bits) over their first n_branches branches, as
+   qb_anneal_branches gives them for a problem of those bits alone (branch_fractions' p_v and
+   classify, src/ising_recon.hpp:160-163, 302-309): branches n_branches x n_bits (nullable),
+   p_void, tri, certainty (n_bits each, nullable), in the span's bit order. */
+int qb_stage_outputs(qb_stage* st, int bit0, int n_bits, int n_branches, uint8_t* branches, float* p_void,
+                     uint8_t* tri, float* certainty);
+/* Sweeps the stage has run. */
+int qb_stage_sweeps(const qb_stage* st, int* sweeps);
+
 /* ==== integer path: weighted MaxCut / Ising ========================================== */
 
 /* WeightedGraph edge list (src/ising_weighted.hpp:35-42), 0-indexed. */
@@ -328,6 +369,55 @@ int qb_scene_gravity(const qb_gravity_scene* s, qb_host_problem** out);
 /* colour[n_vars]: the builder's colouring; QB_E_ARG when the builder gave none. */
 int qb_host_problem_colour(const qb_host_problem* h, int32_t* colour);
 
+/* ---- problem builders on the device --------------------------------------------------
+   A voxel medium (src/muon.cuh:38-58): cells (i, j, k), x fastest, lower corner lo, edge
+   voxel, attenuation mu per cell (host array, copied). A binned sky (src/muon.cuh:177-186). */
+typedef struct {
+    float lo[3];
+    float voxel;
+    int nx, ny, nz;
+    const float* mu;
+} qb_voxel_medium;
+
+typedef struct {
+    float c_min;              /* cos of the zenith cap (0.34202: 70 degrees) */
+    float th_max;             /* the cap, radians (1.22173) */
+    int n_az, n_th;           /* azimuth x zenith bins; bin = ith * n_az + iaz */
+} qb_muon_sky;
+
+/* muon_expose_binned (src/muon.cuh:195-224) on the device: candidates 0 .. n_candidates - 1
+   from the chamber, each a hash-sampled cos^2 sky direction binned into the sky, marched
+   through the medium (src/muon.cuh:66-134) and detected when its hash draw falls below
+   exp(-tau). Adds the open and detected counts per bin to open[] and det[] (n_az x n_th).
+   The direction, binning and detection use the device's float cbrt, sin, cos, acos and exp,
+   as the reference's kernel does, so a candidate near a bin edge or a detection threshold
+   may land differently from a host's libm. */
+int qb_muon_expose(const qb_voxel_medium* m, const float chamber[3], const qb_muon_sky* sky, int64_t n_candidates,
+                   uint32_t seed, uint64_t* open, uint64_t* det);
+/* muon_rows (src/op_muon.hpp:22-50) plus assemble (src/engine.hpp:120-149) and the
+   6-neighbour lists (BitField::grid, src/engine.hpp:61-85) on the device, for n_views views:
+   chambers 3 floats per view, dirs 3 floats per (view, bin) (the bin directions, as
+   MuonBinnedData::bin_direction gives them), open / det counts per (view, bin), site_bit per
+   cell of the model medium (-1 outside the domain; the domain cells numbered 0, 1, ... in
+   cell order). A (view, bin) with open >= min_open whose march of the model medium crosses a
+   bit is a row, in (view, bin) order: d = tau0 - t, t = -log(max(det, 1) / open), w =
+   max(det, 1), entries a = a_scale * segment. Entries are grouped by bit in row order (the
+   reference's std::sort leaves the order inside a bit to its library). The result is a host
+   problem (qb_host_problem_view) with lambda and kappa. */
+int qb_muon_rows_device(const qb_voxel_medium* model, int n_views, const float* chambers, const float* dirs,
+                        int n_bins, const uint64_t* open, const uint64_t* det, const int32_t* site_bit,
+                        double min_open, float a_scale, double lambda, double kappa, qb_host_problem** out);
+/* The reference's muon pipeline for the scene of qb_muon_scene, built on the device: the
+   pyramid's true medium (with the void) exposed from each chamber by 2^log2_muons candidate
+   muons, rows of the model medium (without the void), assembly and neighbour lists. Its
+   counts are the reference's candidate Monte Carlo, not qb_scene_muon's per-bin Poisson
+   draws, so the two problems share their bits, rows and entries but not d and w. *build_ms
+   (nullable): host wall time of the build. */
+int qb_scene_muon_device(const qb_muon_scene* s, qb_host_problem** out, double* build_ms);
+/* qb_scene_gravity with its rows (row_ptr, ray, a) built on the device: the same problem,
+   array for array. */
+int qb_scene_gravity_device(const qb_gravity_scene* s, qb_host_problem** out, double* build_ms);
+
 /* ---- batched sparse QUBOs ---------------------------------------------------------
    No single counterpart in the reference: it solves one QUBO per call (anneal_branch /
    branch_fractions, src/ising_recon.hpp:121-166, with a QUBO's pair terms encoded as
@@ -364,6 +454,12 @@ typedef struct {
     int workgroups;           /* persistent workgroups; 0 = as many as fit the device */
     int block;                /* threads per workgroup: 256 or 1024; 0 with replica_block 0: the
                                  (block, replica block) shape that runs a short prefix fastest */
+    int graph;                /* 1: the anneal's launch schedule (state reset, the persistent launch or
+                                 the per-instance launches) is recorded once into a hipGraph per
+                                 (shape, mode, sweeps, workgroups) and replayed by later calls, which
+                                 then make one graph launch; results are those of graph = 0. No
+                                 counterpart in the reference, whose chunked relaunch loop is plain
+                                 (src/ising_gpu.cuh:424-445) */
 } qb_qubo_opts;
 
 typedef struct {
@@ -374,6 +470,7 @@ typedef struct {
     int jobs;
     int block;                /* the threads per workgroup it used */
     double spin_updates;      /* sum over instances of replicas x sweeps x n_k */
+    int graph_replayed;       /* 1 when the anneal replayed a graph recorded by an earlier call */
 } qb_qubo_report;
 
 /* Anneals every replica of every instance. x: sum_k R_k n_k bytes, instance-major then

@@ -29,6 +29,7 @@ _i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
 
 
 class Problem(C.Structure):
@@ -48,6 +49,67 @@ class Mode(C.Structure):
     _fields_ = [("rng", C.c_int), ("order", C.c_void_p), ("class_ptr", C.c_void_p),
                 ("n_classes", C.c_int), ("precision", C.c_int), ("recompute_every", C.c_int),
                 ("segment", C.c_int)]
+
+
+class Medium(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("voxel", C.c_float), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("mu", C.c_void_p)]
+
+
+def medium(lo, voxel, shape, mu):
+    """A voxel medium over mu (float32, x fastest; kept alive by the returned pair)."""
+    mu = np.ascontiguousarray(mu, np.float32) if mu is not None else None
+    m = Medium()
+    m.lo[:] = [float(v) for v in lo]
+    m.voxel = float(voxel)
+    m.nx, m.ny, m.nz = (int(v) for v in shape)
+    m.mu = mu.ctypes.data if mu is not None else None
+    return m, mu
+
+
+def march(med, p, d):
+    """(tau, cells, segs) of src/muon.cuh's float march from p along d."""
+    m, _ = med
+    cap = 3 * (m.nx + m.ny + m.nz) + 16
+    cells = np.zeros(cap, np.int32)
+    segs = np.zeros(cap, np.float32)
+    n = C.c_int()
+    tau = lib().orc_march(C.byref(m), np.asarray(p, np.float32), np.asarray(d, np.float32), cells, segs, cap,
+                          C.byref(n))
+    return tau, cells[:n.value].copy(), segs[:n.value].copy()
+
+
+def muon_expose(med, chamber, c_min, th_max, n_az, n_th, n_cand, seed):
+    m, _ = med
+    o = np.zeros(n_az * n_th, np.uint64)
+    d = np.zeros(n_az * n_th, np.uint64)
+    lib().orc_muon_expose(C.byref(m), np.asarray(chamber, np.float32), c_min, th_max, n_az, n_th, n_cand, seed, o, d)
+    return o, d
+
+
+def muon_rows(med, chamber, dirs, open_, det, site_bit, min_open, a_scale, row0=0):
+    """(d, w, ent_bit, ent_row, ent_a) of one view's rows (src/op_muon.hpp:22-50)."""
+    m, _ = med
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1)
+    n_bins = dirs.size // 3
+    cap_r, cap_e = n_bins, n_bins * (3 * (m.nx + m.ny + m.nz) + 16)
+    d, w = np.zeros(cap_r), np.zeros(cap_r)
+    eb, er, ea = np.zeros(cap_e, np.int32), np.zeros(cap_e, np.int32), np.zeros(cap_e, np.float32)
+    nr, ne = C.c_int(), C.c_int()
+    lib().orc_muon_rows(C.byref(m), np.asarray(chamber, np.float32), dirs, n_bins, np.ascontiguousarray(open_, np.uint64),
+                        np.ascontiguousarray(det, np.uint64), np.ascontiguousarray(site_bit, np.int32), float(min_open),
+                        float(a_scale), int(row0), d, w, cap_r, eb, er, ea, cap_e, C.byref(nr), C.byref(ne))
+    return d[:nr.value], w[:nr.value], eb[:ne.value], er[:ne.value], ea[:ne.value]
+
+
+def assemble_stable(n_bits, ent_bit, ent_row, ent_a):
+    eb = np.ascontiguousarray(ent_bit, np.int32)
+    row_ptr = np.zeros(n_bits + 1, np.int32)
+    ray = np.zeros(len(eb), np.int32)
+    a = np.zeros(len(eb), np.float32)
+    lib().orc_assemble_stable(n_bits, len(eb), eb, np.ascontiguousarray(ent_row, np.int32),
+                              np.ascontiguousarray(ent_a, np.float32), row_ptr, ray, a)
+    return row_ptr, ray, a
 
 
 class Graph(C.Structure):
@@ -119,6 +181,18 @@ def lib():
         L.orc_qubo_energy.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _u8p]
         L.orc_qubo_anneal.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _i32p, C.POINTER(Schedule),
                                       C.c_uint64, C.c_int, _u8p]
+        L.orc_hash_u32.restype = C.c_uint32
+        L.orc_hash_u32.argtypes = [C.c_uint32]
+        L.orc_hash_unit.restype = C.c_float
+        L.orc_hash_unit.argtypes = [C.c_uint32]
+        L.orc_march.restype = C.c_float
+        L.orc_march.argtypes = [C.POINTER(Medium), _f32p, _f32p, _i32p, _f32p, C.c_int, C.POINTER(C.c_int)]
+        L.orc_muon_expose.argtypes = [C.POINTER(Medium), _f32p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int64,
+                                      C.c_uint32, _u64p, _u64p]
+        L.orc_muon_rows.argtypes = [C.POINTER(Medium), _f32p, _f32p, C.c_int, _u64p, _u64p, _i32p, C.c_double,
+                                    C.c_float, C.c_int, _f64p, _f64p, C.c_int, _i32p, _i32p, _f32p, C.c_int,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_assemble_stable.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f32p, _i32p, _i32p, _f32p]
         L.orc_roi_enumerate.restype = C.c_double
         L.orc_roi_enumerate.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                         C.POINTER(C.c_double), C.POINTER(C.c_uint64)]

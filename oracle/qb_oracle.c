@@ -846,3 +846,165 @@ double orc_roi_enumerate(int n, const double* h, const double* J, double beta, d
     *map_index = s_min;
     return e_min;
 }
+
+
+/* ---- problem builders ------------------------------------------------------------------ */
+
+uint32_t orc_hash_u32(uint32_t x) {   /* src/lidar.cuh:90-97 */
+    x = (x ^ 61u) ^ (x >> 16);
+    x *= 9u;
+    x = x ^ (x >> 4);
+    x *= 0x27d4eb2du;
+    x = x ^ (x >> 15);
+    return x;
+}
+
+float orc_hash_unit(uint32_t x) { return (float)(orc_hash_u32(x) >> 8) * (1.0f / 16777216.0f); }
+
+static int medium_index(const orc_medium* m, int i, int j, int k) { return (k * m->ny + j) * m->nx + i; }
+
+/* src/muon.cuh:66-134, every operation in float as written there */
+float orc_march(const orc_medium* m, const float p[3], const float d[3], int* cells, float* segs, int cap,
+                int* n_cells) {
+    const float eps = 1e-12f;
+    float t0 = -1e30f, t1 = 1e30f;
+    const int nc[3] = {m->nx, m->ny, m->nz};
+    int count = 0;
+    *n_cells = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (fabsf(d[a]) < eps) {
+            if (p[a] < m->lo[a] || p[a] > m->lo[a] + (float)nc[a] * m->voxel) return 0.0f;
+        } else {
+            const float ta = (m->lo[a] - p[a]) / d[a];
+            const float tb = (m->lo[a] + (float)nc[a] * m->voxel - p[a]) / d[a];
+            t0 = fmaxf(t0, fminf(ta, tb));
+            t1 = fminf(t1, fmaxf(ta, tb));
+        }
+    }
+    if (t1 <= fmaxf(t0, 0.0f)) return 0.0f;
+    const float t_start = fmaxf(t0, 0.0f);
+    const float t_end = t1 - t_start;
+    float q[3];
+    for (int a = 0; a < 3; ++a) q[a] = p[a] + d[a] * t_start;
+    int vox[3];
+    float tmax[3], tdelta[3];
+    for (int a = 0; a < 3; ++a) vox[a] = (int)floorf((q[a] - m->lo[a]) / m->voxel);
+    for (int a = 0; a < 3; ++a) {
+        if (vox[a] < 0) vox[a] = 0;
+        if (vox[a] >= nc[a]) vox[a] = nc[a] - 1;
+        if (fabsf(d[a]) < eps) {
+            tmax[a] = 1e30f;
+            tdelta[a] = 0.0f;
+        } else {
+            const float boundary = (d[a] > 0.0f ? (float)(vox[a] + 1) : (float)vox[a]) * m->voxel + m->lo[a];
+            tmax[a] = (boundary - p[a]) / d[a];
+            tdelta[a] = m->voxel / fabsf(d[a]);
+            if (tmax[a] < 0.0f) tmax[a] = 0.0f;
+        }
+    }
+    float tau = 0.0f, t_done = 0.0f;
+    for (int steps = 0; steps < 3 * (m->nx + m->ny + m->nz) + 16; ++steps) {
+        const int axis = (tmax[0] <= tmax[1]) ? ((tmax[0] <= tmax[2]) ? 0 : 2) : ((tmax[1] <= tmax[2]) ? 1 : 2);
+        const float t_next = fminf(tmax[axis], t_end);
+        const float seg = t_next - t_done;
+        if (seg > 0.0f) {
+            const int idx = medium_index(m, vox[0], vox[1], vox[2]);
+            if (m->mu) tau += m->mu[idx] * seg;
+            if (count < cap) {
+                cells[count] = idx;
+                segs[count] = seg;
+            }
+            ++count;
+        }
+        if (t_next >= t_end) break;
+        t_done = t_next;
+        vox[axis] += (d[axis] > 0.0f) ? 1 : -1;
+        tmax[axis] += tdelta[axis];
+        if (vox[axis] < 0 || vox[axis] >= nc[axis]) break;
+    }
+    *n_cells = count;
+    return tau;
+}
+
+/* src/muon.cuh:160-224 (sample_sky, muon_expose_binned), float throughout */
+void orc_muon_expose(const orc_medium* m, const float c[3], float c_min, float th_max, int n_az, int n_th,
+                     int64_t n_cand, uint32_t seed, uint64_t* open, uint64_t* det) {
+    for (int64_t i = 0; i < n_cand; ++i) {
+        const uint32_t key = orc_hash_u32((uint32_t)i * 0x9E3779B9u);
+        const float u1 = orc_hash_unit(key);
+        const float u2 = orc_hash_unit(key ^ 0x68BC21EBu);
+        const float c3 = c_min * c_min * c_min;
+        const float cz = cbrtf(c3 + u1 * (1.0f - c3));
+        const float sn = sqrtf(fmaxf(0.0f, 1.0f - cz * cz));
+        const float az = 6.28318530718f * u2;
+        const float d[3] = {sn * cosf(az), sn * sinf(az), cz};
+        const float th = acosf(fminf(1.0f, fmaxf(-1.0f, cz)));
+        int ith = (int)(th / th_max * (float)n_th);
+        if (ith < 0) ith = 0;
+        if (ith >= n_th) ith = n_th - 1;
+        int iaz = (int)(az * 0.15915494f * (float)n_az);
+        if (iaz < 0) iaz = 0;
+        if (iaz >= n_az) iaz = n_az - 1;
+        const int bin = ith * n_az + iaz;
+        open[bin] += 1;
+        int dummy = 0;
+        const float tau = orc_march(m, c, d, NULL, NULL, 0, &dummy);
+        const float u3 = orc_hash_unit(key ^ 0x2545F491u ^ seed);
+        if (u3 < expf(-tau)) det[bin] += 1;
+    }
+}
+
+/* src/op_muon.hpp:22-50 for one view */
+void orc_muon_rows(const orc_medium* model, const float c[3], const float* dirs, int n_bins, const uint64_t* open,
+                   const uint64_t* det, const int32_t* site_bit, double min_open, float a_scale, int row0,
+                   double* d, double* w, int cap_rows, int32_t* ent_bit, int32_t* ent_row, float* ent_a, int cap_ent,
+                   int* n_rows, int* n_ent) {
+    const int cap_cells = 3 * (model->nx + model->ny + model->nz) + 16;
+    int* cells = (int*)malloc(sizeof(int) * (size_t)cap_cells);
+    float* segs = (float*)malloc(sizeof(float) * (size_t)cap_cells);
+    int rows = 0, ents = 0;
+    for (int b = 0; b < n_bins; ++b) {
+        if ((double)open[b] < min_open) continue;
+        int nc = 0;
+        const float tau0 = orc_march(model, c, dirs + 3 * b, cells, segs, cap_cells, &nc);
+        const double dt = (double)det[b] > 1.0 ? (double)det[b] : 1.0;
+        const double t = -log(dt / (double)open[b]);
+        int touches = 0;
+        for (int k = 0; k < nc && k < cap_cells; ++k) {
+            const int bit = site_bit[cells[k]];
+            if (bit < 0) continue;
+            if (ents < cap_ent) {
+                ent_bit[ents] = bit;
+                ent_row[ents] = row0 + rows;
+                ent_a[ents] = a_scale * segs[k];
+            }
+            ++ents;
+            touches = 1;
+        }
+        if (!touches) continue;
+        if (rows < cap_rows) {
+            d[rows] = (double)tau0 - t;
+            w[rows] = dt;
+        }
+        ++rows;
+    }
+    free(cells);
+    free(segs);
+    *n_rows = rows;
+    *n_ent = ents;
+}
+
+void orc_assemble_stable(int n_bits, int n_ent, const int32_t* ent_bit, const int32_t* ent_row, const float* ent_a,
+                         int32_t* row_ptr, int32_t* ray, float* a) {
+    for (int i = 0; i <= n_bits; ++i) row_ptr[i] = 0;
+    for (int k = 0; k < n_ent; ++k) row_ptr[ent_bit[k] + 1]++;
+    for (int i = 0; i < n_bits; ++i) row_ptr[i + 1] += row_ptr[i];
+    int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n_bits > 0 ? n_bits : 1));
+    for (int i = 0; i < n_bits; ++i) fill[i] = row_ptr[i];
+    for (int k = 0; k < n_ent; ++k) {
+        const int q = fill[ent_bit[k]]++;
+        ray[q] = ent_row[k];
+        a[q] = ent_a[k];
+    }
+    free(fill);
+}

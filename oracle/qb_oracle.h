@@ -12,6 +12,8 @@
  *   src/ising_weighted.hpp:83-303 parse_rudy, cut_of, energy_scratch, State,
  *                                anneal, tts99, tts99_bootstrap
  *   src/ising_weighted.hpp:307-385 SHA-256 of the '0'/'1' assignment string
+ *   src/muon.cuh:66-224, op_muon.hpp:22-50, engine.hpp:120-149  the muon exposure,
+ *                                rows and assembly (the device builders' checker)
  *
  * Two trajectory definitions share one code path:
  *   ORC_RNG_REFERENCE  visit order = bit index, one splitmix64 stream per
@@ -150,6 +152,42 @@ void orc_dense_anneal(int n, const float* Q, const float* h, const orc_schedule*
    *log_z, *map_index (first state of minimum energy, bit i = x_i). n <= 30. */
 double orc_roi_enumerate(int n, const double* h, const double* J, double beta, double* marginal, double* log_z,
                          uint64_t* map_index);
+
+/* ---- problem builders (src/muon.cuh:66-224, src/op_muon.hpp:22-50, src/engine.hpp:120-149) ----
+   A voxel medium: cells (i, j, k), x fastest, lower corner lo, edge voxel, attenuation mu. */
+typedef struct {
+    float lo[3];
+    float voxel;
+    int nx, ny, nz;
+    const float* mu;
+} orc_medium;
+
+/* The Wang hash and its unit float (src/lidar.cuh:90-101). */
+uint32_t orc_hash_u32(uint32_t x);
+float orc_hash_unit(uint32_t x);
+/* The Amanatides-Woo march of src/muon.cuh:66-134 in float: optical depth sum mu * segment
+   from p along unit d; cells[k], segs[k] receive the crossed cells with a positive segment
+   (cap entries at most; *n_cells the count, which may exceed cap). */
+float orc_march(const orc_medium* m, const float p[3], const float d[3], int* cells, float* segs, int cap,
+                int* n_cells);
+/* muon_expose_binned (src/muon.cuh:195-224): candidates 0 .. n_cand - 1 from chamber c, each
+   a hash-sampled cos^2 sky direction binned into n_th x n_az bins up to th_max, counted open,
+   and detected when hash_unit(key ^ 0x2545F491 ^ seed) < exp(-tau). Adds to open and det. */
+void orc_muon_expose(const orc_medium* m, const float c[3], float c_min, float th_max, int n_az, int n_th,
+                     int64_t n_cand, uint32_t seed, uint64_t* open, uint64_t* det);
+/* muon_rows (src/op_muon.hpp:22-50) for one view, bin directions given (dirs: 3 floats per
+   bin): for each bin with open >= min_open whose march of the model medium crosses a bit,
+   one row: d = tau0 - t with t = -log(max(det, 1) / open), w = max(det, 1), and an entry
+   (bit, row, a_scale * seg) per crossed bit in march order. Appends at most cap_rows rows
+   and cap_ent entries; returns the counts through *n_rows, *n_ent. */
+void orc_muon_rows(const orc_medium* model, const float c[3], const float* dirs, int n_bins, const uint64_t* open,
+                   const uint64_t* det, const int32_t* site_bit, double min_open, float a_scale, int row0,
+                   double* d, double* w, int cap_rows, int32_t* ent_bit, int32_t* ent_row, float* ent_a, int cap_ent,
+                   int* n_rows, int* n_ent);
+/* assemble's CSR (src/engine.hpp:120-149) with entries grouped by bit in their given order
+   inside a bit (a stable sort by bit). */
+void orc_assemble_stable(int n_bits, int n_ent, const int32_t* ent_bit, const int32_t* ent_row, const float* ent_a,
+                         int32_t* row_ptr, int32_t* ray, float* a);
 
 #ifdef __cplusplus
 }

@@ -85,12 +85,14 @@ class QuboBatchDesc(C.Structure):
 
 
 class QuboOpts(C.Structure):
-    _fields_ = [("mode", C.c_int), ("replica_block", C.c_int), ("workgroups", C.c_int), ("block", C.c_int)]
+    _fields_ = [("mode", C.c_int), ("replica_block", C.c_int), ("workgroups", C.c_int), ("block", C.c_int),
+                ("graph", C.c_int)]
 
 
 class QuboReport(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
-                ("replica_block", C.c_int), ("jobs", C.c_int), ("block", C.c_int), ("spin_updates", C.c_double)]
+                ("replica_block", C.c_int), ("jobs", C.c_int), ("block", C.c_int), ("spin_updates", C.c_double),
+                ("graph_replayed", C.c_int)]
 
 
 class DenseReport(C.Structure):
@@ -125,6 +127,15 @@ class GravityScene(C.Structure):
                 ("body_hi", C.c_double * 3), ("lam", C.c_double), ("kappa", C.c_double), ("seed", C.c_uint64)]
 
 
+class VoxelMedium(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("voxel", C.c_float), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("mu", C.c_void_p)]
+
+
+class MuonSky(C.Structure):
+    _fields_ = [("c_min", C.c_float), ("th_max", C.c_float), ("n_az", C.c_int), ("n_th", C.c_int)]
+
+
 _LIB = None
 
 
@@ -149,6 +160,18 @@ def lib():
                                          vp, C.POINTER(Report)]),
             "qb_energy": (i32, [vp, vp, i32, vp, vp, vp]),
             "qb_energy_noalloc": (i32, [vp, vp, i32, vp, vp, vp]),
+            "qb_stage_create": (i32, [vp, C.POINTER(Schedule), i32, C.POINTER(AnnealOpts), C.POINTER(vp)]),
+            "qb_stage_destroy": (i32, [vp]),
+            "qb_stage_set_span": (i32, [vp, i32, i32, i32, i32]),
+            "qb_stage_import": (i32, [vp, vp, i32, i32, i32, i32, i32, i32]),
+            "qb_stage_run": (i32, [vp, i32, C.POINTER(Report)]),
+            "qb_stage_outputs": (i32, [vp, i32, i32, i32, vp, vp, vp, vp]),
+            "qb_stage_sweeps": (i32, [vp, C.POINTER(i32)]),
+            "qb_freeze": (i32, [vp, dbl, vp, C.POINTER(i32)]),
+            "qb_anneal_branches_reference": (i32, [vp, C.POINTER(Schedule), i32, u64, vp, vp, vp, vp, vp,
+                                                   C.POINTER(Report)]),
+            "qb_anneal_branches_reference_split": (i32, [vp, C.POINTER(Schedule), i32, u64, i32, vp, vp, vp, vp, vp,
+                                                         C.POINTER(Report)]),
             "qb_graph_create": (i32, [C.POINTER(GraphDesc), i32, C.POINTER(vp)]),
             "qb_graph_destroy": (i32, [vp]),
             "qb_graph_get_plan": (i32, [vp, C.POINTER(Plan)]),
@@ -166,6 +189,11 @@ def lib():
             "qb_scene_gravity": (i32, [C.POINTER(GravityScene), C.POINTER(vp)]),
             "qb_host_problem_colour": (i32, [vp, vp]),
             "qb_scene_er_graph": (i32, [i32, dbl, u64, vp, vp, vp, i64, C.POINTER(i64)]),
+            "qb_muon_expose": (i32, [C.POINTER(VoxelMedium), vp, C.POINTER(MuonSky), i64, C.c_uint32, vp, vp]),
+            "qb_muon_rows_device": (i32, [C.POINTER(VoxelMedium), i32, vp, vp, i32, vp, vp, vp, dbl, C.c_float, dbl,
+                                          dbl, C.POINTER(vp)]),
+            "qb_scene_muon_device": (i32, [C.POINTER(MuonScene), C.POINTER(vp), C.POINTER(dbl)]),
+            "qb_scene_gravity_device": (i32, [C.POINTER(GravityScene), C.POINTER(vp), C.POINTER(dbl)]),
             "qb_qubo_batch_create": (i32, [C.POINTER(QuboBatchDesc), C.POINTER(vp)]),
             "qb_qubo_batch_destroy": (i32, [vp]),
             "qb_qubo_batch_anneal": (i32, [vp, C.POINTER(Schedule), C.POINTER(QuboOpts), vp, vp,
@@ -391,6 +419,57 @@ class Problem:
         return tot, dat, pri
 
 
+class Stage:
+    """The branch state of a resident problem (usually a union of requests) kept on the
+    device between calls (qb_stage_*): a few sweeps at a time, spans of bits with their own
+    clocks, state imported from another stage, per-span outputs."""
+
+    def __init__(self, problem, sched, n_branches, precision=FP64, recompute_every=0, workgroups=0, group_width=0):
+        self.problem = problem          # keeps the problem alive while the stage exists
+        self.h = C.c_void_p()
+        self.n_branches = n_branches
+        opts = AnnealOpts(precision, recompute_every, workgroups, 0, group_width)
+        _check(lib().qb_stage_create(problem.h, C.byref(sched), n_branches, C.byref(opts), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qb_stage_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_span(self, bit0, n_bits, sweeps_done=0, n_branches=0):
+        _check(lib().qb_stage_set_span(self.h, int(bit0), int(n_bits), int(sweeps_done), int(n_branches)))
+
+    def import_span(self, src, src_bit0, dst_bit0, n_bits, src_ray0, dst_ray0, n_rays):
+        _check(lib().qb_stage_import(self.h, src.h, int(src_bit0), int(dst_bit0), int(n_bits), int(src_ray0),
+                                     int(dst_ray0), int(n_rays)))
+
+    def run(self, n_sweeps):
+        rep = Report()
+        _check(lib().qb_stage_run(self.h, int(n_sweeps), C.byref(rep)))
+        return {f: getattr(rep, f) for f, _ in Report._fields_}
+
+    @property
+    def sweeps(self):
+        v = C.c_int()
+        _check(lib().qb_stage_sweeps(self.h, C.byref(v)))
+        return v.value
+
+    def outputs(self, bit0, n_bits, n_branches, branches=True):
+        br = np.zeros((n_branches, n_bits), np.uint8) if branches else None
+        pv = np.zeros(n_bits, np.float32)
+        tri = np.zeros(n_bits, np.uint8)
+        cert = np.zeros(n_bits, np.float32)
+        _check(lib().qb_stage_outputs(self.h, int(bit0), int(n_bits), int(n_branches), _p(br), _p(pv), _p(tri),
+                                      _p(cert)))
+        return dict(branches=br, p_void=pv, tri=tri, certainty=cert)
+
+
 class GraphArrays:
     def __init__(self, n, eu, ev, ew):
         self.n = int(n)
@@ -518,6 +597,72 @@ def gravity_scene(**overrides):
     h = C.c_void_p()
     _check(lib().qb_scene_gravity(C.byref(s), C.byref(h)))
     return _host_problem(h, True)
+
+
+def voxel_medium(lo, voxel, shape, mu):
+    """A qb_voxel_medium over mu (float32, x fastest); returns (struct, array) -- keep both."""
+    mu = np.ascontiguousarray(mu, np.float32)
+    m = VoxelMedium()
+    m.lo[:] = [float(v) for v in lo]
+    m.voxel = float(voxel)
+    m.nx, m.ny, m.nz = (int(v) for v in shape)
+    m.mu = mu.ctypes.data
+    return m, mu
+
+
+def muon_expose(medium, chamber, n_az, n_th, n_candidates, seed, c_min=0.34202, th_max=1.22173):
+    """muon_expose_binned on the device: (open, det) uint64 counts per sky bin."""
+    m, _keep = medium
+    sky = MuonSky(c_min, th_max, n_az, n_th)
+    o = np.zeros(n_az * n_th, np.uint64)
+    d = np.zeros(n_az * n_th, np.uint64)
+    _check(lib().qb_muon_expose(C.byref(m), _p(np.asarray(chamber, np.float32)), C.byref(sky), int(n_candidates),
+                                int(seed) & 0xFFFFFFFF, _p(o), _p(d)))
+    return o, d
+
+
+def muon_rows_device(model, chambers, dirs, open_, det, site_bit, min_open, a_scale, lam=2.0, kappa=float(np.log(999.0))):
+    """muon_rows + assemble + BitField::grid on the device (views x bins rays); ProblemArrays."""
+    m, _keep = model
+    ch = np.ascontiguousarray(chambers, np.float32).reshape(-1)
+    dr = np.ascontiguousarray(dirs, np.float32).reshape(-1)
+    n_views = ch.size // 3
+    n_bins = dr.size // 3 // n_views
+    o = np.ascontiguousarray(open_, np.uint64).reshape(-1)
+    d = np.ascontiguousarray(det, np.uint64).reshape(-1)
+    sb = np.ascontiguousarray(site_bit, np.int32)
+    h = C.c_void_p()
+    _check(lib().qb_muon_rows_device(C.byref(m), n_views, _p(ch), _p(dr), n_bins, _p(o), _p(d), _p(sb),
+                                     float(min_open), float(a_scale), float(lam), float(kappa), C.byref(h)))
+    return _host_problem(h, False)
+
+
+def muon_scene_device(**overrides):
+    """The reference's muon pipeline (candidate exposure, rows, assembly) on the device for
+    the muon scene; returns (ProblemArrays with .truth, build milliseconds)."""
+    s = MuonScene()
+    _check(lib().qb_muon_scene_defaults(C.byref(s)))
+    for k, v in overrides.items():
+        setattr(s, "lam" if k == "lambda" else k, v)
+    h = C.c_void_p()
+    ms = C.c_double()
+    _check(lib().qb_scene_muon_device(C.byref(s), C.byref(h), C.byref(ms)))
+    return _host_problem(h, False), ms.value
+
+
+def gravity_scene_device(**overrides):
+    """gravity_scene with its rows built on the device; returns (ProblemArrays, build ms)."""
+    s = GravityScene()
+    _check(lib().qb_gravity_scene_defaults(C.byref(s)))
+    for k, v in overrides.items():
+        if k in ("body_lo", "body_hi"):
+            getattr(s, k)[:] = [float(t) for t in v]
+        else:
+            setattr(s, "lam" if k == "lambda" else k, v)
+    h = C.c_void_p()
+    ms = C.c_double()
+    _check(lib().qb_scene_gravity_device(C.byref(s), C.byref(h), C.byref(ms)))
+    return _host_problem(h, True), ms.value
 
 
 def er_graph(n=800, p=0.06, seed=1):
@@ -655,9 +800,10 @@ class QuboBatch:
         _check(lib().qb_qubo_batch_colouring(self.h, int(k), _p(order), _p(cp), C.byref(nc)))
         return order, cp[:nc.value + 1].copy()
 
-    def anneal(self, sched, mode=0, replica_block=0, workgroups=0, x=True, energy=True, block=0):
-        """mode 0: one persistent launch; 1: one launch per instance. Returns x (per
-        instance uint8[R, n]), energy (per instance float64[R]) and the report."""
+    def anneal(self, sched, mode=0, replica_block=0, workgroups=0, x=True, energy=True, block=0, graph=False):
+        """mode 0: one persistent launch; 1: one launch per instance; graph: record the
+        launch schedule into a hipGraph once and replay it. Returns x (per instance
+        uint8[R, n]), energy (per instance float64[R]) and the report."""
         total_x = int(sum(q.replicas * q.n for q in self.instances))
         total_r = int(self._R.sum())
         xb = np.zeros(total_x, np.uint8) if x else None
@@ -666,7 +812,7 @@ class QuboBatch:
         if (block == 0) != (replica_block == 0):
             block = block or 256
             replica_block = replica_block or 8
-        o = QuboOpts(int(mode), int(replica_block), int(workgroups), int(block))
+        o = QuboOpts(int(mode), int(replica_block), int(workgroups), int(block), int(bool(graph)))
         _check(lib().qb_qubo_batch_anneal(self.h, C.byref(sched), C.byref(o), _p(xb), _p(eb), C.byref(rep)))
         out = {"report": {f: getattr(rep, f) for f, _ in QuboReport._fields_}}
         if x:

@@ -44,6 +44,11 @@ static int fail(int code, const char* fmt, ...) {
     } while (0)
 
 extern "C" const char* qb_last_error(void) { return g_err.c_str(); }
+
+namespace qb {
+// A device builder's failure (qb_build.hip), recorded for qb_last_error.
+int build_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace qb
 extern "C" int qb_version(void) { return 100; }
 
 // Device buffer that frees itself.
@@ -644,11 +649,11 @@ extern "C" int qb_problem_colouring(const qb_problem* p, int32_t* order, int32_t
 }
 
 // Workgroups per XCD team: `requested` total (a multiple of 8), or one per CU.
-static int team_size_for(int requested, int precision, int bw, size_t table_bytes) {
+static int team_size_for(int requested, int precision, int bw, size_t table_bytes, bool staged = false) {
     hipDeviceProp_t prop;
     int rc = device_props(prop);
     if (rc) return -rc;
-    const int per_cu = ray_sweep_blocks_per_cu(precision, bw, table_bytes);
+    const int per_cu = ray_sweep_blocks_per_cu(precision, bw, table_bytes, staged);
     if (per_cu <= 0)
         return -fail(QB_E_LIMIT, "the sweep kernel does not fit a CU with %zu bytes of dynamic LDS (class tables, "
                                  "multi-pass staging)", table_bytes);
@@ -929,6 +934,205 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
         rep->spin_updates = double(R) * double(total) * double(n);
         rep->fp32_residual_err = rerr;
     }
+    return QB_OK;
+}
+
+// ==== staged solves ==========================================================================
+// The branch state of one problem (in practice a union of requests) kept on the device
+// between calls: stages of a few sweeps at a time, requests joining with their state
+// imported from the previous union's stage, and leaving with their outputs read.
+
+struct qb_stage {
+    qb_problem* P = nullptr;
+    int precision = QB_FP64, recompute_every = 0, R = 0, bw = 0, n_groups = 0, team_size = 0, total = 0;
+    int sweeps = 0;                  // stage sweeps run so far
+    const DevBatches* bt = nullptr;  // owned by P (cached per group width and team size)
+    DevBuf x, r, beta, bar, err, clock, out_pv, out_tri, out_cert, out_br;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~qb_stage() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+};
+
+extern "C" int qb_stage_create(qb_problem* P, const qb_schedule* s, int R, const qb_anneal_opts* opts,
+                               qb_stage** out) {
+    if (!P || !out) return fail(QB_E_ARG, "qb_stage_create: null argument");
+    *out = nullptr;
+    int rc = check_schedule(s);
+    if (rc) return rc;
+    if (R <= 0) return fail(QB_E_ARG, "n_branches must be positive (got %d)", R);
+    qb_anneal_opts o{};
+    if (opts) o = *opts;
+    if (o.precision != QB_FP64 && o.precision != QB_FP32) return fail(QB_E_ARG, "precision must be QB_FP64 or QB_FP32");
+    if (o.recompute_every < 0 || o.workgroups < 0) return fail(QB_E_ARG, "negative option");
+    if (o.group_width != 0 && o.group_width != 2 && o.group_width != 4 && o.group_width != 8 && o.group_width != 16)
+        return fail(QB_E_ARG, "group_width must be 0, 2, 4, 8 or 16 (got %d)", o.group_width);
+    const int total = s->anneal_sweeps + s->hold_sweeps;
+    if (total > int(kClockOffsetMask)) return fail(QB_E_LIMIT, "a staged schedule has at most %u sweeps", kClockOffsetMask);
+    auto S = std::make_unique<qb_stage>();
+    S->P = P;
+    S->precision = o.precision;
+    S->recompute_every = o.precision == QB_FP32 ? o.recompute_every : 0;
+    S->R = R;
+    S->total = total;
+    S->bw = group_width_for(P, R, o.group_width, o.precision);
+    S->n_groups = (R + S->bw - 1) / S->bw;
+    S->team_size = team_size_for(o.workgroups, o.precision, S->bw, 0, true);
+    if (S->team_size <= 0) return -S->team_size;
+    if ((rc = batches_for(P, S->bw, S->team_size, &S->bt))) return rc;
+    {
+        const int checked = team_size_for(o.workgroups, o.precision, S->bw,
+                                          sweep_dynamic_lds(P->n_classes, S->bw, S->bt->heavy_lds != 0), true);
+        if (checked <= 0) return -checked;
+    }
+    const int n = P->n, nr = P->n_rays;
+    const size_t tsz = o.precision == QB_FP32 ? sizeof(float) : sizeof(double);
+    if ((rc = S->x.reserve(sizeof(uint16_t) * size_t(S->n_groups) * size_t(std::max(n, 1)))) ||
+        (rc = S->r.reserve(tsz * size_t(S->n_groups) * size_t(std::max(nr, 1)) * size_t(S->bw))) ||
+        (rc = S->beta.reserve(sizeof(double) * size_t(total))) ||
+        (rc = S->bar.reserve(sizeof(unsigned) * (kBarWords + kTicketWords))) || (rc = S->err.reserve(sizeof(int))) ||
+        (rc = S->clock.reserve(sizeof(uint32_t) * size_t(std::max(n, 1)))))
+        return rc;
+    QB_HIP(hipEventCreate(&S->ev0));
+    QB_HIP(hipEventCreate(&S->ev1));
+    hipStream_t st = P->stream;
+    const std::vector<double> beta = betas_of(*s);
+    QB_HIP(hipMemcpyAsync(S->beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice, st));
+    QB_HIP(launch_ray_init(P->dev, S->precision, S->bw, S->x.p, S->r.p, S->n_groups, st));
+    QB_HIP(hipMemsetAsync(S->clock.p, 0, sizeof(uint32_t) * size_t(std::max(n, 1)), st));
+    QB_HIP(hipMemsetAsync(S->err.p, 0, sizeof(int), st));
+    QB_HIP(hipStreamSynchronize(st));
+    *out = S.release();
+    return QB_OK;
+}
+
+extern "C" int qb_stage_destroy(qb_stage* S) {
+    if (S && S->P) (void)hipStreamSynchronize(S->P->stream);
+    delete S;
+    return QB_OK;
+}
+
+// A span of bits of a fresh stage: in range, the stage not yet run.
+static int check_span(const qb_stage* S, int bit0, int n_bits, const char* what) {
+    if (S->sweeps != 0) return fail(QB_E_ARG, "%s: the stage has run %d sweeps; spans are set before the first", what,
+                                    S->sweeps);
+    if (bit0 < 0 || n_bits < 0 || int64_t(bit0) + n_bits > S->P->n)
+        return fail(QB_E_ARG, "%s: bits [%d, %d) outside [0, %d)", what, bit0, bit0 + n_bits, S->P->n);
+    return QB_OK;
+}
+
+extern "C" int qb_stage_set_span(qb_stage* S, int bit0, int n_bits, int sweeps_done, int n_branches) {
+    if (!S) return fail(QB_E_ARG, "qb_stage_set_span: null stage");
+    if (int rc = check_span(S, bit0, n_bits, "qb_stage_set_span")) return rc;
+    if (sweeps_done < 0 || sweeps_done > S->total)
+        return fail(QB_E_ARG, "sweeps_done %d outside [0, %d]", sweeps_done, S->total);
+    if (S->recompute_every > 0 && sweeps_done % S->recompute_every != 0)
+        return fail(QB_E_ARG, "sweeps_done %d is not a multiple of the recompute period %d (the recompute after "
+                              "every k-th own sweep must fall on the stage's)", sweeps_done, S->recompute_every);
+    if (n_branches < 0 || n_branches > S->R || n_branches > kClockMaxBranches)
+        return fail(QB_E_ARG, "n_branches %d outside [0, min(%d, %d)]", n_branches, S->R, kClockMaxBranches);
+    const uint32_t word = uint32_t(sweeps_done) | (uint32_t(n_branches) << 24);
+    QB_HIP(launch_stage_clock(S->P->dev, bit0, n_bits, word, S->clock.as<uint32_t>(), S->P->stream));
+    QB_HIP(hipStreamSynchronize(S->P->stream));
+    return QB_OK;
+}
+
+extern "C" int qb_stage_import(qb_stage* D, const qb_stage* S, int src_bit0, int dst_bit0, int n_bits, int src_ray0,
+                               int dst_ray0, int n_rays) {
+    if (!D || !S) return fail(QB_E_ARG, "qb_stage_import: null stage");
+    if (D == S) return fail(QB_E_ARG, "qb_stage_import: a stage cannot import from itself");
+    if (int rc = check_span(D, dst_bit0, n_bits, "qb_stage_import")) return rc;
+    if (src_bit0 < 0 || int64_t(src_bit0) + n_bits > S->P->n)
+        return fail(QB_E_ARG, "qb_stage_import: source bits [%d, %d) outside [0, %d)", src_bit0, src_bit0 + n_bits,
+                    S->P->n);
+    if (n_rays < 0 || src_ray0 < 0 || dst_ray0 < 0 || int64_t(src_ray0) + n_rays > S->P->n_rays ||
+        int64_t(dst_ray0) + n_rays > D->P->n_rays)
+        return fail(QB_E_ARG, "qb_stage_import: ray span outside a problem's rays");
+    if (D->precision != S->precision || D->bw != S->bw)
+        return fail(QB_E_ARG, "qb_stage_import: stages differ in precision or group width (%d/%d, %d/%d)",
+                    D->precision, S->precision, D->bw, S->bw);
+    if (D->total != S->total) return fail(QB_E_ARG, "qb_stage_import: stages differ in schedule length");
+    if (D->recompute_every != S->recompute_every ||
+        (S->recompute_every > 0 && S->sweeps % S->recompute_every != 0))
+        return fail(QB_E_ARG, "qb_stage_import: a span moves only at a multiple of the recompute period (source at "
+                              "sweep %d, period %d)", S->sweeps, S->recompute_every);
+    if (int64_t(S->sweeps) + S->total > int64_t(kClockOffsetMask))
+        return fail(QB_E_LIMIT, "qb_stage_import: the source stage has run too many sweeps for a clock offset");
+    QB_HIP(hipStreamSynchronize(S->P->stream));
+    QB_HIP(launch_stage_import(S->P->dev, S->x.p, S->r.p, S->clock.as<uint32_t>(), S->sweeps, D->P->dev, D->x.p, D->r.p,
+                               D->clock.as<uint32_t>(), D->precision, D->bw, src_bit0, dst_bit0, n_bits, src_ray0,
+                               dst_ray0, n_rays, std::min(S->n_groups, D->n_groups), D->P->stream));
+    QB_HIP(hipStreamSynchronize(D->P->stream));
+    return QB_OK;
+}
+
+extern "C" int qb_stage_run(qb_stage* S, int n_sweeps, qb_report* rep) {
+    if (!S) return fail(QB_E_ARG, "qb_stage_run: null stage");
+    if (n_sweeps <= 0) return fail(QB_E_ARG, "n_sweeps must be positive (got %d)", n_sweeps);
+    if (int64_t(S->sweeps) + n_sweeps > int64_t(kClockOffsetMask) - S->total)
+        return fail(QB_E_LIMIT, "qb_stage_run: the stage has run too many sweeps for its clocks");
+    qb_problem* P = S->P;
+    hipStream_t st = P->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    QB_HIP(hipEventRecord(S->ev0, st));
+    QB_HIP(hipMemsetAsync(S->bar.p, 0, sizeof(unsigned) * (kBarWords + kTicketWords), st));
+    QB_HIP(launch_ray_sweeps(P->dev, *S->bt, S->precision, S->bw, S->x.p, S->r.p, S->n_groups, S->R, 0,
+                             S->beta.as<double>(), S->sweeps, S->sweeps + n_sweeps, S->recompute_every,
+                             S->bar.as<unsigned>(), S->bar.as<unsigned>() + kBarWords, S->err.as<int>(), S->team_size,
+                             st, S->clock.as<uint32_t>(), S->total));
+    QB_HIP(hipEventRecord(S->ev1, st));
+    int err_flag = 0;
+    QB_HIP(hipMemcpyAsync(&err_flag, S->err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    if (err_flag == 2)
+        return fail(QB_E_TIMEOUT, "an XCD team came up short of %d workgroups (workgroup placement)", S->team_size);
+    if (err_flag) return fail(QB_E_TIMEOUT, "a sweep barrier wait exceeded its limit (%d workgroups)", 8 * S->team_size);
+    S->sweeps += n_sweeps;
+    if (rep) {
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+        rep->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rep->kernel_ms = ms;
+        rep->launches = 1;
+        rep->spin_updates = double(S->R) * double(n_sweeps) * double(P->n);
+        rep->fp32_residual_err = 0.0;
+    }
+    return QB_OK;
+}
+
+extern "C" int qb_stage_outputs(qb_stage* S, int bit0, int n_bits, int n_branches, uint8_t* branches, float* p_void,
+                                uint8_t* tri, float* certainty) {
+    if (!S) return fail(QB_E_ARG, "qb_stage_outputs: null stage");
+    if (bit0 < 0 || n_bits < 0 || int64_t(bit0) + n_bits > S->P->n)
+        return fail(QB_E_ARG, "qb_stage_outputs: bits [%d, %d) outside [0, %d)", bit0, bit0 + n_bits, S->P->n);
+    if (n_branches <= 0 || n_branches > S->R)
+        return fail(QB_E_ARG, "qb_stage_outputs: n_branches %d outside [1, %d]", n_branches, S->R);
+    const size_t nb = size_t(std::max(n_bits, 1));
+    int rc;
+    if ((rc = S->out_pv.reserve(sizeof(float) * nb)) || (rc = S->out_tri.reserve(nb)) ||
+        (rc = S->out_cert.reserve(sizeof(float) * nb)) ||
+        (rc = S->out_br.reserve(branches ? size_t(n_branches) * nb : 1)))
+        return rc;
+    hipStream_t st = S->P->stream;
+    QB_HIP(launch_stage_outputs(S->P->dev, S->x.as<uint16_t>(), S->bw, bit0, n_bits, n_branches, S->out_pv.as<float>(),
+                                S->out_tri.as<uint8_t>(), S->out_cert.as<float>(),
+                                branches ? S->out_br.as<uint8_t>() : nullptr, st));
+    if (n_bits > 0) {
+        if (p_void) QB_HIP(hipMemcpyAsync(p_void, S->out_pv.p, sizeof(float) * size_t(n_bits), hipMemcpyDeviceToHost, st));
+        if (tri) QB_HIP(hipMemcpyAsync(tri, S->out_tri.p, size_t(n_bits), hipMemcpyDeviceToHost, st));
+        if (certainty)
+            QB_HIP(hipMemcpyAsync(certainty, S->out_cert.p, sizeof(float) * size_t(n_bits), hipMemcpyDeviceToHost, st));
+        if (branches)
+            QB_HIP(hipMemcpyAsync(branches, S->out_br.p, size_t(n_branches) * size_t(n_bits), hipMemcpyDeviceToHost, st));
+    }
+    QB_HIP(hipStreamSynchronize(st));
+    return QB_OK;
+}
+
+extern "C" int qb_stage_sweeps(const qb_stage* S, int* sweeps) {
+    if (!S || !sweeps) return fail(QB_E_ARG, "qb_stage_sweeps: null argument");
+    *sweeps = S->sweeps;
     return QB_OK;
 }
 
@@ -1502,8 +1706,21 @@ struct qb_qubo_batch {
     std::map<int, std::unique_ptr<Jobs>> jobs;   // by replica block
     DevBuf w_mask, w_beta, w_queue, w_energy;
     std::vector<uint32_t> h_mask;
+    // Recorded launch schedules: one per (shape, mode, sweeps, workgroups) and the buffers
+    // they were recorded on (a grown buffer makes a new recording).
+    struct Recorded {
+        int rb, block, n_sweeps, mode, workgroups;
+        const void *mask, *queue, *beta;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Recorded> recorded;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~qb_qubo_batch() {
+        for (Recorded& r : recorded) {
+            if (r.exec) (void)hipGraphExecDestroy(r.exec);
+            if (r.graph) (void)hipGraphDestroy(r.graph);
+        }
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1761,19 +1978,11 @@ static DevQuboBatch qubo_dev(const qb_qubo_batch* B, const qb_qubo_batch::Jobs& 
 
 // One anneal of the whole batch at replica block rb: spin words cleared (all-zero x),
 // then the persistent launch or one launch per instance, timed with events.
-static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int block, int n_sweeps, int mode,
-                    int workgroups, int* launches, float* ms) {
+// The anneal's launches: the persistent launch, or one per instance.
+static int qubo_launches(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int block, int n_sweeps, int mode,
+                         int workgroups, int* launches) {
     hipStream_t st = B->stream;
-    // one set of XCD queue counters per launch (kXcds lines each)
     const size_t set = size_t(kXcds) * size_t(kQueueStride);
-    const size_t queue_bytes = sizeof(unsigned) * set * size_t(mode == 0 ? 1 : B->n_inst);
-    int rc;
-    if ((rc = B->w_mask.reserve(sizeof(uint32_t) * size_t(Jb.mask_words))) ||
-        (rc = B->w_queue.reserve(queue_bytes)))
-        return rc;
-    QB_HIP(hipMemsetAsync(B->w_mask.p, 0, sizeof(uint32_t) * size_t(Jb.mask_words), st));
-    QB_HIP(hipMemsetAsync(B->w_queue.p, 0, queue_bytes, st));
-    QB_HIP(hipEventRecord(B->ev0, st));
     *launches = 0;
     if (mode == 0) {
         const DevQuboBatch q = qubo_dev(B, Jb, true);
@@ -1792,6 +2001,62 @@ static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int
             ++*launches;
         }
     }
+    return QB_OK;
+}
+
+// Resets the state (spin words, queue counters) and runs the launches; with `graph`, the
+// reset (by fill kernels: a recorded graph holds no memset node) and the launches are
+// recorded into a hipGraph on first use and replayed after.
+static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int block, int n_sweeps, int mode,
+                    int workgroups, bool graph, int* launches, float* ms, bool* replayed) {
+    hipStream_t st = B->stream;
+    // one set of XCD queue counters per launch (kXcds lines each)
+    const size_t set = size_t(kXcds) * size_t(kQueueStride);
+    const size_t queue_bytes = sizeof(unsigned) * set * size_t(mode == 0 ? 1 : B->n_inst);
+    int rc;
+    if ((rc = B->w_mask.reserve(sizeof(uint32_t) * size_t(Jb.mask_words))) ||
+        (rc = B->w_queue.reserve(queue_bytes)))
+        return rc;
+    if (replayed) *replayed = false;
+    if (!graph) {
+        QB_HIP(hipMemsetAsync(B->w_mask.p, 0, sizeof(uint32_t) * size_t(Jb.mask_words), st));
+        QB_HIP(hipMemsetAsync(B->w_queue.p, 0, queue_bytes, st));
+        QB_HIP(hipEventRecord(B->ev0, st));
+        if ((rc = qubo_launches(B, Jb, rb, block, n_sweeps, mode, workgroups, launches))) return rc;
+    } else {
+        qb_qubo_batch::Recorded* hit = nullptr;
+        for (auto& r : B->recorded)
+            if (r.rb == rb && r.block == block && r.n_sweeps == n_sweeps && r.mode == mode &&
+                r.workgroups == workgroups && r.mask == B->w_mask.p && r.queue == B->w_queue.p && r.beta == B->w_beta.p)
+                hit = &r;
+        if (!hit) {
+            qb_qubo_batch::Recorded r{rb, block, n_sweeps, mode, workgroups, B->w_mask.p, B->w_queue.p, B->w_beta.p};
+            QB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int lrc = QB_OK;
+            hipError_t e = launch_fill_u32(B->w_mask.as<uint32_t>(), size_t(Jb.mask_words), 0u, st);
+            if (e == hipSuccess) e = launch_fill_u32(B->w_queue.as<uint32_t>(), queue_bytes / sizeof(uint32_t), 0u, st);
+            if (e == hipSuccess) lrc = qubo_launches(B, Jb, rb, block, n_sweeps, mode, workgroups, launches);
+            const hipError_t ee = hipStreamEndCapture(st, &r.graph);
+            if (e != hipSuccess) return fail(QB_E_HIP, "recording the anneal's graph: %s", hipGetErrorString(e));
+            if (lrc) {
+                if (r.graph) (void)hipGraphDestroy(r.graph);
+                return lrc;
+            }
+            if (ee != hipSuccess) return fail(QB_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ee));
+            const hipError_t ie = hipGraphInstantiate(&r.exec, r.graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(r.graph);
+                return fail(QB_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+            }
+            B->recorded.push_back(r);
+            hit = &B->recorded.back();
+        } else {
+            *launches = mode == 0 ? 1 : B->n_inst;
+            if (replayed) *replayed = true;
+        }
+        QB_HIP(hipEventRecord(B->ev0, st));
+        QB_HIP(hipGraphLaunch(hit->exec, st));
+    }
     QB_HIP(hipEventRecord(B->ev1, st));
     QB_HIP(hipEventSynchronize(B->ev1));
     QB_HIP(hipGetLastError());
@@ -1804,13 +2069,14 @@ extern "C" int qb_qubo_batch_anneal(qb_qubo_batch* B, const qb_schedule* s, cons
     if (!B) return fail(QB_E_ARG, "qb_qubo_batch_anneal: null batch");
     int rc = check_schedule(s);
     if (rc) return rc;
-    qb_qubo_opts o{0, 0, 0, 0};
+    qb_qubo_opts o{0, 0, 0, 0, 0};
     if (opts) o = *opts;
     if (o.mode != 0 && o.mode != 1) return fail(QB_E_ARG, "mode must be 0 (persistent) or 1 (per instance)");
     if (o.replica_block != 0 && o.replica_block != 4 && o.replica_block != 8 && o.replica_block != 16 &&
         o.replica_block != 32)
         return fail(QB_E_ARG, "replica_block must be 0, 4, 8, 16 or 32 (got %d)", o.replica_block);
     if (o.workgroups < 0) return fail(QB_E_ARG, "negative workgroups");
+    if (o.graph != 0 && o.graph != 1) return fail(QB_E_ARG, "graph must be 0 or 1 (got %d)", o.graph);
     if (o.block != 0 && o.block != 256 && o.block != 1024)
         return fail(QB_E_ARG, "block must be 0, 256 or 1024 (got %d)", o.block);
     if ((o.block == 0) != (o.replica_block == 0))
@@ -1837,7 +2103,8 @@ extern "C" int qb_qubo_batch_anneal(qb_qubo_batch* B, const qb_schedule* s, cons
             if ((rc = qubo_jobs_for(B, sh[1], &Jb))) return rc;
             int l = 0;
             float ms = 0.0f;
-            if ((rc = qubo_run(B, *Jb, sh[1], sh[0], std::min(n_sweeps, 8), o.mode, grid_for(sh[1], sh[0]), &l, &ms)))
+            if ((rc = qubo_run(B, *Jb, sh[1], sh[0], std::min(n_sweeps, 8), o.mode, grid_for(sh[1], sh[0]), false, &l,
+                               &ms, nullptr)))
                 return rc;
             if (rb == 0 || ms < best) {
                 rb = sh[1];
@@ -1850,7 +2117,10 @@ extern "C" int qb_qubo_batch_anneal(qb_qubo_batch* B, const qb_schedule* s, cons
     if ((rc = qubo_jobs_for(B, rb, &Jb))) return rc;
     int launches = 0;
     float ms = 0.0f;
-    if ((rc = qubo_run(B, *Jb, rb, block, n_sweeps, o.mode, grid_for(rb, block), &launches, &ms))) return rc;
+    bool replayed = false;
+    if ((rc = qubo_run(B, *Jb, rb, block, n_sweeps, o.mode, grid_for(rb, block), o.graph != 0, &launches, &ms,
+                       &replayed)))
+        return rc;
     hipStream_t st = B->stream;
     if (energy) {
         if ((rc = B->w_energy.reserve(sizeof(double) * size_t(B->total_replicas)))) return rc;
@@ -1882,6 +2152,7 @@ extern "C" int qb_qubo_batch_anneal(qb_qubo_batch* B, const qb_schedule* s, cons
         rep->replica_block = rb;
         rep->block = block;
         rep->jobs = int(Jb->host.size());
+        rep->graph_replayed = replayed ? 1 : 0;
         double upd = 0.0;
         for (int k = 0; k < B->n_inst; ++k) upd += double(B->replicas[size_t(k)]) * double(B->n_bits[size_t(k)]);
         rep->spin_updates = upd * n_sweeps;

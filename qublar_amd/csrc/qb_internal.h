@@ -128,7 +128,16 @@ struct DevState {
     unsigned* bar;          // [0]: grid barrier; [64 + 64 t]: team t's arrivals, [96 + 64 t] its release word
     unsigned* tickets;      // [64 t]: team t's member count
     int* err;               // 1: a barrier wait timed out; 2: a team came up short
+    // A staged solve (null clock otherwise): per position, the bit's own sweep is the
+    // launch's sweep plus clock & kClockOffsetMask, and clock >> 24 is the number of
+    // branches its request anneals (0: all). A bit whose own sweep has reached
+    // total_sweeps, or a branch past its request's, stays put (no draw, no flip); beta
+    // is then indexed by the own sweep.
+    const uint32_t* clock;
+    int total_sweeps;
 };
+constexpr uint32_t kClockOffsetMask = 0xFFFFFFu;
+constexpr int kClockMaxBranches = 255;
 
 // Unsigned words of the barrier array and the ticket array (zeroed before a launch).
 constexpr int kBarWords = 64 * 9;
@@ -154,14 +163,16 @@ struct DevGraph {
 hipError_t launch_ray_init(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups,
                            hipStream_t st);
 // The persistent sweep on team_size * 8 workgroups (team_size per XCD).
+// With a non-null `clock` the staged form runs (DevState::clock): sweep_begin .. sweep_end
+// count the stage's own sweeps and every bit keeps its own.
 hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int precision, int bw, void* x, void* r,
                              int n_groups, int R,
                              uint64_t seed_offset, const double* beta, int sweep_begin, int sweep_end,
                              int recompute_every, unsigned* bar, unsigned* tickets, int* err, int team_size,
-                             hipStream_t st);
+                             hipStream_t st, const uint32_t* clock = nullptr, int total_sweeps = 0);
 // Co-resident sweep workgroups per CU for this precision and group width, with
-// `table_bytes` of dynamic LDS (0 when they do not fit).
-int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes);
+// `table_bytes` of dynamic LDS (0 when they do not fit), plain or staged form.
+int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes, bool staged = false);
 constexpr int kSweepBlock = 256;   // threads per sweep workgroup
 // A multi-pass bit whose chains fit kHeavyPasses rounds of a workgroup's slots is staged
 // whole in LDS. The staging area follows the class tables in the sweep's dynamic LDS
@@ -198,6 +209,16 @@ hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const
                              long long* energy, bool lds_state, hipStream_t st);
 // freeze_provable_rock on the device: per-position data bounds, then one fixpoint round
 // (frozen -> next, *changed counts the bits newly frozen).
+// Staged solves: a span's clock word; a span's state (spins, clocks, residuals of groups
+// [0, groups)) from one stage into another; a span's branch agreement over rq branches.
+hipError_t launch_stage_clock(const DevProblem& p, int bit0, int n_bits, uint32_t word, uint32_t* clock,
+                              hipStream_t st);
+hipError_t launch_stage_import(const DevProblem& src, const void* sx, const void* sr, const uint32_t* sclock,
+                               int src_sweeps, const DevProblem& dst, void* dx, void* dr, uint32_t* dclock,
+                               int precision, int bw, int sb0, int db0, int n_bits, int sr0, int dr0, int n_rays,
+                               int groups, hipStream_t st);
+hipError_t launch_stage_outputs(const DevProblem& p, const uint16_t* x, int bw, int bit0, int n_bits, int rq,
+                                float* p_void, uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st);
 hipError_t launch_freeze_bound(const DevProblem& p, double* lb, hipStream_t st);
 hipError_t launch_freeze_round(const DevProblem& p, const double* lb, double margin, const uint8_t* frozen,
                                uint8_t* next, unsigned* changed, hipStream_t st);
@@ -268,6 +289,8 @@ hipError_t launch_qubo_batch(const DevQuboBatch& q, int rb, int block, uint32_t*
                              int n_sweeps, unsigned* queue, const QuboQueues& segs, int workgroups, hipStream_t st);
 int qubo_blocks_per_cu(int rb, int block);
 hipError_t launch_qubo_energy(const DevQuboBatch& q, const uint32_t* masks, double* energy, hipStream_t st);
+// p[0 .. n) = v: a kernel, so a recorded graph holds no memset node.
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
 
 // ---- dense QUBO field engine (qb_dense.hip) ----
 struct DevDense {

@@ -20,6 +20,9 @@
 //
 // Every file of the engine is compiled with -ffp-contract=off: each floating-point
 // expression is evaluated as written, in the order of the CPU oracle.
+#include <algorithm>
+#include <type_traits>
+
 #include "qb_internal.h"
 #include "qb_rng.h"
 
@@ -237,6 +240,30 @@ __device__ __forceinline__ uint16_t segment_flips(bool acc) {
     return uint16_t((bal >> base) & ((uint64_t(1) << BW) - 1));
 }
 
+// The sweep and temperature one bit's proposal uses. In the plain form every bit is at
+// the launch's sweep. In the staged form (DevState::clock) the bit's own sweep is the
+// launch's plus its clock offset, beta is read at it, and the proposal is live only while
+// that sweep is inside the schedule and the branch is one of its request's.
+template <bool STAGED, typename T>
+__device__ __forceinline__ bool bit_time(const DevState<T>& s, uint32_t clk, int branch, int sweep, double beta,
+                                         int& own, double& bt) {
+    if constexpr (!STAGED) {
+        (void)s;
+        (void)clk;
+        (void)branch;
+        own = sweep;
+        bt = beta;
+        return true;
+    } else {
+        (void)beta;
+        own = sweep + int(clk & kClockOffsetMask);
+        const int rq = int(clk >> 24);
+        const bool live = own < s.total_sweeps && (rq == 0 || branch < rq);
+        bt = live ? s.beta[own] : 0.0;
+        return live;
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Packed steps. A slot's static operands -- its descriptor, its chain's entries, and
 // for a bit's first chain the neighbour positions, lin and the original bit index --
@@ -254,6 +281,7 @@ struct SlotStatic {
     double lin;
     int32_t bit;          // draw counter index
     uint64_t key;         // draw key offset
+    uint32_t clk;         // the bit's clock (staged form)
 };
 
 __device__ __forceinline__ int slot_len(uint32_t i) { return int(i & 31u); }
@@ -268,7 +296,8 @@ __device__ __forceinline__ void fetch_desc(const DevBatches& bt, int batch, int 
     S.d = SlotDesc{v.x, v.y, v.z, uint32_t(v.w)};
 }
 
-__device__ __forceinline__ void fetch_static(const DevProblem& p, SlotStatic& S) {
+template <bool STAGED>
+__device__ __forceinline__ void fetch_static(const DevProblem& p, SlotStatic& S, const uint32_t* clock) {
     if (S.d.pos < 0) return;
     const int len = slot_len(S.d.info);
     if (len > 0) {
@@ -299,6 +328,7 @@ __device__ __forceinline__ void fetch_static(const DevProblem& p, SlotStatic& S)
         S.lin = p.lin[S.d.pos];
         S.bit = p.draw_bit[S.d.pos];
         S.key = p.draw_key[S.d.pos];
+        if constexpr (STAGED) S.clk = clock[S.d.pos];
     }
 }
 
@@ -307,7 +337,7 @@ __device__ __forceinline__ void fetch_static(const DevProblem& p, SlotStatic& S)
 // adds the run, the bit's first slot adds the runs and the neighbour terms and decides; each slot then scatters its own chain from
 // registers. `after_issue` runs once the state loads are issued, `after_compute` once
 // this slot's chain sum is in LDS (the caller's prefetch of the next step).
-template <typename T, int BW, int BLOCK, typename F1, typename F2>
+template <typename T, int BW, int BLOCK, bool STAGED, typename F1, typename F2>
 __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<T>& s, const SlotStatic& S, int g,
                                             int sweep, double beta, F1&& after_issue, F2&& after_compute) {
     constexpr int kSlots = BLOCK / BW;
@@ -359,9 +389,13 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
     }
     after_issue();
     // the draw depends only on (branch, sweep, bit): made while the loads are in flight
-    double u = 0.0;
-    if (pos >= 0 && j == 0)
-        u = draw_u01(s.seed_offset + S.key + uint64_t(g * BW + b) + 1u, uint32_t(sweep), uint32_t(S.bit));
+    double u = 0.0, bt = beta;
+    bool live = false;
+    if (pos >= 0 && j == 0) {
+        int own = sweep;
+        live = bit_time<STAGED>(s, S.clk, g * BW + b, sweep, beta, own, bt);
+        u = draw_u01(s.seed_offset + S.key + uint64_t(g * BW + b) + 1u, uint32_t(own), uint32_t(S.bit));
+    }
     // the residuals back in chain order
 #pragma unroll
     for (int u = 0; u < kSeg; ++u) {
@@ -416,7 +450,7 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
             const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
             dE += p.lambda * (before ? -1.0 : 1.0);
         }
-        const bool acc = decide_drawn<T>(s, g * BW + b, u, beta, dE);
+        const bool acc = live && decide_drawn<T>(s, g * BW + b, u, bt, dE);
         accepted[slot][b] = acc ? 1 : 0;
         const uint16_t flips = segment_flips<BW>(acc);
         if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
@@ -454,7 +488,7 @@ __device__ __forceinline__ HeavyBit heavy_bit(const DevBatches& bt, int batch, i
 // passes: the chains are spread over the block's BW-lane slots, their sums meet in
 // LDS, slot 0 adds them (in runs of kChainRun) and decides, and the residual scatter is spread
 // over the block again.
-template <typename T, int BW, int BLOCK>
+template <typename T, int BW, int BLOCK, bool STAGED>
 __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState<T>& s, const HeavyBit& h, int g,
                                              int sweep, double beta) {
     const int pos = h.pos;
@@ -494,9 +528,12 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
     }
     if (slot == 0) {
         dE = neighbour_terms(p, xg, h.n0, h.n1, b, xb, dE);
-        const double u = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(sweep),
+        int own = sweep;
+        double bt = beta;
+        const bool live = bit_time<STAGED>(s, STAGED ? s.clock[pos] : 0u, g * BW + b, sweep, beta, own, bt);
+        const double u = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(own),
                                   uint32_t(p.draw_bit[pos]));
-        const bool acc = decide_drawn<T>(s, g * BW + b, u, beta, dE);
+        const bool acc = live && decide_drawn<T>(s, g * BW + b, u, bt, dE);
         accepted[b] = acc ? 1 : 0;
         const uint16_t flips = segment_flips<BW>(acc);
         if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
@@ -540,7 +577,7 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
 // pass-by-pass form costs two dependent round trips per pass and again per pass to
 // scatter. `hent` is the staging area in the sweep's dynamic LDS (heavy_stage_bytes).
 
-template <typename T, int BW, int BLOCK>
+template <typename T, int BW, int BLOCK, bool STAGED>
 __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevState<T>& s, const HeavyBit& h, int g,
                                                  int sweep, double beta, RayEnt* hent) {
     constexpr int kSlots = BLOCK / BW;
@@ -606,10 +643,14 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
                 for (int u = 12; u < 16; ++u) rv[qq][u] = ld_l2(rg + size_t(e[min(u, len - 1)].ray) * BW + b);
             }
         }
-        double u01 = 0.0;
-        if (slot == 0)
-            u01 = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(sweep),
+        double u01 = 0.0, bt = beta;
+        bool live = false;
+        if (slot == 0) {
+            int own = sweep;
+            live = bit_time<STAGED>(s, STAGED ? s.clock[pos] : 0u, g * BW + b, sweep, beta, own, bt);
+            u01 = draw_u01(s.seed_offset + p.draw_key[pos] + uint64_t(g * BW + b) + 1u, uint32_t(own),
                            uint32_t(p.draw_bit[pos]));
+        }
         const double lin = p.lin[pos];
 #pragma unroll
         for (int qq = 0; qq < kHeavyPasses; ++qq) {
@@ -656,7 +697,7 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
                 const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
                 dE += p.lambda * (before ? -1.0 : 1.0);
             }
-            const bool acc = decide_drawn<T>(s, g * BW + b, u01, beta, dE);
+            const bool acc = live && decide_drawn<T>(s, g * BW + b, u01, bt, dE);
             hacc[b] = acc ? 1 : 0;
             const uint16_t flips = segment_flips<BW>(acc);
             if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
@@ -695,15 +736,15 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
 // Dispatch of a multi-pass batch: staged in LDS when its chains fit kHeavyPasses
 // rounds of the workgroup's slots (and the launch reserved the staging area), else
 // pass by pass.
-template <typename T, int BW, int BLOCK>
+template <typename T, int BW, int BLOCK, bool STAGED>
 __device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T>& s, const DevBatches& bt, int batch,
                                            int g, int sweep, double beta, RayEnt* hent) {
     constexpr int kSlots = BLOCK / BW;
     const HeavyBit h = heavy_bit(bt, batch, kSlots);
     if (bt.heavy_lds && h.k1 - h.k0 <= kHeavyPasses * kSlots * kSeg)
-        heavy_lds_update<T, BW, BLOCK>(p, s, h, g, sweep, beta, hent);
+        heavy_lds_update<T, BW, BLOCK, STAGED>(p, s, h, g, sweep, beta, hent);
     else
-        heavy_update<T, BW, BLOCK>(p, s, h, g, sweep, beta);
+        heavy_update<T, BW, BLOCK, STAGED>(p, s, h, g, sweep, beta);
 }
 
 // The light bits of one class, [lb, le): no ray entry, at most kLightNb neighbours,
@@ -716,7 +757,7 @@ __device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T
 constexpr int kLightU = 4;
 constexpr int kTeams = 8;   // XCD teams of the sweep (group g belongs to team g mod 8)
 
-template <typename T, int BW>
+template <typename T, int BW, bool STAGED>
 __device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<T>& s, int lb, int le, int my_slot,
                                             int n_slots, int my_groups, int team, int sweep, double beta) {
     const int n_light = le - lb;
@@ -730,6 +771,7 @@ __device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<
         double lin[kLightU];
         int bit[kLightU];
         uint64_t key[kLightU];
+        uint32_t clk[kLightU];
         uint16_t xm[kLightU];
 #pragma unroll
         for (int u = 0; u < kLightU; ++u) {
@@ -745,6 +787,7 @@ __device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<
             lin[u] = p.lin[pos[u]];
             bit[u] = p.draw_bit[pos[u]];
             key[u] = p.draw_key[pos[u]];
+            clk[u] = STAGED ? s.clock[pos[u]] : 0u;
             xm[u] = ld_l2(s.x + size_t(grp[u]) * size_t(p.n) + pos[u]);
         }
         uint16_t m[kLightU][kLightNb];
@@ -768,8 +811,11 @@ __device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<
                 dE = q[v] >= 0 ? nxt : dE;
             }
             const int branch = grp[u] * BW + b;
-            const double draw = draw_u01(s.seed_offset + key[u] + uint64_t(branch) + 1u, uint32_t(sweep), uint32_t(bit[u]));
-            const bool acc = valid[u] && decide_drawn<T>(s, branch, draw, beta, dE);
+            int own = sweep;
+            double bt = beta;
+            const bool live = bit_time<STAGED>(s, clk[u], branch, sweep, beta, own, bt);
+            const double draw = draw_u01(s.seed_offset + key[u] + uint64_t(branch) + 1u, uint32_t(own), uint32_t(bit[u]));
+            const bool acc = valid[u] && live && decide_drawn<T>(s, branch, draw, bt, dE);
             const uint16_t flips = segment_flips<BW>(acc);
             if (valid[u] && b == 0 && flips) s.x[size_t(grp[u]) * size_t(p.n) + pos[u]] = uint16_t(xm[u] ^ flips);
         }
@@ -804,7 +850,7 @@ __device__ __forceinline__ void ray_recompute(const DevProblem& p, const DevStat
 // once, behind an agent-scope grid barrier, that every team is complete. Team t
 // anneals groups t, t + 8, ... through every class of every sweep.
 
-template <typename T, int BW, int BLOCK>
+template <typename T, int BW, int BLOCK, bool STAGED>
 __global__ void __launch_bounds__(BLOCK)
 ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int sweep_end, int recompute_every,
                 int team_size) {
@@ -919,12 +965,13 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
     nxt.d = SlotDesc{-1, 0, 0, 0u};
     if (a1 && !h1) {
         fetch_desc<kSlots>(bt, b1, slot, cur);
-        fetch_static(p, cur);
+        fetch_static<STAGED>(p, cur, s.clock);
     }
     if (a2 && !h2) fetch_desc<kSlots>(bt, b2, slot, nxt);
     unsigned target = 0;
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
-        const double beta = s.beta[sweep];
+        // the staged form reads beta per bit, at its own sweep
+        const double beta = STAGED ? 0.0 : s.beta[sweep];
         for (int c = 0; c < p.n_classes; ++c) {
             while (a1 && s1 == sweep && c1 == c) {
                 const int batch = b1, g = g1;
@@ -945,22 +992,22 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 SlotStatic nn;
                 nn.d = SlotDesc{-1, 0, 0, 0u};
                 if (heavy) {
-                    if (fetch1) fetch_static(p, nxt);
+                    if (fetch1) fetch_static<STAGED>(p, nxt, s.clock);
                     if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
-                    heavy_step<T, BW, BLOCK>(p, s, bt, batch, g, sweep, beta, hent);
+                    heavy_step<T, BW, BLOCK, STAGED>(p, s, bt, batch, g, sweep, beta, hent);
                 } else {
-                    packed_step<T, BW, BLOCK>(
+                    packed_step<T, BW, BLOCK, STAGED>(
                         p, s, cur, g, sweep, beta,
                         [] {},
                         [&] {
-                            if (fetch1) fetch_static(p, nxt);
+                            if (fetch1) fetch_static<STAGED>(p, nxt, s.clock);
                             if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
                         });
                 }
                 cur = nxt;
                 nxt.d = nn.d;
             }
-            light_sweep<T, BW>(p, s, lbeg[c], cend[c + 1], my_slot, n_slots, my_groups, team, sweep, beta);
+            light_sweep<T, BW, STAGED>(p, s, lbeg[c], cend[c + 1], my_slot, n_slots, my_groups, team, sweep, beta);
             target += unsigned(team_size);
             if (!team_barrier(bar, target, s.err)) return;
         }
@@ -1236,10 +1283,12 @@ hipError_t launch_ray_init(const DevProblem& p, int precision, int bw, void* x, 
 hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int precision, int bw, void* x, void* r,
                              int n_groups, int R,
                              uint64_t seed_offset, const double* beta, int sb, int se, int recompute_every,
-                             unsigned* bar, unsigned* tickets, int* err, int team_size, hipStream_t st) {
+                             unsigned* bar, unsigned* tickets, int* err, int team_size, hipStream_t st,
+                             const uint32_t* clock, int total_sweeps) {
     return with_bw(bw, [&]<int BW>() {
-        auto go = [&](auto zero) -> hipError_t {
+        auto go = [&](auto zero, auto staged) -> hipError_t {
             using T = decltype(zero);
+            constexpr bool STAGED = decltype(staged)::value;
             DevState<T> s;
             s.n_groups = n_groups;
             s.R = R;
@@ -1250,6 +1299,8 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
             s.bar = bar;
             s.tickets = tickets;
             s.err = err;
+            s.clock = clock;
+            s.total_sweeps = total_sweeps;
             // A plain launch of at most the co-resident workgroup count (the host checks it
             // against the occupancy with this dynamic LDS). Not a cooperative launch: the
             // kernel's barriers are its own atomics, and a cooperative launch takes the
@@ -1257,22 +1308,29 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
             // second process on the GPU would be refused. A workgroup that is not resident
             // leaves its team short or a barrier waiting: both end in the error flag.
             const size_t lds = sweep_dynamic_lds(p.n_classes, BW, bt.heavy_lds != 0);
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock>),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            const hipError_t e =
+                hipFuncSetAttribute(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock, STAGED>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
             if (e != hipSuccess) return e;
-            ray_team_kernel<T, BW, kSweepBlock><<<dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), lds, st>>>(
-                p, bt, s, sb, se, recompute_every, team_size);
+            ray_team_kernel<T, BW, kSweepBlock, STAGED>
+                <<<dim3(unsigned(team_size * kTeams)), dim3(kSweepBlock), lds, st>>>(p, bt, s, sb, se, recompute_every,
+                                                                                   team_size);
             return hipGetLastError();
         };
-        return precision == 1 ? go(0.0f) : go(0.0);
+        if (clock)
+            return precision == 1 ? go(0.0f, std::true_type{}) : go(0.0, std::true_type{});
+        return precision == 1 ? go(0.0f, std::false_type{}) : go(0.0, std::false_type{});
     });
 }
 
-int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes) {
+int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes, bool staged) {
     int per_cu = 0;
     const hipError_t e = with_bw(bw, [&]<int BW>() {
-        const void* k = precision == 1 ? reinterpret_cast<const void*>(&ray_team_kernel<float, BW, kSweepBlock>)
-                                       : reinterpret_cast<const void*>(&ray_team_kernel<double, BW, kSweepBlock>);
+        const void* k =
+            staged ? (precision == 1 ? reinterpret_cast<const void*>(&ray_team_kernel<float, BW, kSweepBlock, true>)
+                                     : reinterpret_cast<const void*>(&ray_team_kernel<double, BW, kSweepBlock, true>))
+                   : (precision == 1 ? reinterpret_cast<const void*>(&ray_team_kernel<float, BW, kSweepBlock, false>)
+                                     : reinterpret_cast<const void*>(&ray_team_kernel<double, BW, kSweepBlock, false>));
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kSweepBlock, table_bytes);
     });
     return e == hipSuccess ? per_cu : 0;
@@ -1678,6 +1736,102 @@ hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, cons
                           int sweep_end, uint8_t* xs, double* rs, uint64_t* sm, hipStream_t st) {
     ray_ref_kernel<<<unsigned((R + 3) / 4), 256, 0, st>>>(p, R, seed_offset, temp, sweep_begin, sweep_end, xs, rs,
                                                           sm);
+    return hipGetLastError();
+}
+
+// ---- staged solves: bit clocks, state carried between stages, per-request outputs ---------
+// A request occupies original bits [bit0, bit0 + n_bits) and rays [ray0, ray0 + n_rays) of a
+// union. Its clock words (DevState::clock) say where its bits are in their own schedule and
+// how many branches it anneals; its state moves between unions span by span.
+
+__global__ void stage_clock_kernel(const int32_t* bit_pos, int bit0, int n_bits, uint32_t word, uint32_t* clock) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n_bits) clock[bit_pos[bit0 + i]] = word;
+}
+
+// Spins and clocks of bits src [sb0, sb0 + n_bits) -> dst [db0, ...), residuals of rays src
+// [sr0, sr0 + n_rays) -> dst [dr0, ...), groups [0, groups). A bit's own sweep is the same
+// before and after: its destination offset is its source offset plus the source stage's
+// sweeps (the destination stage has run none).
+template <typename T, int BW>
+__global__ void stage_import_kernel(const int32_t* sbit_pos, int sn, int sn_rays, const uint16_t* sx, const T* sr,
+                                    const uint32_t* sclock, uint32_t src_sweeps, const int32_t* dbit_pos, int dn,
+                                    int dn_rays, uint16_t* dx, T* dr, uint32_t* dclock, int sb0, int db0, int n_bits,
+                                    int sr0, int dr0, int n_rays, int groups) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int g = int(blockIdx.y);
+    if (g >= groups) return;
+    if (i < size_t(n_bits)) {
+        const int sp = sbit_pos[sb0 + int(i)], dp = dbit_pos[db0 + int(i)];
+        dx[size_t(g) * size_t(dn) + size_t(dp)] = sx[size_t(g) * size_t(sn) + size_t(sp)];
+        if (g == 0) {
+            const uint32_t c = sclock[sp];
+            dclock[dp] = (c & ~kClockOffsetMask) | ((c & kClockOffsetMask) + src_sweeps);
+        }
+    }
+    if (i < size_t(n_rays) * BW) {
+        const size_t ray = i / BW, b = i % BW;
+        dr[(size_t(g) * size_t(dn_rays) + size_t(dr0) + ray) * BW + b] =
+            sr[(size_t(g) * size_t(sn_rays) + size_t(sr0) + ray) * BW + b];
+    }
+}
+
+// Branch agreement of one request over its own first rq branches (branch_fractions' p_v
+// and classify, src/ising_recon.hpp:160-163, 302-309), outputs in the span's bit order.
+__global__ void stage_outputs_kernel(DevProblem p, const uint16_t* x, int bw, int bit0, int n_bits, int rq,
+                                     float* p_void, uint8_t* tri, float* certainty, uint8_t* branches) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n_bits) return;
+    const int pos = p.bit_pos[bit0 + i];
+    int cnt = 0;
+    for (int g = 0; g * bw < rq; ++g) {
+        const int live = rq - g * bw < bw ? rq - g * bw : bw;
+        const unsigned m = unsigned(x[size_t(g) * size_t(p.n) + pos]) & ((1u << live) - 1u);
+        cnt += __popc(m);
+        if (branches)
+            for (int b = 0; b < live; ++b)
+                branches[size_t(g * bw + b) * size_t(n_bits) + size_t(i)] = uint8_t((m >> b) & 1u);
+    }
+    const float pv = float(cnt) / float(rq);
+    p_void[i] = pv;
+    tri[i] = pv >= 0.9f ? 1 : (pv <= 0.1f ? 0 : 2);
+    certainty[i] = pv > 1.0f - pv ? pv : 1.0f - pv;
+}
+
+hipError_t launch_stage_clock(const DevProblem& p, int bit0, int n_bits, uint32_t word, uint32_t* clock,
+                              hipStream_t st) {
+    if (n_bits > 0)
+        stage_clock_kernel<<<unsigned((n_bits + 255) / 256), 256, 0, st>>>(p.bit_pos, bit0, n_bits, word, clock);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_import(const DevProblem& src, const void* sx, const void* sr, const uint32_t* sclock,
+                               int src_sweeps, const DevProblem& dst, void* dx, void* dr, uint32_t* dclock,
+                               int precision, int bw, int sb0, int db0, int n_bits, int sr0, int dr0, int n_rays,
+                               int groups, hipStream_t st) {
+    const size_t items = std::max(size_t(n_bits), size_t(n_rays) * size_t(bw));
+    if (items == 0 || groups <= 0) return hipSuccess;
+    const dim3 grid{unsigned((items + 255) / 256), unsigned(groups), 1u};
+    return with_bw(bw, [&]<int BW>() {
+        if (precision == 1)
+            stage_import_kernel<float, BW><<<grid, 256, 0, st>>>(
+                src.bit_pos, src.n, src.n_rays, static_cast<const uint16_t*>(sx), static_cast<const float*>(sr), sclock,
+                uint32_t(src_sweeps), dst.bit_pos, dst.n, dst.n_rays, static_cast<uint16_t*>(dx),
+                static_cast<float*>(dr), dclock, sb0, db0, n_bits, sr0, dr0, n_rays, groups);
+        else
+            stage_import_kernel<double, BW><<<grid, 256, 0, st>>>(
+                src.bit_pos, src.n, src.n_rays, static_cast<const uint16_t*>(sx), static_cast<const double*>(sr),
+                sclock, uint32_t(src_sweeps), dst.bit_pos, dst.n, dst.n_rays, static_cast<uint16_t*>(dx),
+                static_cast<double*>(dr), dclock, sb0, db0, n_bits, sr0, dr0, n_rays, groups);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_stage_outputs(const DevProblem& p, const uint16_t* x, int bw, int bit0, int n_bits, int rq,
+                                float* p_void, uint8_t* tri, float* certainty, uint8_t* branches, hipStream_t st) {
+    if (n_bits > 0)
+        stage_outputs_kernel<<<unsigned((n_bits + 255) / 256), 256, 0, st>>>(p, x, bw, bit0, n_bits, rq, p_void, tri,
+                                                                            certainty, branches);
     return hipGetLastError();
 }
 

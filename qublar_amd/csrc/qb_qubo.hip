@@ -22,6 +22,8 @@
 // and a flip is accepted iff dE <= 0 or u < qexp(-dE * beta) -- the ray x voxel engine's
 // test, decided through a float exponential with a guard band (uphill_accept). Replicas 2m and 2m + 1 share one Philox4x32-10 call keyed by the instance seed with
 // counter (bit, sweep, m, tag), one 53-bit uniform each (draw2_u01).
+#include <algorithm>
+
 #include "qb_internal.h"
 #include "qb_rng.h"
 
@@ -240,6 +242,18 @@ int qubo_blocks_per_cu(int rb, int block) {
           : rb == 16 ? (const void*)&qubo_batch_kernel<16, 1024> : (const void*)&qubo_batch_kernel<32, 1024>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess) return 0;
     return per_cu;
+}
+
+// A fill kernel in place of a memset where the launch schedule is recorded into a graph.
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    fill_u32_kernel<<<unsigned(blocks), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_qubo_energy(const DevQuboBatch& q, const uint32_t* masks, double* energy, hipStream_t st) {

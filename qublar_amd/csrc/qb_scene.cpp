@@ -12,6 +12,7 @@
 // det ~ Poisson(N0 P(bin) exp(-tau_true)) with N0 muons per chamber under a
 // cos^2(theta) sky, tau_true marched through the medium that holds the void.
 #include "qublar.h"
+#include "qb_host_problem.h"
 
 #include <algorithm>
 #include <cmath>
@@ -21,15 +22,6 @@
 #include <string>
 #include <vector>
 
-struct qb_host_problem {
-    int n_vars = 0, n_rays = 0;
-    std::vector<int32_t> row_ptr, ray, nbr_ptr, nbr;
-    std::vector<float> a;
-    std::vector<double> d, w;
-    double lambda = 0.0, kappa = 0.0;
-    std::vector<uint8_t> truth;
-    std::vector<int32_t> colour;   // empty unless the builder colours its problem
-};
 
 namespace {
 
@@ -307,6 +299,10 @@ static int layer_period(const qb_gravity_scene* s, int k) {
     return cap;
 }
 
+namespace qb {
+int gravity_finish(const qb_gravity_scene* s, qb_host_problem* H);
+}
+
 extern "C" int qb_gravity_scene_defaults(qb_gravity_scene* s) {
     if (!s) return QB_E_ARG;
     *s = qb_gravity_scene{};
@@ -367,6 +363,33 @@ extern "C" int qb_scene_gravity(const qb_gravity_scene* s, qb_host_problem** out
     }
     H->ray.resize(size_t(H->row_ptr[size_t(n)]));
     H->a.resize(H->ray.size());
+    for (int b = 0; b < n; ++b) {
+        const int i = b % nx, j = (b / nx) % ny, k = b / n_st;
+        int32_t e = H->row_ptr[size_t(b)];
+        for (size_t q = 0; q < off[size_t(k)].size(); ++q) {
+            const int si = i + off[size_t(k)][q].first, sj = j + off[size_t(k)][q].second;
+            if (si < 0 || si >= nx || sj < 0 || sj >= ny) continue;
+            H->ray[size_t(e)] = sj * nx + si;
+            H->a[size_t(e)] = val[size_t(k)][q];
+            ++e;
+        }
+    }
+    if (int rc = qb::gravity_finish(s, H.get())) return rc;
+    *out = H.release();
+    return QB_OK;
+}
+
+namespace qb {
+
+int gravity_reach_of(const qb_gravity_scene* s) { return gravity_reach(s); }
+
+// The rest of the gravimetry problem once its rows (row_ptr, ray, a) are in place: the
+// truth, d = the true body's anomaly (summed over the body's bits in bit and CSR order)
+// plus Gaussian noise, w, the 6-neighbour lists, the layer-lattice colouring and the prior.
+int gravity_finish(const qb_gravity_scene* s, qb_host_problem* H) {
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    const int n = nx * ny * nz, n_st = nx * ny;
+    H->truth.assign(size_t(n), 0);
     std::vector<double> model(size_t(n_st), 0.0);
     for (int b = 0; b < n; ++b) {
         const int i = b % nx, j = (b / nx) % ny, k = b / n_st;
@@ -374,15 +397,9 @@ extern "C" int qb_scene_gravity(const qb_gravity_scene* s, qb_host_problem** out
         bool body = true;
         for (int a = 0; a < 3; ++a) body &= c[a] >= s->body_lo[a] && c[a] <= s->body_hi[a];
         H->truth[size_t(b)] = body;
-        int32_t e = H->row_ptr[size_t(b)];
-        for (size_t q = 0; q < off[size_t(k)].size(); ++q) {
-            const int si = i + off[size_t(k)][q].first, sj = j + off[size_t(k)][q].second;
-            if (si < 0 || si >= nx || sj < 0 || sj >= ny) continue;
-            H->ray[size_t(e)] = sj * nx + si;
-            H->a[size_t(e)] = val[size_t(k)][q];
-            if (body) model[size_t(sj * nx + si)] += double(val[size_t(k)][q]);
-            ++e;
-        }
+        if (body)
+            for (int32_t e = H->row_ptr[size_t(b)]; e < H->row_ptr[size_t(b) + 1]; ++e)
+                model[size_t(H->ray[size_t(e)])] += double(H->a[size_t(e)]);
     }
     // d = the true body's anomaly plus Gaussian noise (Box-Muller over splitmix64)
     H->d.resize(size_t(n_st));
@@ -393,6 +410,7 @@ extern "C" int qb_scene_gravity(const qb_gravity_scene* s, qb_host_problem** out
         H->d[size_t(st)] = model[size_t(st)] + s->sigma * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
     }
     // 6-neighbour adjacency, candidates in the order -x, +x, -y, +y, -z, +z
+    H->nbr.clear();
     H->nbr_ptr.assign(size_t(n) + 1, 0);
     for (int b = 0; b < n; ++b) {
         const int i = b % nx, j = (b / nx) % ny, k = b / n_st;
@@ -413,9 +431,10 @@ extern "C" int qb_scene_gravity(const qb_gravity_scene* s, qb_host_problem** out
     }
     H->lambda = s->lambda;
     H->kappa = s->kappa;
-    *out = H.release();
     return QB_OK;
 }
+
+}  // namespace qb
 
 extern "C" int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* v) {
     if (!h || !v) return QB_E_ARG;

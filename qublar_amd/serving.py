@@ -1,14 +1,20 @@
-"""A serving loop for ray x voxel QUBO solves on one GPU.
+"""Serving loops for ray x voxel QUBO solves on one GPU.
 
-Requests (a problem, its branch count, its seed) arrive over time. In batched mode every
-launch anneals the block-diagonal union of the requests waiting when the GPU comes free,
-admitted in arrival order while the union's device footprint stays within an HBM budget;
-a host thread assembles and uploads the next union while the GPU anneals the current one,
-so host preparation and device work overlap. A request's branches in a union are, spin
-for spin, its branches annealed alone with its own seed (per-bit draw keys), and its
-p_void averages its own branches, so batching changes when a result arrives, never what
-it is. Sequential mode solves one request per
-launch in arrival order, for comparison.
+Requests (a problem, its branch count, its seed) arrive over time.
+
+ContinuousServer batches continuously. The GPU anneals the block-diagonal union of the
+requests in flight a stage (a few sweeps) at a time, with their branch state kept on the
+device. Between stages, requests that have finished their schedule leave: their outputs
+are reduced on the device over their own branches. When a host thread has prepared the
+next union (the requests still running plus those that arrived and fit the HBM budget),
+it replaces the current one at a stage boundary: the running requests' spins, residuals
+and clocks are imported on the device, so nobody restarts, and newcomers start at their
+own sweep 0. Every bit anneals at its own sweep (temperature, Philox counter) and keys
+its draws by its request's seed and local index, so a request's result is, spin for spin,
+its result annealed alone, whenever it joins and whatever it shares the GPU with.
+
+Server is the earlier form: a union is formed only when the GPU comes free and annealed
+to the end in one launch; or, in sequential mode, one request per launch, for comparison.
 
 This is new around the reference: QuBLAR solves one problem per branch_fractions call
 (src/ising_recon.hpp:148-166); batching many requests into one device solve has no
@@ -21,7 +27,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import FP32, Problem, schedule, union
+from . import FP32, Problem, Stage, schedule, union
 
 
 @dataclass
@@ -35,6 +41,8 @@ class Request:
     p_void: np.ndarray = field(default=None, repr=False)
     tri: np.ndarray = field(default=None, repr=False)
     colour: object = field(default=None, repr=False)   # a future of the request's colouring
+    keep_branches: bool = False                          # ContinuousServer: also return the branches
+    branches_out: np.ndarray = field(default=None, repr=False)
 
 
 def device_bytes_estimate(P, branches):
@@ -85,7 +93,10 @@ class Server:
         out = prob.anneal_branches(self.sched, R, seed, self.precision, self.k, branches=True, outputs=False,
                                    energy=False)
         for q, (b0, _) in zip(batch, offs):
-            q.p_void, q.tri = branch_agreement(out["branches"][:q.branches, b0:b0 + q.problem.n_vars])
+            mine = out["branches"][:q.branches, b0:b0 + q.problem.n_vars]
+            q.p_void, q.tri = branch_agreement(mine)
+            if q.keep_branches:
+                q.branches_out = mine.copy()
         now = time.perf_counter() - t0
         for q in batch:
             q.done = now
@@ -141,6 +152,133 @@ class Server:
     def _wait_admitted(batch):
         while any(q.colour is None for q in batch):
             time.sleep(0.0005)
+
+
+@dataclass
+class _Member:
+    req: Request
+    bit0: int          # the request's first bit and ray in the union
+    ray0: int
+    start: int         # own sweeps done when the union's stage started
+
+
+class ContinuousServer:
+    """Continuous batching over stages of `stage_sweeps` sweeps (see the module notes).
+
+    With FP32 residuals and a recompute period k, stage_sweeps must be a multiple of k:
+    requests join and move only at stage boundaries, so every request's FP64 recompute
+    falls on its own k-th sweeps as when it is annealed alone."""
+
+    def __init__(self, sched, precision=FP32, recompute_every=25, stage_sweeps=25, hbm_budget=64 << 30,
+                 max_union_bits=4_000_000, colour_threads=8, group_width=2):
+        self.sched = sched
+        self.total = sched.anneal_sweeps + sched.hold_sweeps
+        self.precision = precision
+        self.k = recompute_every if precision == FP32 else 0
+        if self.k and stage_sweeps % self.k:
+            raise ValueError("stage_sweeps must be a multiple of the recompute period")
+        self.stage_sweeps = stage_sweeps
+        self.budget = hbm_budget
+        self.max_bits = max_union_bits
+        self.group_width = group_width
+        self.pool = ThreadPoolExecutor(colour_threads)
+
+    def _union(self, running, newcomers):
+        """(problem, stage, members) of a new union: the running members (with their own
+        sweeps done) followed by the newcomers."""
+        reqs = [m.req for m in running] + newcomers
+        U, offs = union([q.problem for q in reqs], [q.seed for q in reqs], [q.colour.result() for q in reqs])
+        prob = Problem(U)
+        R = max(q.branches for q in reqs)
+        stage = Stage(prob, self.sched, R, self.precision, self.k, group_width=self.group_width)
+        members = [_Member(q, b0, r0, 0) for q, (b0, r0) in zip(reqs, offs)]
+        return prob, stage, members, len(running)
+
+    def _take(self, queue, now, used, bits):
+        batch = []
+        while queue and queue[0].arrival <= now:
+            q = queue[0]
+            b = device_bytes_estimate(q.problem, q.branches)
+            if (batch or used) and (used + b > self.budget or bits + q.problem.n_vars > self.max_bits):
+                break
+            batch.append(queue.pop(0))
+            used += b
+            bits += q.problem.n_vars
+        return batch
+
+    def run(self, requests):
+        """Serves `requests` in real time; returns per-stage reports."""
+        queue = sorted(requests, key=lambda q: q.arrival)
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0
+        threading.Thread(target=Server._admit, args=(self, list(queue), t0), daemon=True).start()
+        reports = []
+        cur = None                  # (problem, stage, members)
+        prep = None                 # a future of the next union
+        pending_new = []            # the newcomers the union in preparation holds
+        union_index = 0
+        n_left = len(queue)
+        while n_left:
+            if cur is None:
+                if prep is None:
+                    if queue and queue[0].arrival > clock():
+                        time.sleep(queue[0].arrival - clock())
+                    pending_new = self._take(queue, clock(), 0, 0)
+                    Server._wait_admitted(pending_new)
+                    prep = self.pool.submit(self._union, [], pending_new)
+                prob, stage, members, _ = prep.result()
+                prep = None
+                for m in members:
+                    stage.set_span(m.bit0, m.req.problem.n_vars, 0, m.req.branches)
+                cur = (prob, stage, members)
+                union_index += 1
+            prob, stage, members = cur
+            running = [m for m in members if m.req.done < 0]
+            # prepare the next union while this stage runs: when requests are waiting, or
+            # when finished requests hold a large share of the union
+            if prep is None:
+                used = sum(device_bytes_estimate(m.req.problem, m.req.branches) for m in running)
+                bits = sum(m.req.problem.n_vars for m in running)
+                fresh = self._take(queue, clock(), used, bits)
+                idle_bits = sum(m.req.problem.n_vars for m in members if m.req.done >= 0)
+                if fresh or (running and idle_bits > bits):
+                    Server._wait_admitted(fresh)
+                    pending_new = fresh
+                    prep = self.pool.submit(self._union, running, fresh)
+            rep = stage.run(self.stage_sweeps)
+            rep["union"] = union_index
+            rep["requests"] = len(running)
+            reports.append(rep)
+            done_sweeps = stage.sweeps
+            now = clock()
+            for m in members:
+                if m.req.done < 0 and m.start + done_sweeps >= self.total:
+                    out = stage.outputs(m.bit0, m.req.problem.n_vars, m.req.branches, branches=m.req.keep_branches)
+                    m.req.p_void, m.req.tri = out["p_void"], out["tri"]
+                    m.req.branches_out = out["branches"]
+                    m.req.done = clock()
+                    m.req.batch = union_index
+                    n_left -= 1
+            if prep is not None and prep.done():
+                nprob, nstage, nmembers, n_running = prep.result()
+                prep = None
+                old = {id(m.req): m for m in members}
+                for nm in nmembers[:n_running]:
+                    om = old[id(nm.req)]
+                    nm.start = om.start + done_sweeps
+                    nstage.import_span(stage, om.bit0, nm.bit0, nm.req.problem.n_vars, om.ray0, nm.ray0,
+                                       nm.req.problem.n_rays)
+                for nm in nmembers[n_running:]:
+                    nstage.set_span(nm.bit0, nm.req.problem.n_vars, 0, nm.req.branches)
+                stage.close()
+                prob.close()
+                cur = (nprob, nstage, nmembers)
+                union_index += 1
+            elif not any(m.req.done < 0 for m in members) and prep is None:
+                stage.close()
+                prob.close()
+                cur = None
+        return reports
 
 
 def branch_agreement(branches):

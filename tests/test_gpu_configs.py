@@ -9,6 +9,8 @@ CPU oracle with the engine's colouring and the same Philox keys.
   replicas) in the tuned launch shape and at replica blocks 16 and 32, which send the
   instances whose spin words exceed the workgroup's LDS to global spin words.
 - The dense N = 4096 QUBO with 64 replicas, FP32 fields with an FP64 recompute.
+- The serving stream: requests of 2k-235k voxels arriving at 50 per second, served by the
+  continuous server, each against the oracle on that request alone.
 
 Bar: spins bit-exact; p_void and the tri-state map identical; FP64 energies within
 1e-9 relative (the device reduces in another order than the oracle's sequential sum).
@@ -140,3 +142,43 @@ def test_dense_4096_sixty_four_replicas():
         scratch = h.astype(np.float64) + Qd.T @ x.astype(np.float64)
         assert abs(out["field_err"][b] - np.max(np.abs(f - scratch))) < 1e-9
     assert 0 < sum(int(x.sum()) for x, _ in ref) < n * R
+
+
+def test_serving_stream_against_the_oracle():
+    """The serving stream at its request sizes: Poisson arrivals at 50 per second of ray x
+    voxel requests from a pool whose voxels span the configuration's log-uniform [2k,
+    300k] (one of about 250k), with 1k-75k rays and 8-16 branches, served by the
+    continuous server (requests join running unions at stage boundaries). Every request's
+    branches, p_void and tri-state map equal the oracle's colour mode run on that request
+    alone, with its own colouring and its seed as key. The schedule is short so the oracle
+    can run every request."""
+    from qublar_amd.serving import ContinuousServer, Request
+
+    shapes = [(24, 48, 8), (36, 90, 14), (52, 180, 30), (70, 250, 45), (90, 300, 60), (118, 360, 70)]
+    pool = [qa.muon_scene(grid=g, sky_az=az, sky_th=th, log2_muons=24, seed=500 + i)
+            for i, (g, az, th) in enumerate(shapes)]
+    sizes = sorted(P.n_vars for P in pool)
+    say(f"pool voxels {sizes}, rays {sorted(P.n_rays for P in pool)}")
+    assert sizes[0] < 5000 and sizes[-1] >= 200_000
+    rng = np.random.default_rng(5)
+    t, reqs = 0.0, []
+    while t < 0.3:
+        P = pool[len(reqs) % len(pool)] if len(reqs) < len(pool) else pool[int(rng.integers(len(pool)))]
+        reqs.append(Request(P, int(rng.integers(8, 17)), 300 + len(reqs), t, keep_branches=True))
+        t += rng.exponential(1.0 / 50.0)
+    sched = qa.schedule(50.0, 1.0, 10, 5)
+    srv = ContinuousServer(sched, qa.FP32, 5, stage_sweeps=5)
+    reports = srv.run(reqs)
+    say(f"{len(reqs)} requests, {len(reports)} stages, unions joined {[q.batch for q in reqs]}")
+    o = orc_mod.schedule(50.0, 1.0, 10, 5)
+    colourings = {id(P): P.colouring() for P in pool}
+    ops = {id(P): orc_mod.FloatProblem(P.row_ptr, P.ray, P.a, P.d, P.w, P.nbr_ptr, P.nbr, P.lam, P.kappa)
+           for P in pool}
+    for i, q in enumerate(reqs):
+        order, cp = colourings[id(q.problem)]
+        mode = orc_mod.make_mode(orc_mod.RNG_PHILOX, order, cp, orc_mod.FP32, 5)
+        pv, br = orc_mod.branch_fractions(ops[id(q.problem)], o, q.branches, THREADS, q.seed, mode)
+        assert np.array_equal(q.branches_out, br), f"request {i} ({q.problem.n_vars} bits): branches differ"
+        assert np.array_equal(q.p_void, pv)
+        tri = np.where(pv >= np.float32(0.9), 1, np.where(pv <= np.float32(0.1), 0, 2)).astype(np.uint8)
+        assert np.array_equal(q.tri, tri)

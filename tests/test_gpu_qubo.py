@@ -60,3 +60,33 @@ def test_qubo_launch_shapes_do_not_change_results():
                 assert np.array_equal(got["x"][k], base[k]), (mode, rb, k)
             if mode == 1:
                 assert got["report"]["launches"] == len(inst)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_recorded_graph_replays_equal_eager(mode):
+    """The launch schedule recorded into a hipGraph and replayed: every replay, with eager
+    anneals (other schedules, other shapes) and synchronisations between the replays,
+    gives the eager result, spins and energies bit for bit; a new schedule length records
+    a new graph."""
+    inst = instances()
+    B = qa.QuboBatch(inst)
+    sched = qa.schedule(4.0, 0.1, 14, 2)
+    other = qa.schedule(2.0, 0.2, 9, 1)
+    # a recording holds the workspace it was made on, and a call that grows the workspace
+    # records anew: grow it to the largest shape first
+    B.anneal(other, mode=mode, replica_block=4, block=1024)
+    eager = B.anneal(sched, mode=mode, replica_block=8, block=256)
+    for rep in range(4):
+        say(f"mode {mode}: graph replay {rep}")
+        g = B.anneal(sched, mode=mode, replica_block=8, block=256, graph=True)
+        assert g["report"]["graph_replayed"] == (1 if rep > 0 else 0)
+        for k in range(len(inst)):
+            assert np.array_equal(g["x"][k], eager["x"][k]), (rep, k)
+            assert np.array_equal(g["energy"][k], eager["energy"][k]), (rep, k)
+        B.anneal(other, mode=mode, replica_block=4, block=1024)
+        B.anneal(other, mode=mode, replica_block=8, block=256)
+    g2 = B.anneal(other, mode=mode, replica_block=8, block=256, graph=True)
+    assert g2["report"]["graph_replayed"] == 0
+    e2 = B.anneal(other, mode=mode, replica_block=8, block=256)
+    for k in range(len(inst)):
+        assert np.array_equal(g2["x"][k], e2["x"][k])

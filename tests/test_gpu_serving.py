@@ -76,3 +76,78 @@ def test_server_results_equal_alone():
         alone = qa.Problem(q.problem).anneal_branches(sched, q.branches, q.seed, qa.FP32, 4)
         assert np.array_equal(q.p_void, alone["p_void"]), f"request with {q.branches} branches"
         assert np.array_equal(q.tri, alone["tri"])
+
+
+@pytest.mark.parametrize("precision,k", [(qa.FP64, 0), (qa.FP32, 4)])
+def test_requests_joining_running_unions_equal_alone(precision, k):
+    """Staged solves: request A runs alone for 8 sweeps, B joins it (A's state imported),
+    then C joins both; every request's branches, p_void and tri-state equal its solve
+    alone, although A and B changed unions mid-schedule and each union anneals 16
+    branches while B asks for 8 and C for 12."""
+    probs, keys = requests()
+    branches = [16, 8, 12]
+    sched = qa.schedule(20.0, 1.0, 18, 6)
+    total = 24
+    alone = [qa.Problem(P).anneal_branches(sched, R, key, precision, k) for P, key, R in zip(probs, keys, branches)]
+    held = []
+
+    def stage_of(members):
+        U, offs = qa.union([probs[q] for q in members], [keys[q] for q in members])
+        prob = qa.Problem(U)
+        st = qa.Stage(prob, sched, max(branches[q] for q in members), precision, k, group_width=2)
+        held.append((prob, st))
+        return st, dict(zip(members, offs))
+
+    s1, o1 = stage_of([0])
+    s1.set_span(o1[0][0], probs[0].n_vars, 0, branches[0])
+    s1.run(4)
+    s1.run(4)
+    say("A: 8 sweeps alone")
+    s2, o2 = stage_of([0, 1])
+    s2.import_span(s1, o1[0][0], o2[0][0], probs[0].n_vars, o1[0][1], o2[0][1], probs[0].n_rays)
+    s2.set_span(o2[1][0], probs[1].n_vars, 0, branches[1])
+    s2.run(8)
+    say("A and B: 8 sweeps together")
+    s3, o3 = stage_of([1, 2, 0])
+    for q, src in ((0, (s2, o2)), (1, (s2, o2))):
+        st, off = src
+        s3.import_span(st, off[q][0], o3[q][0], probs[q].n_vars, off[q][1], o3[q][1], probs[q].n_rays)
+    s3.set_span(o3[2][0], probs[2].n_vars, 0, branches[2])
+    for _ in range(total // 4):
+        s3.run(4)
+    say("A, B and C: to the end of C's schedule")
+    for q, P in enumerate(probs):
+        out = s3.outputs(o3[q][0], P.n_vars, branches[q])
+        assert np.array_equal(out["branches"], alone[q]["branches"]), f"request {q}"
+        assert np.array_equal(out["p_void"], alone[q]["p_void"])
+        assert np.array_equal(out["tri"], alone[q]["tri"])
+        assert np.array_equal(out["certainty"], alone[q]["certainty"])
+    # a span moves only at a multiple of the recompute period
+    if k:
+        s4, o4 = stage_of([0])
+        s5, o5 = stage_of([0])
+        s4.run(2)
+        with pytest.raises(qa.QublarError):
+            s5.import_span(s4, 0, 0, probs[0].n_vars, 0, 0, probs[0].n_rays)
+
+
+def test_continuous_server_results_equal_alone():
+    """Requests arriving over time are served by the continuous server: each joins a
+    running union at a stage boundary (the unions and join points depend on timing), and
+    each request's branches, p_void and tri-state equal its solve alone."""
+    from qublar_amd.serving import ContinuousServer, Request
+
+    probs, keys = requests()
+    sched = qa.schedule(20.0, 1.0, 30, 10)
+    specs = [(0, 16, 0.0), (1, 8, 0.02), (2, 12, 0.04), (0, 10, 0.06), (1, 16, 0.3), (2, 9, 0.31)]
+    reqs = [Request(probs[p], R, 1000 + 7 * i, t, keep_branches=True) for i, (p, R, t) in enumerate(specs)]
+    srv = ContinuousServer(sched, qa.FP32, 4, stage_sweeps=4)
+    reports = srv.run(reqs)
+    unions = sorted({r["union"] for r in reports})
+    say(f"continuous: {len(reports)} stages over unions {unions}; joined unions {[q.batch for q in reqs]}")
+    assert all(q.done >= q.arrival for q in reqs)
+    for i, q in enumerate(reqs):
+        alone = qa.Problem(q.problem).anneal_branches(sched, q.branches, q.seed, qa.FP32, 4)
+        assert np.array_equal(q.branches_out, alone["branches"]), f"request {i}"
+        assert np.array_equal(q.p_void, alone["p_void"])
+        assert np.array_equal(q.tri, alone["tri"])
