@@ -131,13 +131,24 @@ def serve(mode, pool, sched, rate, seconds, seed=11, stage_sweeps=25, k=25):
     import qublar_amd as qa
     from qublar_amd.serving import ContinuousServer, Server, latency_summary
 
+    import threading
+
     reqs = serving_stream(pool, rate, seconds, seed)
     total = sched.anneal_sweeps + sched.hold_sweeps
     t0 = time.perf_counter()
+    running = threading.Event()
+
+    def progress():   # a line every 20 s while the stream is served
+        while not running.wait(20.0):
+            log(f"serving {mode}: {sum(q.done >= 0 for q in reqs)} of {len(reqs)} requests done, "
+                f"{time.perf_counter() - t0:.0f} s")
+
+    threading.Thread(target=progress, daemon=True).start()
     if mode == "continuous":
         reports = ContinuousServer(sched, qa.FP32, k, stage_sweeps=stage_sweeps).run(reqs)
     else:
         reports = Server(sched, qa.FP32, k).run(reqs, batched=mode == "unions")
+    running.set()
     wall = time.perf_counter() - t0
     updates = sum(q.branches * total * q.problem.n_vars for q in reqs)
     kms = np.array([r["kernel_ms"] for r in reports])
