@@ -125,7 +125,7 @@ def serving_stream(pool, rate, seconds, seed):
     return reqs
 
 
-def serve(mode, pool, sched, rate, seconds, seed=11, stage_sweeps=25, k=25):
+def serve(mode, pool, sched, rate, seconds, seed=11, stage_sweeps=10, k=10, max_union_bits=2_000_000):
     """Serves one stream in `mode` (continuous, unions or sequential); returns the summary
     and the requests (each with its results)."""
     import qublar_amd as qa
@@ -144,8 +144,10 @@ def serve(mode, pool, sched, rate, seconds, seed=11, stage_sweeps=25, k=25):
                 f"{time.perf_counter() - t0:.0f} s")
 
     threading.Thread(target=progress, daemon=True).start()
+    srv = None
     if mode == "continuous":
-        reports = ContinuousServer(sched, qa.FP32, k, stage_sweeps=stage_sweeps).run(reqs)
+        srv = ContinuousServer(sched, qa.FP32, k, stage_sweeps=stage_sweeps, max_union_bits=max_union_bits)
+        reports = srv.run(reqs)
     else:
         reports = Server(sched, qa.FP32, k).run(reqs, batched=mode == "unions")
     running.set()
@@ -161,7 +163,12 @@ def serve(mode, pool, sched, rate, seconds, seed=11, stage_sweeps=25, k=25):
            "spin_updates_per_s": updates / wall, "latency_ms": latency_summary(reqs)}
     if mode == "continuous":
         out["stage_sweeps"] = stage_sweeps
+        out["max_union_bits"] = max_union_bits
         out["unions"] = len({r["union"] for r in reports})
+        pl = srv.prep_log
+        out["union_prep_s"] = {k: float(np.mean([p[k] for p in pl])) for k in ("assemble_s", "upload_s", "stage_s")}
+        out["union_bits_mean"] = float(np.mean([p["bits"] for p in pl]))
+        out["union_requests_mean"] = float(np.mean([p["requests"] for p in pl]))
     return out, reqs
 
 
@@ -193,7 +200,8 @@ def run_serving(args):
     sched = qa.schedule(50.0, 1.0, 200, 50)
     energies = {}
     for mode in ("continuous", "unions", "sequential"):
-        out, reqs = serve(mode, pool, sched, args.rate, args.serve_seconds, stage_sweeps=args.stage_sweeps)
+        out, reqs = serve(mode, pool, sched, args.rate, args.serve_seconds, stage_sweeps=args.stage_sweeps,
+                          k=args.stage_sweeps)
         energies[mode] = final_energies(pool, reqs)
         same = all(np.array_equal(a, b) for a, b in zip(energies[mode], energies["continuous"]))
         out.update({"workload": "serving", "same_final_energies_as_continuous": same,
@@ -291,27 +299,6 @@ def run_gravity_precision(args):
               "scene_s": t_scene, "upload_s": t_upload, "rows": [r for r, _ in rows],
               "fp32_k10_more_branches": wide,
               "data": "synthetic: the body's truncated anomaly plus N(0, 10 microGal) noise"}
-    if args.headline_serve_seconds > 0:
-        # the metric's batched solve latency: the serving stream (50 requests/s, 2k-300k
-        # voxels, 8-16 branches, the same schedule) under continuous batching, beside one
-        # solve per launch of the same requests; both end at the same final energies
-        pool = serving_pool(args.pool, 7)
-        lat = {}
-        for mode in ("continuous", "sequential"):
-            o, reqs = serve(mode, pool, sched, args.rate, args.headline_serve_seconds, stage_sweeps=args.stage_sweeps)
-            lat[mode] = (o, final_energies(pool, reqs, 16))
-            log(f"serving {mode}: p50 {o['latency_ms']['p50_ms']:.0f} ms, p99 {o['latency_ms']['p99_ms']:.0f} ms")
-        result["batched_latency_ms"] = {
-            "p50": lat["continuous"][0]["latency_ms"]["p50_ms"], "p99": lat["continuous"][0]["latency_ms"]["p99_ms"],
-            "sequential_p50": lat["sequential"][0]["latency_ms"]["p50_ms"],
-            "sequential_p99": lat["sequential"][0]["latency_ms"]["p99_ms"],
-            "requests": lat["continuous"][0]["requests"], "arrival_window_s": args.headline_serve_seconds,
-            "offered_req_per_s": args.rate,
-            "served_spin_updates_per_s": lat["continuous"][0]["spin_updates_per_s"],
-            "sequential_spin_updates_per_s": lat["sequential"][0]["spin_updates_per_s"],
-            "same_final_energies": all(np.array_equal(a, b) for a, b in zip(lat["continuous"][1], lat["sequential"][1])),
-            "kind": "serving stream, continuous batching vs one solve per launch",
-        }
     if not args.no_cpu:
         threads = max(1, min(16, (os.cpu_count() or 2) - 2))
         log(f"cpu baseline on {threads} threads ...")
@@ -665,7 +652,8 @@ def main():
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window")
     ap.add_argument("--pool", type=int, default=24, help="serving: distinct request problems")
-    ap.add_argument("--stage-sweeps", type=int, default=25, help="serving: sweeps per continuous-batching stage")
+    ap.add_argument("--stage-sweeps", type=int, default=10,
+                    help="serving: sweeps per continuous-batching stage (also the FP64 recompute period there)")
     ap.add_argument("--headline-serve-seconds", type=float, default=8.0,
                     help="muon: arrival window of the batched-serving latency beside the headline (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -790,20 +778,25 @@ def main():
         # solve per launch of the same requests; both end at the same final energies
         pool = serving_pool(args.pool, 7)
         lat = {}
-        for mode in ("continuous", "sequential"):
-            o, reqs = serve(mode, pool, sched, args.rate, args.headline_serve_seconds, stage_sweeps=args.stage_sweeps)
+        for mode in ("unions", "continuous", "sequential"):
+            o, reqs = serve(mode, pool, sched, args.rate, args.headline_serve_seconds, stage_sweeps=args.stage_sweeps,
+                            k=args.stage_sweeps)
             lat[mode] = (o, final_energies(pool, reqs, 16))
             log(f"serving {mode}: p50 {o['latency_ms']['p50_ms']:.0f} ms, p99 {o['latency_ms']['p99_ms']:.0f} ms")
+        same = all(all(np.array_equal(a, b) for a, b in zip(lat[m][1], lat["sequential"][1]))
+                   for m in ("unions", "continuous"))
+
+        def summary(m):
+            return {"p50": lat[m][0]["latency_ms"]["p50_ms"], "p99": lat[m][0]["latency_ms"]["p99_ms"],
+                    "served_spin_updates_per_s": lat[m][0]["spin_updates_per_s"]}
+
         result["batched_latency_ms"] = {
-            "p50": lat["continuous"][0]["latency_ms"]["p50_ms"], "p99": lat["continuous"][0]["latency_ms"]["p99_ms"],
-            "sequential_p50": lat["sequential"][0]["latency_ms"]["p50_ms"],
-            "sequential_p99": lat["sequential"][0]["latency_ms"]["p99_ms"],
-            "requests": lat["continuous"][0]["requests"], "arrival_window_s": args.headline_serve_seconds,
-            "offered_req_per_s": args.rate,
-            "served_spin_updates_per_s": lat["continuous"][0]["spin_updates_per_s"],
-            "sequential_spin_updates_per_s": lat["sequential"][0]["spin_updates_per_s"],
-            "same_final_energies": all(np.array_equal(a, b) for a, b in zip(lat["continuous"][1], lat["sequential"][1])),
-            "kind": "serving stream, continuous batching vs one solve per launch",
+            "p50": lat["unions"][0]["latency_ms"]["p50_ms"], "p99": lat["unions"][0]["latency_ms"]["p99_ms"],
+            "kind": "serving stream (50 requests/s, 2k-300k voxels, 8-16 branches, this schedule): requests "
+                    "batched into block-diagonal unions, one launch per union",
+            "continuous_batching": summary("continuous"), "one_solve_per_launch": summary("sequential"),
+            "requests": lat["unions"][0]["requests"], "arrival_window_s": args.headline_serve_seconds,
+            "offered_req_per_s": args.rate, "same_final_energies_in_every_mode": same,
         }
     if not args.no_cpu:
         threads = max(1, min(16, (os.cpu_count() or 2) - 2))

@@ -169,8 +169,8 @@ class ContinuousServer:
     requests join and move only at stage boundaries, so every request's FP64 recompute
     falls on its own k-th sweeps as when it is annealed alone."""
 
-    def __init__(self, sched, precision=FP32, recompute_every=25, stage_sweeps=25, hbm_budget=64 << 30,
-                 max_union_bits=4_000_000, colour_threads=8, group_width=2):
+    def __init__(self, sched, precision=FP32, recompute_every=10, stage_sweeps=10, hbm_budget=64 << 30,
+                 max_union_bits=2_000_000, colour_threads=8, group_width=2):
         self.sched = sched
         self.total = sched.anneal_sweeps + sched.hold_sweeps
         self.precision = precision
@@ -182,26 +182,40 @@ class ContinuousServer:
         self.max_bits = max_union_bits
         self.group_width = group_width
         self.pool = ThreadPoolExecutor(colour_threads)
+        self.prep_log = []
 
     def _union(self, running, newcomers):
         """(problem, stage, members) of a new union: the running members (with their own
         sweeps done) followed by the newcomers."""
+        t0 = time.perf_counter()
         reqs = [m.req for m in running] + newcomers
         U, offs = union([q.problem for q in reqs], [q.seed for q in reqs], [q.colour.result() for q in reqs])
+        t1 = time.perf_counter()
         prob = Problem(U)
+        t2 = time.perf_counter()
         R = max(q.branches for q in reqs)
         stage = Stage(prob, self.sched, R, self.precision, self.k, group_width=self.group_width)
+        t3 = time.perf_counter()
+        self.prep_log.append({"bits": U.n_vars, "requests": len(reqs), "assemble_s": t1 - t0, "upload_s": t2 - t1,
+                              "stage_s": t3 - t2})
         members = [_Member(q, b0, r0, 0) for q, (b0, r0) in zip(reqs, offs)]
         return prob, stage, members, len(running)
 
-    def _take(self, queue, now, used, bits):
+    def _take(self, queue, now, used, bits, ready_only=True):
+        """The arrived requests that fit the budget, in arrival order; with ready_only,
+        only those whose colouring is done (the rest wait for the next union, so the
+        stage loop never waits on a host colouring)."""
         batch = []
-        while queue and queue[0].arrival <= now:
-            q = queue[0]
+        k = 0
+        while k < len(queue) and queue[k].arrival <= now:
+            q = queue[k]
+            if ready_only and (q.colour is None or not q.colour.done()):
+                k += 1
+                continue
             b = device_bytes_estimate(q.problem, q.branches)
             if (batch or used) and (used + b > self.budget or bits + q.problem.n_vars > self.max_bits):
                 break
-            batch.append(queue.pop(0))
+            batch.append(queue.pop(k))
             used += b
             bits += q.problem.n_vars
         return batch
@@ -215,17 +229,18 @@ class ContinuousServer:
         reports = []
         cur = None                  # (problem, stage, members)
         prep = None                 # a future of the next union
-        pending_new = []            # the newcomers the union in preparation holds
+        leaving = []                # running members the union in preparation leaves out
         union_index = 0
         n_left = len(queue)
+        prep_est, stage_est = 0.5, 0.05   # running estimates of a preparation and a stage, s
         while n_left:
             if cur is None:
                 if prep is None:
                     if queue and queue[0].arrival > clock():
                         time.sleep(queue[0].arrival - clock())
-                    pending_new = self._take(queue, clock(), 0, 0)
-                    Server._wait_admitted(pending_new)
-                    prep = self.pool.submit(self._union, [], pending_new)
+                    fresh = self._take(queue, clock(), 0, 0, ready_only=False)
+                    Server._wait_admitted(fresh)
+                    prep = self.pool.submit(self._union, [], fresh)
                 prob, stage, members, _ = prep.result()
                 prep = None
                 for m in members:
@@ -234,18 +249,25 @@ class ContinuousServer:
                 union_index += 1
             prob, stage, members = cur
             running = [m for m in members if m.req.done < 0]
-            # prepare the next union while this stage runs: when requests are waiting, or
-            # when finished requests hold a large share of the union
+            # Prepare the next union while this stage runs, when requests are waiting or
+            # finished ones hold a large share of the union. It leaves out the members that
+            # will finish before it is ready (their remaining sweeps fit the stages a
+            # preparation takes), and the switch waits for them, so finished requests do
+            # not idle in a union.
             if prep is None:
-                used = sum(device_bytes_estimate(m.req.problem, m.req.branches) for m in running)
-                bits = sum(m.req.problem.n_vars for m in running)
+                horizon = self.stage_sweeps * (int(np.ceil(prep_est / max(stage_est, 1e-3))) + 1)
+                keep = [m for m in running if self.total - (m.start + stage.sweeps) > horizon]
+                used = sum(device_bytes_estimate(m.req.problem, m.req.branches) for m in keep)
+                bits = sum(m.req.problem.n_vars for m in keep)
                 fresh = self._take(queue, clock(), used, bits)
                 idle_bits = sum(m.req.problem.n_vars for m in members if m.req.done >= 0)
-                if fresh or (running and idle_bits > bits):
+                if (fresh or (running and idle_bits > bits)) and (keep or fresh):
                     Server._wait_admitted(fresh)
-                    pending_new = fresh
-                    prep = self.pool.submit(self._union, running, fresh)
+                    keep_ids = {id(m) for m in keep}
+                    leaving = [m for m in running if id(m) not in keep_ids]
+                    prep = self.pool.submit(self._union, keep, fresh)
             rep = stage.run(self.stage_sweeps)
+            stage_est = 0.8 * stage_est + 0.2 * rep["wall_s"]
             rep["union"] = union_index
             rep["requests"] = len(running)
             reports.append(rep)
@@ -259,9 +281,12 @@ class ContinuousServer:
                     m.req.done = clock()
                     m.req.batch = union_index
                     n_left -= 1
-            if prep is not None and prep.done():
+            if prep is not None and prep.done() and all(m.req.done >= 0 for m in leaving):
                 nprob, nstage, nmembers, n_running = prep.result()
                 prep = None
+                leaving = []
+                last = self.prep_log[-1]
+                prep_est = 0.7 * prep_est + 0.3 * (last["assemble_s"] + last["upload_s"] + last["stage_s"])
                 old = {id(m.req): m for m in members}
                 for nm in nmembers[:n_running]:
                     om = old[id(nm.req)]
