@@ -198,6 +198,25 @@ def run_serving(args):
     pool = serving_pool(args.pool, 7)
     log("pool sizes:", sorted(P.n_vars for P in pool))
     sched = qa.schedule(50.0, 1.0, 200, 50)
+    # the staged sweep kernel against the plain one on one union of 40 pool requests: 25
+    # sweeps each, every bit at its own sweep 0 in the staged form
+    rng = np.random.default_rng(3)
+    probs = [pool[int(i)] for i in rng.integers(0, len(pool), 40)]
+    U, offs = qa.union(probs, list(range(100, 140)))
+    prob = qa.Problem(U)
+    plain, staged = [], []
+    for _ in range(3):
+        plain.append(prob.anneal_branches(qa.schedule(50.0, 1.0, 20, 5), 16, 0, qa.FP32, 25, branches=False,
+                                          outputs=False, energy=False)["report"]["kernel_ms"])
+        st = qa.Stage(prob, sched, 16, qa.FP32, 25, group_width=2)
+        for (b0, _), P in zip(offs, probs):
+            st.set_span(b0, P.n_vars, 0, 12)
+        staged.append(st.run(25)["kernel_ms"])
+        st.close()
+    prob.close()
+    print(json.dumps({"workload": "serving", "staged_kernel_overhead": {
+        "union_bits": U.n_vars, "sweeps": 25, "plain_kernel_ms": float(np.median(plain)),
+        "staged_kernel_ms": float(np.median(staged))}}), flush=True)
     energies = {}
     for mode in ("continuous", "unions", "sequential"):
         out, reqs = serve(mode, pool, sched, args.rate, args.serve_seconds, stage_sweeps=args.stage_sweeps,
@@ -511,6 +530,23 @@ def run_batched(args):
     per = B.anneal(short, mode=1, replica_block=rb, block=blk, x=False, energy=False)["report"]
     log(f"one launch per instance, {short.anneal_sweeps} sweeps: kernel {per['kernel_ms']:.1f} ms, "
         f"{per['launches']} launches")
+    # the short schedule's launches recorded into a hipGraph and replayed (the first call
+    # records): host wall per call, eager against replayed, in both modes
+    graphs = {}
+    for mode in (0, 1):
+        eager, replay = [], []
+        B.anneal(short, mode=mode, replica_block=rb, block=blk, x=False, energy=False, graph=True)   # records
+        for _ in range(3):
+            eager.append(B.anneal(short, mode=mode, replica_block=rb, block=blk, x=False, energy=False)["report"])
+            replay.append(B.anneal(short, mode=mode, replica_block=rb, block=blk, x=False, energy=False,
+                                   graph=True)["report"])
+        assert all(r["graph_replayed"] for r in replay)
+        graphs["persistent" if mode == 0 else "per_instance"] = {
+            "eager_wall_ms": float(np.mean([r["wall_s"] for r in eager])) * 1e3,
+            "replay_wall_ms": float(np.mean([r["wall_s"] for r in replay])) * 1e3,
+            "eager_kernel_ms": float(np.mean([r["kernel_ms"] for r in eager])),
+            "replay_kernel_ms": float(np.mean([r["kernel_ms"] for r in replay]))}
+        log(f"mode {mode}: eager {graphs['persistent' if mode == 0 else 'per_instance']}")
     upd_short = float((n * R).sum()) * short.anneal_sweeps
     kern = float(np.mean([r["kernel_ms"] for r in runs]))
     wall = float(np.mean([r["wall_s"] for r in runs]))
@@ -531,6 +567,7 @@ def run_batched(args):
                                   "updates_per_s": upd_short / (per["kernel_ms"] * 1e-3), "launches": per["launches"],
                                   "persistent_kernel_ms": pers_short["kernel_ms"]},
         "persistent_speedup_vs_per_instance": per["kernel_ms"] / pers_short["kernel_ms"],
+        "recorded_graph": dict(graphs, sweeps=short.anneal_sweeps),
         "replica_block": rb,
         "block": blk,
         "jobs": runs[-1]["jobs"],

@@ -455,10 +455,11 @@ typedef struct {
     int block;                /* threads per workgroup: 256 or 1024; 0 with replica_block 0: the
                                  (block, replica block) shape that runs a short prefix fastest */
     int graph;                /* 1: the anneal's launch schedule (state reset, the persistent launch or
-                                 the per-instance launches) is recorded once into a hipGraph per
-                                 (shape, mode, sweeps, workgroups) and replayed by later calls, which
-                                 then make one graph launch; results are those of graph = 0. No
-                                 counterpart in the reference, whose chunked relaunch loop is plain
+                                 the per-instance launches, these as 4 independent branches that may
+                                 overlap on the device) is recorded once into a hipGraph per (shape,
+                                 mode, sweeps, workgroups) and replayed by later calls, which then make
+                                 one graph launch; results are those of graph = 0. No counterpart in
+                                 the reference, whose chunked relaunch loop is plain
                                  (src/ising_gpu.cuh:424-445) */
 } qb_qubo_opts;
 

@@ -1715,8 +1715,17 @@ struct qb_qubo_batch {
         hipGraphExec_t exec = nullptr;
     };
     std::vector<Recorded> recorded;
+    // side streams and events of a recording's parallel branches (one launch per instance)
+    static constexpr int kBranches = 4;
+    hipStream_t side[kBranches] = {};
+    hipEvent_t fork = nullptr, join[kBranches] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~qb_qubo_batch() {
+        for (int i = 0; i < kBranches; ++i) {
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (join[i]) (void)hipEventDestroy(join[i]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
         for (Recorded& r : recorded) {
             if (r.exec) (void)hipGraphExecDestroy(r.exec);
             if (r.graph) (void)hipGraphDestroy(r.graph);
@@ -1892,6 +1901,11 @@ extern "C" int qb_qubo_batch_create(const qb_qubo_batch_desc* d, qb_qubo_batch**
     QB_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
     QB_HIP(hipEventCreate(&B->ev0));
     QB_HIP(hipEventCreate(&B->ev1));
+    QB_HIP(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+    for (int i = 0; i < qb_qubo_batch::kBranches; ++i) {
+        QB_HIP(hipStreamCreateWithFlags(&B->side[i], hipStreamNonBlocking));
+        QB_HIP(hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming));
+    }
     hipStream_t st = B->stream;
     int rc;
     if ((rc = B->b_inst_pos.upload(B->inst_pos, st)) || (rc = B->b_inst_cls.upload(B->inst_cls, st)) ||
@@ -1978,12 +1992,19 @@ static DevQuboBatch qubo_dev(const qb_qubo_batch* B, const qb_qubo_batch::Jobs& 
 
 // One anneal of the whole batch at replica block rb: spin words cleared (all-zero x),
 // then the persistent launch or one launch per instance, timed with events.
-// The anneal's launches: the persistent launch, or one per instance.
+// The anneal's launches: the persistent launch, or one per instance. With `branches`
+// (while a graph is recorded) the per-instance launches go round-robin to side streams
+// forked from and joined back into the recorded stream, so the graph holds independent
+// branches whose launches may overlap on the device.
 static int qubo_launches(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int block, int n_sweeps, int mode,
-                         int workgroups, int* launches) {
+                         int workgroups, int* launches, bool branches = false) {
     hipStream_t st = B->stream;
     const size_t set = size_t(kXcds) * size_t(kQueueStride);
     *launches = 0;
+    if (mode == 1 && branches) {
+        QB_HIP(hipEventRecord(B->fork, st));
+        for (int i = 0; i < qb_qubo_batch::kBranches; ++i) QB_HIP(hipStreamWaitEvent(B->side[i], B->fork, 0));
+    }
     if (mode == 0) {
         const DevQuboBatch q = qubo_dev(B, Jb, true);
         QB_HIP(launch_qubo_batch(q, rb, block, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
@@ -1996,10 +2017,16 @@ static int qubo_launches(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb
             QuboQueues one{};   // the instance's jobs in queue 0, the others empty
             one.begin[0] = f;
             for (int x = 1; x <= kXcds; ++x) one.begin[x] = f + cnt;
+            hipStream_t ls = branches ? B->side[k % qb_qubo_batch::kBranches] : st;
             QB_HIP(launch_qubo_batch(q, rb, block, B->w_mask.as<uint32_t>(), B->w_beta.as<double>(), n_sweeps,
-                                     B->w_queue.as<unsigned>() + set * size_t(k), one, cnt, st));
+                                     B->w_queue.as<unsigned>() + set * size_t(k), one, cnt, ls));
             ++*launches;
         }
+        if (branches)
+            for (int i = 0; i < qb_qubo_batch::kBranches; ++i) {
+                QB_HIP(hipEventRecord(B->join[i], B->side[i]));
+                QB_HIP(hipStreamWaitEvent(st, B->join[i], 0));
+            }
     }
     return QB_OK;
 }
@@ -2035,7 +2062,7 @@ static int qubo_run(qb_qubo_batch* B, const qb_qubo_batch::Jobs& Jb, int rb, int
             int lrc = QB_OK;
             hipError_t e = launch_fill_u32(B->w_mask.as<uint32_t>(), size_t(Jb.mask_words), 0u, st);
             if (e == hipSuccess) e = launch_fill_u32(B->w_queue.as<uint32_t>(), queue_bytes / sizeof(uint32_t), 0u, st);
-            if (e == hipSuccess) lrc = qubo_launches(B, Jb, rb, block, n_sweeps, mode, workgroups, launches);
+            if (e == hipSuccess) lrc = qubo_launches(B, Jb, rb, block, n_sweeps, mode, workgroups, launches, true);
             const hipError_t ee = hipStreamEndCapture(st, &r.graph);
             if (e != hipSuccess) return fail(QB_E_HIP, "recording the anneal's graph: %s", hipGetErrorString(e));
             if (lrc) {
