@@ -343,6 +343,15 @@ int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* view);
 int qb_host_problem_truth(const qb_host_problem* h, uint8_t* truth);
 int qb_host_problem_destroy(qb_host_problem* h);
 
+/* The reduced problem of freeze_provable_rock (src/ising_recon.hpp:206-233), on the host, from
+   the problem and a frozen set (qb_freeze's): the unfrozen bits in original order with their
+   entries, every ray with its d and w, the neighbour lists between kept bits, and each kept
+   bit's field lin + lambda * (its frozen neighbours). kept[n_kept] (nullable; room for n_vars)
+   maps reduced bits to original ones. The result is a host problem (qb_host_problem_view;
+   its view carries the field). */
+int qb_reduce_frozen(const qb_problem_desc* desc, const uint8_t* frozen, qb_host_problem** out, int32_t* kept,
+                     int* n_kept);
+
 /* The gravimetry scene: the operator of src/op_gravity.hpp:17-74 (a cell of volume h^3
    and contrast drho is a point mass at its centre; a_sb = G drho h^3 (z_s - z_b) / r^3 in
    microGal, one row per station, w = 1 / sigma^2) with every row truncated at `radius`

@@ -185,6 +185,7 @@ def lib():
             "qb_host_problem_view": (i32, [vp, C.POINTER(ProblemDesc)]),
             "qb_host_problem_truth": (i32, [vp, vp]),
             "qb_host_problem_destroy": (i32, [vp]),
+            "qb_reduce_frozen": (i32, [C.POINTER(ProblemDesc), vp, C.POINTER(vp), vp, C.POINTER(i32)]),
             "qb_gravity_scene_defaults": (i32, [C.POINTER(GravityScene)]),
             "qb_scene_gravity": (i32, [C.POINTER(GravityScene), C.POINTER(vp)]),
             "qb_host_problem_colour": (i32, [vp, vp]),
@@ -320,6 +321,20 @@ class ProblemArrays:
     def kwargs(self):
         return dict(row_ptr=self.row_ptr, ray=self.ray, a=self.a, d=self.d, w=self.w, nbr_ptr=self.nbr_ptr,
                     nbr=self.nbr, lam=self.lam, kappa=self.kappa, field=self.field)
+
+    def reduce_frozen(self, frozen):
+        """freeze_provable_rock's reduced problem (src/ising_recon.hpp:206-233) for a frozen
+        set: (ProblemArrays of the unfrozen bits with the frozen bonds folded into their
+        field, kept: reduced bit -> original bit)."""
+        frozen = np.ascontiguousarray(frozen, np.uint8)
+        kept = np.zeros(self.n_vars, np.int32)
+        m = C.c_int()
+        h = C.c_void_p()
+        d = self.desc()
+        _check(lib().qb_reduce_frozen(C.byref(d), _p(frozen), C.byref(h), _p(kept), C.byref(m)))
+        R = _host_problem(h, False)
+        R.truth = self.truth[kept[:m.value]] if self.truth is not None else None
+        return R, kept[:m.value].copy()
 
     def colouring(self):
         """(order, class_ptr) of the engine's colour classes, computed on the host."""
@@ -566,9 +581,10 @@ def _host_problem(h, coloured):
         if coloured:
             colour = np.zeros(n, np.int32)
             _check(lib().qb_host_problem_colour(h, _p(colour)))
+        field = arr(v.field, n, np.float64) if v.field else None
         return ProblemArrays(row_ptr, arr(v.ray, int(row_ptr[-1]), np.int32), arr(v.a, int(row_ptr[-1]), np.float32),
                              arr(v.d, nr, np.float64), arr(v.w, nr, np.float64), nbr_ptr,
-                             arr(v.nbr, int(nbr_ptr[-1]), np.int32), v.lam, v.kappa, None, truth, colour=colour)
+                             arr(v.nbr, int(nbr_ptr[-1]), np.int32), v.lam, v.kappa, field, truth, colour=colour)
     finally:
         lib().qb_host_problem_destroy(h)
 

@@ -15,4 +15,5 @@ struct qb_host_problem {
     double lambda = 0.0, kappa = 0.0;
     std::vector<uint8_t> truth;
     std::vector<int32_t> colour;   // empty unless the builder colours its problem
+    std::vector<double> field;     // per-bit linear field; empty: lin = kappa
 };

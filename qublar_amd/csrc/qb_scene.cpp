@@ -436,6 +436,60 @@ int gravity_finish(const qb_gravity_scene* s, qb_host_problem* H) {
 
 }  // namespace qb
 
+// The reduced problem of freeze_provable_rock (src/ising_recon.hpp:206-233): the unfrozen
+// bits in original order with their entries, every ray with its d and w, the neighbour
+// lists between kept bits, and each kept bit's field lin + lambda * (its frozen
+// neighbours), the bonds to frozen rock folded in.
+extern "C" int qb_reduce_frozen(const qb_problem_desc* d, const uint8_t* frozen, qb_host_problem** out, int32_t* kept,
+                                int* n_kept) {
+    if (!d || !frozen || !out || !d->row_ptr || !d->nbr_ptr || (d->n_vars > 0 && (!d->ray || !d->a)) ||
+        (d->n_rays > 0 && (!d->d || !d->w)))
+        return QB_E_ARG;
+    *out = nullptr;
+    const int n = d->n_vars;
+    auto H = std::unique_ptr<qb_host_problem>(new (std::nothrow) qb_host_problem());
+    if (!H) return QB_E_NOMEM;
+    std::vector<int32_t> new_index(size_t(std::max(n, 0)), -1);
+    int m = 0;
+    for (int i = 0; i < n; ++i)
+        if (!frozen[i]) new_index[size_t(i)] = m++;
+    H->n_vars = m;
+    H->n_rays = d->n_rays;
+    H->d.assign(d->d, d->d + d->n_rays);
+    H->w.assign(d->w, d->w + d->n_rays);
+    H->row_ptr.reserve(size_t(m) + 1);
+    H->nbr_ptr.reserve(size_t(m) + 1);
+    H->row_ptr.push_back(0);
+    H->nbr_ptr.push_back(0);
+    std::vector<double> field;
+    field.reserve(size_t(m));
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+        if (frozen[i]) continue;
+        if (kept) kept[k] = i;
+        ++k;
+        for (int e = d->row_ptr[i]; e < d->row_ptr[i + 1]; ++e) {
+            H->ray.push_back(d->ray[e]);
+            H->a.push_back(d->a[e]);
+        }
+        H->row_ptr.push_back(int32_t(H->ray.size()));
+        int rock = 0;
+        for (int e = d->nbr_ptr[i]; e < d->nbr_ptr[i + 1]; ++e) {
+            if (frozen[d->nbr[e]]) ++rock;
+            else H->nbr.push_back(new_index[size_t(d->nbr[e])]);
+        }
+        H->nbr_ptr.push_back(int32_t(H->nbr.size()));
+        const double lin = d->field ? d->field[i] : d->kappa;
+        field.push_back(lin + d->lambda * rock);
+    }
+    H->lambda = d->lambda;
+    H->kappa = d->kappa;
+    H->field = std::move(field);
+    if (n_kept) *n_kept = m;
+    *out = H.release();
+    return QB_OK;
+}
+
 extern "C" int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* v) {
     if (!h || !v) return QB_E_ARG;
     v->n_vars = h->n_vars;
@@ -449,7 +503,7 @@ extern "C" int qb_host_problem_view(const qb_host_problem* h, qb_problem_desc* v
     v->nbr = h->nbr.data();
     v->lambda = h->lambda;
     v->kappa = h->kappa;
-    v->field = nullptr;
+    v->field = h->field.empty() ? nullptr : h->field.data();
     v->draw_bit = nullptr;
     v->draw_key = nullptr;
     v->colour = h->colour.empty() ? nullptr : h->colour.data();

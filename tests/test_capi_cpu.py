@@ -160,3 +160,30 @@ def test_graph_refuses_repeated_edges(eu, ev):
     assert "repeats an earlier edge" in qa.lib().qb_last_error().decode()
     ok = qa.GraphArrays(4, [0, 1, 2], [1, 2, 3], np.ones(3, np.int32))
     assert qa.lib().qb_colour_graph(C.byref(ok.desc()), order.ctypes.data, cp.ctypes.data, C.byref(C.c_int())) == 0
+
+
+def test_reduce_frozen_energy_identity(orc):
+    """freeze_provable_rock's reduction (src/ising_recon.hpp:206-233) through the C ABI, on the
+    oracle's frozen set: the kept bits are the unfrozen ones in order, and for any state of the
+    kept bits the reduced energy equals the full energy with the frozen bits at 0 -- the
+    frozen bonds folded into the field exactly (rtol 1e-12: the prior sums in another order)."""
+    P = qa.muon_scene(grid=28, sky_az=60, sky_th=12, log2_muons=22)
+    op = orc.FloatProblem(P.row_ptr, P.ray, P.a, P.d, P.w, P.nbr_ptr, P.nbr, P.lam, P.kappa)
+    frozen = np.zeros(P.n_vars, np.uint8)
+    orc.lib().orc_freeze_mask(op.c, 10.0, frozen)
+    assert 0 < frozen.sum() < P.n_vars
+    R, kept = P.reduce_frozen(frozen)
+    assert np.array_equal(kept, np.flatnonzero(frozen == 0))
+    assert R.n_rays == P.n_rays and R.field is not None
+    # a bit with k frozen neighbours carries kappa + lambda * k
+    nb_frozen = np.add.reduceat(frozen[P.nbr].astype(np.int64), P.nbr_ptr[:-1])
+    nb_frozen[np.diff(P.nbr_ptr) == 0] = 0
+    assert np.allclose(R.field, P.kappa + P.lam * nb_frozen[kept], rtol=0, atol=1e-12)
+    orr = orc.FloatProblem(R.row_ptr, R.ray, R.a, R.d, R.w, R.nbr_ptr, R.nbr, R.lam, R.kappa, R.field)
+    rng = np.random.default_rng(2)
+    for _ in range(6):
+        x = (rng.random(R.n_vars) < rng.uniform(0.05, 0.5)).astype(np.uint8)
+        full = np.zeros(P.n_vars, np.uint8)
+        full[kept] = x
+        e_full, e_red = orc.binary_energy(op, full), orc.binary_energy(orr, x)
+        assert abs(e_full - e_red) <= 1e-12 * max(1.0, abs(e_full)), (e_full, e_red)

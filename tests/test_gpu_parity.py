@@ -417,3 +417,31 @@ def test_freeze_equals_reference_fixpoint(gpu):
     neg = small_muon(**{"lambda": -0.5})
     with pytest.raises(qa.QublarError):
         qa.Problem(neg).freeze(1.0)
+
+
+def test_frozen_reduced_problem_anneals_as_the_oracle():
+    """The frozen set on the device, the reduced problem on the host (qb_reduce_frozen,
+    src/ising_recon.hpp:206-233) coloured by the full problem's colouring restricted to the
+    kept bits, then annealed on the device: branches and p_void equal the oracle's on the
+    reduced problem with that colouring, energies within 1e-9."""
+    P = qa.muon_scene(grid=32, sky_az=72, sky_th=16, log2_muons=22)
+    prob = qa.Problem(P)
+    frozen, rounds = prob.freeze(10.0)
+    order, cp = prob.colouring()
+    colour = np.zeros(P.n_vars, np.int32)
+    for c in range(len(cp) - 1):
+        colour[order[cp[c]:cp[c + 1]]] = c
+    R, kept = P.reduce_frozen(frozen)
+    R.colour = colour[kept]
+    say(f"frozen {int(frozen.sum())} of {P.n_vars} bits in {rounds} rounds; {R.n_vars} kept")
+    assert 0 < R.n_vars < P.n_vars
+    rp = qa.Problem(R)
+    sched = qa.schedule(50.0, 1.0, 12, 4)
+    out = rp.anneal_branches(sched, 8, 0, qa.FP32, 4)
+    ro, rcp = rp.colouring()
+    op = orc_mod.FloatProblem(R.row_ptr, R.ray, R.a, R.d, R.w, R.nbr_ptr, R.nbr, R.lam, R.kappa, R.field)
+    mode = orc_mod.make_mode(orc_mod.RNG_PHILOX, ro, rcp, orc_mod.FP32, 4)
+    pv, br = orc_mod.branch_fractions(op, orc_mod.schedule(50.0, 1.0, 12, 4), 8, 8, 0, mode)
+    assert np.array_equal(out["branches"], br) and np.array_equal(out["p_void"], pv)
+    e = np.array([orc_mod.binary_energy(op, b) for b in br])
+    assert np.allclose(out["energy"], e, rtol=1e-9, atol=0.0)
