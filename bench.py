@@ -238,6 +238,7 @@ def precision_rows(qa, prob, R, sched):
     rows = []
     for name, prec, k in runs:
         prob.anneal_branches(qa.schedule(50.0, 1.0, 2, 0), R, 0, prec, k, branches=False)   # warm
+        log(f"{name}: {R} branches x {sched.anneal_sweeps + sched.hold_sweeps} sweeps ...")
         t = time.perf_counter()
         out = prob.anneal_branches(sched, R, 0, prec, k)
         wall = time.perf_counter() - t
@@ -288,7 +289,9 @@ def run_gravity_precision(args):
     info = prob.info()
     log(f"gravity QUBO {P.n_vars} bits, {P.n_rays} stations, {int(P.row_ptr[-1])} entries, {info['n_classes']} "
         f"classes; scene {t_scene:.2f} s, upload {t_upload:.2f} s")
-    R, A, H = 16, 200, 50
+    R = args.gravity_branches
+    A = args.gravity_sweeps * 4 // 5
+    H = args.gravity_sweeps - A
     rows = precision_rows(qa, prob, R, qa.schedule(50.0, 1.0, A, H))
     updates = R * (A + H) * P.n_vars
     # gravity fixes mass well and depth poorly (src/op_gravity.hpp:13-14): count the cells
@@ -303,7 +306,7 @@ def run_gravity_precision(args):
     # the sweep is bound by its 2,586 dependent class steps, not by their width: more
     # branches per solve cost little more time
     wide = {}
-    for R2 in (32, 64):
+    for R2 in ((32, 64) if R == 16 and A + H == 250 else ()):
         prob.anneal_branches(qa.schedule(50.0, 1.0, 2, 0), R2, 0, qa.FP32, 10, branches=False)   # warm
         t = time.perf_counter()
         out = prob.anneal_branches(qa.schedule(50.0, 1.0, A, H), R2, 0, qa.FP32, 10, branches=False)
@@ -312,7 +315,7 @@ def run_gravity_precision(args):
         wide[str(R2)] = {"solve_s": wall, "spin_updates_per_s": R2 * (A + H) * P.n_vars / wall}
     result = {"workload": "gravity-precision",
               "problem": "gravimetry QUBO: 100 x 200 x 10 cells of 5 m, 20,000 stations, rows truncated at 50 m; "
-                         "16 branches, T 50 -> 1 over 200 sweeps + 50 at T = 1",
+                         f"{R} branches, T 50 -> 1 over {A} sweeps + {H} at T = 1",
               "n_vars": P.n_vars, "n_rays": P.n_rays, "entries": int(P.row_ptr[-1]),
               "colour_classes": info["n_classes"], "body_cells": int(P.truth.sum()),
               "scene_s": t_scene, "upload_s": t_upload, "rows": [r for r, _ in rows],
@@ -739,6 +742,9 @@ def main():
                     help="muon: the headline line (default); serving: the batched serving stream; "
                          "batched: many sparse QUBOs in one persistent launch")
     ap.add_argument("--instances", type=int, default=256, help="batched: instances")
+    ap.add_argument("--gravity-branches", type=int, default=16, help="gravity-precision: branches (replicas)")
+    ap.add_argument("--gravity-sweeps", type=int, default=250,
+                    help="gravity-precision: sweeps, 4/5 annealing T 50 -> 1 and 1/5 held at T = 1")
     ap.add_argument("--roi-n", type=int, default=32, help="roi: bits enumerated (2^n states)")
     ap.add_argument("--maxcut-replicas", type=int, default=16384, help="maxcut: replicas per batch")
     ap.add_argument("--ref-branches", type=int, default=4096, help="reference-float: branches")

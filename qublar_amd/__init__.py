@@ -13,7 +13,9 @@ Mirrors the reference's entry points:
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 import os
 import subprocess
 
@@ -137,6 +139,20 @@ class MuonSky(C.Structure):
 
 
 _LIB = None
+# Every live device handle, closed at interpreter exit before the HIP runtime's own teardown
+# (a handle's __del__ could otherwise call into HIP after the runtime has gone).
+_HANDLES = weakref.WeakSet()
+
+
+def _close_handles():
+    for h in sorted(list(_HANDLES), key=lambda o: 0 if isinstance(o, Stage) else 1):   # stages before problems
+        try:
+            h.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_handles)
 
 
 def lib():
@@ -354,6 +370,7 @@ class Problem:
         self.h = C.c_void_p()
         d = arrays.desc()
         _check(lib().qb_problem_create(C.byref(d), C.byref(self.h)))
+        _HANDLES.add(self)
 
     def close(self):
         if self.h:
@@ -445,6 +462,7 @@ class Stage:
         self.n_branches = n_branches
         opts = AnnealOpts(precision, recompute_every, workgroups, 0, group_width)
         _check(lib().qb_stage_create(problem.h, C.byref(sched), n_branches, C.byref(opts), C.byref(self.h)))
+        _HANDLES.add(self)
 
     def close(self):
         if self.h:
@@ -513,6 +531,7 @@ class Graph:
         self.h = C.c_void_p()
         d = arrays.desc()
         _check(lib().qb_graph_create(C.byref(d), int(force_global), C.byref(self.h)))
+        _HANDLES.add(self)
 
     def close(self):
         if self.h:
@@ -796,6 +815,7 @@ class QuboBatch:
                           _p(self._seed))
         self.h = C.c_void_p()
         _check(lib().qb_qubo_batch_create(C.byref(d), C.byref(self.h)))
+        _HANDLES.add(self)
 
     def close(self):
         if self.h:
@@ -868,6 +888,7 @@ class DenseQubo:
         self.n = len(self.hv)
         self.h = C.c_void_p()
         _check(lib().qb_dense_create(self.n, _p(self.Q), _p(self.hv), C.byref(self.h)))
+        _HANDLES.add(self)
 
     def close(self):
         if self.h:
