@@ -59,6 +59,15 @@ int qb_get_device_info(qb_device_info* out);
    read + write stream sustains. */
 int qb_copy_probe(int64_t bytes, int reps, double* ms_per_copy);
 
+/* No counterpart in the reference. A check, on the device, of what the sweep's team barrier
+   relies on: that a team's plain stores, drained before the barrier, are seen by the
+   L2-hitting loads of every member after it. For `steps` steps every thread of every
+   workgroup stores the step number, the team passes the barrier, and every thread reads its
+   word of each member; *stale receives the reads that saw an older step (0 on a device the
+   sweep is correct on), *us_per_step (nullable) the mean step time. workgroups as for
+   qb_anneal_opts. */
+int qb_team_barrier_check(int steps, int workgroups, uint64_t* stale, double* us_per_step);
+
 /* ==== float path: the ray x voxel QUBO ================================================ */
 
 /* BinaryProblem (src/engine.hpp:30-49): bit -> (ray, a) CSR, per-ray d and w, the

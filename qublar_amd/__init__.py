@@ -167,6 +167,7 @@ def lib():
             "qb_version": (i32, []),
             "qb_get_device_info": (i32, [C.POINTER(DeviceInfo)]),
             "qb_copy_probe": (i32, [C.c_int64, i32, C.POINTER(C.c_double)]),
+            "qb_team_barrier_check": (i32, [i32, i32, C.POINTER(u64), C.POINTER(dbl)]),
             "qb_problem_create": (i32, [C.POINTER(ProblemDesc), C.POINTER(vp)]),
             "qb_problem_destroy": (i32, [vp]),
             "qb_problem_get_info": (i32, [vp, C.POINTER(ProblemInfo)]),
@@ -284,6 +285,14 @@ def union(problems, keys, colours=None):
     U = ProblemArrays(cat(row_ptr), cat(ray), cat(a), cat(d), cat(w), cat(nbr_ptr), cat(nbr), lam, 0.0, cat(field),
                       draw_bit=cat(dbit), draw_key=cat(dkey), colour=cat(colour))
     return U, offs
+
+
+def team_barrier_check(steps=20000, workgroups=0):
+    """The team barrier's visibility on this device: (stale reads, microseconds per step)."""
+    stale = C.c_uint64()
+    us = C.c_double()
+    _check(lib().qb_team_barrier_check(int(steps), int(workgroups), C.byref(stale), C.byref(us)))
+    return stale.value, us.value
 
 
 def copy_probe(nbytes, reps=10):

@@ -937,6 +937,55 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     return QB_OK;
 }
 
+// ==== the team barrier's visibility, checked on the device ===================================
+
+extern "C" int qb_team_barrier_check(int steps, int workgroups, uint64_t* stale, double* us_per_step) {
+    if (!stale || steps <= 0) return fail(QB_E_ARG, "qb_team_barrier_check: null output or no steps");
+    const int team_size = team_size_for(workgroups, QB_FP32, 2, 0);
+    if (team_size <= 0) return -team_size;
+    DevBuf words, bar, err, cnt;
+    int rc;
+    if ((rc = words.reserve(sizeof(unsigned) * 256 * size_t(team_size) * kTeams)) ||
+        (rc = bar.reserve(sizeof(unsigned) * (kBarWords + kTicketWords))) || (rc = err.reserve(sizeof(int))) ||
+        (rc = cnt.reserve(sizeof(unsigned long long))))
+        return rc;
+    hipStream_t st = nullptr;
+    QB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct Guard {
+        hipStream_t s;
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Guard() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            (void)hipStreamDestroy(s);
+        }
+    } g{st};
+    QB_HIP(hipEventCreate(&g.a));
+    QB_HIP(hipEventCreate(&g.b));
+    QB_HIP(hipMemsetAsync(words.p, 0, words.bytes, st));
+    QB_HIP(hipMemsetAsync(bar.p, 0, bar.bytes, st));
+    QB_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    QB_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), st));
+    QB_HIP(hipEventRecord(g.a, st));
+    QB_HIP(launch_team_barrier_probe(words.as<unsigned>(), bar.as<unsigned>(), bar.as<unsigned>() + kBarWords,
+                                     err.as<int>(), steps, team_size, cnt.as<unsigned long long>(), st));
+    QB_HIP(hipEventRecord(g.b, st));
+    unsigned long long bad = 0;
+    int flag = 0;
+    QB_HIP(hipMemcpyAsync(&bad, cnt.p, sizeof(bad), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipMemcpyAsync(&flag, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    QB_HIP(hipStreamSynchronize(st));
+    if (flag == 2) return fail(QB_E_TIMEOUT, "an XCD team came up short of %d workgroups (workgroup placement)", team_size);
+    if (flag) return fail(QB_E_TIMEOUT, "a team barrier wait exceeded its limit");
+    *stale = bad;
+    if (us_per_step) {
+        float ms = 0.0f;
+        QB_HIP(hipEventElapsedTime(&ms, g.a, g.b));
+        *us_per_step = double(ms) * 1e3 / steps;
+    }
+    return QB_OK;
+}
+
 // ==== staged solves ==========================================================================
 // The branch state of one problem (in practice a union of requests) kept on the device
 // between calls: stages of a few sweeps at a time, requests joining with their state

@@ -174,6 +174,7 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
 // `table_bytes` of dynamic LDS (0 when they do not fit), plain or staged form.
 int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes, bool staged = false);
 constexpr int kSweepBlock = 256;   // threads per sweep workgroup
+constexpr int kTeams = 8;          // XCD teams of the sweep (group g belongs to team g mod 8)
 // A multi-pass bit whose chains fit kHeavyPasses rounds of a workgroup's slots is staged
 // whole in LDS. The staging area follows the class tables in the sweep's dynamic LDS
 // (16-byte aligned), reserved only when the problem has a multi-pass bit at the group
@@ -209,6 +210,10 @@ hipError_t launch_int_sweeps(const DevGraph& g, int R, uint64_t seed_base, const
                              long long* energy, bool lds_state, hipStream_t st);
 // freeze_provable_rock on the device: per-position data bounds, then one fixpoint round
 // (frozen -> next, *changed counts the bits newly frozen).
+// The team barrier's visibility check: team_size * 8 workgroups, words holding 256
+// unsigned per workgroup, *stale counting the reads that saw an older step.
+hipError_t launch_team_barrier_probe(unsigned* words, unsigned* bar, unsigned* tickets, int* err, int steps,
+                                     int team_size, unsigned long long* stale, hipStream_t st);
 // Staged solves: a span's clock word; a span's state (spins, clocks, residuals of groups
 // [0, groups)) from one stage into another; a span's branch agreement over rq branches.
 hipError_t launch_stage_clock(const DevProblem& p, int bit0, int n_bits, uint32_t word, uint32_t* clock,

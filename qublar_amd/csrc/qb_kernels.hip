@@ -755,7 +755,6 @@ __device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T
 // masks); no descriptor, no LDS and no workgroup sync. Bits of one class are
 // independent, so which slot takes which bit does not matter.
 constexpr int kLightU = 4;
-constexpr int kTeams = 8;   // XCD teams of the sweep (group g belongs to team g mod 8)
 
 template <typename T, int BW, bool STAGED>
 __device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<T>& s, int lb, int le, int my_slot,
@@ -1736,6 +1735,51 @@ hipError_t launch_ray_ref(const DevProblem& p, int R, uint64_t seed_offset, cons
                           int sweep_end, uint8_t* xs, double* rs, uint64_t* sm, hipStream_t st) {
     ray_ref_kernel<<<unsigned((R + 3) / 4), 256, 0, st>>>(p, R, seed_offset, temp, sweep_begin, sweep_end, xs, rs,
                                                           sm);
+    return hipGetLastError();
+}
+
+// ---- the team barrier's visibility, checked on the device ---------------------------------
+// The sweep relies on a team's plain stores, drained before its barrier, being seen by the
+// L2-hitting loads of every member after it. This kernel exercises exactly that: each
+// step, every thread of every member stores the step number into its own word, the team
+// passes team_barrier, and every thread then reads its word of each member with ld_l2; a
+// word below the step is a stale read and is counted. Words only grow, so a member that
+// runs ahead is never miscounted. Teams form as in the sweep (XCC_ID, tickets).
+__global__ void __launch_bounds__(256) team_barrier_probe_kernel(unsigned* words, unsigned* bar, unsigned* tickets,
+                                                                 int* err, int steps, int team_size,
+                                                                 unsigned long long* stale) {
+    __shared__ int s_team, s_rank, s_ok;
+    if (threadIdx.x == 0) {
+        s_team = int(xcc_id() & (kTeams - 1));
+        s_rank = int(__hip_atomic_fetch_add(tickets + s_team * 64, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    if (!grid_barrier(bar, gridDim.x, err)) return;
+    if (threadIdx.x == 0) {
+        const unsigned members = __hip_atomic_load(tickets + s_team * 64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = members >= unsigned(team_size) ? 1 : 0;
+        if (!s_ok) __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const int team = s_team, rank = s_rank;
+    if (!s_ok || rank >= team_size) return;
+    unsigned* tbar = bar + 64 + team * 64;
+    unsigned* mine = words + (size_t(team) * team_size + size_t(rank)) * 256;
+    const unsigned* team_words = words + size_t(team) * team_size * 256;
+    unsigned long long bad = 0;
+    unsigned target = 0;
+    for (int s = 1; s <= steps; ++s) {
+        mine[threadIdx.x] = unsigned(s);
+        target += unsigned(team_size);
+        if (!team_barrier(tbar, target, err)) break;
+        for (int r = 0; r < team_size; ++r) bad += ld_l2(team_words + size_t(r) * 256 + threadIdx.x) < unsigned(s);
+    }
+    if (bad) atomicAdd(stale, bad);
+}
+
+hipError_t launch_team_barrier_probe(unsigned* words, unsigned* bar, unsigned* tickets, int* err, int steps,
+                                     int team_size, unsigned long long* stale, hipStream_t st) {
+    team_barrier_probe_kernel<<<unsigned(team_size * kTeams), 256, 0, st>>>(words, bar, tickets, err, steps, team_size,
+                                                                          stale);
     return hipGetLastError();
 }
 

@@ -255,6 +255,24 @@ def test_build_binary_problem_muon_rows(driver, shape, seed, bins):
             assert P["w"][row] == det
             row += 1
     assert row == len(P["d"]) == len(P["w"]) and crossing_rays > 5
+    # the same rows from the oracle's own float march (oracle/qb_oracle.c, a separate
+    # restatement of src/muon.cuh:66-134 and src/op_muon.hpp:22-50): every entry's a, every
+    # d and w, bit for bit; only the order inside a bit is std::sort's in the drop-in
+    import oracle
+
+    med = oracle.medium(P["lo"], P["voxel"], P["shape"], P["mu"].astype(np.float32))
+    row0 = 0
+    want_rows = {}
+    for v in P["views"]:
+        d, w, eb, er, ea = oracle.muon_rows(med, v["chamber"].astype(np.float32), v["dirs"].astype(np.float32),
+                                            v["open"].astype(np.uint64), v["det"].astype(np.uint64),
+                                            site_bit.astype(np.int32), 30.0, np.float32(P["a_scale"]), row0)
+        for b, r, a in zip(eb, er, ea):
+            want_rows.setdefault(int(r), {})[int(b)] = float(a)
+        assert np.array_equal(P["d"][row0:row0 + len(d)], d) and np.array_equal(P["w"][row0:row0 + len(w)], w)
+        row0 += len(d)
+    assert row0 == len(P["d"])
+    assert {r: {b: float(np.float32(a)) for b, a in bits.items()} for r, bits in rows.items()} == want_rows
 
 
 def test_reference_drivers_link_and_fail_loudly_without_a_device():

@@ -445,3 +445,12 @@ def test_frozen_reduced_problem_anneals_as_the_oracle():
     assert np.array_equal(out["branches"], br) and np.array_equal(out["p_void"], pv)
     e = np.array([orc_mod.binary_energy(op, b) for b in br])
     assert np.allclose(out["energy"], e, rtol=1e-9, atol=0.0)
+
+
+def test_team_barrier_shows_every_members_stores():
+    """What the sweep's team barrier relies on, checked on this device: after every team
+    barrier, each member's drained plain stores are seen by the others' L2-hitting loads --
+    20,000 steps, every thread of every workgroup, no stale read."""
+    stale, us = qa.team_barrier_check(20000)
+    say(f"team barrier: {stale} stale reads, {us:.2f} us per step")
+    assert stale == 0
