@@ -310,7 +310,7 @@ struct qb_problem {
     std::vector<int32_t> col_ptr, nb_ptr;   // host copies, position space
     std::vector<uint8_t> seq_update;
     struct Batches {
-        DevBuf cls, cls_heavy, first, count, pos, slot0, desc;
+        DevBuf cls, cls_heavy, first, count, pos, slot0, desc, blk, blk_meta;
         DevBatches dev{};
     };
     std::map<int, std::unique_ptr<Batches>> batches;   // by group width and team size
@@ -659,8 +659,9 @@ static int team_size_for(int requested, int precision, int bw, size_t table_byte
                                  "multi-pass staging)", table_bytes);
     const int cap = per_cu * prop.multiProcessorCount;
     const int total = requested > 0 ? requested : prop.multiProcessorCount;
-    if (total % 8 != 0 || total > cap)
-        return -fail(QB_E_ARG, "workgroups must be a positive multiple of 8 and at most %d (got %d)", cap, total);
+    const int limit = std::min(cap, 8 * kMaxTeamSize);
+    if (total % 8 != 0 || total > limit)
+        return -fail(QB_E_ARG, "workgroups must be a positive multiple of 8 and at most %d (got %d)", limit, total);
     return total / 8;
 }
 
@@ -682,7 +683,7 @@ static int group_width_for(const qb_problem* P, int R, int requested, int precis
     int bw = 2;
     while (bw < 16 && (R + bw - 1) / bw > 8) bw *= 2;
     while (bw < 16 &&
-           ray_sweep_blocks_per_cu(precision, bw, sweep_dynamic_lds(P->n_classes, bw, has_multipass(P, bw))) <= 0)
+           ray_sweep_blocks_per_cu(precision, bw, sweep_dynamic_lds(P->n_classes, bw, has_multipass(P, bw), precision)) <= 0)
         bw *= 2;
     return bw;
 }
@@ -815,18 +816,38 @@ static int batches_for(qb_problem* P, int bw, int team_size, const DevBatches** 
     const int n_batches = int(first.size());
     if (sweep_table_bytes(K) > size_t(kSweepTableLimit))
         return fail(QB_E_LIMIT, "%d colour classes exceed the sweep's %d-byte LDS table", K, kSweepTableLimit);
+    // every packed batch's block: its used slots (a prefix of the batch) with their
+    // chains and heads, built on the device from the descriptors
+    std::vector<int2> meta(first.size(), int2{0, 0});
+    int64_t granules = 0;
+    for (size_t bi = 0; bi < first.size(); ++bi) {
+        if (count[bi] < 0) continue;
+        int used = 0;
+        while (used < slots && desc[bi * size_t(slots) + size_t(used)].pos >= 0) ++used;
+        for (int q = used; q < slots; ++q)
+            if (desc[bi * size_t(slots) + size_t(q)].pos >= 0)
+                return fail(QB_E_ARG, "batch %zu: its used slots are not a prefix", bi);
+        meta[bi] = int2{int(granules), used};
+        granules += int64_t(used) * (kSeg + kHeadGranules);
+        if (granules > int64_t(INT32_MAX))
+            return fail(QB_E_LIMIT, "the sweep's batch blocks exceed 2^31 granules of 16 bytes");
+    }
     auto B = std::make_unique<qb_problem::Batches>();
     int rc;
     if ((rc = B->cls.upload(cls_batch, P->stream)) || (rc = B->first.upload(first, P->stream)) ||
         (rc = B->count.upload(count, P->stream)) || (rc = B->pos.upload(pos, P->stream)) ||
         (rc = B->slot0.upload(slot0, P->stream)) || (rc = B->desc.upload(desc, P->stream)) ||
-        (rc = B->cls_heavy.upload(cls_heavy, P->stream)))
+        (rc = B->cls_heavy.upload(cls_heavy, P->stream)) || (rc = B->blk_meta.upload(meta, P->stream)) ||
+        (rc = B->blk.reserve(size_t(std::max<int64_t>(granules, 1)) * 16)))
         return rc;
+    QB_HIP(launch_batch_blocks(P->dev, B->desc.as<SlotDesc>(), B->blk_meta.as<int2>(), n_batches, slots,
+                               B->blk.as<uint4>(), P->stream));
     QB_HIP(hipStreamSynchronize(P->stream));
     const bool heavy = std::any_of(count.begin(), count.end(), [](int32_t c) { return c < 0; });
     B->dev = DevBatches{B->cls.as<int32_t>(), B->cls_heavy.as<int32_t>(), B->first.as<int32_t>(),
                         B->count.as<int32_t>(), B->pos.as<int32_t>(), B->slot0.as<int32_t>(),
-                        B->desc.as<SlotDesc>(), n_batches, heavy ? 1 : 0};
+                        B->desc.as<SlotDesc>(), B->blk.as<uint4>(), B->blk_meta.as<int2>(), n_batches,
+                        heavy ? 1 : 0};
     *out = &B->dev;
     P->batches[key] = std::move(B);
     return QB_OK;
@@ -858,7 +879,7 @@ extern "C" int qb_anneal_branches(qb_problem* P, const qb_schedule* s, int R, ui
     {
         // the dynamic LDS must not lower the residency the team size was chosen for
         const int checked =
-            team_size_for(o.workgroups, o.precision, bw, sweep_dynamic_lds(P->n_classes, bw, bt->heavy_lds != 0));
+            team_size_for(o.workgroups, o.precision, bw, sweep_dynamic_lds(P->n_classes, bw, bt->heavy_lds != 0, o.precision));
         if (checked <= 0) return -checked;
     }
     const int n_part = ray_energy_blocks(P->dev, bw);
@@ -1032,7 +1053,7 @@ extern "C" int qb_stage_create(qb_problem* P, const qb_schedule* s, int R, const
     if ((rc = batches_for(P, S->bw, S->team_size, &S->bt))) return rc;
     {
         const int checked = team_size_for(o.workgroups, o.precision, S->bw,
-                                          sweep_dynamic_lds(P->n_classes, S->bw, S->bt->heavy_lds != 0), true);
+                                          sweep_dynamic_lds(P->n_classes, S->bw, S->bt->heavy_lds != 0, o.precision), true);
         if (checked <= 0) return -checked;
     }
     const int n = P->n, nr = P->n_rays;

@@ -87,6 +87,27 @@ struct SlotDesc {
 };
 constexpr int kMaxPackedNeighbours = 1023;
 
+// The static operands of a packed batch as one block, copied whole into LDS a step
+// ahead: its n_used slots (a prefix of the batch) each hold one chain of kSeg entries
+// (past the chain's length: ray -1, a 0, w 0), entry u of slot s at 16-byte granule
+// s * kSeg + (u ^ (s % kSeg)) -- the swizzle keeps the slots' reads of one entry on
+// different LDS banks -- followed by the n_used slots' heads.
+constexpr int kNb = 6;   // neighbour positions carried per bit; more are read from nb
+struct SlotHead {
+    int32_t pos;
+    uint32_t info;
+    int32_t nb0;
+    int32_t bit;          // draw counter index
+    int32_t q[kNb];       // first neighbour positions (the last repeated past the bit's count)
+    double lin;
+    uint64_t key;         // draw key offset
+    int32_t pad[2];
+};
+static_assert(sizeof(SlotHead) == 64, "a head is four 16-byte granules");
+static_assert(sizeof(RayEnt) == 16, "an entry is one 16-byte granule");
+constexpr int kHeadGranules = int(sizeof(SlotHead) / 16);
+__host__ __device__ inline int blk_entry(int slot, int u) { return slot * kSeg + (u ^ (slot & (kSeg - 1))); }
+
 struct DevBatches {
     const int32_t* cls_batch;   // n_classes + 1: class c's batches, its multi-pass batches first
     const int32_t* cls_heavy;   // n_classes: class c's multi-pass batches
@@ -95,6 +116,8 @@ struct DevBatches {
     const int32_t* pos;         // per item: the bit's position
     const int32_t* slot0;       // per item: its first slot in the batch
     const SlotDesc* desc;       // per batch: BLOCK / BW slot descriptors (all empty for count -1)
+    const uint4* blk;           // the packed batches' blocks, in 16-byte granules
+    const int2* blk_meta;       // per batch: {first granule of its block, slots used (0: multi-pass)}
     int n_batches;
     int heavy_lds;              // 1 when some batch is multi-pass: the sweep's dynamic LDS then
                                 // holds the staging area after the class tables
@@ -125,7 +148,7 @@ struct DevState {
     uint16_t* x;            // [g][n] spin masks, bit b = branch g*BW + b
     T* r;                   // [g][n_rays][BW] residuals
     const double* beta;     // 1 / T per sweep
-    unsigned* bar;          // [0]: grid barrier; [64 + 64 t]: team t's arrivals, [96 + 64 t] its release word
+    unsigned* bar;          // [0]: grid barrier; [64 + 64 t + r]: team t member r's arrival word (r < 64)
     unsigned* tickets;      // [64 t]: team t's member count
     int* err;               // 1: a barrier wait timed out; 2: a team came up short
     // A staged solve (null clock otherwise): per position, the bit's own sweep is the
@@ -160,6 +183,9 @@ struct DevGraph {
 };
 
 // ---- launchers (qb_kernels.hip) ----
+// Writes every packed batch's block (DevBatches::blk) from its slot descriptors.
+hipError_t launch_batch_blocks(const DevProblem& p, const SlotDesc* desc, const int2* meta, int n_batches, int slots,
+                               uint4* blk, hipStream_t st);
 hipError_t launch_ray_init(const DevProblem& p, int precision, int bw, void* x, void* r, int n_groups,
                            hipStream_t st);
 // The persistent sweep on team_size * 8 workgroups (team_size per XCD).
@@ -175,19 +201,38 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
 int ray_sweep_blocks_per_cu(int precision, int bw, size_t table_bytes, bool staged = false);
 constexpr int kSweepBlock = 256;   // threads per sweep workgroup
 constexpr int kTeams = 8;          // XCD teams of the sweep (group g belongs to team g mod 8)
+constexpr int kMaxTeamSize = 64;   // members of a team (one arrival word each in its barrier area)
 // A multi-pass bit whose chains fit kHeavyPasses rounds of a workgroup's slots is staged
 // whole in LDS. The staging area follows the class tables in the sweep's dynamic LDS
 // (16-byte aligned), reserved only when the problem has a multi-pass bit at the group
 // width, so problems without one keep that LDS for their tables.
 constexpr int kHeavyPasses = 3;
-__host__ __device__ inline size_t heavy_stage_offset(int n_classes) {
+// The sweep's dynamic LDS: the class tables, two block buffers (the step's and the next
+// one's), and a work area X. A packed step transposes its residuals in X; a multi-pass
+// step stages its column's first kSlots chains in its own (empty) block buffer and the
+// rest, with its chain sums, in X.
+__host__ __device__ inline size_t sweep_tables_bytes_aligned(int n_classes) {
     return (sweep_table_bytes(n_classes) + 15) & ~size_t(15);
 }
-__host__ __device__ inline size_t heavy_stage_bytes(int bw) {
-    return sizeof(RayEnt) * size_t(kHeavyPasses) * size_t(kSweepBlock / bw) * size_t(kSeg);
+__host__ __device__ inline size_t sweep_block_bytes(int bw) {
+    return size_t(kSweepBlock / bw) * (size_t(kSeg) * sizeof(RayEnt) + sizeof(SlotHead));
 }
-inline size_t sweep_dynamic_lds(int n_classes, int bw, bool heavy) {
-    return heavy_stage_offset(n_classes) + (heavy ? heavy_stage_bytes(bw) : 0);
+__host__ __device__ inline size_t sweep_trow_bytes(int precision) {
+    return size_t(kSweepBlock) * size_t(kSeg) * (precision == 1 ? sizeof(float) : sizeof(double));
+}
+__host__ __device__ inline size_t heavy_extra_entries(int bw) {
+    return size_t(kHeavyPasses - 1) * size_t(kSweepBlock / bw) * size_t(kSeg);
+}
+__host__ __device__ inline size_t heavy_extra_bytes(int bw) {
+    return heavy_extra_entries(bw) * sizeof(RayEnt) + size_t(kHeavyPasses) * size_t(kSweepBlock / bw) * size_t(bw) *
+                                                          sizeof(double);
+}
+__host__ __device__ inline size_t sweep_x_bytes(int bw, int precision, bool heavy) {
+    const size_t t = sweep_trow_bytes(precision);
+    return heavy && heavy_extra_bytes(bw) > t ? heavy_extra_bytes(bw) : t;
+}
+inline size_t sweep_dynamic_lds(int n_classes, int bw, bool heavy, int precision) {
+    return sweep_tables_bytes_aligned(n_classes) + 2 * sweep_block_bytes(bw) + sweep_x_bytes(bw, precision, heavy);
 }
 // Entries of a bit with more chains than a sweep workgroup has slots at group width 2
 // (a multi-pass bit there).

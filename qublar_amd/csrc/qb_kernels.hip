@@ -93,52 +93,48 @@ __device__ __forceinline__ V ld_l2(const V* p) {
     return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Team barrier: a monotone arrival counter in L2, bar[0], and a release word on its
-// own 128-byte line, bar[32]; the k-th barrier completes at k * team size arrivals.
-// Every wave drains its stores, then thread 0 arrives with one atomic; the workgroup
-// whose arrival completes the count writes the release word, and the others poll that
-// word, not the counter line the atomics keep busy. While thread 0 polls, every thread
-// runs `after_arrival` (loads that must not be waited for before the arrival, or the
-// drain would wait for them too). The error flag is read only while polling; a wait
-// longer than ~2 s sets *err, as in grid_barrier.
+// Team barrier: every member has its own arrival word in the team's 64-word area of the
+// barrier array, words[rank], and the k-th barrier completes when every member's word
+// has reached k * team size (`target`). Every wave drains its stores, then thread 0
+// publishes the member's arrival with one store; wave 0 then polls all members' words,
+// one per lane in one load, until every one has arrived. No member waits on an atomic's
+// return or on a second hop through a release word. While wave 0 polls, every thread
+// first runs `after_arrival` (work that must not delay the arrival, such as the
+// caller's task cursor). The error flag is read only while polling; a wait longer than
+// ~2 s sets *err, as in grid_barrier.
+static_assert(kMaxTeamSize <= 64, "wave 0 polls one member per lane");
 template <typename F>
-__device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int* err, F&& after_arrival) {
+__device__ __forceinline__ bool team_barrier(unsigned* words, int rank, int team_size, unsigned target, int* err,
+                                             F&& after_arrival) {
     __shared__ int s_ok;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    unsigned seen = 0;
-    unsigned* release = bar + 32;
-    if (threadIdx.x == 0) {
-        const unsigned before = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (before + 1u == target) {
-            __hip_atomic_store(release, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            seen = target;
-        } else {
-            seen = ld_l2(release);
-        }
-    }
+    if (threadIdx.x == 0) __hip_atomic_store(words + rank, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     after_arrival();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        const int lane = int(threadIdx.x);
         int ok = 1;
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
         unsigned polls = 0;
-        while (seen < target && (seen = ld_l2(release)) < target) {
+        while (true) {
+            const unsigned v = lane < team_size ? ld_l2(words + lane) : target;
+            if (__ballot(v < target) == 0) break;
             if ((++polls & 1023u) == 0) {
                 if (ld_l2(err) != 0 || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
-                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
                 }
             }
         }
-        s_ok = ok;
+        if (lane == 0) s_ok = ok;
     }
     __syncthreads();
     return s_ok != 0;
 }
 
-__device__ __forceinline__ bool team_barrier(unsigned* bar, unsigned target, int* err) {
-    return team_barrier(bar, target, err, [] {});
+__device__ __forceinline__ bool team_barrier(unsigned* words, int rank, int team_size, unsigned target, int* err) {
+    return team_barrier(words, rank, team_size, target, err, [] {});
 }
 
 // The dE of a bit sums its CSR entries in chains of kSeg: the first chain starts
@@ -185,6 +181,20 @@ template <typename T>
 __device__ __forceinline__ double chain_sum(const Chain<T>& ch, int len, double delta, double seed, bool seeded) {
     return chain_sum<T>(ch.e, ch.rv, len, delta, seed, seeded);
 }
+// The same sum from the entries' shares and weights held apart.
+template <typename T>
+__device__ __forceinline__ double chain_sum_aw(const float* ea, const double* ew, const T* rv, int len, double delta,
+                                               double seed, bool seeded) {
+    double acc = seed;
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u) {
+        const double ad = double(ea[u]) * delta;
+        const double t = ew[u] * (double(rv[u]) * ad + 0.5 * ad * ad);
+        const double nxt = (u == 0 && !seeded) ? t : acc + t;
+        acc = u < len ? nxt : acc;
+    }
+    return acc;
+}
 
 // On acceptance: r[ray] += a * delta for the chain's entries, from the loaded values
 // (no other bit of the class touches these rays).
@@ -204,20 +214,20 @@ __device__ __forceinline__ void chain_scatter(const Chain<T>& ch, int len, T* rg
 __device__ __forceinline__ double neighbour_terms(const DevProblem& p, const uint16_t* xg, int n0, int n1, int b,
                                                   int xb, double acc) {
     if (n1 <= n0) return acc;
-    constexpr int kNb = 8;
-    int q[kNb];
-    uint16_t m[kNb];
+    constexpr int kNbL = 8;
+    int q[kNbL];
+    uint16_t m[kNbL];
 #pragma unroll
-    for (int u = 0; u < kNb; ++u) q[u] = p.nb[min(n0 + u, n1 - 1)];
+    for (int u = 0; u < kNbL; ++u) q[u] = p.nb[min(n0 + u, n1 - 1)];
 #pragma unroll
-    for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + q[u]);
+    for (int u = 0; u < kNbL; ++u) m[u] = ld_l2(xg + q[u]);
 #pragma unroll
-    for (int u = 0; u < kNb; ++u) {
+    for (int u = 0; u < kNbL; ++u) {
         const int before = ((m[u] >> b) & 1) != xb;
         const double nxt = acc + p.lambda * (before ? -1.0 : 1.0);
         acc = n0 + u < n1 ? nxt : acc;
     }
-    for (int k = n0 + kNb; k < n1; ++k) {
+    for (int k = n0 + kNbL; k < n1; ++k) {
         const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
         acc += p.lambda * (before ? -1.0 : 1.0);
     }
@@ -265,24 +275,13 @@ __device__ __forceinline__ bool bit_time(const DevState<T>& s, uint32_t clk, int
 }
 
 // ---------------------------------------------------------------------------------
-// Packed steps. A slot's static operands -- its descriptor, its chain's entries, and
-// for a bit's first chain the neighbour positions, lin and the original bit index --
-// are fetched a step ahead (fetch_desc while the previous step computes, the rest
-// either then or while the team barrier before this step is pending), so once the
-// barrier opens a step issues only its state loads (the spin mask, the chain's
-// residuals, the neighbours' masks), all at once.
+// Packed steps. A batch's static operands -- every slot's chain entries and head --
+// form one block (DevBatches::blk), which the workgroup copies straight into LDS a
+// step ahead: every wave moves 64 granules of 16 bytes per instruction, in a few
+// instructions, while the previous step decides, scatters and waits at the barrier.
+// Once the barrier opens a step issues only its state loads (the spin masks, the
+// chains' residuals), with their addresses read from LDS.
 // ---------------------------------------------------------------------------------
-constexpr int kNb = 6;
-
-struct SlotStatic {
-    SlotDesc d;
-    RayEnt e[kSeg];
-    int32_t q[kNb];
-    double lin;
-    int32_t bit;          // draw counter index
-    uint64_t key;         // draw key offset
-    uint32_t clk;         // the bit's clock (staged form)
-};
 
 __device__ __forceinline__ int slot_len(uint32_t i) { return int(i & 31u); }
 __device__ __forceinline__ int slot_j(uint32_t i) { return int((i >> 5) & 255u); }
@@ -290,130 +289,169 @@ __device__ __forceinline__ int slot_nch(uint32_t i) { return int((i >> 13) & 255
 __device__ __forceinline__ int slot_nnb(uint32_t i) { return int((i >> 21) & 1023u); }
 __device__ __forceinline__ bool slot_seq(uint32_t i) { return (i >> 31) != 0u; }
 
-template <int KSLOTS>
-__device__ __forceinline__ void fetch_desc(const DevBatches& bt, int batch, int slot, SlotStatic& S) {
-    const int4 v = *reinterpret_cast<const int4*>(bt.desc + size_t(batch) * KSLOTS + size_t(slot));
-    S.d = SlotDesc{v.x, v.y, v.z, uint32_t(v.w)};
-}
-
-template <bool STAGED>
-__device__ __forceinline__ void fetch_static(const DevProblem& p, SlotStatic& S, const uint32_t* clock) {
-    if (S.d.pos < 0) return;
-    const int len = slot_len(S.d.info);
-    if (len > 0) {
-        // in quarters: a lane whose chain ends earlier issues no further loads
-        const int last = S.d.ka + len - 1;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) S.e[u] = p.ent[min(S.d.ka + u, last)];
-        if (len > 4) {
-#pragma unroll
-            for (int u = 4; u < 8; ++u) S.e[u] = p.ent[min(S.d.ka + u, last)];
-        }
-        if (len > 8) {
-#pragma unroll
-            for (int u = 8; u < 12; ++u) S.e[u] = p.ent[min(S.d.ka + u, last)];
-        }
-        if (len > 12) {
-#pragma unroll
-            for (int u = 12; u < 16; ++u) S.e[u] = p.ent[min(S.d.ka + u, last)];
-        }
-    }
-    if (slot_j(S.d.info) == 0) {
-        const int nn = slot_nnb(S.d.info);
-        if (nn > 0) {
-            const int last = S.d.nb0 + nn - 1;
-#pragma unroll
-            for (int u = 0; u < kNb; ++u) S.q[u] = p.nb[min(S.d.nb0 + u, last)];
-        }
-        S.lin = p.lin[S.d.pos];
-        S.bit = p.draw_bit[S.d.pos];
-        S.key = p.draw_key[S.d.pos];
-        if constexpr (STAGED) S.clk = clock[S.d.pos];
+// Copies `n` granules from global `src` into LDS at `dst`, every wave of the workgroup
+// 64 granules per instruction (lanes past the end repeat the last granule, so `dst`
+// must hold n rounded up to 64 granules).
+template <int BLOCK>
+__device__ __forceinline__ void copy_to_lds(const uint4* src, int n, unsigned char* dst) {
+    const int tid = int(threadIdx.x);
+    for (int v0 = (tid >> 6) * 64; v0 < n; v0 += BLOCK) {
+        const uint4* g = src + min(v0 + (tid & 63), n - 1);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)g,
+                                         (__attribute__((address_space(3))) void*)(dst + size_t(v0) * 16), 16, 0, 0);
     }
 }
 
-// One packed batch for the BW branches of group g: every slot evaluates one chain of
-// one bit; the chain sums meet in LDS, the first slot of each run of kChainRun chains
-// adds the run, the bit's first slot adds the runs and the neighbour terms and decides; each slot then scatters its own chain from
-// registers. `after_issue` runs once the state loads are issued, `after_compute` once
-// this slot's chain sum is in LDS (the caller's prefetch of the next step).
+// Issues the copy of a batch's block ({first granule, slots used} = meta) into the LDS
+// buffer pb: the entries to pb, the heads to pb + KSLOTS * kSeg granules. The copy
+// lands asynchronously: a wave's vmcnt covers it, so a wait for every memory operation
+// and a workgroup barrier (as the team barrier makes) precede any read of the buffer.
+template <int BLOCK, int KSLOTS>
+__device__ __forceinline__ void issue_block(const uint4* blk, int2 meta, unsigned char* pb) {
+    static_assert((KSLOTS * kSeg) % 64 == 0 && (KSLOTS * kHeadGranules) % 64 == 0,
+                  "the buffer's parts hold whole 64-granule copies");
+    const int first = __builtin_amdgcn_readfirstlane(meta.x), n = __builtin_amdgcn_readfirstlane(meta.y);
+    if (n <= 0) return;
+    copy_to_lds<BLOCK>(blk + first, n * kSeg, pb);
+    copy_to_lds<BLOCK>(blk + first + n * kSeg, n * kHeadGranules, pb + size_t(KSLOTS) * kSeg * 16);
+}
+
+// One packed batch for the BW branches of group g, its block in LDS at pb with n_used
+// slots: every slot evaluates one chain of one bit; the chain sums meet in LDS, the
+// first slot of each run of kChainRun chains adds the run, the bit's first slot adds
+// the runs and the neighbour terms and decides; then the accepted chains are scattered
+// (with two FP32 branches by the lanes that loaded each entry's pair, else by each
+// slot). `after_issue` runs once the state loads are issued, `after_sync` once every
+// slot's chain sum is in LDS (the caller's cursor and its copy of the next block).
+// `trow` is the residual transposition area (kSweepBlock * kSeg values of T), `part`,
+// `accepted` and `fx` the workgroup's chain sums, decisions and per-bit flip records.
 template <typename T, int BW, int BLOCK, bool STAGED, typename F1, typename F2>
-__device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<T>& s, const SlotStatic& S, int g,
-                                            int sweep, double beta, F1&& after_issue, F2&& after_compute) {
+__device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<T>& s, const unsigned char* pb,
+                                            int n_used, T* trow, double (*part)[BW], int (*accepted)[BW],
+                                            uint32_t* fx, int g, int sweep, double beta, F1&& after_issue,
+                                            F2&& after_sync) {
     constexpr int kSlots = BLOCK / BW;
     constexpr int kSpw = 64 / BW;   // slots per wave
-    __shared__ double part[kSlots][BW];
-    __shared__ int accepted[kSlots][BW];
-    // per wave: its chains' rays [slot][entry], and the loaded residuals [slot][lane b][entry]
-    __shared__ __align__(16) int32_t rrow[kSlots * kSeg];
-    __shared__ __align__(16) T trow[kSlots * BW * kSeg];
+    const RayEnt* E = reinterpret_cast<const RayEnt*>(pb);
+    const SlotHead* Hd = reinterpret_cast<const SlotHead*>(pb + size_t(kSlots) * kSeg * sizeof(RayEnt));
     const int tid = int(threadIdx.x);
     const int b = tid % BW, slot = tid / BW;
     const int lane = tid & 63, wave = tid >> 6;
-    const uint32_t info = S.d.info;
-    const int pos = S.d.pos, len = slot_len(info), j = slot_j(info), nn = slot_nnb(info);
+    const bool used = slot < n_used;
+    const int pos = used ? Hd[slot].pos : -1;
+    const uint32_t info = used ? Hd[slot].info : 0u;
+    const int len = slot_len(info), j = slot_j(info), nn = slot_nnb(info);
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
     T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
-    // Residual loads across the wave: load u of lane l is entry q % kSeg of the wave's
-    // chain q / kSeg, q = (64 u + l) / BW, so each load covers whole chains, whose rays
-    // share few lines, instead of one entry of 64 / BW unrelated chains. The rays go
-    // through this wave's LDS rows, and the residuals come back through them in chain
-    // order (a wave's LDS operations complete in order, so no barrier is needed).
-    int32_t* rw = rrow + wave * kSpw * kSeg;
+    // Residual loads across the wave, in whole chains (whose rays share few lines) rather
+    // than one entry of 64 / BW unrelated chains. With two FP32 branches a lane loads an
+    // entry's pair of residuals as one 8-byte word: load u of lane l is entry q % kSeg of
+    // the wave's chain q / kSeg, q = 64 u + l, so a wave issues kSeg / 2 loads. Otherwise
+    // a lane loads one branch's residual: q = (64 u + l) / BW, kSeg loads. The residuals
+    // come back through this wave's rows of trow in chain order (a wave's LDS operations
+    // complete in order, so no barrier is needed).
+    constexpr bool kPair = BW == 2 && sizeof(T) == 4;
+    constexpr int kLoads = kPair ? kSeg / 2 : kSeg;
     T* tw = trow + wave * kSpw * BW * kSeg;
     const int ls = slot - wave * kSpw;
-    if (b == 0) {
-        int4* row = reinterpret_cast<int4*>(rw + ls * kSeg);
+    T rvt[kPair ? 1 : kSeg];
+    uint64_t rpair[kPair ? kLoads : 1];
+    int rray[kPair ? kLoads : 1];
 #pragma unroll
-        for (int u = 0; u < kSeg / 4; ++u) {
-            int r4[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) r4[v] = pos >= 0 && 4 * u + v < len ? S.e[4 * u + v].ray : -1;
-            row[u] = int4{r4[0], r4[1], r4[2], r4[3]};
+    for (int u = 0; u < kLoads; ++u) {
+        const int q = kPair ? u * 64 + lane : (u * 64 + lane) / BW;
+        const int c = wave * kSpw + q / kSeg;
+        const int ray = c < n_used ? E[blk_entry(c, q % kSeg)].ray : -1;
+        if constexpr (kPair) {
+            rray[u] = ray;
+            rpair[u] = ray >= 0 ? ld_l2(reinterpret_cast<const uint64_t*>(rg + size_t(ray) * BW)) : 0ull;
+        } else {
+            rvt[u] = ray >= 0 ? ld_l2(rg + size_t(ray) * BW + b) : T(0);
         }
-    }
-    T rvt[kSeg];
-#pragma unroll
-    for (int u = 0; u < kSeg; ++u) {
-        const int ray = rw[(u * 64 + lane) / BW];
-        rvt[u] = ray >= 0 ? ld_l2(rg + size_t(ray) * BW + b) : T(0);
     }
     uint16_t m[kNb];
     uint16_t xm = 0;
+    uint32_t clk = 0;
+    const bool first = pos >= 0 && j == 0;
     if (pos >= 0) {
         xm = ld_l2(xg + pos);
-        if (j == 0 && nn > 0) {
+        if (first && nn > 0) {
 #pragma unroll
-            for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + S.q[u]);
+            for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + Hd[slot].q[u]);
         }
+        if constexpr (STAGED)
+            if (first) clk = s.clock[pos];
     }
     after_issue();
+    // the chain's shares and weights, from LDS while the loads are in flight; with pairs
+    // also, for each loaded entry, its share and its bit's first slot (-1: no entry, or
+    // a column whose rays repeat, which that slot scatters itself)
+    float ea[kSeg];
+    double ew[kSeg];
+    if (pos >= 0) {
+#pragma unroll
+        for (int v = 0; v < kSeg; ++v) {
+            const RayEnt ev = E[blk_entry(slot, v)];
+            ea[v] = ev.a;
+            ew[v] = ev.w;
+        }
+    }
+    int pbs[kPair ? kLoads : 1];
+    float pa[kPair ? kLoads : 1];
+    if constexpr (kPair) {
+#pragma unroll
+        for (int w = 0; w < kLoads; ++w) {
+            const int q = w * 64 + lane;
+            const int c = wave * kSpw + q / kSeg;
+            pbs[w] = -1;
+            pa[w] = 0.0f;
+            if (rray[w] >= 0) {
+                const uint32_t ci = Hd[c].info;
+                pbs[w] = slot_seq(ci) ? -1 : c - slot_j(ci);
+                pa[w] = E[blk_entry(c, q % kSeg)].a;
+            }
+        }
+    }
     // the draw depends only on (branch, sweep, bit): made while the loads are in flight
-    double u = 0.0, bt = beta;
+    double u = 0.0, bt = beta, lin = 0.0;
     bool live = false;
-    if (pos >= 0 && j == 0) {
+    if (first) {
         int own = sweep;
-        live = bit_time<STAGED>(s, S.clk, g * BW + b, sweep, beta, own, bt);
-        u = draw_u01(s.seed_offset + S.key + uint64_t(g * BW + b) + 1u, uint32_t(own), uint32_t(S.bit));
+        live = bit_time<STAGED>(s, clk, g * BW + b, sweep, beta, own, bt);
+        u = draw_u01(s.seed_offset + Hd[slot].key + uint64_t(g * BW + b) + 1u, uint32_t(own), uint32_t(Hd[slot].bit));
+        lin = Hd[slot].lin;
     }
     // the residuals back in chain order
 #pragma unroll
-    for (int u = 0; u < kSeg; ++u) {
-        const int q = (u * 64 + lane) / BW;
-        tw[((q / kSeg) * BW + b) * kSeg + q % kSeg] = rvt[u];
+    for (int u = 0; u < kLoads; ++u) {
+        if constexpr (kPair) {
+            const int q = u * 64 + lane;
+            T* row = tw + (q / kSeg) * BW * kSeg + q % kSeg;
+            row[0] = __uint_as_float(uint32_t(rpair[u]));
+            row[kSeg] = __uint_as_float(uint32_t(rpair[u] >> 32));
+        } else {
+            const int q = (u * 64 + lane) / BW;
+            tw[((q / kSeg) * BW + b) * kSeg + q % kSeg] = rvt[u];
+        }
     }
     T rv[kSeg];
 #pragma unroll
     for (int u = 0; u < kSeg; ++u) rv[u] = tw[(ls * BW + b) * kSeg + u];
     const int xb = (xm >> b) & 1;
     const double delta = xb ? -1.0 : 1.0;
+    // the first kNb neighbours' disagreement with this branch's spin, one bit each
+    uint32_t unlike = 0;
+    if (first && nn > 0) {
+#pragma unroll
+        for (int v = 0; v < kNb; ++v) unlike |= uint32_t(((m[v] >> b) & 1) != xb) << v;
+    }
     if (pos >= 0) {
-        double partial = j == 0 ? S.lin * delta : 0.0;
-        if (len > 0) partial = chain_sum<T>(S.e, rv, len, delta, partial, j == 0);
+        double partial = j == 0 ? lin * delta : 0.0;
+        if (len > 0) partial = chain_sum_aw<T>(ea, ew, rv, len, delta, partial, j == 0);
         part[slot][b] = partial;
     }
-    after_compute();
     __syncthreads();
+    after_sync();
     // the first slot of every run of kChainRun chains adds the run's sums in order, in
     // place (no other slot reads them)
     if (pos >= 0 && j % kChainRun == 0) {
@@ -421,53 +459,75 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
         if (j + 1 < nch) {
             double v[kChainRun];
 #pragma unroll
-            for (int u = 1; u < kChainRun; ++u) v[u] = part[min(slot + u, kSlots - 1)][b];
-            double g = part[slot][b];
+            for (int w = 1; w < kChainRun; ++w) v[w] = part[min(slot + w, kSlots - 1)][b];
+            double gsum = part[slot][b];
 #pragma unroll
-            for (int u = 1; u < kChainRun; ++u) g = j + u < nch ? g + v[u] : g;
-            part[slot][b] = g;
+            for (int w = 1; w < kChainRun; ++w) gsum = j + w < nch ? gsum + v[w] : gsum;
+            part[slot][b] = gsum;
         }
     }
     __syncthreads();
-    if (pos >= 0 && j == 0) {
+    if (first) {
         const int nch = slot_nch(info);
         double dE = part[slot][b];
         for (int q0 = kChainRun; q0 < nch; q0 += 8 * kChainRun) {
             double v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = part[min(slot + q0 + u * kChainRun, kSlots - 1)][b];
+            for (int w = 0; w < 8; ++w) v[w] = part[min(slot + q0 + w * kChainRun, kSlots - 1)][b];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) dE = q0 + u * kChainRun < nch ? dE + v[u] : dE;
+            for (int w = 0; w < 8; ++w) dE = q0 + w * kChainRun < nch ? dE + v[w] : dE;
         }
         // lambda * (+-1) per neighbour in CSR order
 #pragma unroll
-        for (int u = 0; u < kNb; ++u) {
-            const int before = ((m[u] >> b) & 1) != xb;
-            const double nxt = dE + p.lambda * (before ? -1.0 : 1.0);
-            dE = u < nn ? nxt : dE;
+        for (int v = 0; v < kNb; ++v) {
+            const double nxt = dE + p.lambda * (((unlike >> v) & 1u) ? -1.0 : 1.0);
+            dE = v < nn ? nxt : dE;
         }
-        for (int k = S.d.nb0 + kNb; k < S.d.nb0 + nn; ++k) {
-            const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
-            dE += p.lambda * (before ? -1.0 : 1.0);
+        if (nn > kNb) {
+            const int nb0 = Hd[slot].nb0;
+            for (int k = nb0 + kNb; k < nb0 + nn; ++k) {
+                const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
+                dE += p.lambda * (before ? -1.0 : 1.0);
+            }
         }
         const bool acc = live && decide_drawn<T>(s, g * BW + b, u, bt, dE);
         accepted[slot][b] = acc ? 1 : 0;
         const uint16_t flips = segment_flips<BW>(acc);
         if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
+        if constexpr (kPair)
+            if (b == 0) fx[slot] = uint32_t(flips) | (uint32_t(xm) << 16);
     }
     __syncthreads();
+    if constexpr (kPair) {
+        // every entry's pair from the loading lane: both branches written back, the
+        // flipped ones updated (rays repeating in a column are left to the bit's first slot)
+        uint32_t f[kLoads];
+#pragma unroll
+        for (int w = 0; w < kLoads; ++w) f[w] = pbs[w] >= 0 ? fx[pbs[w]] : 0u;
+#pragma unroll
+        for (int w = 0; w < kLoads; ++w) {
+            if ((f[w] & 3u) == 0) continue;
+            float r0 = __uint_as_float(uint32_t(rpair[w])), r1 = __uint_as_float(uint32_t(rpair[w] >> 32));
+            if (f[w] & 1u) r0 = residual_add<float>(r0, pa[w], (f[w] & 0x10000u) ? -1.0 : 1.0);
+            if (f[w] & 2u) r1 = residual_add<float>(r1, pa[w], (f[w] & 0x20000u) ? -1.0 : 1.0);
+            *reinterpret_cast<uint64_t*>(rg + size_t(rray[w]) * BW) =
+                uint64_t(__float_as_uint(r0)) | (uint64_t(__float_as_uint(r1)) << 32);
+        }
+    }
     if (pos >= 0 && accepted[slot - j][b]) {
         if (slot_seq(info)) {
             // a ray repeats in this column: the bit's first slot scatters it in CSR order
             if (j == 0)
                 for (int k = p.col_ptr[pos]; k < p.col_ptr[pos + 1]; ++k) {
-                    const RayEnt e = p.ent[k];
-                    T* q = rg + size_t(e.ray) * BW + b;
-                    *q = residual_add<T>(ld_l2(q), e.a, delta);
+                    const RayEnt ek = p.ent[k];
+                    T* q = rg + size_t(ek.ray) * BW + b;
+                    *q = residual_add<T>(ld_l2(q), ek.a, delta);
                     __builtin_amdgcn_s_waitcnt(0);
                 }
-        } else if (len > 0) {
-            chain_scatter<T, BW>(S.e, rv, len, rg, b, delta);
+        } else if (!kPair && len > 0) {
+#pragma unroll
+            for (int v = 0; v < kSeg; ++v)
+                if (v < len) rg[size_t(E[blk_entry(slot, v)].ray) * BW + b] = residual_add<T>(rv[v], ea[v], delta);
         }
     }
 }
@@ -488,13 +548,11 @@ __device__ __forceinline__ HeavyBit heavy_bit(const DevBatches& bt, int batch, i
 // passes: the chains are spread over the block's BW-lane slots, their sums meet in
 // LDS, slot 0 adds them (in runs of kChainRun) and decides, and the residual scatter is spread
 // over the block again.
-template <typename T, int BW, int BLOCK, bool STAGED>
+template <typename T, int BW, int BLOCK, bool STAGED, typename F>
 __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState<T>& s, const HeavyBit& h, int g,
-                                             int sweep, double beta) {
+                                             int sweep, double beta, double (*part)[BW], int* accepted, F&& after) {
     const int pos = h.pos;
     constexpr int kSlots = BLOCK / BW;
-    __shared__ double part[kSlots][BW];
-    __shared__ int accepted[BW];
     const int b = threadIdx.x % BW;
     const int slot = threadIdx.x / BW;
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
@@ -566,6 +624,7 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
         }
     }
     __syncthreads();
+    after();
 }
 
 // A bit with more chains than a workgroup has slots, up to kHeavyPasses chains per
@@ -575,18 +634,24 @@ __device__ __forceinline__ void heavy_update(const DevProblem& p, const DevState
 // positions and masks are loaded by the deciding slot at the start, beside the column.
 // A step of such a bit costs the staging plus one round of residual loads, where the
 // pass-by-pass form costs two dependent round trips per pass and again per pass to
-// scatter. `hent` is the staging area in the sweep's dynamic LDS (heavy_stage_bytes).
+// scatter. The staging area is the step's own block buffer, h0 (empty for a multi-pass
+// batch), for the first kSlots chains and the sweep's work area hx for the rest, which
+// also holds the chain sums. `after` runs once the decision is made (the caller's copy
+// of the next block into the other buffer).
 
-template <typename T, int BW, int BLOCK, bool STAGED>
+template <typename T, int BW, int BLOCK, bool STAGED, typename F>
 __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevState<T>& s, const HeavyBit& h, int g,
-                                                 int sweep, double beta, RayEnt* hent) {
+                                                 int sweep, double beta, RayEnt* h0, RayEnt* hx, F&& after) {
     constexpr int kSlots = BLOCK / BW;
-    constexpr int kMaxEnt = kHeavyPasses * kSlots * kSeg;
+    constexpr int kE0 = kSlots * kSeg;   // entries staged in h0; the rest go to hx
+    constexpr int kMaxEnt = kHeavyPasses * kE0;
     constexpr int kNbHeavy = 8;
-    static_assert(kMaxEnt % 64 == 0, "a wave stages 64 entries at a time");
+    static_assert(kE0 % 64 == 0 && kMaxEnt % 64 == 0, "a wave stages 64 entries at a time");
     static_assert(BLOCK == kSweepBlock, "the staging area is sized for the sweep workgroup");
-    __shared__ double hpart[kHeavyPasses * kSlots][BW];
+    double (*hpart)[BW] = reinterpret_cast<double (*)[BW]>(hx + (kHeavyPasses - 1) * kE0);
     __shared__ int hacc[BW];
+    // chain j's entries: the first kSlots chains in h0, the others in hx
+    auto chain_at = [&](int j) -> const RayEnt* { return j < kSlots ? h0 + j * kSeg : hx + (j - kSlots) * kSeg; };
     const int tid = int(threadIdx.x);
     const int b = tid % BW, slot = tid / BW;
     uint16_t* xg = s.x + size_t(g) * size_t(p.n);
@@ -596,12 +661,13 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
     const uint16_t xm = ld_l2(xg + pos);
     // stage the column straight into LDS: wave w copies entries [64 v, 64 v + 64) for
     // v = w, w + 4, ... (lane i to row 64 v + i; rows past deg repeat the last entry and
-    // stay inside hent, as deg <= kMaxEnt and kMaxEnt is a multiple of 64)
+    // stay inside the area, as deg <= kMaxEnt and kE0 is a multiple of 64)
     {
         for (int v0 = (tid >> 6) * 64; v0 < deg; v0 += BLOCK) {
             const RayEnt* src = p.ent + k0 + min(v0 + (tid & 63), deg - 1);
+            RayEnt* dst = v0 < kE0 ? h0 + v0 : hx + (v0 - kE0);
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)(hent + v0), 16, 0, 0);
+                                             (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
         }
         // the deciding slot's neighbour positions, then their masks (neighbours are in
         // other classes, final since the barrier)
@@ -623,7 +689,7 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
         for (int qq = 0; qq < kHeavyPasses; ++qq) {
             const int j = slot + qq * kSlots;
             const int len = j < n_chain ? min(kSeg, deg - j * kSeg) : 0;
-            const RayEnt* e = hent + j * kSeg;
+            const RayEnt* e = chain_at(min(j, n_chain - 1));
 #pragma unroll
             for (int u = 0; u < kSeg; ++u) rv[qq][u] = T(0);
             if (len > 0) {
@@ -658,8 +724,9 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
             if (j < n_chain) {
                 const int len = min(kSeg, deg - j * kSeg);
                 RayEnt e[kSeg];
+                const RayEnt* ej = chain_at(j);
 #pragma unroll
-                for (int u = 0; u < kSeg; ++u) e[u] = hent[j * kSeg + min(u, len - 1)];
+                for (int u = 0; u < kSeg; ++u) e[u] = ej[min(u, len - 1)];
                 hpart[j][b] = chain_sum<T>(e, rv[qq], len, delta, lin * delta, j == 0);
             }
         }
@@ -703,12 +770,13 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
             if (b == 0 && flips) xg[pos] = uint16_t(xm ^ flips);
         }
         __syncthreads();
+        after();
         if (hacc[b]) {
             if (h.seq) {
                 // a ray repeats in this column: its updates must land in CSR order
                 if (slot == 0)
                     for (int k = 0; k < deg; ++k) {
-                        const RayEnt e = hent[k];
+                        const RayEnt e = k < kE0 ? h0[k] : hx[k - kE0];
                         T* qp = rg + size_t(e.ray) * BW + b;
                         *qp = residual_add<T>(ld_l2(qp), e.a, delta);
                         __builtin_amdgcn_s_waitcnt(0);
@@ -719,10 +787,11 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
                     const int j = slot + qq * kSlots;
                     if (j < n_chain) {
                         const int len = min(kSeg, deg - j * kSeg);
+                        const RayEnt* ej = chain_at(j);
 #pragma unroll
                         for (int u = 0; u < kSeg; ++u)
                             if (u < len) {
-                                const RayEnt e = hent[j * kSeg + u];
+                                const RayEnt e = ej[u];
                                 rg[size_t(e.ray) * BW + b] = residual_add<T>(rv[qq][u], e.a, delta);
                             }
                     }
@@ -730,21 +799,22 @@ __device__ __forceinline__ void heavy_lds_update(const DevProblem& p, const DevS
             }
         }
     }
-    __syncthreads();   // hent, hpart and hacc are reused by the next heavy step
+    __syncthreads();   // the staging area, hpart and hacc are reused by the next step
 }
 
 // Dispatch of a multi-pass batch: staged in LDS when its chains fit kHeavyPasses
 // rounds of the workgroup's slots (and the launch reserved the staging area), else
 // pass by pass.
-template <typename T, int BW, int BLOCK, bool STAGED>
+template <typename T, int BW, int BLOCK, bool STAGED, typename F>
 __device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T>& s, const DevBatches& bt, int batch,
-                                           int g, int sweep, double beta, RayEnt* hent) {
+                                           int g, int sweep, double beta, RayEnt* h0, RayEnt* hx, double (*part)[BW],
+                                           int (*accepted)[BW], F&& after) {
     constexpr int kSlots = BLOCK / BW;
     const HeavyBit h = heavy_bit(bt, batch, kSlots);
     if (bt.heavy_lds && h.k1 - h.k0 <= kHeavyPasses * kSlots * kSeg)
-        heavy_lds_update<T, BW, BLOCK, STAGED>(p, s, h, g, sweep, beta, hent);
+        heavy_lds_update<T, BW, BLOCK, STAGED>(p, s, h, g, sweep, beta, h0, hx, after);
     else
-        heavy_update<T, BW, BLOCK, STAGED>(p, s, h, g, sweep, beta);
+        heavy_update<T, BW, BLOCK, STAGED>(p, s, h, g, sweep, beta, part, accepted[0], after);
 }
 
 // The light bits of one class, [lb, le): no ray entry, at most kLightNb neighbours,
@@ -876,9 +946,15 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
     const int n_slots = team_size * kSlots;
     const int my_slot = rank * kSlots + slot;
     if (p.n_classes == 0) return;
-    // class -> batch table and heavy-batch bits, copied to LDS once
+    // class -> batch table and heavy-batch bits, copied to LDS once; then the two block
+    // buffers and the work area (sweep_dynamic_lds)
     extern __shared__ __align__(16) int32_t tables[];
-    RayEnt* hent = reinterpret_cast<RayEnt*>(reinterpret_cast<unsigned char*>(tables) + heavy_stage_offset(p.n_classes));
+    constexpr size_t kBlockBytes = size_t(kSlots) * (kSeg * sizeof(RayEnt) + sizeof(SlotHead));
+    unsigned char* pb0 = reinterpret_cast<unsigned char*>(tables) + sweep_tables_bytes_aligned(p.n_classes);
+    unsigned char* xa = pb0 + 2 * kBlockBytes;
+    __shared__ double s_part[kSlots][BW];
+    __shared__ uint32_t s_fx[kSlots];   // a deciding slot's flips | spin mask << 16 (two FP32 branches)
+    __shared__ int s_acc[kSlots][BW];
     int32_t* cls = tables;                          // n_classes + 1 batch offsets
     int32_t* nheavy = tables + p.n_classes + 1;    // n_classes multi-pass batch counts
     int32_t* lbeg = nheavy + p.n_classes;           // n_classes first light positions
@@ -934,12 +1010,10 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
         return cls[c] + q.nh + (pk - gi * q.np);
     };
     // Two lookahead cursors: (s1, c1, k1) is the task after the one being processed,
-    // (s2, c2, k2) the one after that. Once a packed step's state loads have returned
-    // and its chain sums are made, it issues the static operands of task 1 (whose
-    // descriptor it fetched a step earlier) and the descriptor of task 2; they land
-    // while the step combines, decides and scatters, so neither the drain before the
-    // barrier nor the next step waits long for them, and they do not compete with the
-    // step's own state loads.
+    // (s2, c2, k2) the one after that. A step issues the copy of task 1's block into the
+    // other buffer once its own chain sums are made (its state loads have returned, so
+    // the copy does not delay them), and advances cursor 2 while its state loads are in
+    // flight; the copy lands while the step decides, scatters and waits at the barrier.
     auto advance = [&](int& sw, int& c, int& k) {
         ++k;
         while (k >= n_tasks(c)) {
@@ -951,6 +1025,12 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
         }
         return true;
     };
+    // a batch's {first granule, slots used}, by a vector load (a scalar one would hold up
+    // the step's LDS waits until it returned)
+    auto meta_of = [&](int batch) {
+        const uint64_t v = ld_l2(reinterpret_cast<const uint64_t*>(bt.blk_meta) + batch);
+        return int2{int(uint32_t(v)), int(uint32_t(v >> 32))};
+    };
     int s1 = sweep_begin, c1 = 0, k1 = -1;
     bool a1 = advance(s1, c1, k1);
     int s2 = s1, c2 = c1, k2 = k1;
@@ -959,22 +1039,41 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
     bool h1 = false, h2 = false;
     if (a1) b1 = task_of(c1, k1, g1, h1);
     if (a2) b2 = task_of(c2, k2, g2, h2);
-    SlotStatic cur, nxt;
-    cur.d = SlotDesc{-1, 0, 0, 0u};
-    nxt.d = SlotDesc{-1, 0, 0, 0u};
+    // advances cursor 2 once per task: in the barrier wait after the task, or, when the
+    // next task follows in the same class, before it
+    bool cursor_due = false;
+    auto advance_cursor = [&] {
+        if (cursor_due && a2) {
+            a2 = advance(s2, c2, k2);
+            if (a2) b2 = task_of(c2, k2, g2, h2);
+        }
+        cursor_due = false;
+    };
+    int2 meta_cur = int2{0, 0};
     if (a1 && !h1) {
-        fetch_desc<kSlots>(bt, b1, slot, cur);
-        fetch_static<STAGED>(p, cur, s.clock);
+        meta_cur = meta_of(b1);
+        issue_block<BLOCK, kSlots>(bt.blk, meta_cur, pb0);
     }
-    if (a2 && !h2) fetch_desc<kSlots>(bt, b2, slot, nxt);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int buf = 0;
+    bool synced = true;   // the current block is visible to every wave (a barrier since its copy)
     unsigned target = 0;
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         // the staged form reads beta per bit, at its own sweep
         const double beta = STAGED ? 0.0 : s.beta[sweep];
         for (int c = 0; c < p.n_classes; ++c) {
             while (a1 && s1 == sweep && c1 == c) {
+                if (!synced) {
+                    // a second task of this class: its block was copied during the last step
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __syncthreads();
+                }
+                synced = false;
+                advance_cursor();
                 const int batch = b1, g = g1;
                 const bool heavy = h1;
+                const int n_used = __builtin_amdgcn_readfirstlane(meta_cur.y);
                 // shift the cursors: the next task is the old task 2
                 s1 = s2;
                 c1 = c2;
@@ -983,32 +1082,28 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 b1 = b2;
                 g1 = g2;
                 h1 = h2;
-                if (a2) {
-                    a2 = advance(s2, c2, k2);
-                    if (a2) b2 = task_of(c2, k2, g2, h2);
-                }
-                const bool fetch1 = a1 && !h1, fetch2 = a2 && !h2;
-                SlotStatic nn;
-                nn.d = SlotDesc{-1, 0, 0, 0u};
+                cursor_due = true;
+                const bool pf = a1 && !h1;
+                const int2 meta_nxt = pf ? meta_of(b1) : int2{0, 0};
+                unsigned char* pc = pb0 + size_t(buf) * kBlockBytes;
+                unsigned char* pn = pb0 + size_t(buf ^ 1) * kBlockBytes;
+                auto next_block = [&] {
+                    if (pf) issue_block<BLOCK, kSlots>(bt.blk, meta_nxt, pn);
+                };
                 if (heavy) {
-                    if (fetch1) fetch_static<STAGED>(p, nxt, s.clock);
-                    if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
-                    heavy_step<T, BW, BLOCK, STAGED>(p, s, bt, batch, g, sweep, beta, hent);
+                    heavy_step<T, BW, BLOCK, STAGED>(p, s, bt, batch, g, sweep, beta, reinterpret_cast<RayEnt*>(pc),
+                                                     reinterpret_cast<RayEnt*>(xa), s_part, s_acc, next_block);
                 } else {
-                    packed_step<T, BW, BLOCK, STAGED>(
-                        p, s, cur, g, sweep, beta,
-                        [] {},
-                        [&] {
-                            if (fetch1) fetch_static<STAGED>(p, nxt, s.clock);
-                            if (fetch2) fetch_desc<kSlots>(bt, b2, slot, nn);
-                        });
+                    packed_step<T, BW, BLOCK, STAGED>(p, s, pc, n_used, reinterpret_cast<T*>(xa), s_part, s_acc, s_fx, g,
+                                                      sweep, beta, [] {}, next_block);
                 }
-                cur = nxt;
-                nxt.d = nn.d;
+                meta_cur = meta_nxt;
+                buf ^= 1;
             }
             light_sweep<T, BW, STAGED>(p, s, lbeg[c], cend[c + 1], my_slot, n_slots, my_groups, team, sweep, beta);
             target += unsigned(team_size);
-            if (!team_barrier(bar, target, s.err)) return;
+            if (!team_barrier(bar, rank, team_size, target, s.err, advance_cursor)) return;
+            synced = true;
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
             const int tasks = p.n_rays * my_groups;
@@ -1017,9 +1112,43 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 ray_recompute<T, BW>(p, s, t - gi * p.n_rays, team + gi * kTeams, b);
             }
             target += unsigned(team_size);
-            if (!team_barrier(bar, target, s.err)) return;
+            if (!team_barrier(bar, rank, team_size, target, s.err)) return;
         }
     }
+}
+
+// One thread per (batch, slot) of the packed batches: writes a used slot's chain
+// (entries past its length: ray -1, a 0, w 0) and head into its batch's block.
+__global__ void batch_blocks_kernel(DevProblem p, const SlotDesc* desc, const int2* meta, int n_batches, int slots,
+                                    uint4* blk) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(n_batches) * slots) return;
+    const int bi = int(t / slots), sl = int(t - int64_t(bi) * slots);
+    const int2 m = meta[bi];
+    if (sl >= m.y) return;
+    const SlotDesc d = desc[t];
+    RayEnt* E = reinterpret_cast<RayEnt*>(blk + m.x);
+    const int len = slot_len(d.info), nn = slot_nnb(d.info);
+    for (int u = 0; u < kSeg; ++u) E[blk_entry(sl, u)] = u < len ? p.ent[d.ka + u] : RayEnt{-1, 0.0f, 0.0};
+    SlotHead h;
+    h.pos = d.pos;
+    h.info = d.info;
+    h.nb0 = d.nb0;
+    h.bit = p.draw_bit[d.pos];
+    for (int u = 0; u < kNb; ++u) h.q[u] = nn > 0 ? p.nb[d.nb0 + min(u, nn - 1)] : d.pos;
+    h.lin = p.lin[d.pos];
+    h.key = p.draw_key[d.pos];
+    h.pad[0] = 0;
+    h.pad[1] = 0;
+    reinterpret_cast<SlotHead*>(blk + m.x + int64_t(m.y) * kSeg)[sl] = h;
+}
+
+hipError_t launch_batch_blocks(const DevProblem& p, const SlotDesc* desc, const int2* meta, int n_batches, int slots,
+                               uint4* blk, hipStream_t st) {
+    const int64_t items = int64_t(n_batches) * slots;
+    if (items <= 0) return hipSuccess;
+    batch_blocks_kernel<<<unsigned((items + 255) / 256), 256, 0, st>>>(p, desc, meta, n_batches, slots, blk);
+    return hipGetLastError();
 }
 
 template <typename T, int BW>
@@ -1306,7 +1435,7 @@ hipError_t launch_ray_sweeps(const DevProblem& p, const DevBatches& bt, int prec
             // device's global wave sync resource, which one process holds at a time, so a
             // second process on the GPU would be refused. A workgroup that is not resident
             // leaves its team short or a barrier waiting: both end in the error flag.
-            const size_t lds = sweep_dynamic_lds(p.n_classes, BW, bt.heavy_lds != 0);
+            const size_t lds = sweep_dynamic_lds(p.n_classes, BW, bt.heavy_lds != 0, precision);
             const hipError_t e =
                 hipFuncSetAttribute(reinterpret_cast<const void*>(&ray_team_kernel<T, BW, kSweepBlock, STAGED>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -1770,7 +1899,7 @@ __global__ void __launch_bounds__(256) team_barrier_probe_kernel(unsigned* words
     for (int s = 1; s <= steps; ++s) {
         mine[threadIdx.x] = unsigned(s);
         target += unsigned(team_size);
-        if (!team_barrier(tbar, target, err)) break;
+        if (!team_barrier(tbar, rank, team_size, target, err)) break;
         for (int r = 0; r < team_size; ++r) bad += ld_l2(team_words + size_t(r) * 256 + threadIdx.x) < unsigned(s);
     }
     if (bad) atomicAdd(stale, bad);
