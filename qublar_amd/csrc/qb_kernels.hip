@@ -235,11 +235,12 @@ __device__ __forceinline__ double neighbour_terms(const DevProblem& p, const uin
 }
 
 // The Metropolis test of one branch with its draw u = draw_u01(seed, sweep, bit)
-// already made: accept iff dE <= 0 or u < qexp(-dE * beta).
+// already made: accept iff dE <= 0 or u < qexp(-dE * beta), decided through the float
+// exponential with a guard band (uphill_accept: qexp's decision, bit for bit).
 template <typename T>
 __device__ __forceinline__ bool decide_drawn(const DevState<T>& s, int branch, double u, double beta, double dE) {
     if (branch >= s.R) return false;
-    return dE <= 0.0 || u < qexp(-dE * beta);
+    return dE <= 0.0 || uphill_accept(-dE * beta, u);
 }
 
 // The accepted lanes of this lane's BW-wide segment, as a mask of the group's branches.

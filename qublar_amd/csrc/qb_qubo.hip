@@ -31,22 +31,6 @@ namespace qb {
 
 constexpr int kRowChunk = 16;
 
-// The Metropolis decision u < qexp(-dE * beta) for dE > 0, decided by the hardware float
-// exponential where that is unambiguous. For -80 < x = -dE beta <= 0, e_f = 2^(float(x)
-// log2e) (v_exp_f32) is within 1.5e-5 of e^x (float(x): 4.8e-6 absolute; the product with
-// log2e: 2.1e-5 in the exponent; v_exp_f32: a few ulp), so u < e_f (1 - 2^-14) implies
-// u < qexp(x) and u > e_f (1 + 2^-14) implies u > qexp(x); a draw inside the band takes
-// qexp itself. For x <= -80, qexp(x) < 2e-35 lies below every nonzero 53-bit draw, so
-// only u = 0 can pass. Every decision is therefore qexp's, bit for bit.
-__device__ __forceinline__ bool uphill_accept(double x, double u) {
-    if (!(x > -80.0)) return u == 0.0 && qexp(x) > 0.0;
-    constexpr double kBand = 0x1.0p-14;
-    const double e = double(__builtin_amdgcn_exp2f(float(x) * 1.44269504f));
-    if (u < e * (1.0 - kBand)) return true;
-    if (u > e * (1.0 + kBand)) return false;
-    return u < qexp(x);
-}
-
 // Spin words of a job in LDS: one byte per bit up to 8 replicas per job, 16 or 32 bits
 // beyond; kQuboLdsBytes of them hold instances up to kQuboLdsBytes / sizeof(word) bits.
 template <int RB>
