@@ -317,76 +317,158 @@ __device__ __forceinline__ void issue_block(const uint4* blk, int2 meta, unsigne
     copy_to_lds<BLOCK>(blk + first + n * kSeg, n * kHeadGranules, pb + size_t(KSLOTS) * kSeg * 16);
 }
 
-// One packed batch for the BW branches of group g, its block in LDS at pb with n_used
-// slots: every slot evaluates one chain of one bit; the chain sums meet in LDS, the
-// first slot of each run of kChainRun chains adds the run, the bit's first slot adds
-// the runs and the neighbour terms and decides; then the accepted chains are scattered
-// (with two FP32 branches by the lanes that loaded each entry's pair, else by each
-// slot). `after_issue` runs once the state loads are issued, `after_sync` once every
-// slot's chain sum is in LDS (the caller's cursor and its copy of the next block).
-// `trow` is the residual transposition area (kSweepBlock * kSeg values of T), `part`,
-// `accepted` and `fx` the workgroup's chain sums, decisions and per-bit flip records.
-template <typename T, int BW, int BLOCK, bool STAGED, typename F1, typename F2>
-__device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<T>& s, const unsigned char* pb,
-                                            int n_used, T* trow, double (*part)[BW], int (*accepted)[BW],
-                                            uint32_t* fx, int g, int sweep, double beta, F1&& after_issue,
-                                            F2&& after_sync) {
+// The per-lane operands of a packed task that depend only on its block, its sweep and
+// its group: the slot's bit and chain, the rays of the entries the lane loads, the
+// neighbour positions, the draw and, with pairs, each loaded entry's share and bit. A
+// task's prep is made from its block in LDS before its step -- in the barrier wait before
+// it when a barrier separates it from the previous task -- so once the barrier opens the
+// step only issues its state loads.
+template <typename T, int BW>
+struct PackedPrep {
+    static constexpr bool kPair = BW == 2 && sizeof(T) == 4;
+    static constexpr int kLoads = kPair ? kSeg / 2 : kSeg;
+    int pos;
+    uint32_t info;
+    int ray[kLoads];
+    int q[kNb];
+    int pbs[kPair ? kLoads : 1];
+    float pa[kPair ? kLoads : 1];
+    double u, bt, lin;
+    bool live;
+};
+
+// Residual loads across the wave, in whole chains (whose rays share few lines) rather
+// than one entry of 64 / BW unrelated chains. With two FP32 branches a lane loads an
+// entry's pair of residuals as one 8-byte word: load u of lane l is entry q % kSeg of the
+// wave's chain q / kSeg, q = 64 u + l, so a wave issues kSeg / 2 loads. Otherwise a lane
+// loads one branch's residual: q = (64 u + l) / BW, kSeg loads.
+template <typename T, int BW, int BLOCK, bool STAGED>
+__device__ __forceinline__ void packed_prep(const DevState<T>& s, const unsigned char* pb, int n_used, int g,
+                                            int sweep, double beta, PackedPrep<T, BW>& P) {
     constexpr int kSlots = BLOCK / BW;
-    constexpr int kSpw = 64 / BW;   // slots per wave
+    constexpr int kSpw = 64 / BW;
+    constexpr bool kPair = PackedPrep<T, BW>::kPair;
+    constexpr int kLoads = PackedPrep<T, BW>::kLoads;
     const RayEnt* E = reinterpret_cast<const RayEnt*>(pb);
     const SlotHead* Hd = reinterpret_cast<const SlotHead*>(pb + size_t(kSlots) * kSeg * sizeof(RayEnt));
     const int tid = int(threadIdx.x);
     const int b = tid % BW, slot = tid / BW;
     const int lane = tid & 63, wave = tid >> 6;
+    // every LDS read first, unconditionally (slots past n_used hold stale data, which the
+    // masks below discard), so they are in flight together
     const bool used = slot < n_used;
-    const int pos = used ? Hd[slot].pos : -1;
-    const uint32_t info = used ? Hd[slot].info : 0u;
-    const int len = slot_len(info), j = slot_j(info), nn = slot_nnb(info);
-    uint16_t* xg = s.x + size_t(g) * size_t(p.n);
-    T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
-    // Residual loads across the wave, in whole chains (whose rays share few lines) rather
-    // than one entry of 64 / BW unrelated chains. With two FP32 branches a lane loads an
-    // entry's pair of residuals as one 8-byte word: load u of lane l is entry q % kSeg of
-    // the wave's chain q / kSeg, q = 64 u + l, so a wave issues kSeg / 2 loads. Otherwise
-    // a lane loads one branch's residual: q = (64 u + l) / BW, kSeg loads. The residuals
-    // come back through this wave's rows of trow in chain order (a wave's LDS operations
-    // complete in order, so no barrier is needed).
-    constexpr bool kPair = BW == 2 && sizeof(T) == 4;
-    constexpr int kLoads = kPair ? kSeg / 2 : kSeg;
-    T* tw = trow + wave * kSpw * BW * kSeg;
-    const int ls = slot - wave * kSpw;
-    T rvt[kPair ? 1 : kSeg];
-    uint64_t rpair[kPair ? kLoads : 1];
-    int rray[kPair ? kLoads : 1];
+    const int hpos = Hd[slot].pos;
+    const uint32_t hinfo = Hd[slot].info;
+    const uint64_t hkey = Hd[slot].key;
+    const int hbit = Hd[slot].bit;
+    const double hlin = Hd[slot].lin;
+    int hq[kNb];
+#pragma unroll
+    for (int v = 0; v < kNb; ++v) hq[v] = Hd[slot].q[v];
+    int ray[kLoads];
+    uint32_t cinfo[kPair ? kLoads : 1];
+    float ca[kPair ? kLoads : 1];
 #pragma unroll
     for (int u = 0; u < kLoads; ++u) {
         const int q = kPair ? u * 64 + lane : (u * 64 + lane) / BW;
         const int c = wave * kSpw + q / kSeg;
-        const int ray = c < n_used ? E[blk_entry(c, q % kSeg)].ray : -1;
+        const RayEnt* e = E + blk_entry(c, q % kSeg);
+        ray[u] = e->ray;
         if constexpr (kPair) {
-            rray[u] = ray;
-            rpair[u] = ray >= 0 ? ld_l2(reinterpret_cast<const uint64_t*>(rg + size_t(ray) * BW)) : 0ull;
-        } else {
-            rvt[u] = ray >= 0 ? ld_l2(rg + size_t(ray) * BW + b) : T(0);
+            ca[u] = e->a;
+            cinfo[u] = Hd[c].info;
         }
+    }
+    P.pos = used ? hpos : -1;
+    P.info = used ? hinfo : 0u;
+    const int j = slot_j(P.info), nn = slot_nnb(P.info);
+    const bool first = P.pos >= 0 && j == 0;
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+        const int q = kPair ? u * 64 + lane : (u * 64 + lane) / BW;
+        const int c = wave * kSpw + q / kSeg;
+        P.ray[u] = c < n_used ? ray[u] : -1;
+    }
+#pragma unroll
+    for (int v = 0; v < kNb; ++v) P.q[v] = first && nn > 0 ? hq[v] : 0;
+    if constexpr (kPair) {
+        // each loaded entry's share and its bit's first slot (-1: no entry, or a column
+        // whose rays repeat, which that slot scatters itself)
+#pragma unroll
+        for (int w = 0; w < kLoads; ++w) {
+            const int q = w * 64 + lane;
+            const int c = wave * kSpw + q / kSeg;
+            const bool has = P.ray[w] >= 0;
+            P.pbs[w] = has && !slot_seq(cinfo[w]) ? c - slot_j(cinfo[w]) : -1;
+            P.pa[w] = has ? ca[w] : 0.0f;
+        }
+    }
+    // the draw depends only on (branch, sweep, bit)
+    P.u = 0.0;
+    P.bt = beta;
+    P.lin = 0.0;
+    P.live = false;
+    if (first) {
+        const uint32_t clk = STAGED ? s.clock[P.pos] : 0u;
+        int own = sweep;
+        P.live = bit_time<STAGED>(s, clk, g * BW + b, sweep, beta, own, P.bt);
+        P.u = draw_u01(s.seed_offset + hkey + uint64_t(g * BW + b) + 1u, uint32_t(own), uint32_t(hbit));
+        P.lin = hlin;
+    }
+}
+
+// One packed batch for the BW branches of group g, its block in LDS at pb and its prep
+// made: every slot evaluates one chain of one bit; the chain sums meet in LDS, the first
+// slot of each run of kChainRun chains adds the run, the bit's first slot adds the runs
+// and the neighbour terms and decides; then the accepted chains are scattered (with two
+// FP32 branches by the lanes that loaded each entry's pair, else by each slot).
+// `after_sync` runs once every slot's chain sum is in LDS (the caller's copy of the next
+// block). `trow` is the residual transposition area (kSweepBlock * kSeg values of T),
+// `part`, `accepted` and `fx` the workgroup's chain sums, decisions and per-bit flip
+// records.
+template <typename T, int BW, int BLOCK, bool STAGED, typename F2>
+__device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<T>& s, const unsigned char* pb,
+                                            const PackedPrep<T, BW>& P, T* trow, double (*part)[BW],
+                                            int (*accepted)[BW], uint32_t* fx, int g, F2&& after_sync) {
+    constexpr int kSlots = BLOCK / BW;
+    constexpr int kSpw = 64 / BW;   // slots per wave
+    constexpr bool kPair = PackedPrep<T, BW>::kPair;
+    constexpr int kLoads = PackedPrep<T, BW>::kLoads;
+    const RayEnt* E = reinterpret_cast<const RayEnt*>(pb);
+    const SlotHead* Hd = reinterpret_cast<const SlotHead*>(pb + size_t(kSlots) * kSeg * sizeof(RayEnt));
+    const int tid = int(threadIdx.x);
+    const int b = tid % BW, slot = tid / BW;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int pos = P.pos;
+    const uint32_t info = P.info;
+    const int len = slot_len(info), j = slot_j(info), nn = slot_nnb(info);
+    const bool first = pos >= 0 && j == 0;
+    uint16_t* xg = s.x + size_t(g) * size_t(p.n);
+    T* rg = s.r + size_t(g) * size_t(p.n_rays) * BW;
+    // The residuals come back through this wave's rows of trow in chain order (a wave's
+    // LDS operations complete in order, so no barrier is needed).
+    T* tw = trow + wave * kSpw * BW * kSeg;
+    const int ls = slot - wave * kSpw;
+    T rvt[kPair ? 1 : kSeg];
+    uint64_t rpair[kPair ? kLoads : 1];
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+        const int ray = P.ray[u];
+        if constexpr (kPair)
+            rpair[u] = ray >= 0 ? ld_l2(reinterpret_cast<const uint64_t*>(rg + size_t(ray) * BW)) : 0ull;
+        else
+            rvt[u] = ray >= 0 ? ld_l2(rg + size_t(ray) * BW + b) : T(0);
     }
     uint16_t m[kNb];
     uint16_t xm = 0;
-    uint32_t clk = 0;
-    const bool first = pos >= 0 && j == 0;
     if (pos >= 0) {
         xm = ld_l2(xg + pos);
         if (first && nn > 0) {
 #pragma unroll
-            for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + Hd[slot].q[u]);
+            for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + P.q[u]);
         }
-        if constexpr (STAGED)
-            if (first) clk = s.clock[pos];
     }
-    after_issue();
-    // the chain's shares and weights, from LDS while the loads are in flight; with pairs
-    // also, for each loaded entry, its share and its bit's first slot (-1: no entry, or
-    // a column whose rays repeat, which that slot scatters itself)
+    // the chain's shares and weights, from LDS while the loads are in flight
     float ea[kSeg];
     double ew[kSeg];
     if (pos >= 0) {
@@ -397,31 +479,8 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
             ew[v] = ev.w;
         }
     }
-    int pbs[kPair ? kLoads : 1];
-    float pa[kPair ? kLoads : 1];
-    if constexpr (kPair) {
-#pragma unroll
-        for (int w = 0; w < kLoads; ++w) {
-            const int q = w * 64 + lane;
-            const int c = wave * kSpw + q / kSeg;
-            pbs[w] = -1;
-            pa[w] = 0.0f;
-            if (rray[w] >= 0) {
-                const uint32_t ci = Hd[c].info;
-                pbs[w] = slot_seq(ci) ? -1 : c - slot_j(ci);
-                pa[w] = E[blk_entry(c, q % kSeg)].a;
-            }
-        }
-    }
-    // the draw depends only on (branch, sweep, bit): made while the loads are in flight
-    double u = 0.0, bt = beta, lin = 0.0;
-    bool live = false;
-    if (first) {
-        int own = sweep;
-        live = bit_time<STAGED>(s, clk, g * BW + b, sweep, beta, own, bt);
-        u = draw_u01(s.seed_offset + Hd[slot].key + uint64_t(g * BW + b) + 1u, uint32_t(own), uint32_t(Hd[slot].bit));
-        lin = Hd[slot].lin;
-    }
+    const double u = P.u, bt = P.bt, lin = P.lin;
+    const bool live = P.live;
     // the residuals back in chain order
 #pragma unroll
     for (int u = 0; u < kLoads; ++u) {
@@ -504,14 +563,14 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
         // flipped ones updated (rays repeating in a column are left to the bit's first slot)
         uint32_t f[kLoads];
 #pragma unroll
-        for (int w = 0; w < kLoads; ++w) f[w] = pbs[w] >= 0 ? fx[pbs[w]] : 0u;
+        for (int w = 0; w < kLoads; ++w) f[w] = P.pbs[w] >= 0 ? fx[P.pbs[w]] : 0u;
 #pragma unroll
         for (int w = 0; w < kLoads; ++w) {
             if ((f[w] & 3u) == 0) continue;
             float r0 = __uint_as_float(uint32_t(rpair[w])), r1 = __uint_as_float(uint32_t(rpair[w] >> 32));
-            if (f[w] & 1u) r0 = residual_add<float>(r0, pa[w], (f[w] & 0x10000u) ? -1.0 : 1.0);
-            if (f[w] & 2u) r1 = residual_add<float>(r1, pa[w], (f[w] & 0x20000u) ? -1.0 : 1.0);
-            *reinterpret_cast<uint64_t*>(rg + size_t(rray[w]) * BW) =
+            if (f[w] & 1u) r0 = residual_add<float>(r0, P.pa[w], (f[w] & 0x10000u) ? -1.0 : 1.0);
+            if (f[w] & 2u) r1 = residual_add<float>(r1, P.pa[w], (f[w] & 0x20000u) ? -1.0 : 1.0);
+            *reinterpret_cast<uint64_t*>(rg + size_t(P.ray[w]) * BW) =
                 uint64_t(__float_as_uint(r0)) | (uint64_t(__float_as_uint(r1)) << 32);
         }
     }
@@ -987,8 +1046,17 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
         const int r = rank - q.Hr, n = team_size - q.Hr;
         return r < q.P ? (q.P - r + n - 1) / n : 0;
     };
+    // With one group per team (Hr = 0 always) a rank's k-th task in class c is the class's
+    // batch rank + k * team_size, so the cursor needs no division.
+    const bool one_group = my_groups == 1 && heavy_ranks(1, team_size) == 0;
     // the batch and group of this rank's k-th task in class c; heavy = multi-pass
     auto task_of = [&](int c, int k, int& g, bool& heavy) {
+        if (one_group) {
+            const int t = rank + k * team_size;
+            heavy = t < nheavy[c];
+            g = team;
+            return cls[c] + t;
+        }
         const Split q = split_of(c);
         int h = -1, pk = -1;
         if (q.Hr == 0) {
@@ -1017,6 +1085,16 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
     // flight; the copy lands while the step decides, scatters and waits at the barrier.
     auto advance = [&](int& sw, int& c, int& k) {
         ++k;
+        if (one_group) {
+            while (rank + k * team_size >= cls[c + 1] - cls[c]) {
+                k = 0;
+                if (++c == p.n_classes) {
+                    c = 0;
+                    if (++sw >= sweep_end) return false;
+                }
+            }
+            return true;
+        }
         while (k >= n_tasks(c)) {
             k = 0;
             if (++c == p.n_classes) {
@@ -1059,10 +1137,30 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
     __syncthreads();
     int buf = 0;
     bool synced = true;   // the current block is visible to every wave (a barrier since its copy)
+    // The next packed task's prep: made in the barrier wait before it (its block has
+    // landed and every wave has passed the barrier's drain), else at its start.
+    PackedPrep<T, BW> prep;
+    bool prepped = false;
+    double beta_now = 0.0, beta_next = 0.0;
+    int sweep_now = sweep_begin;
+    auto at_arrival = [&] {
+        if (a1 && !h1 && !prepped) {
+            const double b1beta = STAGED ? 0.0 : s1 == sweep_now ? beta_now : s1 == sweep_now + 1 ? beta_next
+                                                                                            : s.beta[s1];
+            packed_prep<T, BW, BLOCK, STAGED>(s, pb0 + size_t(buf) * kBlockBytes,
+                                              __builtin_amdgcn_readfirstlane(meta_cur.y), g1, s1, b1beta, prep);
+            prepped = true;
+        }
+        advance_cursor();
+    };
     unsigned target = 0;
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
-        // the staged form reads beta per bit, at its own sweep
+        // the staged form reads beta per bit, at its own sweep; the next sweep's is at hand
+        // for a prep made in this sweep's last barrier
         const double beta = STAGED ? 0.0 : s.beta[sweep];
+        beta_now = beta;
+        beta_next = STAGED || sweep + 1 >= sweep_end ? 0.0 : s.beta[sweep + 1];
+        sweep_now = sweep;
         for (int c = 0; c < p.n_classes; ++c) {
             while (a1 && s1 == sweep && c1 == c) {
                 if (!synced) {
@@ -1075,6 +1173,9 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 const int batch = b1, g = g1;
                 const bool heavy = h1;
                 const int n_used = __builtin_amdgcn_readfirstlane(meta_cur.y);
+                if (!heavy && !prepped)
+                    packed_prep<T, BW, BLOCK, STAGED>(s, pb0 + size_t(buf) * kBlockBytes, n_used, g, sweep, beta, prep);
+                prepped = false;
                 // shift the cursors: the next task is the old task 2
                 s1 = s2;
                 c1 = c2;
@@ -1095,15 +1196,15 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                     heavy_step<T, BW, BLOCK, STAGED>(p, s, bt, batch, g, sweep, beta, reinterpret_cast<RayEnt*>(pc),
                                                      reinterpret_cast<RayEnt*>(xa), s_part, s_acc, next_block);
                 } else {
-                    packed_step<T, BW, BLOCK, STAGED>(p, s, pc, n_used, reinterpret_cast<T*>(xa), s_part, s_acc, s_fx, g,
-                                                      sweep, beta, [] {}, next_block);
+                    packed_step<T, BW, BLOCK, STAGED>(p, s, pc, prep, reinterpret_cast<T*>(xa), s_part, s_acc, s_fx, g,
+                                                      next_block);
                 }
                 meta_cur = meta_nxt;
                 buf ^= 1;
             }
             light_sweep<T, BW, STAGED>(p, s, lbeg[c], cend[c + 1], my_slot, n_slots, my_groups, team, sweep, beta);
             target += unsigned(team_size);
-            if (!team_barrier(bar, rank, team_size, target, s.err, advance_cursor)) return;
+            if (!team_barrier(bar, rank, team_size, target, s.err, at_arrival)) return;
             synced = true;
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
