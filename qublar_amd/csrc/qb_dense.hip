@@ -28,16 +28,17 @@ constexpr int kDenseWin = 64;
 constexpr int kRankChunk = 8;
 constexpr int kFieldGroup = 4;   // fields a thread updates together (i, i + block, ...)
 
-// Fields i = g0 + q kDenseBlock (q < kFieldGroup, i < n) from scratch:
+// Fields i = g0 + q STRIDE (q < kFieldGroup, i < n) from scratch:
 // h_i + sum_j x_j Q_ji in FP64, j ascending (Q_ji = Q_ij), additions of unset bits
 // skipped; every field's loads of a 16-column chunk in flight together.
+template <int STRIDE = kDenseBlock>
 __device__ __forceinline__ void scratch_fields(const DevDense& d, const uint8_t* x, int g0, double (&s)[kFieldGroup]) {
     constexpr int kChunk = 16;
     const int n = d.n;
     int col[kFieldGroup];
 #pragma unroll
     for (int q = 0; q < kFieldGroup; ++q) {
-        col[q] = min(g0 + q * kDenseBlock, n - 1);
+        col[q] = min(g0 + q * STRIDE, n - 1);
         s[q] = double(d.h[col[q]]);
     }
     for (int j0 = 0; j0 < n; j0 += kChunk) {
@@ -56,6 +57,36 @@ __device__ __forceinline__ void scratch_fields(const DevDense& d, const uint8_t*
             }
         }
     }
+}
+
+// One window's sequential decisions by one wave, lane l holding bit i0 + l: its field fl,
+// spin xl, its row of the window's Q block qb[t] = Q[i0 + t][i0 + l] and its -log draw L.
+// At step t lane t decides (dE = (1 - 2 x_t) f_t; accept iff dE <= 0 or dE beta < L_t);
+// on a flip every lane reads lane t's spin and adds Q[i0 + t][i0 + l] delta_t to its field,
+// so later bits see it. Lane 0 records the flips (s_t, s_delta); returns their count.
+template <typename T>
+__device__ __forceinline__ int decide_window(int wl, double b, double L, const float (&qb)[kDenseWin], T& fl, int& xl,
+                                             int* s_t, float* s_delta) {
+    const int lane = int(threadIdx.x) & 63;
+    int k = 0;
+#pragma unroll
+    for (int t = 0; t < kDenseWin; ++t) {
+        if (t < wl) {
+            const double dE = xl ? -double(fl) : double(fl);
+            const bool flip = lane == t && (dE <= 0.0 || dE * b < L);
+            if (__ballot(flip)) {
+                const float dl = __builtin_amdgcn_readlane(xl, t) ? -1.0f : 1.0f;
+                fl = fl + T(qb[t]) * T(dl);
+                if (lane == t) xl ^= 1;
+                if (lane == 0) {
+                    s_t[k] = t;
+                    s_delta[k] = dl;
+                }
+                ++k;
+            }
+        }
+    }
+    return k;
 }
 
 template <typename T>
@@ -91,24 +122,7 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
                 const double L = -qlog(draw_u01(key, uint32_t(sweep), uint32_t(i)));
                 T fl = f[i];
                 int xl = x[i];
-                int k = 0;
-#pragma unroll
-                for (int t = 0; t < kDenseWin; ++t) {
-                    if (t < w) {
-                        const double dE = xl ? -double(fl) : double(fl);
-                        const bool flip = lane == t && (dE <= 0.0 || dE * b < L);
-                        if (__ballot(flip)) {
-                            const float dl = __shfl(xl ? -1.0f : 1.0f, t);
-                            fl = fl + T(qb[t]) * T(dl);
-                            if (lane == t) xl ^= 1;
-                            if (lane == 0) {
-                                s_t[k] = t;
-                                s_delta[k] = dl;
-                            }
-                            ++k;
-                        }
-                    }
-                }
+                const int k = decide_window<T>(w, b, L, qb, fl, xl, s_t, s_delta);
                 if (live) {
                     f[i] = fl;
                     x[i] = uint8_t(xl);
@@ -202,6 +216,256 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
     }
 }
 
+// ---- several workgroups per replica ----------------------------------------------------
+// kDenseParts workgroups on one XCD anneal one replica together. Part p owns a contiguous
+// run of windows and the fields of their bits; every part keeps the whole spin vector.
+// Windows go in order: the window's owner decides it as above (its fields are its own) and
+// publishes the window's flips -- a count and one byte per flip, t | 0x80 for 1 -> 0 -- in a
+// slot of the replica's ring (two sweeps deep), then raises the replica's published count.
+// Every other part waits for that count, reads the flips and applies them to its spins;
+// then every part adds the flipped rows, in flip order, to its fields. Each field therefore
+// receives the same additions in the same order as with one workgroup, and the spins are
+// the same bit for bit; only the final energy's partial sums are grouped by part. The
+// parts share state through the XCD's L2 (stores drained before the count is raised, sc1
+// loads), like the ray sweep's teams.
+//
+// The ring's two sweeps suffice: the owner of a window in sweep s + 2 decides only after
+// every part has published its windows of sweep s + 1, which each did after applying the
+// same window of sweep s.
+constexpr int kDenseSlotWords = 1 + kDenseWin / 4;
+constexpr int kTeamRankChunk = 16;
+constexpr int kDenseTeamBlock = 512;   // two waves per SIMD: room for a window's Q block in registers
+
+struct DenseTeamWork {
+    unsigned* tickets;   // [8 * 64]: per-XCD tickets
+    unsigned* pub;       // [R * 32]: replica r's published window count at pub[32 r]
+    unsigned* done;      // [R * 32]: parts finished, at done[32 r]
+    double* part_e;      // [R * kDenseParts]
+    double* part_err;    // [R * kDenseParts]
+    uint32_t* slots;     // [R][2][n_win][kDenseSlotWords]
+};
+
+__host__ __device__ inline int dense_part_fields_max(int n) {
+    const int n_win = (n + kDenseWin - 1) / kDenseWin;
+    return ((n_win + kDenseParts - 1) / kDenseParts) * kDenseWin;
+}
+
+size_t dense_team_work_bytes(int n, int R) {
+    const size_t n_win = size_t((n + kDenseWin - 1) / kDenseWin);
+    return sizeof(unsigned) * (8 * 64 + 2 * 32 * size_t(R)) + sizeof(double) * 2 * size_t(R) * kDenseParts +
+           sizeof(uint32_t) * size_t(R) * 2 * n_win * kDenseSlotWords;
+}
+
+static DenseTeamWork dense_team_work(void* work, int n, int R) {
+    unsigned char* p = static_cast<unsigned char*>(work);
+    DenseTeamWork w;
+    w.tickets = reinterpret_cast<unsigned*>(p);
+    w.pub = w.tickets + 8 * 64;
+    w.done = w.pub + 32 * size_t(R);
+    w.part_e = reinterpret_cast<double*>(w.done + 32 * size_t(R));
+    w.part_err = w.part_e + size_t(R) * kDenseParts;
+    w.slots = reinterpret_cast<uint32_t*>(w.part_err + size_t(R) * kDenseParts);
+    (void)n;
+    return w;
+}
+
+size_t dense_team_lds_bytes(int n, int precision) {
+    return size_t(dense_part_fields_max(n)) * (precision == 1 ? sizeof(float) : sizeof(double)) + size_t(n);
+}
+
+// Waits (all lanes of one wave) until *p >= v; false (and *err set) after ~2 s or when
+// another workgroup has set *err.
+__device__ __forceinline__ bool dense_wait(const unsigned* p, unsigned v, int* err) {
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    unsigned polls = 0;
+    while (ld_l2(p) < v) {
+        if ((++polls & 1023u) == 0 &&
+            (ld_l2(err) != 0 || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull)) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+    }
+    return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d, DenseTeamWork tw, int R,
+                                                                 const double* beta, int n_sweeps,
+                                                                 int recompute_every, uint64_t seed_offset,
+                                                                 uint8_t* x_out, double* energy, double* field_err,
+                                                                 int* err) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n = d.n;
+    const int n_win = (n + kDenseWin - 1) / kDenseWin;
+    __shared__ int s_xcd, s_rank, s_k, s_ok;
+    __shared__ int s_t[kDenseWin];
+    __shared__ float s_delta[kDenseWin];
+    __shared__ double s_red[kDenseTeamBlock / 64][2];
+    const int tid = int(threadIdx.x);
+    if (tid == 0) {
+        s_xcd = int(xcc_id() & 7u);
+        s_rank = int(__hip_atomic_fetch_add(tw.tickets + s_xcd * 64, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        s_ok = 1;
+    }
+    __syncthreads();
+    const int rank = s_rank;
+    if (rank >= kDenseTeamSlots * kDenseParts) return;
+    const int r = s_xcd + 8 * (rank / kDenseParts);
+    if (r >= R) return;
+    const int part = rank % kDenseParts;
+    const int wb = part * n_win / kDenseParts, we = (part + 1) * n_win / kDenseParts;
+    const int ib = wb * kDenseWin, ie = min(we * kDenseWin, n);
+    T* f = reinterpret_cast<T*>(smem);   // fields of bits [ib, ie)
+    uint8_t* x = reinterpret_cast<uint8_t*>(f + dense_part_fields_max(n));
+    unsigned* pub = tw.pub + 32 * size_t(r);
+    const uint64_t key = seed_offset + uint64_t(r) + 1u;
+    for (int i = tid; i < n; i += kDenseTeamBlock) x[i] = 0;
+    for (int i = ib + tid; i < ie; i += kDenseTeamBlock) f[i - ib] = T(d.h[i]);
+    __syncthreads();
+    for (int sweep = 0; sweep < n_sweeps; ++sweep) {
+        const double b = beta[sweep];
+        for (int w = 0; w < n_win; ++w) {
+            const int i0 = w * kDenseWin;
+            const int wl = min(kDenseWin, n - i0);
+            const bool own = w >= wb && w < we;
+            const unsigned idx = unsigned(sweep) * unsigned(n_win) + unsigned(w) + 1u;
+            uint32_t* slot = tw.slots + ((size_t(r) * 2 + size_t(sweep & 1)) * size_t(n_win) + size_t(w)) * kDenseSlotWords;
+            if (tid < 64) {
+                const int lane = tid;
+                if (own) {
+                    const bool live = lane < wl;
+                    const int i = i0 + (live ? lane : 0);
+                    float qb[kDenseWin];
+#pragma unroll
+                    for (int t = 0; t < kDenseWin; ++t) qb[t] = d.Q[size_t(i0 + min(t, wl - 1)) * size_t(n) + size_t(i)];
+                    const double L = -qlog(draw_u01(key, uint32_t(sweep), uint32_t(i)));
+                    T fl = f[i - ib];
+                    int xl = x[i];
+                    const int k = decide_window<T>(wl, b, L, qb, fl, xl, s_t, s_delta);
+                    if (live) {
+                        f[i - ib] = fl;
+                        x[i] = uint8_t(xl);
+                    }
+                    if (lane == 0) s_k = k;
+                    // publish the flips: lane q packs flips 4q .. 4q + 3
+                    __builtin_amdgcn_s_waitcnt(0);   // s_t / s_delta written (LDS) before they are read
+                    if (lane < kDenseWin / 4) {
+                        uint32_t word = 0;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int m = 4 * lane + u;
+                            if (m < k) word |= uint32_t(s_t[m] | (s_delta[m] < 0.0f ? 0x80 : 0)) << (8 * u);
+                        }
+                        slot[1 + lane] = word;
+                    }
+                    if (lane == 0) slot[0] = uint32_t(k);
+                    __builtin_amdgcn_s_waitcnt(0);   // the slot is in L2 before the count is raised
+                    if (lane == 0) __hip_atomic_store(pub, idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    if (!dense_wait(pub, idx, err)) {
+                        if (lane == 0) s_ok = 0;
+                    } else {
+                        const int k = int(ld_l2(slot));
+                        const uint32_t word = lane < kDenseWin / 4 ? ld_l2(slot + 1 + lane) : 0u;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int m = 4 * lane + u;
+                            if (lane < kDenseWin / 4 && m < k) {
+                                const uint32_t c = (word >> (8 * u)) & 0xFFu;
+                                const int t = int(c & 0x7Fu);
+                                s_t[m] = t;
+                                s_delta[m] = (c & 0x80u) ? -1.0f : 1.0f;
+                                x[i0 + t] = (c & 0x80u) ? 0 : 1;
+                            }
+                        }
+                        if (lane == 0) s_k = k;
+                    }
+                }
+            }
+            __syncthreads();
+            if (!s_ok) return;
+            const int k = s_k;
+            if (k > 0) {
+                // this part's fields, the window's own bits excluded (the owner updated them),
+                // kTeamRankChunk flipped rows at a time, the additions in flip order
+                for (int i = ib + tid; i < ie; i += kDenseTeamBlock) {
+                    if (own && i >= i0 && i < i0 + wl) continue;
+                    T acc = f[i - ib];
+                    for (int m0 = 0; m0 < k; m0 += kTeamRankChunk) {
+                        float v[kTeamRankChunk];
+#pragma unroll
+                        for (int u = 0; u < kTeamRankChunk; ++u)
+                            v[u] = d.Q[size_t(i0 + s_t[min(m0 + u, k - 1)]) * size_t(n) + size_t(i)];
+#pragma unroll
+                        for (int u = 0; u < kTeamRankChunk; ++u) {
+                            const T nxt = acc + T(v[u]) * T(s_delta[min(m0 + u, k - 1)]);
+                            acc = m0 + u < k ? nxt : acc;
+                        }
+                    }
+                    f[i - ib] = acc;
+                }
+            }
+            __syncthreads();
+        }
+        if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
+            // this part's fields from scratch in FP64 (all windows of the sweep applied)
+            for (int g0 = ib + tid; g0 < ie; g0 += kFieldGroup * kDenseTeamBlock) {
+                double sc[kFieldGroup];
+                scratch_fields<kDenseTeamBlock>(d, x, g0, sc);
+#pragma unroll
+                for (int q = 0; q < kFieldGroup; ++q)
+                    if (g0 + q * kDenseTeamBlock < ie) f[g0 + q * kDenseTeamBlock - ib] = T(sc[q]);
+            }
+            __syncthreads();
+        }
+    }
+    // outputs: this part's spins, its share of the FP64 energy and of max |f - f_scratch|
+    double e = 0.0, er = 0.0;
+    for (int g0 = ib + tid; g0 < ie; g0 += kFieldGroup * kDenseTeamBlock) {
+        double sc[kFieldGroup];
+        scratch_fields<kDenseTeamBlock>(d, x, g0, sc);
+#pragma unroll
+        for (int q = 0; q < kFieldGroup; ++q) {
+            const int i = g0 + q * kDenseTeamBlock;
+            if (i >= ie) continue;
+            er = fmax(er, fabs(double(f[i - ib]) - sc[q]));
+            if (x[i]) e += double(d.h[i]) + 0.5 * (sc[q] - double(d.h[i]));
+            x_out[size_t(r) * size_t(n) + size_t(i)] = x[i];
+        }
+    }
+    for (int o = 32; o > 0; o /= 2) {
+        e += __shfl_down(e, o);
+        er = fmax(er, __shfl_down(er, o));
+    }
+    if ((tid & 63) == 0) {
+        s_red[tid / 64][0] = e;
+        s_red[tid / 64][1] = er;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double es = 0.0, em = 0.0;
+        for (int wv = 0; wv < kDenseTeamBlock / 64; ++wv) {
+            es += s_red[wv][0];
+            em = fmax(em, s_red[wv][1]);
+        }
+        tw.part_e[size_t(r) * kDenseParts + part] = es;
+        tw.part_err[size_t(r) * kDenseParts + part] = em;
+        __builtin_amdgcn_s_waitcnt(0);
+        // the last part to finish adds the parts' shares in part order
+        const unsigned before =
+            __hip_atomic_fetch_add(tw.done + 32 * size_t(r), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (before + 1u == unsigned(kDenseParts)) {
+            double et = 0.0, mt = 0.0;
+            for (int q = 0; q < kDenseParts; ++q) {
+                et += ld_l2(tw.part_e + size_t(r) * kDenseParts + q);
+                mt = fmax(mt, ld_l2(tw.part_err + size_t(r) * kDenseParts + q));
+            }
+            energy[r] = et;
+            field_err[r] = mt;
+        }
+    }
+}
+
 size_t dense_lds_bytes(int n, int precision) {
     return size_t(n) * (precision == 1 ? sizeof(float) : sizeof(double)) + size_t(n);
 }
@@ -222,6 +486,30 @@ hipError_t launch_dense(const DevDense& d, int precision, int R, const double* b
         if (e != hipSuccess) return e;
         dense_anneal_kernel<double><<<R, kDenseBlock, lds, st>>>(d, beta, n_sweeps, recompute_every, seed_offset, x_out,
                                                                  energy, field_err);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_team(const DevDense& d, int precision, int R, const double* beta, int n_sweeps,
+                             int recompute_every, uint64_t seed_offset, uint8_t* x_out, double* energy,
+                             double* field_err, void* work, int* err, hipStream_t st) {
+    if (R < 1 || R > kTeams * kDenseTeamSlots || d.n < kDenseWin * kDenseParts) return hipErrorInvalidValue;
+    const size_t lds = dense_team_lds_bytes(d.n, precision);
+    const DenseTeamWork tw = dense_team_work(work, d.n, R);
+    // every XCD gets kDenseParts workgroups per replica slot it holds (round-robin dispatch)
+    const unsigned grid = unsigned(kTeams * kDenseParts * ((R + kTeams - 1) / kTeams));
+    if (precision == 1) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dense_team_kernel<float>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        dense_team_kernel<float><<<grid, kDenseTeamBlock, lds, st>>>(d, tw, R, beta, n_sweeps, recompute_every, seed_offset,
+                                                                 x_out, energy, field_err, err);
+    } else {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dense_team_kernel<double>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        dense_team_kernel<double><<<grid, kDenseTeamBlock, lds, st>>>(d, tw, R, beta, n_sweeps, recompute_every,
+                                                                  seed_offset, x_out, energy, field_err, err);
     }
     return hipGetLastError();
 }

@@ -2275,7 +2275,7 @@ extern "C" int qb_qubo_batch_colouring(const qb_qubo_batch* B, int k, int32_t* o
 struct qb_dense {
     hipStream_t stream = nullptr;
     int n = 0;
-    DevBuf b_Q, b_h, w_beta, w_x, w_e, w_err;
+    DevBuf b_Q, b_h, w_beta, w_x, w_e, w_err, w_team, w_terr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~qb_dense() {
         if (ev0) (void)hipEventDestroy(ev0);
@@ -2336,10 +2336,30 @@ extern "C" int qb_dense_anneal(qb_dense* D, const qb_schedule* s, int R, uint64_
     const auto t0 = std::chrono::steady_clock::now();
     QB_HIP(hipMemcpyAsync(D->w_beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice, st));
     const DevDense dd{n, D->b_Q.as<float>(), D->b_h.as<float>()};
+    // up to 64 replicas: kDenseParts workgroups of one XCD per replica, the device's CUs shared
+    // out; more replicas: one workgroup each
+    const bool team = R <= kTeams * kDenseTeamSlots && n >= 64 * kDenseParts;
+    if (team) {
+        if ((rc = D->w_team.reserve(dense_team_work_bytes(n, R))) || (rc = D->w_terr.reserve(sizeof(int)))) return rc;
+        QB_HIP(hipMemsetAsync(D->w_team.p, 0, dense_team_work_bytes(n, R), st));
+        QB_HIP(hipMemsetAsync(D->w_terr.p, 0, sizeof(int), st));
+    }
+    const int k_rec = precision == QB_FP32 ? recompute_every : 0;
     QB_HIP(hipEventRecord(D->ev0, st));
-    QB_HIP(launch_dense(dd, precision, R, D->w_beta.as<double>(), int(beta.size()), precision == QB_FP32 ? recompute_every : 0,
-                        seed_offset, D->w_x.as<uint8_t>(), D->w_e.as<double>(), D->w_err.as<double>(), st));
+    if (team)
+        QB_HIP(launch_dense_team(dd, precision, R, D->w_beta.as<double>(), int(beta.size()), k_rec, seed_offset,
+                                 D->w_x.as<uint8_t>(), D->w_e.as<double>(), D->w_err.as<double>(), D->w_team.p,
+                                 D->w_terr.as<int>(), st));
+    else
+        QB_HIP(launch_dense(dd, precision, R, D->w_beta.as<double>(), int(beta.size()), k_rec, seed_offset,
+                            D->w_x.as<uint8_t>(), D->w_e.as<double>(), D->w_err.as<double>(), st));
     QB_HIP(hipEventRecord(D->ev1, st));
+    if (team) {
+        int terr = 0;
+        QB_HIP(hipMemcpyAsync(&terr, D->w_terr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        QB_HIP(hipStreamSynchronize(st));
+        if (terr) return fail(QB_E_TIMEOUT, "a dense replica's workgroups timed out waiting for each other");
+    }
     if (x) QB_HIP(hipMemcpyAsync(x, D->w_x.p, size_t(R) * size_t(n), hipMemcpyDeviceToHost, st));
     if (energy) QB_HIP(hipMemcpyAsync(energy, D->w_e.p, sizeof(double) * size_t(R), hipMemcpyDeviceToHost, st));
     if (field_err) QB_HIP(hipMemcpyAsync(field_err, D->w_err.p, sizeof(double) * size_t(R), hipMemcpyDeviceToHost, st));

@@ -355,11 +355,27 @@ size_t dense_lds_bytes(int n, int precision);
 hipError_t launch_dense(const DevDense& d, int precision, int R, const double* beta, int n_sweeps,
                         int recompute_every, uint64_t seed_offset, uint8_t* x_out, double* energy, double* field_err,
                         hipStream_t st);
+// The same anneal with kDenseParts workgroups of one XCD per replica (R <= 8 * kDenseTeamSlots,
+// n >= 64 * kDenseParts): bit for bit the same spins and fields. `work` holds
+// dense_team_work_bytes(n, R) zeroed bytes; *err (zeroed) is set when a wait times out.
+constexpr int kDenseParts = 4;
+constexpr int kDenseTeamSlots = 8;   // replicas per XCD (32 CUs / kDenseParts)
+size_t dense_team_work_bytes(int n, int R);
+size_t dense_team_lds_bytes(int n, int precision);
+hipError_t launch_dense_team(const DevDense& d, int precision, int R, const double* beta, int n_sweeps,
+                             int recompute_every, uint64_t seed_offset, uint8_t* x_out, double* energy,
+                             double* field_err, void* work, int* err, hipStream_t st);
 
 
 #if defined(__HIPCC__)
 // The XCD (accelerator complex die) the calling wave runs on: HW_REG_XCC_ID bits 0-3.
 __device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
+// A load that skips the CU's L1 and reads the XCD's L2 (sc1): within one XCD, the point of
+// coherence for stores that have completed.
+template <typename V>
+__device__ __forceinline__ V ld_l2(const V* p) {
+    return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 #endif
 
 // ---- exact region-of-interest posterior (qb_roi.hip) -----------------------------------

@@ -88,10 +88,6 @@ __device__ __forceinline__ float residual_add<float>(float r, float a, double de
 // back or invalidated between steps, and teams never touch each other's state.
 // ---------------------------------------------------------------------------------
 
-template <typename V>
-__device__ __forceinline__ V ld_l2(const V* p) {
-    return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Team barrier: every member has its own arrival word in the team's 64-word area of the
 // barrier array, words[rank], and the k-th barrier completes when every member's word
