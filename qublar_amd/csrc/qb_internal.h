@@ -149,7 +149,7 @@ struct DevState {
     T* r;                   // [g][n_rays][BW] residuals
     const double* beta;     // 1 / T per sweep
     unsigned* bar;          // [0]: grid barrier; [64 + 64 t + r]: team t member r's arrival word (r < 64)
-    unsigned* tickets;      // [64 t]: team t's member count
+    unsigned* tickets;      // [64 t]: team t's member count; [64 t + 32]: its light-task counter
     int* err;               // 1: a barrier wait timed out; 2: a team came up short
     // A staged solve (null clock otherwise): per position, the bit's own sweep is the
     // launch's sweep plus clock & kClockOffsetMask, and clock >> 24 is the number of

@@ -875,21 +875,36 @@ __device__ __forceinline__ void heavy_step(const DevProblem& p, const DevState<T
 
 // The light bits of one class, [lb, le): no ray entry, at most kLightNb neighbours,
 // so dE = lin * delta + lambda * (+-1) per neighbour in CSR order -- the same sum the
-// packed step makes for such a bit. Every slot of the team takes (bit, group) tasks in
-// a grid-stride loop, kLightU of them at a time with all their loads in flight
-// (neighbour positions, lin and draw key with the spin mask, then the neighbours'
-// masks); no descriptor, no LDS and no workgroup sync. Bits of one class are
-// independent, so which slot takes which bit does not matter.
-constexpr int kLightU = 4;
+// packed step makes for such a bit. A lane takes a (bit, group) task and decides every
+// branch of the group, so each load serves BW branches. Waves claim chunks of
+// 64 * kLightU tasks from the team's counter (DevState::tickets[64 t + 32]), so ranks
+// that spent the class on multi-pass or packed tasks take fewer. A wave's claims end with
+// its first claim past the class's last chunk, so a class adds exactly n_chunks + n_waves
+// to the counter and every wave knows the next class's base without a reset. Bits of one
+// class are independent, so which lane takes which bit does not matter. No LDS and no
+// workgroup sync.
+constexpr int kLightU = 2;
+constexpr int kLightCounter = 32;   // word of a team's 64-word ticket area (its own 128-byte line)
 
 template <typename T, int BW, bool STAGED>
-__device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<T>& s, int lb, int le, int my_slot,
-                                            int n_slots, int my_groups, int team, int sweep, double beta) {
+__device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<T>& s, int lb, int le, int my_groups,
+                                            int team, int n_waves, unsigned& base, int sweep, double beta) {
     const int n_light = le - lb;
     if (n_light <= 0) return;
-    const int b = int(threadIdx.x) % BW;
+    constexpr int kChunk = 64 * kLightU;
     const int tasks = n_light * my_groups;
-    for (int t0 = my_slot; t0 < tasks; t0 += n_slots * kLightU) {
+    const unsigned n_chunks = unsigned((tasks + kChunk - 1) / kChunk);
+    unsigned* ctr = s.tickets + team * 64 + kLightCounter;
+    const int lane = int(threadIdx.x) & 63;
+    auto claim = [&] {
+        unsigned v = 0;
+        if (lane == 0) v = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return unsigned(__builtin_amdgcn_readfirstlane(int(v)));
+    };
+    unsigned got = claim();
+    while (got - base < n_chunks) {
+        const int c0 = int(got - base) * kChunk;   // the chunk's first task: < tasks
+        const int t0 = c0 + lane;
         int pos[kLightU], grp[kLightU];
         bool valid[kLightU];
         int4 qa[kLightU], qb[kLightU];
@@ -900,9 +915,9 @@ __device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<
         uint16_t xm[kLightU];
 #pragma unroll
         for (int u = 0; u < kLightU; ++u) {
-            const int t = t0 + u * n_slots;
+            const int t = t0 + u * 64;
             valid[u] = t < tasks;
-            const int tt = valid[u] ? t : t0;
+            const int tt = valid[u] ? t : c0;   // lanes past the end load a real task's operands
             const int gi = tt / n_light;
             pos[u] = lb + (tt - gi * n_light);
             grp[u] = team + gi * kTeams;
@@ -915,6 +930,7 @@ __device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<
             clk[u] = STAGED ? s.clock[pos[u]] : 0u;
             xm[u] = ld_l2(s.x + size_t(grp[u]) * size_t(p.n) + pos[u]);
         }
+        got = claim();   // the next chunk, in flight while this one is decided
         uint16_t m[kLightU][kLightNb];
 #pragma unroll
         for (int u = 0; u < kLightU; ++u) {
@@ -926,25 +942,31 @@ __device__ __forceinline__ void light_sweep(const DevProblem& p, const DevState<
 #pragma unroll
         for (int u = 0; u < kLightU; ++u) {
             const int q[kLightNb] = {qa[u].x, qa[u].y, qa[u].z, qa[u].w, qb[u].x, qb[u].y, qb[u].z, qb[u].w};
-            const int xb = (xm[u] >> b) & 1;
-            const double delta = xb ? -1.0 : 1.0;
-            double dE = lin[u] * delta;
+            uint16_t flips = 0;
 #pragma unroll
-            for (int v = 0; v < kLightNb; ++v) {
-                const int before = ((m[u][v] >> b) & 1) != xb;
-                const double nxt = dE + p.lambda * (before ? -1.0 : 1.0);
-                dE = q[v] >= 0 ? nxt : dE;
+            for (int b = 0; b < BW; ++b) {
+                const int xb = (xm[u] >> b) & 1;
+                const double delta = xb ? -1.0 : 1.0;
+                double dE = lin[u] * delta;
+#pragma unroll
+                for (int v = 0; v < kLightNb; ++v) {
+                    const int before = ((m[u][v] >> b) & 1) != xb;
+                    const double nxt = dE + p.lambda * (before ? -1.0 : 1.0);
+                    dE = q[v] >= 0 ? nxt : dE;
+                }
+                const int branch = grp[u] * BW + b;
+                int own = sweep;
+                double bt = beta;
+                const bool live = bit_time<STAGED>(s, clk[u], branch, sweep, beta, own, bt);
+                const double draw =
+                    draw_u01(s.seed_offset + key[u] + uint64_t(branch) + 1u, uint32_t(own), uint32_t(bit[u]));
+                const bool acc = live && decide_drawn<T>(s, branch, draw, bt, dE);
+                flips |= uint16_t(acc ? 1u << b : 0u);
             }
-            const int branch = grp[u] * BW + b;
-            int own = sweep;
-            double bt = beta;
-            const bool live = bit_time<STAGED>(s, clk[u], branch, sweep, beta, own, bt);
-            const double draw = draw_u01(s.seed_offset + key[u] + uint64_t(branch) + 1u, uint32_t(own), uint32_t(bit[u]));
-            const bool acc = valid[u] && live && decide_drawn<T>(s, branch, draw, bt, dE);
-            const uint16_t flips = segment_flips<BW>(acc);
-            if (valid[u] && b == 0 && flips) s.x[size_t(grp[u]) * size_t(p.n) + pos[u]] = uint16_t(xm[u] ^ flips);
+            if (valid[u] && flips) s.x[size_t(grp[u]) * size_t(p.n) + pos[u]] = uint16_t(xm[u] ^ flips);
         }
     }
+    base += n_chunks + unsigned(n_waves);
 }
 
 // Residuals of ray `ray` for branch lane b, from scratch: -d + sum of a over the
@@ -1150,6 +1172,8 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
         advance_cursor();
     };
     unsigned target = 0;
+    unsigned light_base = 0;   // the team's light-task counter at this class's first claim
+    const int n_waves = team_size * (BLOCK / 64);
     for (int sweep = sweep_begin; sweep < sweep_end; ++sweep) {
         // the staged form reads beta per bit, at its own sweep; the next sweep's is at hand
         // for a prep made in this sweep's last barrier
@@ -1198,7 +1222,7 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 meta_cur = meta_nxt;
                 buf ^= 1;
             }
-            light_sweep<T, BW, STAGED>(p, s, lbeg[c], cend[c + 1], my_slot, n_slots, my_groups, team, sweep, beta);
+            light_sweep<T, BW, STAGED>(p, s, lbeg[c], cend[c + 1], my_groups, team, n_waves, light_base, sweep, beta);
             target += unsigned(team_size);
             if (!team_barrier(bar, rank, team_size, target, s.err, at_arrival)) return;
             synced = true;
