@@ -7,10 +7,10 @@
 // flip: f += delta_t Q_t (delta_t = +1 for 0 -> 1, -1 for 1 -> 0). One workgroup anneals
 // one replica with x and f in LDS:
 //
-//   - a window of 64 bits is decided by one wave, one bit per lane: the lanes prefetch the
-//     window's 64 x 64 block of Q into registers and their draws, then step t = 0..63 lets
-//     lane t decide (dE = (1 - 2 x_t) f_t) and, on a flip, every lane adds Q_{t,lane}
-//     delta_t to its field -- so later bits of the window see earlier flips;
+//   - a window of 64 bits is decided by one wave, one bit per lane: the window's 64 x 64
+//     block of Q is copied into LDS while the previous window's fields are updated, bit t
+//     decides on dE = (1 - 2 x_t) f_t and, on a flip, every lane adds Q_{t,lane} delta_t to
+//     its field -- so later bits of the window see earlier flips;
 //   - the whole workgroup then adds the window's flipped rows, in flip order, to every field
 //     outside the window (a rank-k update, k = flips in the window).
 //
@@ -59,32 +59,49 @@ __device__ __forceinline__ void scratch_fields(const DevDense& d, const uint8_t*
     }
 }
 
-// One window's sequential decisions by one wave, lane l holding bit i0 + l: its field fl,
-// spin xl, its row of the window's Q block qb[t] = Q[i0 + t][i0 + l] and its -log draw L.
-// At step t lane t decides (dE = (1 - 2 x_t) f_t; accept iff dE <= 0 or dE beta < L_t);
-// on a flip every lane reads lane t's spin and adds Q[i0 + t][i0 + l] delta_t to its field,
-// so later bits see it. Lane 0 records the flips (s_t, s_delta); returns their count.
-template <typename T>
-__device__ __forceinline__ int decide_window(int wl, double b, double L, const float (&qb)[kDenseWin], T& fl, int& xl,
-                                             int* s_t, float* s_delta) {
+// Issues the copy of the block of the window at i0 (wl bits) into LDS by one wave, one row
+// per load straight into LDS: blk[t * 64 + l] = Q[i0 + min(t, wl - 1)][min(i0 + l, n - 1)].
+// The wave's vmcnt covers the copy; the lanes past wl hold columns nobody reads.
+__device__ __forceinline__ void issue_window_block(const DevDense& d, int i0, int wl, float* blk) {
     const int lane = int(threadIdx.x) & 63;
-    int k = 0;
+    const int col = min(i0 + lane, d.n - 1);
 #pragma unroll
     for (int t = 0; t < kDenseWin; ++t) {
-        if (t < wl) {
-            const double dE = xl ? -double(fl) : double(fl);
-            const bool flip = lane == t && (dE <= 0.0 || dE * b < L);
-            if (__ballot(flip)) {
-                const float dl = __builtin_amdgcn_readlane(xl, t) ? -1.0f : 1.0f;
-                fl = fl + T(qb[t]) * T(dl);
-                if (lane == t) xl ^= 1;
-                if (lane == 0) {
-                    s_t[k] = t;
-                    s_delta[k] = dl;
-                }
-                ++k;
-            }
+        const float* src = d.Q + size_t(i0 + min(t, wl - 1)) * size_t(d.n) + size_t(col);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                         (__attribute__((address_space(3))) void*)(blk + t * kDenseWin), 4, 0, 0);
+    }
+}
+
+// One window's sequential decisions by one wave, lane l holding bit i0 + l: its field fl,
+// spin xl and -log draw L, the window's block in LDS (blk[t * 64 + l] = Q[i0 + t][i0 + l]).
+// Bit t decides on dE = (1 - 2 x_t) f_t -- accept iff dE <= 0 or dE beta < L_t -- with every
+// earlier flip of the window in f_t. Every lane from the first undecided bit on tests its
+// own bit against its current field; the first that accepts is the next flip, and the bits
+// before it decide as they would one at a time, since no flip lies between them. Every lane
+// then adds Q[i0 + t][i0 + l] delta_t and the search goes on after t: a window costs one
+// round per flip plus one. Lane 0 records the flips (s_t, s_delta); returns their count.
+template <typename T>
+__device__ __forceinline__ int decide_window(int wl, double b, double L, const float* blk, T& fl, int& xl, int* s_t,
+                                             float* s_delta) {
+    const int lane = int(threadIdx.x) & 63;
+    int k = 0;
+    int from = 0;
+    while (true) {
+        const double dE = xl ? -double(fl) : double(fl);
+        const bool accept = lane >= from && lane < wl && (dE <= 0.0 || dE * b < L);
+        const uint64_t m = __ballot(accept);
+        if (m == 0) break;
+        const int t = __builtin_ctzll(m);
+        const float dl = __builtin_amdgcn_readlane(xl, t) ? -1.0f : 1.0f;
+        fl = fl + T(blk[t * kDenseWin + lane]) * T(dl);
+        if (lane == t) xl ^= 1;
+        if (lane == 0) {
+            s_t[k] = t;
+            s_delta[k] = dl;
         }
+        ++k;
+        from = t + 1;
     }
     return k;
 }
@@ -95,7 +112,8 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
                                                                     uint8_t* x_out, double* energy, double* field_err) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = d.n;
-    T* f = reinterpret_cast<T*>(smem);
+    float* blk = reinterpret_cast<float*>(smem);   // the window's Q block
+    T* f = reinterpret_cast<T*>(blk + kDenseWin * kDenseWin);
     uint8_t* x = reinterpret_cast<uint8_t*>(f + n);
     __shared__ int s_k;
     __shared__ int s_t[kDenseWin];
@@ -107,6 +125,7 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
         x[i] = 0;
         f[i] = T(d.h[i]);
     }
+    if (tid < 64 && n_sweeps > 0) issue_window_block(d, 0, min(kDenseWin, n), blk);
     __syncthreads();
     for (int sweep = 0; sweep < n_sweeps; ++sweep) {
         const double b = beta[sweep];
@@ -116,18 +135,20 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
                 const int lane = tid;
                 const bool live = lane < w;
                 const int i = i0 + (live ? lane : 0);
-                float qb[kDenseWin];
-#pragma unroll
-                for (int t = 0; t < kDenseWin; ++t) qb[t] = d.Q[size_t(i0 + min(t, w - 1)) * size_t(n) + size_t(i)];
                 const double L = -qlog(draw_u01(key, uint32_t(sweep), uint32_t(i)));
                 T fl = f[i];
                 int xl = x[i];
-                const int k = decide_window<T>(w, b, L, qb, fl, xl, s_t, s_delta);
+                __builtin_amdgcn_s_waitcnt(0);   // the window's block has landed
+                const int k = decide_window<T>(w, b, L, blk, fl, xl, s_t, s_delta);
                 if (live) {
                     f[i] = fl;
                     x[i] = uint8_t(xl);
                 }
                 if (lane == 0) s_k = k;
+                // the next window's block (after the last, the next sweep's first), copied
+                // while this window's flips are added to the fields
+                const int n0 = i0 + kDenseWin < n ? i0 + kDenseWin : 0;
+                if (n0 > 0 || sweep + 1 < n_sweeps) issue_window_block(d, n0, min(kDenseWin, n - n0), blk);
             }
             __syncthreads();
             const int k = s_k;
@@ -234,7 +255,7 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
 // same window of sweep s.
 constexpr int kDenseSlotWords = 1 + kDenseWin / 4;
 constexpr int kTeamRankChunk = 16;
-constexpr int kDenseTeamBlock = 512;   // two waves per SIMD: room for a window's Q block in registers
+constexpr int kDenseTeamBlock = 512;
 
 struct DenseTeamWork {
     unsigned* tickets;   // [8 * 64]: per-XCD tickets
@@ -270,7 +291,8 @@ static DenseTeamWork dense_team_work(void* work, int n, int R) {
 }
 
 size_t dense_team_lds_bytes(int n, int precision) {
-    return size_t(dense_part_fields_max(n)) * (precision == 1 ? sizeof(float) : sizeof(double)) + size_t(n);
+    return sizeof(float) * kDenseWin * kDenseWin +
+           size_t(dense_part_fields_max(n)) * (precision == 1 ? sizeof(float) : sizeof(double)) + size_t(n);
 }
 
 // Waits (all lanes of one wave) until *p >= v; false (and *err set) after ~2 s or when
@@ -315,12 +337,14 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
     const int part = rank % kDenseParts;
     const int wb = part * n_win / kDenseParts, we = (part + 1) * n_win / kDenseParts;
     const int ib = wb * kDenseWin, ie = min(we * kDenseWin, n);
-    T* f = reinterpret_cast<T*>(smem);   // fields of bits [ib, ie)
+    float* blk = reinterpret_cast<float*>(smem);                  // the block of this part's next window
+    T* f = reinterpret_cast<T*>(blk + kDenseWin * kDenseWin);   // fields of bits [ib, ie)
     uint8_t* x = reinterpret_cast<uint8_t*>(f + dense_part_fields_max(n));
     unsigned* pub = tw.pub + 32 * size_t(r);
     const uint64_t key = seed_offset + uint64_t(r) + 1u;
     for (int i = tid; i < n; i += kDenseTeamBlock) x[i] = 0;
     for (int i = ib + tid; i < ie; i += kDenseTeamBlock) f[i - ib] = T(d.h[i]);
+    if (tid < 64 && n_sweeps > 0) issue_window_block(d, ib, min(kDenseWin, n - ib), blk);
     __syncthreads();
     for (int sweep = 0; sweep < n_sweeps; ++sweep) {
         const double b = beta[sweep];
@@ -335,13 +359,11 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                 if (own) {
                     const bool live = lane < wl;
                     const int i = i0 + (live ? lane : 0);
-                    float qb[kDenseWin];
-#pragma unroll
-                    for (int t = 0; t < kDenseWin; ++t) qb[t] = d.Q[size_t(i0 + min(t, wl - 1)) * size_t(n) + size_t(i)];
                     const double L = -qlog(draw_u01(key, uint32_t(sweep), uint32_t(i)));
                     T fl = f[i - ib];
                     int xl = x[i];
-                    const int k = decide_window<T>(wl, b, L, qb, fl, xl, s_t, s_delta);
+                    __builtin_amdgcn_s_waitcnt(0);   // the window's block has landed
+                    const int k = decide_window<T>(wl, b, L, blk, fl, xl, s_t, s_delta);
                     if (live) {
                         f[i - ib] = fl;
                         x[i] = uint8_t(xl);
@@ -361,6 +383,11 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                     if (lane == 0) slot[0] = uint32_t(k);
                     __builtin_amdgcn_s_waitcnt(0);   // the slot is in L2 before the count is raised
                     if (lane == 0) __hip_atomic_store(pub, idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // this part's next window (after its last, its first of the next sweep), copied
+                    // while the flips are added to the fields
+                    const int wn = w + 1 < we ? w + 1 : wb;
+                    if (wn != wb || sweep + 1 < n_sweeps)
+                        issue_window_block(d, wn * kDenseWin, min(kDenseWin, n - wn * kDenseWin), blk);
                 } else {
                     if (!dense_wait(pub, idx, err)) {
                         if (lane == 0) s_ok = 0;
@@ -467,7 +494,8 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
 }
 
 size_t dense_lds_bytes(int n, int precision) {
-    return size_t(n) * (precision == 1 ? sizeof(float) : sizeof(double)) + size_t(n);
+    return sizeof(float) * kDenseWin * kDenseWin + size_t(n) * (precision == 1 ? sizeof(float) : sizeof(double)) +
+           size_t(n);
 }
 
 hipError_t launch_dense(const DevDense& d, int precision, int R, const double* beta, int n_sweeps,

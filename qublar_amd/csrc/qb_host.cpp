@@ -2289,7 +2289,7 @@ extern "C" int qb_dense_create(int n, const float* Q, const float* h, qb_dense**
     *out = nullptr;
     if (n <= 0 || !Q || !h) return fail(QB_E_ARG, "qb_dense_create: n must be positive and Q, h given");
     if (dense_lds_bytes(n, 0) > size_t(150 * 1024))
-        return fail(QB_E_LIMIT, "n = %d: the FP64 fields and spins of one replica must fit 150 KB of LDS", n);
+        return fail(QB_E_LIMIT, "n = %d: the FP64 fields, spins and window block of one replica must fit 150 KB of LDS", n);
     for (int i = 0; i < n; ++i) {
         if (Q[size_t(i) * size_t(n) + size_t(i)] != 0.0f) return fail(QB_E_ARG, "Q[%d][%d] is not zero", i, i);
         if (!std::isfinite(h[i])) return fail(QB_E_ARG, "h[%d] is not finite", i);
