@@ -8,9 +8,9 @@
 // one replica with x and f in LDS:
 //
 //   - a window of 64 bits is decided by one wave, one bit per lane: the window's 64 x 64
-//     block of Q is copied into LDS while the previous window's fields are updated, bit t
-//     decides on dE = (1 - 2 x_t) f_t and, on a flip, every lane adds Q_{t,lane} delta_t to
-//     its field -- so later bits of the window see earlier flips;
+//     block of Q and its draws were staged in LDS by the other waves during the previous
+//     window; bit t decides on dE = (1 - 2 x_t) f_t and, on a flip, every lane adds
+//     Q_{t,lane} delta_t to its field -- so later bits of the window see earlier flips;
 //   - the whole workgroup then adds the window's flipped rows, in flip order, to every field
 //     outside the window (a rank-k update, k = flips in the window).
 //
@@ -59,18 +59,23 @@ __device__ __forceinline__ void scratch_fields(const DevDense& d, const uint8_t*
     }
 }
 
-// Issues the copy of the block of the window at i0 (wl bits) into LDS by one wave, one row
-// per load straight into LDS: blk[t * 64 + l] = Q[i0 + min(t, wl - 1)][min(i0 + l, n - 1)].
-// The wave's vmcnt covers the copy; the lanes past wl hold columns nobody reads.
-__device__ __forceinline__ void issue_window_block(const DevDense& d, int i0, int wl, float* blk) {
-    const int lane = int(threadIdx.x) & 63;
-    const int col = min(i0 + lane, d.n - 1);
-#pragma unroll
-    for (int t = 0; t < kDenseWin; ++t) {
-        const float* src = d.Q + size_t(i0 + min(t, wl - 1)) * size_t(d.n) + size_t(col);
+// Staging of the window at i0n (in sweep sw) into one of two buffers while wave 0 decides
+// the current window: waves 1.. of the workgroup copy its block, one row per direct-to-LDS
+// load, bq[t * 64 + l] = Q[i0n + min(t, wl - 1)][min(i0n + l, n - 1)] (lanes past the
+// window hold columns nobody reads), and wave 1 makes its -log draws L[l]. The copy is
+// covered by the issuing waves' vmcnt: they drain it before the barrier that makes the
+// buffer visible.
+template <int BLOCK>
+__device__ __forceinline__ void stage_window(const DevDense& d, uint64_t key, int i0n, int sw, float* bq, double* L) {
+    const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+    const int n = d.n, wln = min(kDenseWin, n - i0n);
+    const int col = min(i0n + lane, n - 1);
+    for (int t = wave - 1; t < kDenseWin; t += BLOCK / 64 - 1) {
+        const float* src = d.Q + size_t(i0n + min(t, wln - 1)) * size_t(n) + size_t(col);
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                         (__attribute__((address_space(3))) void*)(blk + t * kDenseWin), 4, 0, 0);
+                                         (__attribute__((address_space(3))) void*)(bq + t * kDenseWin), 4, 0, 0);
     }
+    if (wave == 1) L[lane] = -qlog(draw_u01(key, uint32_t(sw), uint32_t(i0n + (lane < wln ? lane : 0))));
 }
 
 // One window's sequential decisions by one wave, lane l holding bit i0 + l: its field fl,
@@ -112,9 +117,10 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
                                                                     uint8_t* x_out, double* energy, double* field_err) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = d.n;
-    float* blk = reinterpret_cast<float*>(smem);   // the window's Q block
-    T* f = reinterpret_cast<T*>(blk + kDenseWin * kDenseWin);
+    float* blk = reinterpret_cast<float*>(smem);   // two staging buffers: the next windows' Q blocks
+    T* f = reinterpret_cast<T*>(blk + 2 * kDenseWin * kDenseWin);
     uint8_t* x = reinterpret_cast<uint8_t*>(f + n);
+    __shared__ double s_L[2][kDenseWin];   // and their -log draws
     __shared__ int s_k;
     __shared__ int s_t[kDenseWin];
     __shared__ float s_delta[kDenseWin];
@@ -125,30 +131,33 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
         x[i] = 0;
         f[i] = T(d.h[i]);
     }
-    if (tid < 64 && n_sweeps > 0) issue_window_block(d, 0, min(kDenseWin, n), blk);
+    if (tid >= 64 && n_sweeps > 0) stage_window<kDenseBlock>(d, key, 0, 0, blk, s_L[0]);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    int nbuf = 0;   // the buffer of the current window
     for (int sweep = 0; sweep < n_sweeps; ++sweep) {
         const double b = beta[sweep];
         for (int i0 = 0; i0 < n; i0 += kDenseWin) {
             const int w = min(kDenseWin, n - i0);
+            // the next window (after the last, the next sweep's first), staged during this one
+            const int n0 = i0 + kDenseWin < n ? i0 + kDenseWin : 0;
+            const bool staging = n0 > 0 || sweep + 1 < n_sweeps;
+            if (tid >= 64 && staging)
+                stage_window<kDenseBlock>(d, key, n0, n0 > 0 ? sweep : sweep + 1,
+                                          blk + (nbuf ^ 1) * kDenseWin * kDenseWin, s_L[nbuf ^ 1]);
             if (tid < 64) {
                 const int lane = tid;
                 const bool live = lane < w;
                 const int i = i0 + (live ? lane : 0);
-                const double L = -qlog(draw_u01(key, uint32_t(sweep), uint32_t(i)));
+                const double L = s_L[nbuf][lane];
                 T fl = f[i];
                 int xl = x[i];
-                __builtin_amdgcn_s_waitcnt(0);   // the window's block has landed
-                const int k = decide_window<T>(w, b, L, blk, fl, xl, s_t, s_delta);
+                const int k = decide_window<T>(w, b, L, blk + nbuf * kDenseWin * kDenseWin, fl, xl, s_t, s_delta);
                 if (live) {
                     f[i] = fl;
                     x[i] = uint8_t(xl);
                 }
                 if (lane == 0) s_k = k;
-                // the next window's block (after the last, the next sweep's first), copied
-                // while this window's flips are added to the fields
-                const int n0 = i0 + kDenseWin < n ? i0 + kDenseWin : 0;
-                if (n0 > 0 || sweep + 1 < n_sweeps) issue_window_block(d, n0, min(kDenseWin, n - n0), blk);
             }
             __syncthreads();
             const int k = s_k;
@@ -189,7 +198,9 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
                         if (upd[q]) f[g0 + q * kDenseBlock] = acc[q];
                 }
             }
+            if (staging) __builtin_amdgcn_s_waitcnt(0);   // the staged block has landed before the barrier
             __syncthreads();
+            nbuf ^= 1;
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
             // f = h + Q x from scratch in FP64, j ascending (column i of Q = row i)
@@ -242,24 +253,23 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
 // run of windows and the fields of their bits; every part keeps the whole spin vector.
 // Windows go in order: the window's owner decides it as above (its fields are its own) and
 // publishes the window's flips -- a count and one byte per flip, t | 0x80 for 1 -> 0 -- in a
-// slot of the replica's ring (two sweeps deep), then raises the replica's published count.
-// Every other part waits for that count, reads the flips and applies them to its spins;
-// then every part adds the flipped rows, in flip order, to its fields. Each field therefore
-// receives the same additions in the same order as with one workgroup, and the spins are
-// the same bit for bit; only the final energy's partial sums are grouped by part. The
-// parts share state through the XCD's L2 (stores drained before the count is raised, sc1
-// loads), like the ray sweep's teams.
+// slot of the replica's ring (two sweeps deep). Every word of the slot carries the window's
+// tag, so the owner goes on without waiting for its stores; every other part polls the slot
+// until each word has the tag, reads the flips and applies them to its spins. Then every
+// part adds the flipped rows, in flip order, to its fields. Each field therefore receives the
+// same additions in the same order as with one workgroup, and the spins are the same bit for
+// bit; only the final energy's partial sums are grouped by part. The parts share state
+// through the XCD's L2 (sc1 loads), like the ray sweep's teams.
 //
 // The ring's two sweeps suffice: the owner of a window in sweep s + 2 decides only after
 // every part has published its windows of sweep s + 1, which each did after applying the
 // same window of sweep s.
-constexpr int kDenseSlotWords = 1 + kDenseWin / 4;
+constexpr int kDenseSlotWords = 1 + kDenseWin / 2;   // a header word, then two flips a word
 constexpr int kTeamRankChunk = 16;
 constexpr int kDenseTeamBlock = 512;
 
 struct DenseTeamWork {
     unsigned* tickets;   // [8 * 64]: per-XCD tickets
-    unsigned* pub;       // [R * 32]: replica r's published window count at pub[32 r]
     unsigned* done;      // [R * 32]: parts finished, at done[32 r]
     double* part_e;      // [R * kDenseParts]
     double* part_err;    // [R * kDenseParts]
@@ -273,7 +283,7 @@ __host__ __device__ inline int dense_part_fields_max(int n) {
 
 size_t dense_team_work_bytes(int n, int R) {
     const size_t n_win = size_t((n + kDenseWin - 1) / kDenseWin);
-    return sizeof(unsigned) * (8 * 64 + 2 * 32 * size_t(R)) + sizeof(double) * 2 * size_t(R) * kDenseParts +
+    return sizeof(unsigned) * (8 * 64 + 32 * size_t(R)) + sizeof(double) * 2 * size_t(R) * kDenseParts +
            sizeof(uint32_t) * size_t(R) * 2 * n_win * kDenseSlotWords;
 }
 
@@ -281,8 +291,7 @@ static DenseTeamWork dense_team_work(void* work, int n, int R) {
     unsigned char* p = static_cast<unsigned char*>(work);
     DenseTeamWork w;
     w.tickets = reinterpret_cast<unsigned*>(p);
-    w.pub = w.tickets + 8 * 64;
-    w.done = w.pub + 32 * size_t(R);
+    w.done = w.tickets + 8 * 64;
     w.part_e = reinterpret_cast<double*>(w.done + 32 * size_t(R));
     w.part_err = w.part_e + size_t(R) * kDenseParts;
     w.slots = reinterpret_cast<uint32_t*>(w.part_err + size_t(R) * kDenseParts);
@@ -291,23 +300,31 @@ static DenseTeamWork dense_team_work(void* work, int n, int R) {
 }
 
 size_t dense_team_lds_bytes(int n, int precision) {
-    return sizeof(float) * kDenseWin * kDenseWin +
+    return 2 * sizeof(float) * kDenseWin * kDenseWin +
            size_t(dense_part_fields_max(n)) * (precision == 1 ? sizeof(float) : sizeof(double)) + size_t(n);
 }
 
-// Waits (all lanes of one wave) until *p >= v; false (and *err set) after ~2 s or when
-// another workgroup has set *err.
-__device__ __forceinline__ bool dense_wait(const unsigned* p, unsigned v, int* err) {
+// A slot word's tag for the idx-th window (idx from 1), in its low 16 bits: never 0, the
+// zeroed ring, and different from the slot's previous use, 2 n_win windows earlier, while
+// n_win < 16384 (n is far smaller: its fields fit LDS).
+__device__ __forceinline__ uint32_t dense_tag(unsigned idx) { return (idx & 0x7FFFu) | 0x8000u; }
+
+// Waits (one wave, lane q reading word q) until every word of the slot carries `tag`; the
+// lane's word is left in `word`. False (and *err set) after ~2 s or when another workgroup
+// has set *err.
+__device__ __forceinline__ bool dense_wait_slot(const uint32_t* slot, uint32_t tag, uint32_t& word, int* err) {
+    const int lane = int(threadIdx.x) & 63;
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     unsigned polls = 0;
-    while (ld_l2(p) < v) {
+    while (true) {
+        word = lane < kDenseSlotWords ? ld_l2(slot + lane) : tag;
+        if (__ballot((word & 0xFFFFu) != tag) == 0) return true;
         if ((++polls & 1023u) == 0 &&
             (ld_l2(err) != 0 || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull)) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
     }
-    return true;
 }
 
 template <typename T>
@@ -337,15 +354,21 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
     const int part = rank % kDenseParts;
     const int wb = part * n_win / kDenseParts, we = (part + 1) * n_win / kDenseParts;
     const int ib = wb * kDenseWin, ie = min(we * kDenseWin, n);
-    float* blk = reinterpret_cast<float*>(smem);                  // the block of this part's next window
-    T* f = reinterpret_cast<T*>(blk + kDenseWin * kDenseWin);   // fields of bits [ib, ie)
+    // two staging buffers for the part's next owned windows: their Q blocks and -log draws
+    float* blk = reinterpret_cast<float*>(smem);
+    T* f = reinterpret_cast<T*>(blk + 2 * kDenseWin * kDenseWin);   // fields of bits [ib, ie)
     uint8_t* x = reinterpret_cast<uint8_t*>(f + dense_part_fields_max(n));
-    unsigned* pub = tw.pub + 32 * size_t(r);
+    __shared__ double s_L[2][kDenseWin];
     const uint64_t key = seed_offset + uint64_t(r) + 1u;
+    auto stage = [&](int wn, int sw, int q) {
+        stage_window<kDenseTeamBlock>(d, key, wn * kDenseWin, sw, blk + q * kDenseWin * kDenseWin, s_L[q]);
+    };
     for (int i = tid; i < n; i += kDenseTeamBlock) x[i] = 0;
     for (int i = ib + tid; i < ie; i += kDenseTeamBlock) f[i - ib] = T(d.h[i]);
-    if (tid < 64 && n_sweeps > 0) issue_window_block(d, ib, min(kDenseWin, n - ib), blk);
+    if (tid >= 64 && wb == 0 && n_sweeps > 0) stage(0, 0, 0);   // later parts stage their first in the loop
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    int nbuf = 0;   // the buffer of the part's next owned window
     for (int sweep = 0; sweep < n_sweeps; ++sweep) {
         const double b = beta[sweep];
         for (int w = 0; w < n_win; ++w) {
@@ -354,51 +377,51 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
             const bool own = w >= wb && w < we;
             const unsigned idx = unsigned(sweep) * unsigned(n_win) + unsigned(w) + 1u;
             uint32_t* slot = tw.slots + ((size_t(r) * 2 + size_t(sweep & 1)) * size_t(n_win) + size_t(w)) * kDenseSlotWords;
+            // the next window, if this part owns it, is staged during this one
+            const int wn = w + 1 < n_win ? w + 1 : 0, swn = w + 1 < n_win ? sweep : sweep + 1;
+            const bool staging = wn >= wb && wn < we && swn < n_sweeps;
+            if (tid >= 64 && staging) stage(wn, swn, own ? nbuf ^ 1 : nbuf);
             if (tid < 64) {
                 const int lane = tid;
                 if (own) {
                     const bool live = lane < wl;
                     const int i = i0 + (live ? lane : 0);
-                    const double L = -qlog(draw_u01(key, uint32_t(sweep), uint32_t(i)));
+                    const double L = s_L[nbuf][lane];
                     T fl = f[i - ib];
                     int xl = x[i];
-                    __builtin_amdgcn_s_waitcnt(0);   // the window's block has landed
-                    const int k = decide_window<T>(wl, b, L, blk, fl, xl, s_t, s_delta);
+                    const int k = decide_window<T>(wl, b, L, blk + nbuf * kDenseWin * kDenseWin, fl, xl, s_t, s_delta);
                     if (live) {
                         f[i - ib] = fl;
                         x[i] = uint8_t(xl);
                     }
                     if (lane == 0) s_k = k;
-                    // publish the flips: lane q packs flips 4q .. 4q + 3
+                    // publish the flips, every word tagged: lane 0 the count, lane q > 0 flips
+                    // 2q - 2 and 2q - 1 in bits 16-23 and 24-31
                     __builtin_amdgcn_s_waitcnt(0);   // s_t / s_delta written (LDS) before they are read
-                    if (lane < kDenseWin / 4) {
-                        uint32_t word = 0;
+                    if (lane < kDenseSlotWords) {
+                        uint32_t word = dense_tag(idx);
+                        if (lane == 0) {
+                            word |= uint32_t(k) << 16;
+                        } else {
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int m = 4 * lane + u;
-                            if (m < k) word |= uint32_t(s_t[m] | (s_delta[m] < 0.0f ? 0x80 : 0)) << (8 * u);
+                            for (int u = 0; u < 2; ++u) {
+                                const int m = 2 * (lane - 1) + u;
+                                if (m < k) word |= uint32_t(s_t[m] | (s_delta[m] < 0.0f ? 0x80 : 0)) << (16 + 8 * u);
+                            }
                         }
-                        slot[1 + lane] = word;
+                        __hip_atomic_store(slot + lane, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
-                    if (lane == 0) slot[0] = uint32_t(k);
-                    __builtin_amdgcn_s_waitcnt(0);   // the slot is in L2 before the count is raised
-                    if (lane == 0) __hip_atomic_store(pub, idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    // this part's next window (after its last, its first of the next sweep), copied
-                    // while the flips are added to the fields
-                    const int wn = w + 1 < we ? w + 1 : wb;
-                    if (wn != wb || sweep + 1 < n_sweeps)
-                        issue_window_block(d, wn * kDenseWin, min(kDenseWin, n - wn * kDenseWin), blk);
                 } else {
-                    if (!dense_wait(pub, idx, err)) {
+                    uint32_t word = 0;
+                    if (!dense_wait_slot(slot, dense_tag(idx), word, err)) {
                         if (lane == 0) s_ok = 0;
                     } else {
-                        const int k = int(ld_l2(slot));
-                        const uint32_t word = lane < kDenseWin / 4 ? ld_l2(slot + 1 + lane) : 0u;
+                        const int k = int((__builtin_amdgcn_readfirstlane(int(word)) >> 16) & 0xFF);
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int m = 4 * lane + u;
-                            if (lane < kDenseWin / 4 && m < k) {
-                                const uint32_t c = (word >> (8 * u)) & 0xFFu;
+                        for (int u = 0; u < 2; ++u) {
+                            const int m = 2 * (lane - 1) + u;
+                            if (lane >= 1 && lane < kDenseSlotWords && m < k) {
+                                const uint32_t c = (word >> (16 + 8 * u)) & 0xFFu;
                                 const int t = int(c & 0x7Fu);
                                 s_t[m] = t;
                                 s_delta[m] = (c & 0x80u) ? -1.0f : 1.0f;
@@ -410,7 +433,10 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                 }
             }
             __syncthreads();
-            if (!s_ok) return;
+            if (!s_ok) {
+                __builtin_amdgcn_s_waitcnt(0);   // no staging load outlives the workgroup
+                return;
+            }
             const int k = s_k;
             if (k > 0) {
                 // this part's fields, the window's own bits excluded (the owner updated them),
@@ -432,7 +458,9 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                     f[i - ib] = acc;
                 }
             }
+            if (staging) __builtin_amdgcn_s_waitcnt(0);   // the staged block has landed before the barrier
             __syncthreads();
+            if (own) nbuf ^= 1;
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
             // this part's fields from scratch in FP64 (all windows of the sweep applied)
@@ -494,7 +522,7 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
 }
 
 size_t dense_lds_bytes(int n, int precision) {
-    return sizeof(float) * kDenseWin * kDenseWin + size_t(n) * (precision == 1 ? sizeof(float) : sizeof(double)) +
+    return 2 * sizeof(float) * kDenseWin * kDenseWin + size_t(n) * (precision == 1 ? sizeof(float) : sizeof(double)) +
            size_t(n);
 }
 
