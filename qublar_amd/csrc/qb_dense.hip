@@ -111,6 +111,46 @@ __device__ __forceinline__ int decide_window(int wl, double b, double L, const f
     return k;
 }
 
+// Fields i = g0 + q kDenseBlock (q < kFieldGroup, i < n, outside the window [i0, i0 + w))
+// plus the window's k flipped rows in flip order, CHUNK rows' loads at a time (past the last
+// flip the index repeats it; its value is unused).
+template <typename T, int CHUNK>
+__device__ __forceinline__ void add_rows_group(const DevDense& d, int i0, int w, int g0, int k, const int* s_t,
+                                               const float* s_delta, T* f) {
+    const int n = d.n;
+    int col[kFieldGroup];
+    bool upd[kFieldGroup];
+    T acc[kFieldGroup];
+#pragma unroll
+    for (int q = 0; q < kFieldGroup; ++q) {
+        const int i = g0 + q * kDenseBlock;
+        upd[q] = i < n && !(i >= i0 && i < i0 + w);
+        col[q] = min(i, n - 1);
+        acc[q] = f[col[q]];
+    }
+    for (int m0 = 0; m0 < k; m0 += CHUNK) {
+        float v[kFieldGroup][CHUNK];
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) {
+            const size_t row = size_t(i0 + s_t[min(m0 + u, k - 1)]) * size_t(n);
+#pragma unroll
+            for (int q = 0; q < kFieldGroup; ++q) v[q][u] = d.Q[row + size_t(col[q])];
+        }
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) {
+            const T dl = T(s_delta[min(m0 + u, k - 1)]);
+#pragma unroll
+            for (int q = 0; q < kFieldGroup; ++q) {
+                const T nxt = acc[q] + T(v[q][u]) * dl;
+                acc[q] = m0 + u < k ? nxt : acc[q];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kFieldGroup; ++q)
+        if (upd[q]) f[g0 + q * kDenseBlock] = acc[q];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, const double* beta, int n_sweeps,
                                                                     int recompute_every, uint64_t seed_offset,
@@ -162,40 +202,13 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
             __syncthreads();
             const int k = s_k;
             if (k > 0) {
-                // kRankChunk flipped rows at a time: their loads in flight together, the
-                // additions in flip order
+                // up to kRankChunk flipped rows at a time (fewer when fewer flipped): their
+                // loads in flight together, the additions in flip order
                 for (int g0 = tid; g0 < n; g0 += kFieldGroup * kDenseBlock) {
-                    int col[kFieldGroup];
-                    bool upd[kFieldGroup];
-                    T acc[kFieldGroup];
-#pragma unroll
-                    for (int q = 0; q < kFieldGroup; ++q) {
-                        const int i = g0 + q * kDenseBlock;
-                        upd[q] = i < n && !(i >= i0 && i < i0 + w);
-                        col[q] = min(i, n - 1);
-                        acc[q] = f[col[q]];
-                    }
-                    for (int m0 = 0; m0 < k; m0 += kRankChunk) {
-                        float v[kFieldGroup][kRankChunk];
-#pragma unroll
-                        for (int u = 0; u < kRankChunk; ++u) {
-                            const size_t row = size_t(i0 + s_t[min(m0 + u, k - 1)]) * size_t(n);
-#pragma unroll
-                            for (int q = 0; q < kFieldGroup; ++q) v[q][u] = d.Q[row + size_t(col[q])];
-                        }
-#pragma unroll
-                        for (int u = 0; u < kRankChunk; ++u) {
-                            const T dl = T(s_delta[min(m0 + u, k - 1)]);
-#pragma unroll
-                            for (int q = 0; q < kFieldGroup; ++q) {
-                                const T nxt = acc[q] + T(v[q][u]) * dl;
-                                acc[q] = m0 + u < k ? nxt : acc[q];
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int q = 0; q < kFieldGroup; ++q)
-                        if (upd[q]) f[g0 + q * kDenseBlock] = acc[q];
+                    if (k <= 1) add_rows_group<T, 1>(d, i0, w, g0, k, s_t, s_delta, f);
+                    else if (k <= 2) add_rows_group<T, 2>(d, i0, w, g0, k, s_t, s_delta, f);
+                    else if (k <= 4) add_rows_group<T, 4>(d, i0, w, g0, k, s_t, s_delta, f);
+                    else add_rows_group<T, kRankChunk>(d, i0, w, g0, k, s_t, s_delta, f);
                 }
             }
             if (staging) __builtin_amdgcn_s_waitcnt(0);   // the staged block has landed before the barrier
@@ -327,6 +340,24 @@ __device__ __forceinline__ bool dense_wait_slot(const uint32_t* slot, uint32_t t
     }
 }
 
+// Field i plus the window's k flipped rows, acc + Q[i0 + s_t[m]][i] s_delta[m] in flip order,
+// CHUNK rows' loads at a time (past the last flip the index repeats it; its value is unused).
+template <typename T, int CHUNK>
+__device__ __forceinline__ T add_rows(const DevDense& d, int i0, int i, int k, const int* s_t, const float* s_delta,
+                                      T acc) {
+    for (int m0 = 0; m0 < k; m0 += CHUNK) {
+        float v[CHUNK];
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) v[u] = d.Q[size_t(i0 + s_t[min(m0 + u, k - 1)]) * size_t(d.n) + size_t(i)];
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) {
+            const T nxt = acc + T(v[u]) * T(s_delta[min(m0 + u, k - 1)]);
+            acc = m0 + u < k ? nxt : acc;
+        }
+    }
+    return acc;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d, DenseTeamWork tw, int R,
                                                                  const double* beta, int n_sweeps,
@@ -443,19 +474,13 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                 // kTeamRankChunk flipped rows at a time, the additions in flip order
                 for (int i = ib + tid; i < ie; i += kDenseTeamBlock) {
                     if (own && i >= i0 && i < i0 + wl) continue;
-                    T acc = f[i - ib];
-                    for (int m0 = 0; m0 < k; m0 += kTeamRankChunk) {
-                        float v[kTeamRankChunk];
-#pragma unroll
-                        for (int u = 0; u < kTeamRankChunk; ++u)
-                            v[u] = d.Q[size_t(i0 + s_t[min(m0 + u, k - 1)]) * size_t(n) + size_t(i)];
-#pragma unroll
-                        for (int u = 0; u < kTeamRankChunk; ++u) {
-                            const T nxt = acc + T(v[u]) * T(s_delta[min(m0 + u, k - 1)]);
-                            acc = m0 + u < k ? nxt : acc;
-                        }
-                    }
-                    f[i - ib] = acc;
+                    // the rows' loads of a chunk in flight together; few flips take a short chunk
+                    const T a = f[i - ib];
+                    f[i - ib] = k <= 1   ? add_rows<T, 1>(d, i0, i, k, s_t, s_delta, a)
+                                : k <= 2 ? add_rows<T, 2>(d, i0, i, k, s_t, s_delta, a)
+                                : k <= 4 ? add_rows<T, 4>(d, i0, i, k, s_t, s_delta, a)
+                                : k <= 8 ? add_rows<T, 8>(d, i0, i, k, s_t, s_delta, a)
+                                         : add_rows<T, kTeamRankChunk>(d, i0, i, k, s_t, s_delta, a);
                 }
             }
             if (staging) __builtin_amdgcn_s_waitcnt(0);   // the staged block has landed before the barrier
