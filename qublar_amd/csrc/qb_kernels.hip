@@ -460,8 +460,14 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
     if (pos >= 0) {
         xm = ld_l2(xg + pos);
         if (first && nn > 0) {
+            if constexpr (BW == 2) {
+                // the slot's two lanes load half of the masks each (swapped below)
 #pragma unroll
-            for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + P.q[u]);
+                for (int u = 0; u < kNb / 2; ++u) m[u] = ld_l2(xg + (b ? P.q[u + kNb / 2] : P.q[u]));
+            } else {
+#pragma unroll
+                for (int u = 0; u < kNb; ++u) m[u] = ld_l2(xg + P.q[u]);
+            }
         }
     }
     // the chain's shares and weights, from LDS while the loads are in flight
@@ -498,8 +504,21 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
     // the first kNb neighbours' disagreement with this branch's spin, one bit each
     uint32_t unlike = 0;
     if (first && nn > 0) {
+        if constexpr (BW == 2) {
+            // both lanes of the slot are here: each takes the other's half (DPP quad_perm 1,0,3,2)
+            uint16_t mm[kNb];
 #pragma unroll
-        for (int v = 0; v < kNb; ++v) unlike |= uint32_t(((m[v] >> b) & 1) != xb) << v;
+            for (int u = 0; u < kNb / 2; ++u) {
+                const uint16_t o = uint16_t(__builtin_amdgcn_mov_dpp(int(m[u]), 0xB1, 0xF, 0xF, false));
+                mm[u] = b ? o : m[u];
+                mm[u + kNb / 2] = b ? m[u] : o;
+            }
+#pragma unroll
+            for (int v = 0; v < kNb; ++v) unlike |= uint32_t(((mm[v] >> b) & 1) != xb) << v;
+        } else {
+#pragma unroll
+            for (int v = 0; v < kNb; ++v) unlike |= uint32_t(((m[v] >> b) & 1) != xb) << v;
+        }
     }
     if (pos >= 0) {
         double partial = j == 0 ? lin * delta : 0.0;
