@@ -72,3 +72,16 @@ def test_dense_recompute_returns_scratch_fields(orc):
     x2, f2 = orc.dense_anneal(Q, h, orc.schedule(1.0, 0.05, 6, 0), 9, 1, 0, fields=True)
     s2 = h.astype(np.float64) + Q.astype(np.float64) @ x2.astype(np.float64)
     assert 0 < np.max(np.abs(f2 - s2)) < 1e-4
+
+
+def test_dense_create_refuses_n_beyond_the_lds_limit():
+    # One replica's FP64 fields and spins and two staged 64 x 64 window blocks must fit 150 KB of
+    # LDS: 32768 + 9 n <= 153600, so n = 13,425 is the largest accepted. The size is refused
+    # before Q is read, so a small buffer stands in for the matrix.
+    import ctypes as C
+    Q = np.zeros(16, np.float32)
+    h = np.zeros(16, np.float32)
+    out = C.c_void_p()
+    rc = qa.lib().qb_dense_create(13426, Q.ctypes.data, h.ctypes.data, C.byref(out))
+    assert rc == qa.QB_E_LIMIT and not out.value
+    assert "150 KB" in qa.lib().qb_last_error().decode()
