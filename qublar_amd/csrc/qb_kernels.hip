@@ -115,7 +115,7 @@ __device__ __forceinline__ bool team_barrier(unsigned* words, int rank, int team
         while (true) {
             const unsigned v = lane < team_size ? ld_l2(words + lane) : target;
             if (__ballot(v < target) == 0) break;
-            if ((++polls & 1023u) == 0) {
+            if ((++polls & 1023u) == 0) [[unlikely]] {
                 if (ld_l2(err) != 0 || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
                     if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
@@ -558,7 +558,7 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
             const double nxt = dE + p.lambda * (((unlike >> v) & 1u) ? -1.0 : 1.0);
             dE = v < nn ? nxt : dE;
         }
-        if (nn > kNb) {
+        if (nn > kNb) [[unlikely]] {
             const int nb0 = Hd[slot].nb0;
             for (int k = nb0 + kNb; k < nb0 + nn; ++k) {
                 const int before = ((ld_l2(xg + p.nb[k]) >> b) & 1) != xb;
@@ -590,7 +590,7 @@ __device__ __forceinline__ void packed_step(const DevProblem& p, const DevState<
         }
     }
     if (pos >= 0 && accepted[slot - j][b]) {
-        if (slot_seq(info)) {
+        if (slot_seq(info)) [[unlikely]] {
             // a ray repeats in this column: the bit's first slot scatters it in CSR order
             if (j == 0)
                 for (int k = p.col_ptr[pos]; k < p.col_ptr[pos + 1]; ++k) {
@@ -1231,7 +1231,7 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 auto next_block = [&] {
                     if (pf) issue_block<BLOCK, kSlots>(bt.blk, meta_nxt, pn);
                 };
-                if (heavy) {
+                if (heavy) [[unlikely]] {
                     heavy_step<T, BW, BLOCK, STAGED>(p, s, bt, batch, g, sweep, beta, reinterpret_cast<RayEnt*>(pc),
                                                      reinterpret_cast<RayEnt*>(xa), s_part, s_acc, next_block);
                 } else {
@@ -1241,12 +1241,13 @@ ray_team_kernel(DevProblem p, DevBatches bt, DevState<T> s, int sweep_begin, int
                 meta_cur = meta_nxt;
                 buf ^= 1;
             }
-            light_sweep<T, BW, STAGED>(p, s, lbeg[c], cend[c + 1], my_groups, team, n_waves, light_base, sweep, beta);
+            if (lbeg[c] < cend[c + 1]) [[unlikely]]   // few classes hold light bits: keep their code out of line
+                light_sweep<T, BW, STAGED>(p, s, lbeg[c], cend[c + 1], my_groups, team, n_waves, light_base, sweep, beta);
             target += unsigned(team_size);
             if (!team_barrier(bar, rank, team_size, target, s.err, at_arrival)) return;
             synced = true;
         }
-        if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
+        if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) [[unlikely]] {
             const int tasks = p.n_rays * my_groups;
             for (int t = my_slot; t < tasks; t += n_slots) {
                 const int gi = t / p.n_rays;

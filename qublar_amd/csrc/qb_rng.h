@@ -144,12 +144,12 @@ QB_HD bool metropolis_uphill(double dE, double beta, uint64_t seed, uint32_t swe
 // qexp itself. For x <= -80, qexp(x) < 2e-35 lies below every nonzero 53-bit draw, so
 // only u = 0 can pass. Every decision is therefore qexp's, bit for bit.
 __device__ __forceinline__ bool uphill_accept(double x, double u) {
-    if (!(x > -80.0)) return u == 0.0 && qexp(x) > 0.0;
+    if (!(x > -80.0)) [[unlikely]] return u == 0.0 && qexp(x) > 0.0;
     constexpr double kBand = 0x1.0p-14;
     const double e = double(__builtin_amdgcn_exp2f(float(x) * 1.44269504f));
-    if (u < e * (1.0 - kBand)) return true;
-    if (u > e * (1.0 + kBand)) return false;
-    return u < qexp(x);
+    const bool below = u < e * (1.0 - kBand), above = u > e * (1.0 + kBand);
+    if (!below && !above) [[unlikely]] return u < qexp(x);   // inside the band
+    return below;
 }
 #endif
 
