@@ -19,9 +19,11 @@ def say(msg):
 
 
 # Up to 64 replicas (and n >= 256) each replica is annealed by four workgroups sharing its
-# fields; 66 replicas take one workgroup each. Both give the oracle's spins.
+# fields; 66 replicas take one workgroup each. Both give the oracle's spins, also when the
+# last window is short and odd (301, 259).
 @pytest.mark.parametrize("n,R,prec,k", [(300, 6, qa.FP64, 0), (300, 6, qa.FP32, 3), (300, 6, qa.FP32, 0),
-                                        (1024, 4, qa.FP32, 10), (300, 66, qa.FP32, 3), (200, 3, qa.FP64, 0)])
+                                        (1024, 4, qa.FP32, 10), (300, 66, qa.FP32, 3), (200, 3, qa.FP64, 0),
+                                        (301, 5, qa.FP32, 2), (259, 66, qa.FP64, 0)])
 def test_dense_equals_oracle(n, R, prec, k):
     Q, h = qa.dense_qubo(n, 7)
     D = qa.DenseQubo(Q, h)
