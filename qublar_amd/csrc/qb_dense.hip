@@ -65,17 +65,17 @@ __device__ __forceinline__ void scratch_fields(const DevDense& d, const uint8_t*
 // window hold columns nobody reads), and wave 1 makes its -log draws L[l]. The copy is
 // covered by the issuing waves' vmcnt: they drain it before the barrier that makes the
 // buffer visible.
-template <int BLOCK>
+template <int FIRST, int NW>
 __device__ __forceinline__ void stage_window(const DevDense& d, uint64_t key, int i0n, int sw, float* bq, double* L) {
     const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
     const int n = d.n, wln = min(kDenseWin, n - i0n);
     const int col = min(i0n + lane, n - 1);
-    for (int t = wave - 1; t < kDenseWin; t += BLOCK / 64 - 1) {
+    for (int t = wave - FIRST; t < kDenseWin; t += NW) {
         const float* src = d.Q + size_t(i0n + min(t, wln - 1)) * size_t(n) + size_t(col);
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                          (__attribute__((address_space(3))) void*)(bq + t * kDenseWin), 4, 0, 0);
     }
-    if (wave == 1) L[lane] = -qlog(draw_u01(key, uint32_t(sw), uint32_t(i0n + (lane < wln ? lane : 0))));
+    if (wave == FIRST) L[lane] = -qlog(draw_u01(key, uint32_t(sw), uint32_t(i0n + (lane < wln ? lane : 0))));
 }
 
 // One window's sequential decisions by one wave, lane l holding bit i0 + l: its field fl,
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
         x[i] = 0;
         f[i] = T(d.h[i]);
     }
-    if (tid >= 64 && n_sweeps > 0) stage_window<kDenseBlock>(d, key, 0, 0, blk, s_L[0]);
+    if (tid >= 64 && n_sweeps > 0) stage_window<1, kDenseBlock / 64 - 1>(d, key, 0, 0, blk, s_L[0]);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     int nbuf = 0;   // the buffer of the current window
@@ -183,8 +183,8 @@ __global__ void __launch_bounds__(kDenseBlock) dense_anneal_kernel(DevDense d, c
             const int n0 = i0 + kDenseWin < n ? i0 + kDenseWin : 0;
             const bool staging = n0 > 0 || sweep + 1 < n_sweeps;
             if (tid >= 64 && staging)
-                stage_window<kDenseBlock>(d, key, n0, n0 > 0 ? sweep : sweep + 1,
-                                          blk + (nbuf ^ 1) * kDenseWin * kDenseWin, s_L[nbuf ^ 1]);
+                stage_window<1, kDenseBlock / 64 - 1>(d, key, n0, n0 > 0 ? sweep : sweep + 1,
+                                                      blk + (nbuf ^ 1) * kDenseWin * kDenseWin, s_L[nbuf ^ 1]);
             if (tid < 64) {
                 const int lane = tid;
                 const bool live = lane < w;
@@ -313,7 +313,7 @@ static DenseTeamWork dense_team_work(void* work, int n, int R) {
 }
 
 size_t dense_team_lds_bytes(int n, int precision) {
-    return 2 * sizeof(float) * kDenseWin * kDenseWin +
+    return 3 * sizeof(float) * kDenseWin * kDenseWin +
            size_t(dense_part_fields_max(n)) * (precision == 1 ? sizeof(float) : sizeof(double)) + size_t(n);
 }
 
@@ -367,15 +367,12 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = d.n;
     const int n_win = (n + kDenseWin - 1) / kDenseWin;
-    __shared__ int s_xcd, s_rank, s_k, s_ok;
-    __shared__ int s_t[kDenseWin];
-    __shared__ float s_delta[kDenseWin];
+    __shared__ int s_xcd, s_rank;
     __shared__ double s_red[kDenseTeamBlock / 64][2];
     const int tid = int(threadIdx.x);
     if (tid == 0) {
         s_xcd = int(xcc_id() & 7u);
         s_rank = int(__hip_atomic_fetch_add(tw.tickets + s_xcd * 64, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        s_ok = 1;
     }
     __syncthreads();
     const int rank = s_rank;
@@ -385,50 +382,101 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
     const int part = rank % kDenseParts;
     const int wb = part * n_win / kDenseParts, we = (part + 1) * n_win / kDenseParts;
     const int ib = wb * kDenseWin, ie = min(we * kDenseWin, n);
-    // two staging buffers for the part's next owned windows: their Q blocks and -log draws
+    // three staging buffers (window G's block and draws in buffer G % 3)
     float* blk = reinterpret_cast<float*>(smem);
-    T* f = reinterpret_cast<T*>(blk + 2 * kDenseWin * kDenseWin);   // fields of bits [ib, ie)
+    T* f = reinterpret_cast<T*>(blk + 3 * kDenseWin * kDenseWin);   // fields of bits [ib, ie)
     uint8_t* x = reinterpret_cast<uint8_t*>(f + dense_part_fields_max(n));
-    __shared__ double s_L[2][kDenseWin];
+    __shared__ double s_L[3][kDenseWin];
+    // the flip ring: window G's flips (t, delta) and count in slot G % 4
+    __shared__ int s_rt[4][kDenseWin];
+    __shared__ float s_rd[4][kDenseWin];
+    __shared__ int s_rk[4];
+    // progress words: windows whose flips are in the ring; windows whose next-window update is
+    // done (wave 1); windows whose bulk update and staging are done (waves 2..7, one word each)
+    __shared__ int s_flips, s_aprog, s_abort;
+    __shared__ int s_bprog[kDenseTeamBlock / 64 - 2];
+    constexpr int kNB = kDenseTeamBlock / 64 - 2;
     const uint64_t key = seed_offset + uint64_t(r) + 1u;
-    auto stage = [&](int wn, int sw, int q) {
-        stage_window<kDenseTeamBlock>(d, key, wn * kDenseWin, sw, blk + q * kDenseWin * kDenseWin, s_L[q]);
+    const int wave = tid >> 6, lane = tid & 63;
+    const int T_total = n_sweeps * n_win;
+    auto own_g = [&](int G) { const int w = G % n_win; return w >= wb && w < we; };
+    auto stage = [&](int G) {
+        stage_window<2, kNB>(d, key, (G % n_win) * kDenseWin, G / n_win, blk + (G % 3) * kDenseWin * kDenseWin,
+                             s_L[G % 3]);
+    };
+    // Waits (one wave) until *p >= v; false once the workgroup aborts or after ~2 s (then it
+    // aborts, with *err set).
+    auto wait_ge = [&](int* p, int v) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        unsigned polls = 0;
+        while (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < v) {
+            if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+            __builtin_amdgcn_s_sleep(1);
+            if ((++polls & 1023u) == 0 && __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                if (lane == 0) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                return false;
+            }
+        }
+        return true;
+    };
+    auto wait_b = [&](int v) {
+        for (int j = 0; j < kNB; ++j)
+            if (!wait_ge(&s_bprog[j], v)) return false;
+        return true;
+    };
+    auto release = [&](int* p, int v) {
+        if (lane == 0) __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
     for (int i = tid; i < n; i += kDenseTeamBlock) x[i] = 0;
     for (int i = ib + tid; i < ie; i += kDenseTeamBlock) f[i - ib] = T(d.h[i]);
-    if (tid >= 64 && wb == 0 && n_sweeps > 0) stage(0, 0, 0);   // later parts stage their first in the loop
+    if (tid == 0) {
+        s_flips = 0;
+        s_aprog = 0;
+        s_abort = 0;
+    }
+    if (tid < kNB) s_bprog[tid] = 0;
+    if (wave >= 2) {
+        for (int G = 0; G < 2 && G < T_total; ++G)
+            if (own_g(G)) stage(G);
+    }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    int nbuf = 0;   // the buffer of the part's next owned window
     for (int sweep = 0; sweep < n_sweeps; ++sweep) {
         const double b = beta[sweep];
         for (int w = 0; w < n_win; ++w) {
+            const int G = sweep * n_win + w;
             const int i0 = w * kDenseWin;
             const int wl = min(kDenseWin, n - i0);
             const bool own = w >= wb && w < we;
-            const unsigned idx = unsigned(sweep) * unsigned(n_win) + unsigned(w) + 1u;
-            uint32_t* slot = tw.slots + ((size_t(r) * 2 + size_t(sweep & 1)) * size_t(n_win) + size_t(w)) * kDenseSlotWords;
-            // the next window, if this part owns it, is staged during this one
-            const int wn = w + 1 < n_win ? w + 1 : 0, swn = w + 1 < n_win ? sweep : sweep + 1;
-            const bool staging = wn >= wb && wn < we && swn < n_sweeps;
-            if (tid >= 64 && staging) stage(wn, swn, own ? nbuf ^ 1 : nbuf);
-            if (tid < 64) {
-                const int lane = tid;
+            const int q = G & 3;
+            const bool next_own = G + 1 < T_total && own_g(G + 1);
+            if (wave == 0) {
+                // the ring slot is free once window G - 4 is fully applied
+                if (!wait_ge(&s_aprog, G - 3) || !wait_b(G - 3)) return;
+                const unsigned idx = unsigned(G) + 1u;
+                uint32_t* slot = tw.slots + ((size_t(r) * 2 + size_t(sweep & 1)) * size_t(n_win) + size_t(w)) * kDenseSlotWords;
                 if (own) {
+                    // the window's fields hold every earlier window's flips once wave 1 has
+                    // added window G - 1's (it did so after the bulk of window G - 2)
+                    if (!wait_ge(&s_aprog, G)) return;
                     const bool live = lane < wl;
                     const int i = i0 + (live ? lane : 0);
-                    const double L = s_L[nbuf][lane];
+                    const double L = s_L[G % 3][lane];
                     T fl = f[i - ib];
                     int xl = x[i];
-                    const int k = decide_window<T>(wl, b, L, blk + nbuf * kDenseWin * kDenseWin, fl, xl, s_t, s_delta);
+                    const int k = decide_window<T>(wl, b, L, blk + (G % 3) * kDenseWin * kDenseWin, fl, xl, s_rt[q],
+                                                   s_rd[q]);
                     if (live) {
                         f[i - ib] = fl;
                         x[i] = uint8_t(xl);
                     }
-                    if (lane == 0) s_k = k;
+                    if (lane == 0) s_rk[q] = k;
                     // publish the flips, every word tagged: lane 0 the count, lane q > 0 flips
                     // 2q - 2 and 2q - 1 in bits 16-23 and 24-31
-                    __builtin_amdgcn_s_waitcnt(0);   // s_t / s_delta written (LDS) before they are read
+                    __builtin_amdgcn_s_waitcnt(0);   // the ring written (LDS) before it is read
                     if (lane < kDenseSlotWords) {
                         uint32_t word = dense_tag(idx);
                         if (lane == 0) {
@@ -437,7 +485,7 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
 #pragma unroll
                             for (int u = 0; u < 2; ++u) {
                                 const int m = 2 * (lane - 1) + u;
-                                if (m < k) word |= uint32_t(s_t[m] | (s_delta[m] < 0.0f ? 0x80 : 0)) << (16 + 8 * u);
+                                if (m < k) word |= uint32_t(s_rt[q][m] | (s_rd[q][m] < 0.0f ? 0x80 : 0)) << (16 + 8 * u);
                             }
                         }
                         __hip_atomic_store(slot + lane, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -445,60 +493,81 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                 } else {
                     uint32_t word = 0;
                     if (!dense_wait_slot(slot, dense_tag(idx), word, err)) {
-                        if (lane == 0) s_ok = 0;
-                    } else {
-                        const int k = int((__builtin_amdgcn_readfirstlane(int(word)) >> 16) & 0xFF);
+                        if (lane == 0) __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        return;
+                    }
+                    const int k = int((__builtin_amdgcn_readfirstlane(int(word)) >> 16) & 0xFF);
 #pragma unroll
-                        for (int u = 0; u < 2; ++u) {
-                            const int m = 2 * (lane - 1) + u;
-                            if (lane >= 1 && lane < kDenseSlotWords && m < k) {
-                                const uint32_t c = (word >> (16 + 8 * u)) & 0xFFu;
-                                const int t = int(c & 0x7Fu);
-                                s_t[m] = t;
-                                s_delta[m] = (c & 0x80u) ? -1.0f : 1.0f;
-                                x[i0 + t] = (c & 0x80u) ? 0 : 1;
-                            }
+                    for (int u = 0; u < 2; ++u) {
+                        const int m = 2 * (lane - 1) + u;
+                        if (lane >= 1 && lane < kDenseSlotWords && m < k) {
+                            const uint32_t c = (word >> (16 + 8 * u)) & 0xFFu;
+                            const int t = int(c & 0x7Fu);
+                            s_rt[q][m] = t;
+                            s_rd[q][m] = (c & 0x80u) ? -1.0f : 1.0f;
+                            x[i0 + t] = (c & 0x80u) ? 0 : 1;
                         }
-                        if (lane == 0) s_k = k;
+                    }
+                    if (lane == 0) s_rk[q] = k;
+                }
+                release(&s_flips, G + 1);
+            } else if (wave == 1) {
+                // the next window's 64 fields get this window's flips, after the bulk of window
+                // G - 1 has given them that window's
+                if (!wait_ge(&s_flips, G + 1)) return;
+                if (next_own) {
+                    if (!wait_b(G)) return;
+                    const int k = s_rk[q];
+                    const int i = ((G + 1) % n_win) * kDenseWin + lane;
+                    if (k > 0 && i < min(ie, n)) {
+                        const T a = f[i - ib];
+                        f[i - ib] = k <= 1   ? add_rows<T, 1>(d, i0, i, k, s_rt[q], s_rd[q], a)
+                                    : k <= 2 ? add_rows<T, 2>(d, i0, i, k, s_rt[q], s_rd[q], a)
+                                    : k <= 4 ? add_rows<T, 4>(d, i0, i, k, s_rt[q], s_rd[q], a)
+                                    : k <= 8 ? add_rows<T, 8>(d, i0, i, k, s_rt[q], s_rd[q], a)
+                                             : add_rows<T, kTeamRankChunk>(d, i0, i, k, s_rt[q], s_rd[q], a);
                     }
                 }
-            }
-            __syncthreads();
-            if (!s_ok) {
-                __builtin_amdgcn_s_waitcnt(0);   // no staging load outlives the workgroup
-                return;
-            }
-            const int k = s_k;
-            if (k > 0) {
-                // this part's fields, the window's own bits excluded (the owner updated them),
-                // kTeamRankChunk flipped rows at a time, the additions in flip order
-                for (int i = ib + tid; i < ie; i += kDenseTeamBlock) {
-                    if (own && i >= i0 && i < i0 + wl) continue;
-                    // the rows' loads of a chunk in flight together; few flips take a short chunk
-                    const T a = f[i - ib];
-                    f[i - ib] = k <= 1   ? add_rows<T, 1>(d, i0, i, k, s_t, s_delta, a)
-                                : k <= 2 ? add_rows<T, 2>(d, i0, i, k, s_t, s_delta, a)
-                                : k <= 4 ? add_rows<T, 4>(d, i0, i, k, s_t, s_delta, a)
-                                : k <= 8 ? add_rows<T, 8>(d, i0, i, k, s_t, s_delta, a)
-                                         : add_rows<T, kTeamRankChunk>(d, i0, i, k, s_t, s_delta, a);
+                __builtin_amdgcn_s_waitcnt(0);
+                release(&s_aprog, G + 1);
+            } else {
+                // waves 2..: stage window G + 2 (its buffer's last reader, window G - 1, is
+                // decided), then add this window's flips to the part's other fields
+                if (G + 2 < T_total && own_g(G + 2)) stage(G + 2);
+                if (!wait_ge(&s_flips, G + 1)) return;
+                const int k = s_rk[q];
+                if (k > 0) {
+                    const int i_next = next_own ? ((G + 1) % n_win) * kDenseWin : -1 - kDenseWin;
+                    for (int i = ib + tid - 128; i < ie; i += kDenseTeamBlock - 128) {
+                        if (own && i >= i0 && i < i0 + wl) continue;
+                        if (i >= i_next && i < i_next + kDenseWin) continue;
+                        const T a = f[i - ib];
+                        f[i - ib] = k <= 1   ? add_rows<T, 1>(d, i0, i, k, s_rt[q], s_rd[q], a)
+                                    : k <= 2 ? add_rows<T, 2>(d, i0, i, k, s_rt[q], s_rd[q], a)
+                                    : k <= 4 ? add_rows<T, 4>(d, i0, i, k, s_rt[q], s_rd[q], a)
+                                    : k <= 8 ? add_rows<T, 8>(d, i0, i, k, s_rt[q], s_rd[q], a)
+                                             : add_rows<T, kTeamRankChunk>(d, i0, i, k, s_rt[q], s_rd[q], a);
+                    }
                 }
+                __builtin_amdgcn_s_waitcnt(0);   // the staged rows have landed too
+                release(&s_bprog[wave - 2], G + 1);
             }
-            if (staging) __builtin_amdgcn_s_waitcnt(0);   // the staged block has landed before the barrier
-            __syncthreads();
-            if (own) nbuf ^= 1;
         }
         if (recompute_every > 0 && (sweep + 1) % recompute_every == 0) {
-            // this part's fields from scratch in FP64 (all windows of the sweep applied)
+            // every wave is past the sweep's last window: the fields from scratch in FP64
+            __syncthreads();
             for (int g0 = ib + tid; g0 < ie; g0 += kFieldGroup * kDenseTeamBlock) {
                 double sc[kFieldGroup];
                 scratch_fields<kDenseTeamBlock>(d, x, g0, sc);
 #pragma unroll
-                for (int q = 0; q < kFieldGroup; ++q)
-                    if (g0 + q * kDenseTeamBlock < ie) f[g0 + q * kDenseTeamBlock - ib] = T(sc[q]);
+                for (int qq = 0; qq < kFieldGroup; ++qq)
+                    if (g0 + qq * kDenseTeamBlock < ie) f[g0 + qq * kDenseTeamBlock - ib] = T(sc[qq]);
             }
             __syncthreads();
         }
     }
+    __syncthreads();
+    if (s_abort) return;
     // outputs: this part's spins, its share of the FP64 energy and of max |f - f_scratch|
     double e = 0.0, er = 0.0;
     for (int g0 = ib + tid; g0 < ie; g0 += kFieldGroup * kDenseTeamBlock) {
