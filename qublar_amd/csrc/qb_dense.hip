@@ -358,6 +358,32 @@ __device__ __forceinline__ T add_rows(const DevDense& d, int i0, int i, int k, c
     return acc;
 }
 
+// Fields col[j] (j < NF) plus the window's k flipped rows, acc[j] + Q[i0 + s_t[m]][col[j]]
+// s_delta[m] in flip order, the CHUNK rows' loads of every field in flight together (past the
+// last flip the index repeats it; its value is unused).
+template <typename T, int CHUNK, int NF>
+__device__ __forceinline__ void add_rows_fields(const DevDense& d, int i0, const int (&col)[NF], int k, const int* s_t,
+                                                const float* s_delta, T (&acc)[NF]) {
+    for (int m0 = 0; m0 < k; m0 += CHUNK) {
+        float v[NF][CHUNK];
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) {
+            const size_t row = size_t(i0 + s_t[min(m0 + u, k - 1)]) * size_t(d.n);
+#pragma unroll
+            for (int j = 0; j < NF; ++j) v[j][u] = d.Q[row + size_t(col[j])];
+        }
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) {
+            const T dl = T(s_delta[min(m0 + u, k - 1)]);
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                const T nxt = acc[j] + T(v[j][u]) * dl;
+                acc[j] = m0 + u < k ? nxt : acc[j];
+            }
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d, DenseTeamWork tw, int R,
                                                                  const double* beta, int n_sweeps,
@@ -396,6 +422,8 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
     __shared__ int s_flips, s_aprog, s_abort;
     __shared__ int s_bprog[kDenseTeamBlock / 64 - 2];
     constexpr int kNB = kDenseTeamBlock / 64 - 2;
+    constexpr int kBulk = kDenseTeamBlock - 128;   // threads of the bulk update
+    constexpr int kBulkFields = 3;                 // fields a bulk thread updates together
     const uint64_t key = seed_offset + uint64_t(r) + 1u;
     const int wave = tid >> 6, lane = tid & 63;
     const int T_total = n_sweeps * n_win;
@@ -537,16 +565,28 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                 if (!wait_ge(&s_flips, G + 1)) return;
                 const int k = s_rk[q];
                 if (k > 0) {
+                    // kBulkFields fields a thread, their row loads in flight together; a field
+                    // of the own or the next own window takes part in the loads only
                     const int i_next = next_own ? ((G + 1) % n_win) * kDenseWin : -1 - kDenseWin;
-                    for (int i = ib + tid - 128; i < ie; i += kDenseTeamBlock - 128) {
-                        if (own && i >= i0 && i < i0 + wl) continue;
-                        if (i >= i_next && i < i_next + kDenseWin) continue;
-                        const T a = f[i - ib];
-                        f[i - ib] = k <= 1   ? add_rows<T, 1>(d, i0, i, k, s_rt[q], s_rd[q], a)
-                                    : k <= 2 ? add_rows<T, 2>(d, i0, i, k, s_rt[q], s_rd[q], a)
-                                    : k <= 4 ? add_rows<T, 4>(d, i0, i, k, s_rt[q], s_rd[q], a)
-                                    : k <= 8 ? add_rows<T, 8>(d, i0, i, k, s_rt[q], s_rd[q], a)
-                                             : add_rows<T, kTeamRankChunk>(d, i0, i, k, s_rt[q], s_rd[q], a);
+                    for (int g = ib + tid - 128; g < ie; g += kBulkFields * kBulk) {
+                        int col[kBulkFields];
+                        bool use[kBulkFields];
+                        T a[kBulkFields];
+#pragma unroll
+                        for (int j = 0; j < kBulkFields; ++j) {
+                            const int i = g + j * kBulk;
+                            use[j] = i < ie && !(own && i >= i0 && i < i0 + wl) && !(i >= i_next && i < i_next + kDenseWin);
+                            col[j] = i < ie ? i : ib;
+                            a[j] = f[col[j] - ib];
+                        }
+                        if (k <= 1) add_rows_fields<T, 1, kBulkFields>(d, i0, col, k, s_rt[q], s_rd[q], a);
+                        else if (k <= 2) add_rows_fields<T, 2, kBulkFields>(d, i0, col, k, s_rt[q], s_rd[q], a);
+                        else if (k <= 4) add_rows_fields<T, 4, kBulkFields>(d, i0, col, k, s_rt[q], s_rd[q], a);
+                        else if (k <= 8) add_rows_fields<T, 8, kBulkFields>(d, i0, col, k, s_rt[q], s_rd[q], a);
+                        else add_rows_fields<T, kTeamRankChunk, kBulkFields>(d, i0, col, k, s_rt[q], s_rd[q], a);
+#pragma unroll
+                        for (int j = 0; j < kBulkFields; ++j)
+                            if (use[j]) f[col[j] - ib] = a[j];
                     }
                 }
                 __builtin_amdgcn_s_waitcnt(0);   // the staged rows have landed too
