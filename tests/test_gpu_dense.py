@@ -23,7 +23,8 @@ def say(msg):
 # last window is short and odd (301, 259).
 @pytest.mark.parametrize("n,R,prec,k", [(300, 6, qa.FP64, 0), (300, 6, qa.FP32, 3), (300, 6, qa.FP32, 0),
                                         (1024, 4, qa.FP32, 10), (300, 66, qa.FP32, 3), (200, 3, qa.FP64, 0),
-                                        (301, 5, qa.FP32, 2), (259, 66, qa.FP64, 0)])
+                                        (301, 5, qa.FP32, 2), (259, 66, qa.FP64, 0), (520, 64, qa.FP32, 5),
+                                        (1000, 7, qa.FP64, 0)])
 def test_dense_equals_oracle(n, R, prec, k):
     Q, h = qa.dense_qubo(n, 7)
     D = qa.DenseQubo(Q, h)
