@@ -450,9 +450,24 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
         }
         return true;
     };
-    auto wait_b = [&](int v) {
-        for (int j = 0; j < kNB; ++j)
-            if (!wait_ge(&s_bprog[j], v)) return false;
+    // Waits (one wave) until s_aprog >= va and every s_bprog[j] >= vb: lane j < kNB reads
+    // s_bprog[j] and lane kNB s_aprog, all in one load a poll. False as for wait_ge.
+    auto wait_ab = [&](int va, int vb) {
+        int* w = lane < kNB ? &s_bprog[lane] : &s_aprog;
+        const int need = lane < kNB ? vb : lane == kNB ? va : INT_MIN;
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        unsigned polls = 0;
+        while (__ballot(__hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) != 0) {
+            if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+            __builtin_amdgcn_s_sleep(1);
+            if ((++polls & 1023u) == 0 && __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                if (lane == 0) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                return false;
+            }
+        }
         return true;
     };
     auto release = [&](int* p, int v) {
@@ -483,7 +498,7 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
             const bool next_own = G + 1 < T_total && own_g(G + 1);
             if (wave == 0) {
                 // the ring slot is free once window G - 4 is fully applied
-                if (!wait_ge(&s_aprog, G - 3) || !wait_b(G - 3)) return;
+                if (!wait_ab(G - 3, G - 3)) return;
                 const unsigned idx = unsigned(G) + 1u;
                 uint32_t* slot = tw.slots + ((size_t(r) * 2 + size_t(sweep & 1)) * size_t(n_win) + size_t(w)) * kDenseSlotWords;
                 if (own) {
@@ -544,7 +559,7 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                 // G - 1 has given them that window's
                 if (!wait_ge(&s_flips, G + 1)) return;
                 if (next_own) {
-                    if (!wait_b(G)) return;
+                    if (!wait_ab(0, G)) return;
                     const int k = s_rk[q];
                     const int i = ((G + 1) % n_win) * kDenseWin + lane;
                     if (k > 0 && i < min(ie, n)) {
