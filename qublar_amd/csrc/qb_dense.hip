@@ -517,6 +517,9 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                         x[i] = uint8_t(xl);
                     }
                     if (lane == 0) s_rk[q] = k;
+                    // the workgroup's waves need only the ring: raise the word before the slot
+                    // stores, so that its release does not wait for them
+                    release(&s_flips, G + 1);
                     // publish the flips, every word tagged: lane 0 the count, lane q > 0 flips
                     // 2q - 2 and 2q - 1 in bits 16-23 and 24-31
                     __builtin_amdgcn_s_waitcnt(0);   // the ring written (LDS) before it is read
@@ -552,8 +555,8 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                         }
                     }
                     if (lane == 0) s_rk[q] = k;
+                    release(&s_flips, G + 1);
                 }
-                release(&s_flips, G + 1);
             } else if (wave == 1) {
                 // the next window's 64 fields get this window's flips, after the bulk of window
                 // G - 1 has given them that window's
