@@ -450,11 +450,12 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
         }
         return true;
     };
-    // Waits (one wave) until s_aprog >= va and every s_bprog[j] >= vb: lane j < kNB reads
-    // s_bprog[j] and lane kNB s_aprog, all in one load a poll. False as for wait_ge.
-    auto wait_ab = [&](int va, int vb) {
-        int* w = lane < kNB ? &s_bprog[lane] : &s_aprog;
-        const int need = lane < kNB ? vb : lane == kNB ? va : INT_MIN;
+    // Waits (one wave) until s_aprog >= va, every s_bprog[j] >= vb and s_flips >= vf: lane
+    // j < kNB reads s_bprog[j], lane kNB s_aprog and lane kNB + 1 s_flips, all in one load a
+    // poll. False as for wait_ge.
+    auto wait_ab = [&](int va, int vb, int vf) {
+        int* w = lane < kNB ? &s_bprog[lane] : lane == kNB ? &s_aprog : &s_flips;
+        const int need = lane < kNB ? vb : lane == kNB ? va : lane == kNB + 1 ? vf : INT_MIN;
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
         unsigned polls = 0;
         while (__ballot(__hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) != 0) {
@@ -498,13 +499,12 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
             const bool next_own = G + 1 < T_total && own_g(G + 1);
             if (wave == 0) {
                 // the ring slot is free once window G - 4 is fully applied
-                if (!wait_ab(G - 3, G - 3)) return;
+                // (an owned window's fields hold every earlier window's flips once wave 1 has
+                // added window G - 1's, after the bulk of window G - 2)
+                if (!wait_ab(own ? G : G - 3, G - 3, INT_MIN)) return;
                 const unsigned idx = unsigned(G) + 1u;
                 uint32_t* slot = tw.slots + ((size_t(r) * 2 + size_t(sweep & 1)) * size_t(n_win) + size_t(w)) * kDenseSlotWords;
                 if (own) {
-                    // the window's fields hold every earlier window's flips once wave 1 has
-                    // added window G - 1's (it did so after the bulk of window G - 2)
-                    if (!wait_ge(&s_aprog, G)) return;
                     const bool live = lane < wl;
                     const int i = i0 + (live ? lane : 0);
                     const double L = s_L[G % 3][lane];
@@ -560,9 +560,8 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
             } else if (wave == 1) {
                 // the next window's 64 fields get this window's flips, after the bulk of window
                 // G - 1 has given them that window's
-                if (!wait_ge(&s_flips, G + 1)) return;
+                if (!wait_ab(0, next_own ? G : INT_MIN, G + 1)) return;
                 if (next_own) {
-                    if (!wait_ab(0, G)) return;
                     const int k = s_rk[q];
                     const int i = ((G + 1) % n_win) * kDenseWin + lane;
                     if (k > 0 && i < min(ie, n)) {
