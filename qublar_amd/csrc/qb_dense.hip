@@ -384,6 +384,25 @@ __device__ __forceinline__ void add_rows_fields(const DevDense& d, int i0, const
     }
 }
 
+// Field i plus the window's k flipped rows in flip order, as add_rows, with flip m's bit and
+// sign in lane m of tl and dl (read across the wave), CHUNK rows' loads at a time.
+template <typename T, int CHUNK>
+__device__ __forceinline__ T add_rows_lanes(const DevDense& d, int i0, int i, int k, int tl, float dl, T acc) {
+    for (int m0 = 0; m0 < k; m0 += CHUNK) {
+        float v[CHUNK];
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u)
+            v[u] = d.Q[size_t(i0 + __builtin_amdgcn_readlane(tl, min(m0 + u, k - 1))) * size_t(d.n) + size_t(i)];
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) {
+            const float dm = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dl), min(m0 + u, k - 1)));
+            const T nxt = acc + T(v[u]) * T(dm);
+            acc = m0 + u < k ? nxt : acc;
+        }
+    }
+    return acc;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d, DenseTeamWork tw, int R,
                                                                  const double* beta, int n_sweeps,
@@ -562,15 +581,21 @@ __global__ void __launch_bounds__(kDenseTeamBlock) dense_team_kernel(DevDense d,
                 // G - 1 has given them that window's
                 if (!wait_ab(0, next_own ? G : INT_MIN, G + 1)) return;
                 if (next_own) {
-                    const int k = s_rk[q];
+                    // the count and every lane's flip read together; the branch is uniform, so
+                    // every lane holds its flip when the rows are read across the wave
+                    const int k = __builtin_amdgcn_readfirstlane(s_rk[q]);
+                    const int tl = s_rt[q][lane];
+                    const float dl = s_rd[q][lane];
                     const int i = ((G + 1) % n_win) * kDenseWin + lane;
-                    if (k > 0 && i < min(ie, n)) {
-                        const T a = f[i - ib];
-                        f[i - ib] = k <= 1   ? add_rows<T, 1>(d, i0, i, k, s_rt[q], s_rd[q], a)
-                                    : k <= 2 ? add_rows<T, 2>(d, i0, i, k, s_rt[q], s_rd[q], a)
-                                    : k <= 4 ? add_rows<T, 4>(d, i0, i, k, s_rt[q], s_rd[q], a)
-                                    : k <= 8 ? add_rows<T, 8>(d, i0, i, k, s_rt[q], s_rd[q], a)
-                                             : add_rows<T, kTeamRankChunk>(d, i0, i, k, s_rt[q], s_rd[q], a);
+                    if (k > 0) {
+                        const int ic = min(i, min(ie, n) - 1);
+                        const T a = f[ic - ib];
+                        const T r = k <= 1   ? add_rows_lanes<T, 1>(d, i0, ic, k, tl, dl, a)
+                                    : k <= 2 ? add_rows_lanes<T, 2>(d, i0, ic, k, tl, dl, a)
+                                    : k <= 4 ? add_rows_lanes<T, 4>(d, i0, ic, k, tl, dl, a)
+                                    : k <= 8 ? add_rows_lanes<T, 8>(d, i0, ic, k, tl, dl, a)
+                                             : add_rows_lanes<T, kTeamRankChunk>(d, i0, ic, k, tl, dl, a);
+                        if (i < min(ie, n)) f[i - ib] = r;
                     }
                 }
                 __builtin_amdgcn_s_waitcnt(0);
